@@ -1,0 +1,211 @@
+/*
+ * ocfd_hip.h -- C-ABI of the MI355X (gfx950) implementation of OpenCFD-Comb's
+ * per-step hot path.
+ *
+ * Every entry point replaces one reference subroutine (cited as
+ * file:line relative to the OpenCFD-Comb source tree).  The reference works
+ * on Fortran module globals; this library keeps one global context holding
+ * device-resident mirrors of those globals, so the argument lists below stay
+ * as close to the reference's as a C ABI allows:
+ *   - reals are double (OCFD_REAL_KIND=8, code/head/OCFD_precision.f90:7),
+ *     integers int32_t (default INTEGER);
+ *   - host arrays are in Fortran layout, i fastest, variable index slowest;
+ *     "halo" arrays span (1-LAP:nx+LAP, 1-LAP:ny+LAP, 1-LAP:nz+LAP) and the
+ *     pointer is to element (1-LAP,1-LAP,1-LAP); "interior" arrays span
+ *     (1:nx,1:ny,1:nz) (code/head/OCFD_allocate_data.f90:7-22);
+ *   - every call returns int32 status: 0 ok, >0 error, with the text in
+ *     ocfd_last_error().  The reference prints and STOPs on errors
+ *     (e.g. code/mpi-io/OCFD_readpara.f90:312-315); the Fortran shim in the
+ *     integration notes maps a non-zero status to print + stop.
+ *   - not thread-safe: one host thread, one HIP stream (the library's own).
+ *
+ * Only one GPU and one rank: the MPI halo exchange
+ * (code/mpi-io/OCFD_mpi.f90:186-236) becomes on-device periodic index
+ * wrapping, and MPI_ALLREDUCE in the hybrid sensor becomes a device
+ * reduction.
+ */
+#ifndef OCFD_HIP_H
+#define OCFD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Scheme / splitting / BC codes: identical values to OCFD_constants
+ * (code/head/MODULE_parameter.f90:14-26). */
+enum {
+  OCFD_SCHEME_WENO5 = 1, OCFD_SCHEME_WENO7 = 2, OCFD_SCHEME_OMP6 = 3,
+  OCFD_SCHEME_UD7L = 4, OCFD_SCHEME_CD6 = 5, OCFD_SCHEME_CD8 = 6,
+  OCFD_SCHEME_HYBRID = 10
+};
+enum { OCFD_SPLIT_SW = 1, OCFD_SPLIT_LLF = 2 };
+enum { OCFD_BC_NONE = 0, OCFD_BC_BOUNDARYLAYER = 2, OCFD_BC_JET = 4 };
+
+/* Run configuration: the subset of TYPE_Para / TYPE_Scheme / Transport
+ * (code/head/MODULE_parameter.f90:49-94, code/Chemical/TRANSPORT_modules.f90:16-21)
+ * that the hot path reads.  Field meanings and defaults follow the namelist
+ * /control_opencfd/ (code/mpi-io/OCFD_readpara.f90:41-110). */
+typedef struct ocfd_cfg {
+  int32_t nx, ny, nz;          /* interior points (global == local: one GPU) */
+  int32_t lap;                 /* halo width LAP; must be 4 */
+  int32_t n_spec;              /* N_SPEC */
+  int32_t periodic[3];         /* Iperiodic_X/Y/Z */
+  int32_t scheme_invis;        /* OCFD_SCHEME_* */
+  int32_t scheme_vis;          /* OCFD_SCHEME_CD6 or OCFD_SCHEME_CD8 */
+  int32_t flux_splitting;      /* OCFD_SPLIT_SW or OCFD_SPLIT_LLF */
+  int32_t if_viscous;          /* IF_Viscous */
+  int32_t if_scheme_character; /* IF_Scheme_Character (only 0 supported) */
+  int32_t scheme_boundary[6];  /* Scheme_boundary(1:6): 0, 1 or -1 */
+  int32_t ghost_cell[6];       /* Ghost_Cell(1:6) (only 0 supported) */
+  int32_t ibc;                 /* OCFD_BC_* */
+  int32_t if_mass_force;       /* IF_Mass_Force */
+  int32_t if_real_gas;         /* IF_RealGas (0 perfect gas, 1 NASA fits) */
+  int32_t flag_vis, flag_cond, flag_diff; /* Trans%Flag_* (0: Sutherland/Pr/Sc) */
+  int32_t wall_catalysis;      /* Para%Wall_Catalysis (only 0 supported) */
+  int32_t pad0;
+  double  ud7l_diss;           /* UD7L_Diss */
+  double  hybrid_para[16];     /* Hybrid_para(1:16) */
+  double  amu0, pr, sc;        /* Trans%Amu0, Trans%Pr, Trans%Sc */
+  double  mass_force[3];       /* Mass_Force(1:3) */
+  double  bc_para[32];         /* BC_Para(1:32) */
+} ocfd_cfg;
+
+/* Field identifiers for ocfd_upload_field / ocfd_download_field. */
+enum {
+  OCFD_FIELD_F = 0,   /* f(nx,ny,nz,5)            interior x5      */
+  OCFD_FIELD_FN,      /* fn(nx,ny,nz,5)           interior x5      */
+  OCFD_FIELD_DU,      /* du(nx,ny,nz,5)           interior x5      */
+  OCFD_FIELD_DI,      /* di(halo,N_SPEC)          halo x N_SPEC    */
+  OCFD_FIELD_D, OCFD_FIELD_U, OCFD_FIELD_V, OCFD_FIELD_W,
+  OCFD_FIELD_T, OCFD_FIELD_P, OCFD_FIELD_ET,            /* halo       */
+  OCFD_FIELD_AMU, OCFD_FIELD_AMK,                       /* interior   */
+  OCFD_FIELD_RHYBRID,                                   /* halo       */
+  OCFD_FIELD_AKX, OCFD_FIELD_AKY, OCFD_FIELD_AKZ,       /* halo metrics */
+  OCFD_FIELD_AIX, OCFD_FIELD_AIY, OCFD_FIELD_AIZ,
+  OCFD_FIELD_ASX, OCFD_FIELD_ASY, OCFD_FIELD_ASZ, OCFD_FIELD_AJAC,
+  OCFD_FIELD_AKX1, OCFD_FIELD_AKY1, OCFD_FIELD_AKZ1,
+  OCFD_FIELD_AIX1, OCFD_FIELD_AIY1, OCFD_FIELD_AIZ1,
+  OCFD_FIELD_ASX1, OCFD_FIELD_ASY1, OCFD_FIELD_ASZ1,
+  OCFD_FIELD_DIN,     /* din(nx,ny,nz,N_SPEC)     interior x N_SPEC */
+  OCFD_FIELD_COUNT
+};
+
+/* Hybrid-scheme masks (Scm_Hbx/y/z, code/flow-solver/OCFD_inviscous.f90:63-68),
+ * kept as uint8 per interface: x (0:nx,ny,nz), y (nx,0:ny,nz), z (nx,ny,0:nz). */
+enum { OCFD_MASK_X = 0, OCFD_MASK_Y = 1, OCFD_MASK_Z = 2 };
+
+/* Options for ocfd_set_option. */
+enum {
+  /* Global mean of |grad p| for hybrid sensor type != 1
+   * (MPI_ALLREDUCE at code/schemes/OCFD_SetHybrid.f90:64):
+   *   0 = ordered: one sequential sum in (k,j,i) order, bit-identical to a
+   *       single-rank reference run (slow: one wavefront);
+   *   1 = fast: parallel tree sum (mask may differ where R is within
+   *       rounding of a threshold). */
+  OCFD_OPT_HYBRID_SUM_MODE = 1,
+  /* 1 = capture the RK3 step in a hipGraph on first use and replay it. */
+  OCFD_OPT_USE_GRAPH = 2,
+  /* 1 = bracket every kernel launch with HIP events (per-kernel timing). */
+  OCFD_OPT_KERNEL_TIMING = 3
+};
+
+/* ---- lifetime ---------------------------------------------------------- */
+/* Replaces allocate_flow_data / allocate_inviscous_data / allocate_vicous_data
+ * and set_parameters (code/head/OCFD_allocate_data.f90:2-68,
+ * code/flow-solver/OCFD_inviscous.f90:5-31, code/flow-solver/OCFD_viscous.f90:5-16,
+ * code/mpi-io/OCFD_readpara.f90:347-431).  Device arrays get the reference's
+ * initial values (d=u=v=w=p=Et=1, T=300 in the halo, metrics 1). */
+int32_t ocfd_init(const ocfd_cfg* cfg);
+int32_t ocfd_finalize(void);
+const char* ocfd_last_error(void);
+int32_t ocfd_set_option(int32_t key, int32_t value);
+
+/* Species thermo (read_Spec, code/Chemical/CHEM_thermdyn.f90:16-51):
+ * mi[n_spec] (kg/mol); coef[k][j][7] = (A,B,C,D,E,F,G) of range j of
+ * species k (n_spec x n_tct x 7); tct[n_tct-1]; tct_max.  For a perfect
+ * gas only mi is used (coef may be NULL). */
+int32_t ocfd_set_species(int32_t n_tct, const double* tct, double tct_max,
+                         const double* mi, const double* coef);
+
+/* Mesh: coordinates Axx, Ayy, Azz (halo arrays; only the interior is read).
+ * Computes the halo coordinate fill, periodic span shift, the 9 metrics and
+ * Jacobian and the Akx1 = Akx/Ajac (etc.) arrays on the device
+ * (init_mesh / comput_Jacobian3d / updata_Axyz_periodic,
+ * code/flow-solver/OCFD_mesh.f90:1-185).  span[9] = Periodic_ISpan(1:3),
+ * Periodic_JSpan(1:3), Periodic_KSpan(1:3). */
+int32_t ocfd_set_mesh(const double* axx, const double* ayy, const double* azz,
+                      const double* span);
+
+/* ---- state transfer ---------------------------------------------------- */
+int32_t ocfd_upload_field(int32_t field, const double* host);
+int32_t ocfd_download_field(int32_t field, double* host);
+int32_t ocfd_download_mask(int32_t dir, uint8_t* host);
+
+/* Boundary-layer BC data (ocfd_bc_BoundaryLayer and its init routines,
+ * code/boundary-condition/OCFD_BC_Boundarylayers.f90:20-243,258-489):
+ *   inlet1d[(5+N_SPEC) x ny]  (column-major (ny,5+N_SPEC): d,u,v,w,T,di..),
+ *                               already rescaled so sum(di)=d;
+ *   upper0d[5+N_SPEC];
+ *   nonreflect_upper[nx*nz]    (int32, 1 = non-reflecting, 0 = Dirichlet);
+ *   wall_pertb[nx*nz]          (fx*gz of the Rai blow/suction).
+ * The random phases of the reference are drawn by the caller. */
+int32_t ocfd_set_bc_boundarylayer(const double* inlet1d, const double* upper0d,
+                                  const int32_t* nonreflect_upper,
+                                  const double* wall_pertb);
+/* ht(tt) of get_ht_multifrequancy (OCFD_BC_Boundarylayers.f90:594-621),
+ * evaluated by the caller once per time step (it depends only on tt). */
+int32_t ocfd_set_bc_time(double ht);
+
+/* ---- the hot path ------------------------------------------------------ */
+/* One TVD-RK3 step of size dt1 (flow_time_advance,
+ * code/flow-solver/OCFD_Time_Adv.f90:7-92).  Non-reacting, N_SPEC==1 path. */
+int32_t ocfd_flow_time_advance(double dt1);
+
+/* Per-stage pieces, for a host that keeps the reference's own stage loop
+ * (OCFD_Time_Adv.f90:33-91).  Called in this order they reproduce
+ * ocfd_flow_time_advance exactly:
+ *   ocfd_save_stage0();
+ *   for krk = 1..3:
+ *     ocfd_stage_begin(krk); ocfd_du_inviscous(); ocfd_du_viscous();
+ *     ocfd_time_adv_rk3(krk, dt1); ocfd_comput_flow_variables(); ocfd_bc(); */
+int32_t ocfd_save_stage0(void);             /* fn=f, din=di (OCFD_Time_Adv.f90:10-29) */
+/* Stage prologue, OCFD_Time_Adv.f90:35-62: comput_Amu_Amk_AMD, the halo
+ * exchanges (periodic wrap, nothing to do), di=d (N_SPEC=1) and, at krk=1
+ * with the hybrid scheme, comput_Rhybrid. */
+int32_t ocfd_stage_begin(int32_t krk);
+int32_t ocfd_comput_amu_amk(void);          /* Comput_flow_variables.f90:69-98 */
+int32_t ocfd_comput_rhybrid(void);          /* OCFD_SetHybrid.f90:17-143 + masks */
+int32_t ocfd_du_inviscous(void);            /* OCFD_inviscous.f90:47-202 */
+int32_t ocfd_du_viscous(void);              /* OCFD_viscous.f90:27-131 */
+int32_t ocfd_time_adv_rk3(int32_t krk, double dt); /* OCFD_Time_Adv.f90:98-130 */
+int32_t ocfd_comput_flow_variables(void);   /* Comput_flow_variables.f90:3-46 */
+int32_t ocfd_bc(void);                      /* OCFD_BC.f90:2-188 */
+
+/* Largest number of cells repaired by handle_NegativeEt in one
+ * comput_flow_variables since the last ocfd_save_stage0 / step start
+ * (code/others/OCFD_handle_NegativeT.f90:1-20).  More than 10 makes the next
+ * synchronising call (and the next step) return an error, as the reference
+ * stops.  Stepping calls are asynchronous: errors surface at
+ * ocfd_synchronize, a download or the next step. */
+int32_t ocfd_negative_et_count(void);
+
+/* ---- timing ------------------------------------------------------------ */
+/* Synchronises the library stream. */
+int32_t ocfd_synchronize(void);
+/* The library's HIP stream (hipStream_t) as an opaque pointer. */
+void*   ocfd_stream(void);
+/* Per-kernel accumulated time (ms) and launch count since the last reset,
+ * recorded when OCFD_OPT_KERNEL_TIMING is on.  Kernel names are returned
+ * through ocfd_kernel_name(i) for i < ocfd_kernel_count(). */
+int32_t ocfd_kernel_count(void);
+const char* ocfd_kernel_name(int32_t i);
+int32_t ocfd_kernel_stats(int32_t i, double* total_ms, int64_t* launches);
+int32_t ocfd_kernel_stats_reset(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OCFD_HIP_H */

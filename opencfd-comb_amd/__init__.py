@@ -1,0 +1,274 @@
+"""MI355X-native per-step hot path of OpenCFD-Comb (host side).
+
+The compute lives in ``libocfd_hip.so`` (HIP, gfx950), built from ``csrc/`` by
+``build.py`` and reached through the C-ABI declared in
+``include/ocfd_hip.h``.  This module is the Python mirror of the reference's
+subroutine interface for the path: ``FlowSolver`` exposes
+``flow_time_advance(dt1)`` and the per-stage pieces under the reference's
+names (``code/flow-solver/OCFD_Time_Adv.f90:7-130``,
+``code/flow-solver/OCFD_inviscous.f90:47``, ``code/flow-solver/OCFD_viscous.f90:27``,
+``code/boundary-condition/OCFD_BC.f90:2``).
+
+Arrays follow the reference's Fortran layout: halo arrays are numpy arrays of
+shape ``(nx+8, ny+8, nz+8)`` in Fortran order (index 0 is the reference's
+``1-LAP``), interior 5-variable arrays have shape ``(nx, ny, nz, 5)``.
+
+There is no CPU fallback: if the HIP library is missing every entry point
+raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libocfd_hip.so")
+LAP = 4
+
+# codes of OCFD_constants (code/head/MODULE_parameter.f90:14-26)
+SCHEME_WENO5, SCHEME_WENO7, SCHEME_OMP6, SCHEME_UD7L, SCHEME_CD6, SCHEME_CD8 = 1, 2, 3, 4, 5, 6
+SCHEME_HYBRID = 10
+SPLIT_SW, SPLIT_LLF = 1, 2
+BC_NONE, BC_BOUNDARYLAYER, BC_JET = 0, 2, 4
+
+# field identifiers (include/ocfd_hip.h)
+FIELDS = ["F", "FN", "DU", "DI", "D", "U", "V", "W", "T", "P", "ET", "AMU", "AMK", "RHYBRID",
+          "AKX", "AKY", "AKZ", "AIX", "AIY", "AIZ", "ASX", "ASY", "ASZ", "AJAC",
+          "AKX1", "AKY1", "AKZ1", "AIX1", "AIY1", "AIZ1", "ASX1", "ASY1", "ASZ1", "DIN"]
+FIELD = {name: i for i, name in enumerate(FIELDS)}
+METRICS = FIELDS[FIELD["AKX"]:FIELD["ASZ1"] + 1]
+
+OPT_HYBRID_SUM_MODE, OPT_USE_GRAPH, OPT_KERNEL_TIMING = 1, 2, 3
+
+
+class OcfdCfg(ctypes.Structure):
+    """Mirror of ``ocfd_cfg`` (include/ocfd_hip.h)."""
+
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("lap", ctypes.c_int32), ("n_spec", ctypes.c_int32),
+        ("periodic", ctypes.c_int32 * 3),
+        ("scheme_invis", ctypes.c_int32), ("scheme_vis", ctypes.c_int32),
+        ("flux_splitting", ctypes.c_int32), ("if_viscous", ctypes.c_int32),
+        ("if_scheme_character", ctypes.c_int32),
+        ("scheme_boundary", ctypes.c_int32 * 6), ("ghost_cell", ctypes.c_int32 * 6),
+        ("ibc", ctypes.c_int32), ("if_mass_force", ctypes.c_int32), ("if_real_gas", ctypes.c_int32),
+        ("flag_vis", ctypes.c_int32), ("flag_cond", ctypes.c_int32), ("flag_diff", ctypes.c_int32),
+        ("wall_catalysis", ctypes.c_int32), ("pad0", ctypes.c_int32),
+        ("ud7l_diss", ctypes.c_double), ("hybrid_para", ctypes.c_double * 16),
+        ("amu0", ctypes.c_double), ("pr", ctypes.c_double), ("sc", ctypes.c_double),
+        ("mass_force", ctypes.c_double * 3), ("bc_para", ctypes.c_double * 32),
+    ]
+
+
+def make_cfg(nx, ny, nz, *, periodic=(1, 1, 1), scheme_invis=SCHEME_WENO7, scheme_vis=SCHEME_CD6,
+             flux_splitting=SPLIT_SW, if_viscous=1, ibc=BC_NONE, ud7l_diss=1.0, hybrid_para=None,
+             amu0=1.789e-5, pr=0.72, sc=1.0, mass_force=None, bc_para=None, scheme_boundary=None,
+             n_spec=1, if_real_gas=0):
+    """Build an ``ocfd_cfg`` with the namelist defaults of the reference
+    (code/mpi-io/OCFD_readpara.f90:65-110) for fields not given."""
+    c = OcfdCfg()
+    c.nx, c.ny, c.nz, c.lap, c.n_spec = nx, ny, nz, LAP, n_spec
+    for a in range(3):
+        c.periodic[a] = int(periodic[a])
+    c.scheme_invis, c.scheme_vis, c.flux_splitting = scheme_invis, scheme_vis, flux_splitting
+    c.if_viscous, c.if_scheme_character, c.ibc = if_viscous, 0, ibc
+    for a in range(6):
+        c.scheme_boundary[a] = 0 if scheme_boundary is None else int(scheme_boundary[a])
+        c.ghost_cell[a] = 0
+    c.if_mass_force = 1 if mass_force is not None else 0
+    c.if_real_gas = if_real_gas
+    c.flag_vis = c.flag_cond = c.flag_diff = 0
+    c.wall_catalysis = 0
+    c.ud7l_diss = ud7l_diss
+    hp = [1.0] * 16 if hybrid_para is None else list(hybrid_para) + [1.0] * (16 - len(hybrid_para))
+    for a in range(16):
+        c.hybrid_para[a] = hp[a]
+    c.amu0, c.pr, c.sc = amu0, pr, sc
+    mf = mass_force or (0.0, 0.0, 0.0)
+    for a in range(3):
+        c.mass_force[a] = mf[a]
+    bp = list(bc_para or []) + [0.0] * (32 - len(bc_para or []))
+    for a in range(32):
+        c.bc_para[a] = bp[a]
+    return c
+
+
+_LIB = None
+
+
+def load_library(path=None):
+    """Load the HIP library; raise if it has not been built."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"HIP library not built: {p} (run __graft_entry__.build())")
+    lib = ctypes.CDLL(p)
+    i32, dbl, vp = ctypes.c_int32, ctypes.c_double, ctypes.c_void_p
+    P = ctypes.POINTER
+    sig = {
+        "ocfd_init": (i32, [P(OcfdCfg)]), "ocfd_finalize": (i32, []),
+        "ocfd_last_error": (ctypes.c_char_p, []), "ocfd_set_option": (i32, [i32, i32]),
+        "ocfd_set_species": (i32, [i32, vp, dbl, vp, vp]),
+        "ocfd_set_mesh": (i32, [vp, vp, vp, vp]),
+        "ocfd_upload_field": (i32, [i32, vp]), "ocfd_download_field": (i32, [i32, vp]),
+        "ocfd_download_mask": (i32, [i32, vp]),
+        "ocfd_set_bc_boundarylayer": (i32, [vp, vp, vp, vp]), "ocfd_set_bc_time": (i32, [dbl]),
+        "ocfd_flow_time_advance": (i32, [dbl]), "ocfd_save_stage0": (i32, []),
+        "ocfd_stage_begin": (i32, [i32]), "ocfd_comput_amu_amk": (i32, []),
+        "ocfd_comput_rhybrid": (i32, []), "ocfd_du_inviscous": (i32, []), "ocfd_du_viscous": (i32, []),
+        "ocfd_time_adv_rk3": (i32, [i32, dbl]), "ocfd_comput_flow_variables": (i32, []),
+        "ocfd_bc": (i32, []), "ocfd_negative_et_count": (i32, []), "ocfd_synchronize": (i32, []),
+        "ocfd_stream": (vp, []), "ocfd_kernel_count": (i32, []), "ocfd_kernel_name": (ctypes.c_char_p, [i32]),
+        "ocfd_kernel_stats": (i32, [i32, P(dbl), P(ctypes.c_int64)]), "ocfd_kernel_stats_reset": (i32, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype, fn.argtypes = res, args
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class OcfdError(RuntimeError):
+    pass
+
+
+class FlowSolver:
+    """Device-resident flow state and the hot-path entry points.
+
+    Method names follow the reference subroutines; every call goes through
+    the HIP library (there is no host fallback)."""
+
+    def __init__(self, cfg, lib=None):
+        self.lib = lib or load_library()
+        self.cfg = cfg
+        self.nx, self.ny, self.nz = cfg.nx, cfg.ny, cfg.nz
+        self.halo_shape = (cfg.nx + 2 * LAP, cfg.ny + 2 * LAP, cfg.nz + 2 * LAP)
+        self._check(self.lib.ocfd_init(ctypes.byref(cfg)))
+
+    # -- plumbing -------------------------------------------------------
+    def _check(self, rc):
+        if rc != 0:
+            raise OcfdError(self.lib.ocfd_last_error().decode())
+
+    def close(self):
+        self.lib.ocfd_finalize()
+
+    def set_option(self, key, value):
+        self._check(self.lib.ocfd_set_option(key, int(value)))
+
+    def field_shape(self, name):
+        if name in ("F", "FN", "DU"):
+            return (self.nx, self.ny, self.nz, 5)
+        if name in ("AMU", "AMK"):
+            return (self.nx, self.ny, self.nz)
+        if name == "DIN":
+            return (self.nx, self.ny, self.nz, self.cfg.n_spec)
+        if name == "DI":
+            return self.halo_shape + (self.cfg.n_spec,)
+        return self.halo_shape
+
+    def upload(self, name, arr):
+        a = np.asfortranarray(arr, dtype=np.float64)
+        if a.shape != self.field_shape(name):
+            raise ValueError(f"{name}: expected shape {self.field_shape(name)}, got {a.shape}")
+        self._check(self.lib.ocfd_upload_field(FIELD[name], _ptr(a)))
+
+    def download(self, name):
+        a = np.zeros(self.field_shape(name), dtype=np.float64, order="F")
+        self._check(self.lib.ocfd_download_field(FIELD[name], _ptr(a)))
+        return a
+
+    def download_mask(self, direction):
+        shp = [self.nx, self.ny, self.nz]
+        shp[direction] += 1
+        a = np.zeros(tuple(shp), dtype=np.uint8, order="F")
+        self._check(self.lib.ocfd_download_mask(direction, _ptr(a)))
+        return a
+
+    # -- setup ----------------------------------------------------------
+    def set_species(self, mi, n_tct=1, tct=None, tct_max=5000.0, coef=None):
+        mi = np.ascontiguousarray(mi, dtype=np.float64)
+        t = None if tct is None else np.ascontiguousarray(tct, dtype=np.float64)
+        cf = None if coef is None else np.ascontiguousarray(coef, dtype=np.float64)
+        self._check(self.lib.ocfd_set_species(n_tct, None if t is None else _ptr(t), tct_max, _ptr(mi),
+                                              None if cf is None else _ptr(cf)))
+
+    def set_mesh(self, axx, ayy, azz, span=None):
+        arrs = [np.asfortranarray(a, dtype=np.float64) for a in (axx, ayy, azz)]
+        sp = np.zeros(9) if span is None else np.ascontiguousarray(span, dtype=np.float64)
+        self._check(self.lib.ocfd_set_mesh(_ptr(arrs[0]), _ptr(arrs[1]), _ptr(arrs[2]), _ptr(sp)))
+
+    def set_bc_boundarylayer(self, inlet1d, upper0d, nonreflect_upper, wall_pertb):
+        inl = np.asfortranarray(inlet1d, dtype=np.float64)
+        up = np.ascontiguousarray(upper0d, dtype=np.float64)
+        nr = np.asfortranarray(nonreflect_upper, dtype=np.int32)
+        wp = np.asfortranarray(wall_pertb, dtype=np.float64)
+        self._check(self.lib.ocfd_set_bc_boundarylayer(_ptr(inl), _ptr(up), _ptr(nr), _ptr(wp)))
+
+    def set_bc_time(self, ht):
+        self._check(self.lib.ocfd_set_bc_time(float(ht)))
+
+    def upload_state(self, state):
+        """state: dict with F, DI, D, U, V, W, T, P, ET (reference layouts)."""
+        for name in ("F", "DI", "D", "U", "V", "W", "T", "P", "ET"):
+            self.upload(name, state[name])
+
+    # -- hot path (reference names) -------------------------------------
+    def flow_time_advance(self, dt1):
+        self._check(self.lib.ocfd_flow_time_advance(float(dt1)))
+
+    def save_stage0(self):
+        self._check(self.lib.ocfd_save_stage0())
+
+    def stage_begin(self, krk):
+        self._check(self.lib.ocfd_stage_begin(int(krk)))
+
+    def comput_Amu_Amk_AMD(self):
+        self._check(self.lib.ocfd_comput_amu_amk())
+
+    def comput_Rhybrid(self):
+        self._check(self.lib.ocfd_comput_rhybrid())
+
+    def du_inviscous(self):
+        self._check(self.lib.ocfd_du_inviscous())
+
+    def du_viscous(self):
+        self._check(self.lib.ocfd_du_viscous())
+
+    def OCFD_time_adv_RK3(self, krk, dt):
+        self._check(self.lib.ocfd_time_adv_rk3(int(krk), float(dt)))
+
+    def comput_flow_variables(self):
+        self._check(self.lib.ocfd_comput_flow_variables())
+
+    def OCFD_bc(self):
+        self._check(self.lib.ocfd_bc())
+
+    def negative_et_count(self):
+        return int(self.lib.ocfd_negative_et_count())
+
+    def synchronize(self):
+        self._check(self.lib.ocfd_synchronize())
+
+    # -- timing ---------------------------------------------------------
+    def kernel_stats(self):
+        out = {}
+        for i in range(self.lib.ocfd_kernel_count()):
+            ms, n = ctypes.c_double(), ctypes.c_int64()
+            self._check(self.lib.ocfd_kernel_stats(i, ctypes.byref(ms), ctypes.byref(n)))
+            out[self.lib.ocfd_kernel_name(i).decode()] = (ms.value, n.value)
+        return out
+
+    def kernel_stats_reset(self):
+        self._check(self.lib.ocfd_kernel_stats_reset())
+
+
+__all__ = ["OcfdCfg", "make_cfg", "load_library", "FlowSolver", "OcfdError", "FIELD", "FIELDS", "LAP"]

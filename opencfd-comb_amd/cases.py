@@ -1,0 +1,275 @@
+"""Case set-up for the flow path: meshes, initial states and BC data.
+
+Host-side numpy code that plays the role of the reference's case generators
+(cases/isotropic/init-isotropic.f90, cases/flat-2.25/sc2-data/Grid-and-init.f90)
+and of init3d (code/flow-solver/OCFD_init.f90:1-25).  It runs once, before the
+hot path, and its outputs are uploaded through the C-ABI.
+
+Synthetic inputs (no data files are read):
+  * isotropic(n): decaying compressible isotropic turbulence in a periodic box
+    of side 2*pi, calorically perfect air, uniform rho and T, a seeded random
+    solenoidal velocity with E(k) ~ k^4 exp(-2 (k/k0)^2), k0 = 4, scaled to
+    turbulent Mach 0.5 and Re_lambda = 72 (definitions of ana_box,
+    code/others/OCFD_ANA.f90:197-303).
+  * flat_plate(nx, ny, nz): Mach 2.25 flat-plate boundary layer, T_inf =
+    169.44 K, Re = 635000/in, isothermal wall T_w = 1.9 T_inf, geometric
+    wall-normal stretching, compressible laminar similarity inflow, Rai
+    blow/suction strip with seeded phases, periodic span.
+"""
+import math
+
+import numpy as np
+
+from . import (LAP, BC_BOUNDARYLAYER, SCHEME_HYBRID, SCHEME_WENO7, SCHEME_CD6, SPLIT_SW, make_cfg)
+
+R0 = 8.314          # CHEM_modules.f90:20
+MI_AIR = 0.02897    # kg/mol
+R_AIR = R0 / MI_AIR
+GAMMA = 1.4
+SUTH_TA, SUTH_TC = 288.15, 110.4   # TRANSPORT_modules.f90:265
+AMU0 = 1.789e-5
+
+
+def sutherland(T, amu0=AMU0):
+    tr = T / SUTH_TA
+    return amu0 * np.sqrt(tr * tr * tr) * (SUTH_TA + SUTH_TC) / (SUTH_TC + T)
+
+
+def halo_zeros(nx, ny, nz, value=0.0):
+    return np.full((nx + 2 * LAP, ny + 2 * LAP, nz + 2 * LAP), value, dtype=np.float64, order="F")
+
+
+def interior(a):
+    return a[LAP:-LAP, LAP:-LAP, LAP:-LAP]
+
+
+def state_from_primitives(d, u, v, w, T, R=R_AIR):
+    """Host image of init3d (OCFD_init.f90:1-25) for N_SPEC=1, perfect gas.
+
+    d,u,v,w,T are interior arrays.  Returns the dict of reference arrays
+    (halo arrays carry the allocate_flow_data ghost values d=u=v=w=p=Et=1,
+    T=300, OCFD_allocate_data.f90:53-55)."""
+    nx, ny, nz = d.shape
+    st = {}
+    for name, val, src in (("D", 1.0, d), ("U", 1.0, u), ("V", 1.0, v), ("W", 1.0, w), ("T", 300.0, T)):
+        a = halo_zeros(nx, ny, nz, val)
+        interior(a)[...] = src
+        st[name] = a
+    di = halo_zeros(nx, ny, nz, 1.0)
+    interior(di)[...] = d
+    # Rescaling_di with one species: di*(d/di)
+    interior(di)[...] = interior(di) * (d / interior(di))
+    st["DI"] = di[..., None].copy(order="F")
+    Et = 0.0 + 2.5 * R * interior(di) * T       # comput_E_pgas
+    f = np.zeros((nx, ny, nz, 5), order="F")
+    f[..., 0] = d
+    f[..., 1] = d * u
+    f[..., 2] = d * v
+    f[..., 3] = d * w
+    f[..., 4] = Et + d * (u * u + v * v + w * w) * 0.5
+    st["F"] = f
+    st["P"] = halo_zeros(nx, ny, nz, 1.0)
+    st["ET"] = halo_zeros(nx, ny, nz, 1.0)
+    return st
+
+
+def mesh_1d(x, y, z):
+    """GRID1D coordinates as halo arrays (read_mesh1d layout)."""
+    nx, ny, nz = len(x), len(y), len(z)
+    axx, ayy, azz = (halo_zeros(nx, ny, nz, 1.0) for _ in range(3))
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    interior(axx)[...] = X
+    interior(ayy)[...] = Y
+    interior(azz)[...] = Z
+    return axx, ayy, azz
+
+
+# --------------------------------------------------------------------------
+def isotropic(n=64, seed=1, k0=4.0, mach_t=0.5, re_lambda=72.0, T0=288.15, scheme_invis=SCHEME_WENO7,
+              cfl=0.5):
+    """Decaying isotropic turbulence (the 64^3 WENO7 / Steger-Warming / CD6 case)."""
+    L = 2.0 * math.pi
+    h = L / n
+    x = np.arange(n) * h
+    cfg = make_cfg(n, n, n, periodic=(1, 1, 1), scheme_invis=scheme_invis, scheme_vis=SCHEME_CD6,
+                   flux_splitting=SPLIT_SW, if_viscous=1, pr=0.72, amu0=AMU0)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    kv = np.fft.fftfreq(n, d=1.0 / n)
+    KX, KY, KZ = np.meshgrid(kv, kv, kv, indexing="ij")
+    K2 = KX ** 2 + KY ** 2 + KZ ** 2
+    K = np.sqrt(K2)
+    Ek = np.where(K > 0, K ** 4 * np.exp(-2.0 * (K / k0) ** 2), 0.0)
+    # random complex field, made solenoidal by projection, amplitude per shell
+    U = [rng.standard_normal(K.shape) + 1j * rng.standard_normal(K.shape) for _ in range(3)]
+    K2s = np.where(K2 > 0, K2, 1.0)
+    kdotu = KX * U[0] + KY * U[1] + KZ * U[2]
+    U = [U[0] - KX * kdotu / K2s, U[1] - KY * kdotu / K2s, U[2] - KZ * kdotu / K2s]
+    amp = np.sqrt(np.sum([np.abs(c) ** 2 for c in U], axis=0))
+    scale = np.where((K > 0) & (amp > 0), np.sqrt(Ek / np.maximum(4 * np.pi * K2s, 1e-300)) / np.maximum(amp, 1e-300), 0.0)
+    vel = [np.real(np.fft.ifftn(c * scale)) for c in U]
+    c0 = math.sqrt(GAMMA * R_AIR * T0)
+    urms2 = np.mean(vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2)
+    s = mach_t * c0 / math.sqrt(urms2)
+    vel = [a * s for a in vel]
+    # Re_lambda = q * lambda * rho / mu, lambda = q / rms(du/dx)   (ana_box)
+    q = math.sqrt(np.mean(vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2) / 3.0)
+    dudx = np.real(np.fft.ifftn(1j * KX * np.fft.fftn(vel[0])))
+    lam = q / math.sqrt(np.mean(dudx ** 2))
+    mu0 = float(sutherland(T0))
+    rho0 = re_lambda * mu0 / (q * lam)
+    d = np.full((n, n, n), rho0)
+    T = np.full((n, n, n), T0)
+    st = state_from_primitives(d, vel[0], vel[1], vel[2], T)
+    mesh = mesh_1d(x, x, x)
+    span = np.array([L, 0, 0, 0, L, 0, 0, 0, L], dtype=np.float64)
+    umax = (np.abs(vel[0]) + c0) / h + (np.abs(vel[1]) + c0) / h + (np.abs(vel[2]) + c0) / h
+    dt = cfl / float(np.max(umax))
+    info = dict(name="isotropic", rho0=rho0, T0=T0, mach_t=mach_t, re_lambda=re_lambda, dt=dt, n=n)
+    return cfg, mesh, span, st, info
+
+
+# --------------------------------------------------------------------------
+def similarity_profile(mach, T_inf, T_wall, pr=0.72, n=2001, eta_max=12.0):
+    """Compressible laminar boundary-layer similarity solution (Levy-Lees
+    variables, Sutherland viscosity, constant Pr):
+        (C f'')' + f f'' = 0
+        (C th'/Pr)' + f th' + (gamma-1) M^2 C f''^2 = 0,  C = th^-1 mu(th)/mu_inf
+    with f(0)=f'(0)=0, th(0)=T_w/T_inf, f'(inf)=th(inf)=1.  Returns eta,
+    f, f' and th = T/T_inf."""
+    from scipy.integrate import solve_bvp
+
+    tw = T_wall / T_inf
+
+    def C(th):
+        T = th * T_inf
+        return sutherland(T) / sutherland(T_inf) / th
+
+    def rhs(eta, y):
+        f, fp, g, th, q = y  # g = C f'', q = C th'/Pr
+        c = C(th)
+        fpp = g / c
+        thp = q * pr / c
+        return np.vstack([fp, fpp, -f * fpp, thp, -f * thp - (GAMMA - 1.0) * mach ** 2 * c * fpp ** 2])
+
+    def bc(ya, yb):
+        return np.array([ya[0], ya[1], ya[3] - tw, yb[1] - 1.0, yb[3] - 1.0])
+
+    eta = np.linspace(0.0, eta_max, 400)
+    fp0 = np.tanh(eta)
+    th0 = tw + (1.0 - tw) * np.tanh(eta)
+    y0 = np.vstack([eta - np.tanh(eta) * 0.9, fp0, 0.5 * (1 - fp0 ** 2), th0, 0.1 * (1 - fp0 ** 2)])
+    sol = solve_bvp(rhs, bc, eta, y0, tol=1e-8, max_nodes=100000)
+    if not sol.success:
+        raise RuntimeError("similarity solver did not converge: " + sol.message)
+    e = np.linspace(0.0, eta_max, n)
+    y = sol.sol(e)
+    return e, y[0], y[1], y[3]
+
+
+def stretched_y(ny, dy1, ly):
+    """Geometric wall-normal spacing with first spacing dy1 and height ly."""
+    lo, hi = 1.0 + 1e-9, 2.0
+    for _ in range(200):
+        r = 0.5 * (lo + hi)
+        if dy1 * (r ** (ny - 1) - 1.0) / (r - 1.0) > ly:
+            hi = r
+        else:
+            lo = r
+    r = 0.5 * (lo + hi)
+    return np.concatenate([[0.0], np.cumsum(dy1 * r ** np.arange(ny - 1))])
+
+
+def flat_plate(nx=640, ny=120, nz=64, seed=2, x0=0.1, lx=0.2, ly=0.012, lz=0.004445, theta=(2.0, 10.0),
+               cfl=0.5):
+    """Mach 2.25 spatially developing flat-plate boundary layer (hybrid
+    UD7L/WENO7/WENO5 scheme switched by the pressure-gradient sensor)."""
+    mach, T_inf = 2.25, 169.44
+    re_per_m = 635000.0 / 0.0254
+    c_inf = math.sqrt(GAMMA * R_AIR * T_inf)
+    U_inf = mach * c_inf
+    mu_inf = float(sutherland(T_inf))
+    rho_inf = re_per_m * mu_inf / U_inf
+    p_inf = rho_inf * R_AIR * T_inf
+    T_wall = 1.9 * T_inf
+    # mesh: x uniform, y stretched (first cell y+ ~ 0.5 for Cf ~ 2e-3), z uniform periodic
+    rho_w = p_inf / (R_AIR * T_wall)
+    nu_w = float(sutherland(T_wall)) / rho_w
+    u_tau = U_inf * math.sqrt(1.0e-3)
+    dy1 = 0.5 * nu_w / u_tau
+    x = x0 + np.arange(nx) * (lx / (nx - 1))
+    y = stretched_y(ny, dy1, ly)
+    z = np.arange(nz) * (lz / nz)
+    eta, f, fp, th = similarity_profile(mach, T_inf, T_wall)
+    # physical wall distance of each eta at station x: y = sqrt(2 nu x / U) * int_0^eta th
+    Y_eta = np.concatenate([[0.0], np.cumsum(0.5 * (th[1:] + th[:-1]) * np.diff(eta))])
+
+    def profile_at(xs):
+        scale = math.sqrt(2.0 * mu_inf / rho_inf * xs / U_inf)
+        yy = Y_eta * scale
+        uu = np.interp(y, yy, fp, right=1.0) * U_inf
+        tt = np.interp(y, yy, th, right=1.0) * T_inf
+        vv = np.zeros_like(y)  # wall-normal velocity of the inflow profile is set to zero
+        dd = p_inf / (R_AIR * tt)
+        return dd, uu, vv, tt
+
+    d = np.empty((nx, ny, nz))
+    u = np.empty_like(d)
+    v = np.zeros_like(d)
+    w = np.zeros_like(d)
+    T = np.empty_like(d)
+    for i in range(nx):
+        dd, uu, vv, tt = profile_at(x[i])
+        d[i] = dd[:, None]
+        u[i] = uu[:, None]
+        T[i] = tt[:, None]
+    st = state_from_primitives(d, u, v, w, T)
+    mesh = mesh_1d(x, y, z)
+    span = np.array([0, 0, 0, 0, 0, 0, 0, 0, lz], dtype=np.float64)
+    # inlet data (ny, 5+N_SPEC): d,u,v,w,T,di (di rescaled so sum = d)
+    dd, uu, vv, tt = profile_at(x[0])
+    inlet = np.zeros((ny, 6), order="F")
+    inlet[:, 0], inlet[:, 1], inlet[:, 2], inlet[:, 4], inlet[:, 5] = dd, uu, vv, tt, dd
+    inlet[:, 5] = inlet[:, 5] * inlet[:, 0] / inlet[:, 5]
+    upper = np.array([rho_inf, U_inf, 0.0, 0.0, T_inf, rho_inf])
+    # upper boundary: auto type with a flat top -> non-reflecting everywhere
+    # (init_bc_nonRef, OCFD_BC_Boundarylayers.f90:406-443)
+    nonref = np.ones((nx, nz), dtype=np.int32, order="F")
+    # Rai blow/suction strip (init_bc_wall_perturbation, :446-489), seeded phases
+    rng = np.random.Generator(np.random.PCG64(seed))
+    mz = mt = 10
+    xb, xe = 0.1143, 0.1270
+    faiz = rng.random(mz)
+    zl = np.array([1.25 ** (-k) for k in range(mz)])
+    zl /= zl.sum()
+    X = x[:, None] * np.ones((1, nz))
+    Z = np.ones((nx, 1)) * z[None, :]
+    seta = 2.0 * math.pi * (X - xb) / (xe - xb)
+    fx = np.where((X >= xb) & (X <= xe), 4.0 / math.sqrt(27.0) * np.sin(seta) * (1.0 - np.cos(seta)), 0.0)
+    gz = np.zeros_like(X)
+    for m in range(1, mz + 1):
+        gz += zl[m - 1] * np.sin(2.0 * math.pi * m * (Z / lz + faiz[m - 1]))
+    pertb = np.asfortranarray(fx * gz)
+    fait = rng.random(mt)
+    tm = np.array([1.25 ** (-k) for k in range(mt)])
+    tm /= tm.sum()
+    freq = 28861.58
+
+    def ht(tt):  # get_ht_multifrequancy, OCFD_BC_Boundarylayers.f90:594-621
+        return float(sum(tm[m - 1] * math.sin(m * freq * tt * 2.0 * math.pi + 2.0 * math.pi * fait[m - 1])
+                         for m in range(1, mt + 1)))
+
+    bc_para = [1, 0, 0, 0, T_wall, 0.0, 1, 0.01 * U_inf, xb, xe, mt, mz, lz, freq]
+    hp = [2.0, theta[0], theta[1], 0.0]
+    cfg = make_cfg(nx, ny, nz, periodic=(0, 0, 1), scheme_invis=SCHEME_HYBRID, scheme_vis=SCHEME_CD6,
+                   flux_splitting=SPLIT_SW, if_viscous=1, ibc=BC_BOUNDARYLAYER, hybrid_para=hp, pr=0.72,
+                   amu0=AMU0, bc_para=bc_para, ud7l_diss=1.0)
+    dxm = min(np.min(np.diff(x)), 1.0)
+    dym = np.diff(y)
+    dzm = lz / nz
+    c_all = np.sqrt(GAMMA * R_AIR * T)
+    lam = ((np.abs(u) + c_all) / dxm + (np.abs(v) + c_all) / dym.min() + (np.abs(w) + c_all) / dzm)
+    dt = cfl / float(np.max(lam))
+    info = dict(name="flat_plate", U_inf=U_inf, rho_inf=rho_inf, p_inf=p_inf, T_inf=T_inf, T_wall=T_wall,
+                dy1=dy1, dt=dt, ht=ht)
+    bc = dict(inlet1d=inlet, upper0d=upper, nonreflect_upper=nonref, wall_pertb=pertb)
+    return cfg, mesh, span, st, bc, info

@@ -1,0 +1,1414 @@
+// ocfd_flow.hip -- MI355X (gfx950) kernels and C-ABI for OpenCFD-Comb's
+// per-step flow path: flux-split upwind inviscid derivatives, central viscous
+// terms, the TVD-RK3 update, primitive recovery and the boundary conditions.
+//
+// Data stay resident in HBM in the reference's Fortran layout (i fastest,
+// halo arrays of width LAP=4).  The MPI halo exchange of the reference is not
+// performed: periodic faces are read through wrapped indices and non-periodic
+// ghosts keep the values the host uploaded.  FP64 throughout, no FMA
+// contraction (-ffp-contract=off), same operation order as the reference.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/ocfd_hip.h"
+#include "ocfd_device.h"
+
+// ===========================================================================
+// Inviscid sweep (du_inviscous, OCFD_inviscous.f90:47-202).
+//
+// One workgroup owns NP pencils x TS cells along DIR.  Phase 1 evaluates the
+// flux splitting once per point of the tile (TS+8 points per pencil) into LDS;
+// phase 2 reconstructs the 10 interface values (5 components x flux+/-) at the
+// TS+1 interfaces; phase 3 differences them into du.  For DIR 0 the pencils
+// are i-rows (coalesced along the sweep); for DIR 1/2 the NP pencils are
+// adjacent in i, so loads coalesce across pencils.
+// ===========================================================================
+template <int DIR>
+struct SweepTile;
+template <> struct SweepTile<0> { static constexpr int NP = 4, TS = 64; };
+template <> struct SweepTile<1> { static constexpr int NP = 16, TS = 16; };
+template <> struct SweepTile<2> { static constexpr int NP = 16, TS = 16; };
+
+template <int DIR>
+__device__ __forceinline__ void pencil_ijk(const DevState& S, long pen, int& a0, int& a1) {
+  // DIR 0: pen -> (j,k); DIR 1: pen -> (i,k); DIR 2: pen -> (i,j)
+  const int m = DIR == 0 ? S.ny : S.nx;
+  a0 = (int)(pen % m) + 1;
+  a1 = (int)(pen / m) + 1;
+}
+
+template <int DIR>
+__device__ __forceinline__ long sweep_hq(const DevState& S, int a0, int a1, int s) {
+  const int t = wrap(s, S.n[DIR], S.per[DIR]);
+  if (DIR == 0) return hq(S, t, a0, a1);
+  if (DIR == 1) return hq(S, a0, t, a1);
+  return hq(S, a0, a1, t);
+}
+
+template <int DIR>
+__device__ __forceinline__ long sweep_iq(const DevState& S, int a0, int a1, int s) {
+  if (DIR == 0) return iq(S, s, a0, a1);
+  if (DIR == 1) return iq(S, a0, s, a1);
+  return iq(S, a0, a1, s);
+}
+
+template <int DIR>
+__device__ __forceinline__ long mask_index(const DevState& S, long pen, int s) {
+  if (DIR == 0) return (long)s + (long)(S.nx + 1) * pen;
+  if (DIR == 1) {
+    const long i0 = pen % S.nx, k0 = pen / S.nx;
+    return i0 + (long)S.nx * ((long)s + (long)(S.ny + 1) * k0);
+  }
+  return pen + (long)S.nx * S.ny * (long)s;
+}
+
+// Flux splitting at one point (split_Stager_Warming, OCFD_flux_split.f90:6-72;
+// split_Local_LaxFriedrichs, :76-103).  The sound speed is comput_cc's
+// (Comput_flow_variables.f90:50-63), evaluated on the fly.
+__device__ __forceinline__ void split_point(int kind, double Et1, double p1, double d1, double u1, double v1,
+                                            double w1, double A1, double A2, double A3, double* fp, double* fm) {
+  const double gamma = p1 / Et1 + 1.0;
+  const double c1 = sqrt(gamma * p1 / d1);
+  if (kind == OCFD_SPLIT_SW) {
+    const double tmp1 = 2.0 * (gamma - 1.0);
+    const double tmp3 = (3.0 - gamma) / (2.0 * (gamma - 1.0));
+    const double ss = sqrt(A1 * A1 + A2 * A2 + A3 * A3);
+    const double ss1 = 1.0 / ss;
+    const double ak1 = A1 * ss1, ak2 = A2 * ss1, ak3 = A3 * ss1;
+    const double vs = A1 * u1 + A2 * v1 + A3 * w1;
+    const double E1 = vs, E2 = vs - c1 * ss, E3 = vs + c1 * ss;
+    const double E1P = (E1 + fabs(E1)) * 0.5, E2P = (E2 + fabs(E2)) * 0.5, E3P = (E3 + fabs(E3)) * 0.5;
+    const double E1M = E1 - E1P, E2M = E2 - E2P, E3M = E3 - E3P;
+    const double tmp0 = d1 / (2.0 * gamma);
+    const double uc1 = u1 - c1 * ak1, uc2 = u1 + c1 * ak1;
+    const double vc1 = v1 - c1 * ak2, vc2 = v1 + c1 * ak2;
+    const double wc1 = w1 - c1 * ak3, wc2 = w1 + c1 * ak3;
+    const double vvc1 = (uc1 * uc1 + vc1 * vc1 + wc1 * wc1) * 0.5;
+    const double vvc2 = (uc2 * uc2 + vc2 * vc2 + wc2 * wc2) * 0.5;
+    const double vv = (gamma - 1.0) * (u1 * u1 + v1 * v1 + w1 * w1);
+    const double W2 = tmp3 * c1 * c1;
+    fp[0] = tmp0 * (tmp1 * E1P + E2P + E3P);
+    fp[1] = tmp0 * (tmp1 * E1P * u1 + E2P * uc1 + E3P * uc2);
+    fp[2] = tmp0 * (tmp1 * E1P * v1 + E2P * vc1 + E3P * vc2);
+    fp[3] = tmp0 * (tmp1 * E1P * w1 + E2P * wc1 + E3P * wc2);
+    fp[4] = tmp0 * (E1P * vv + E2P * vvc1 + E3P * vvc2 + W2 * (E2P + E3P));
+    fm[0] = tmp0 * (tmp1 * E1M + E2M + E3M);
+    fm[1] = tmp0 * (tmp1 * E1M * u1 + E2M * uc1 + E3M * uc2);
+    fm[2] = tmp0 * (tmp1 * E1M * v1 + E2M * vc1 + E3M * vc2);
+    fm[3] = tmp0 * (tmp1 * E1M * w1 + E2M * wc1 + E3M * wc2);
+    fm[4] = tmp0 * (E1M * vv + E2M * vvc1 + E3M * vvc2 + W2 * (E2M + E3M));
+  } else {
+    const double ss = sqrt(A1 * A1 + A2 * A2 + A3 * A3);
+    const double un = A1 * u1 + A2 * v1 + A3 * w1;
+    const double lamda0 = fabs(un) + ss * c1;
+    const double E0 = Et1 + 0.5 * d1 * (u1 * u1 + v1 * v1 + w1 * w1);
+    fp[0] = 0.5 * (d1 * un + lamda0 * d1);
+    fp[1] = 0.5 * (d1 * u1 * un + A1 * p1 + lamda0 * d1 * u1);
+    fp[2] = 0.5 * (d1 * v1 * un + A2 * p1 + lamda0 * d1 * v1);
+    fp[3] = 0.5 * (d1 * w1 * un + A3 * p1 + lamda0 * d1 * w1);
+    fp[4] = 0.5 * ((E0 + p1) * un + lamda0 * E0);
+    fm[0] = 0.5 * (d1 * un - lamda0 * d1);
+    fm[1] = 0.5 * (d1 * u1 * un + A1 * p1 - lamda0 * d1 * u1);
+    fm[2] = 0.5 * (d1 * v1 * un + A2 * p1 - lamda0 * d1 * v1);
+    fm[3] = 0.5 * (d1 * w1 * un + A3 * p1 - lamda0 * d1 * w1);
+    fm[4] = 0.5 * ((E0 + p1) * un - lamda0 * E0);
+  }
+}
+
+template <int DIR>
+__global__ __launch_bounds__(256) void k_sweep(DevState S) {
+  constexpr int NP = SweepTile<DIR>::NP, TS = SweepTile<DIR>::TS;
+  constexpr int NPT = TS + 8, NIF = TS + 1;
+  __shared__ double sF[10 * NP * NPT];
+  __shared__ double sH[10 * NP * NIF];
+  const int n = S.n[DIR];
+  const int c0 = (int)blockIdx.x * TS + 1;
+  const int ncell = min(TS, n - c0 + 1);
+  const long npen = DIR == 0 ? (long)S.ny * S.nz : DIR == 1 ? (long)S.nx * S.nz : (long)S.nx * S.ny;
+  const long pen0 = (long)blockIdx.y * NP;
+  auto lp = [](int pl, int s) { return DIR == 0 ? pl * NPT + s : s * NP + pl; };
+  auto lh = [](int pl, int l) { return DIR == 0 ? pl * NIF + l : l * NP + pl; };
+  const double* M1 = DIR == 0 ? S.Akx1 : DIR == 1 ? S.Aix1 : S.Asx1;
+  const double* M2 = DIR == 0 ? S.Aky1 : DIR == 1 ? S.Aiy1 : S.Asy1;
+  const double* M3 = DIR == 0 ? S.Akz1 : DIR == 1 ? S.Aiz1 : S.Asz1;
+
+  // phase 1: split fluxes at the tile's points s = c0-4 .. c0+ncell+3
+  for (int t = threadIdx.x; t < NP * NPT; t += blockDim.x) {
+    const int pl = DIR == 0 ? t / NPT : t % NP;
+    const int sl = DIR == 0 ? t % NPT : t / NP;
+    const long pen = pen0 + pl;
+    const int s = c0 - 4 + sl;
+    if (pen >= npen || sl > ncell + 7) continue;
+    int a0, a1;
+    pencil_ijk<DIR>(S, pen, a0, a1);
+    const long q = sweep_hq<DIR>(S, a0, a1, s);
+    double fp[5], fm[5];
+    split_point(S.split, S.Et[q], S.p[q], S.d[q], S.u[q], S.v[q], S.w[q], M1[q], M2[q], M3[q], fp, fm);
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      sF[m * NP * NPT + lp(pl, sl)] = fp[m];
+      sF[(5 + m) * NP * NPT + lp(pl, sl)] = fm[m];
+    }
+  }
+  __syncthreads();
+
+  // phase 2: reconstruct at interfaces s_if = c0-1 .. c0+ncell-1
+  const bool hyb = S.scheme == OCFD_SCHEME_HYBRID;
+  for (int t = threadIdx.x; t < 10 * NP * NIF; t += blockDim.x) {
+    const int cs = t / (NP * NIF);
+    const int r = t % (NP * NIF);
+    const int pl = DIR == 0 ? r / NIF : r % NP;
+    const int l = DIR == 0 ? r % NIF : r / NP;
+    const long pen = pen0 + pl;
+    if (pen >= npen || l > ncell) continue;
+    const int s_if = c0 - 1 + l;
+    const int side = cs / 5;
+    const int mask = hyb ? (int)S.mask[DIR][mask_index<DIR>(S, pen, s_if)] : 0;
+    const int kind = recon_kind(S, DIR, s_if, side, mask);
+    const double* F = sF + cs * NP * NPT;
+    double w[8];
+#pragma unroll
+    for (int qq = 0; qq < 8; qq++) w[qq] = F[lp(pl, side ? l + 7 - qq : l + qq)];
+    sH[cs * NP * NIF + lh(pl, l)] = recon(kind, w, S.ud7l);
+  }
+  __syncthreads();
+
+  // phase 3: du from interface differences
+  const double h1 = S.h_1[DIR];
+  for (int t = threadIdx.x; t < 5 * NP * TS; t += blockDim.x) {
+    const int m = t / (NP * TS);
+    const int r = t % (NP * TS);
+    const int pl = DIR == 0 ? r / TS : r % NP;
+    const int c = DIR == 0 ? r % TS : r / NP;
+    const long pen = pen0 + pl;
+    if (pen >= npen || c >= ncell) continue;
+    const double hp1 = sH[m * NP * NIF + lh(pl, c + 1)], hp0 = sH[m * NP * NIF + lh(pl, c)];
+    const double hm1 = sH[(5 + m) * NP * NIF + lh(pl, c + 1)], hm0 = sH[(5 + m) * NP * NIF + lh(pl, c)];
+    const double val = (hp1 - hp0 + hm1 - hm0) * h1;
+    int a0, a1;
+    pencil_ijk<DIR>(S, pen, a0, a1);
+    const long ci = sweep_iq<DIR>(S, a0, a1, c0 + c) + (long)m * S.ni;
+    if (DIR == 0) {
+      S.du[ci] = val;
+    } else if (DIR == 1) {
+      S.du[ci] = S.du[ci] + val;
+    } else {
+      const long q = sweep_hq<DIR>(S, a0, a1, c0 + c);
+      S.du[ci] = -(S.du[ci] + val) * S.Ajac[q];
+    }
+  }
+}
+
+// ===========================================================================
+// Viscous terms, part 1 (du_viscous, OCFD_viscous.f90:27-95, with the
+// transport coefficients of comput_Amu_Amk_AMD, Comput_flow_variables.f90:69-98):
+// per cell, 12 central derivatives of u,v,w,T, Sutherland viscosity,
+// kappa = mu*Cp/Pr, stresses and the 12 flux components Ev1/Ev2/Ev3(1:4).
+// ===========================================================================
+__device__ __forceinline__ double pw3(double x) { double x2 = x * x; return x2 * x; }
+
+__global__ __launch_bounds__(256) void k_visc_prep(DevState S) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= S.ni) return;
+  const int i = (int)(c % S.nx) + 1;
+  const int j = (int)((c / S.nx) % S.ny) + 1;
+  const int k = (int)(c / ((long)S.nx * S.ny)) + 1;
+  const long q = hq(S, i, j, k);
+  // transport (N_SPEC = 1): comput_vis_coeff Sutherland (TRANSPORT_modules.f90:265-269),
+  // comput_Cp_pgas (CHEM_thermdyn.f90:125-136), Amk = Amu*Cp/Pr (:326-327)
+  const double TA = 288.15, TC = 110.4;
+  const double T0 = S.T[q];
+  const double tr = T0 / TA;
+  const double amu = S.amu0 * sqrt(pw3(tr)) * (TA + TC) / (TC + T0);
+  const double Cp = (0.0 + 3.5 * S.Ri0 * S.di[q]) / S.d[q];
+  const double amk = amu * Cp / S.pr;
+  S.Amu[c] = amu;
+  S.Amk[c] = amk;
+
+  auto dh = [&](const double* F, int dir, int s) {
+    return deriv1(S, dir, s, [&](int o) {
+      const int t = wrap(s + o, S.n[dir], S.per[dir]);
+      return F[q + (long)(t - s) * S.sh[dir]];
+    });
+  };
+  const double uk = dh(S.u, 0, i), vk = dh(S.v, 0, i), wk = dh(S.w, 0, i), Tk = dh(S.T, 0, i);
+  const double ui = dh(S.u, 1, j), vi = dh(S.v, 1, j), wi = dh(S.w, 1, j), Ti = dh(S.T, 1, j);
+  const double us = dh(S.u, 2, k), vs = dh(S.v, 2, k), ws = dh(S.w, 2, k), Ts = dh(S.T, 2, k);
+  const double Akx = S.Akx[q], Aky = S.Aky[q], Akz = S.Akz[q];
+  const double Aix = S.Aix[q], Aiy = S.Aiy[q], Aiz = S.Aiz[q];
+  const double Asx = S.Asx[q], Asy = S.Asy[q], Asz = S.Asz[q];
+  const double ux = uk * Akx + ui * Aix + us * Asx;
+  const double vx = vk * Akx + vi * Aix + vs * Asx;
+  const double wx = wk * Akx + wi * Aix + ws * Asx;
+  const double Tx = Tk * Akx + Ti * Aix + Ts * Asx;
+  const double uy = uk * Aky + ui * Aiy + us * Asy;
+  const double vy = vk * Aky + vi * Aiy + vs * Asy;
+  const double wy = wk * Aky + wi * Aiy + ws * Asy;
+  const double Ty = Tk * Aky + Ti * Aiy + Ts * Asy;
+  const double uz = uk * Akz + ui * Aiz + us * Asz;
+  const double vz = vk * Akz + vi * Aiz + vs * Asz;
+  const double wz = wk * Akz + wi * Aiz + ws * Asz;
+  const double Tz = Tk * Akz + Ti * Aiz + Ts * Asz;
+  const double div = ux + vy + wz;
+  const double tmp2_3 = 2.0 / 3.0;
+  const double s11 = (2.0 * ux - tmp2_3 * div) * amu;
+  const double s22 = (2.0 * vy - tmp2_3 * div) * amu;
+  const double s33 = (2.0 * wz - tmp2_3 * div) * amu;
+  const double s12 = (uy + vx) * amu, s13 = (uz + wx) * amu, s23 = (vz + wy) * amu;
+  const double uu = S.u[q], vv = S.v[q], ww = S.w[q];
+  // Qpx/Qpy/Qpz are zero without species diffusion (OCFD_viscous.f90:13-15)
+  const double E1 = uu * s11 + vv * s12 + ww * s13 + amk * Tx + 0.0;
+  const double E2 = uu * s12 + vv * s22 + ww * s23 + amk * Ty + 0.0;
+  const double E3 = uu * s13 + vv * s23 + ww * s33 + amk * Tz + 0.0;
+  const double Akx1 = S.Akx1[q], Aky1 = S.Aky1[q], Akz1 = S.Akz1[q];
+  const double Aix1 = S.Aix1[q], Aiy1 = S.Aiy1[q], Aiz1 = S.Aiz1[q];
+  const double Asx1 = S.Asx1[q], Asy1 = S.Asy1[q], Asz1 = S.Asz1[q];
+  double* Ev = S.Ev;
+  const long ni = S.ni;
+  Ev[c + 0 * ni] = Akx1 * s11 + Aky1 * s12 + Akz1 * s13;
+  Ev[c + 1 * ni] = Akx1 * s12 + Aky1 * s22 + Akz1 * s23;
+  Ev[c + 2 * ni] = Akx1 * s13 + Aky1 * s23 + Akz1 * s33;
+  Ev[c + 3 * ni] = Akx1 * E1 + Aky1 * E2 + Akz1 * E3;
+  Ev[c + 4 * ni] = Aix1 * s11 + Aiy1 * s12 + Aiz1 * s13;
+  Ev[c + 5 * ni] = Aix1 * s12 + Aiy1 * s22 + Aiz1 * s23;
+  Ev[c + 6 * ni] = Aix1 * s13 + Aiy1 * s23 + Aiz1 * s33;
+  Ev[c + 7 * ni] = Aix1 * E1 + Aiy1 * E2 + Aiz1 * E3;
+  Ev[c + 8 * ni] = Asx1 * s11 + Asy1 * s12 + Asz1 * s13;
+  Ev[c + 9 * ni] = Asx1 * s12 + Asy1 * s22 + Asz1 * s23;
+  Ev[c + 10 * ni] = Asx1 * s13 + Asy1 * s23 + Asz1 * s33;
+  Ev[c + 11 * ni] = Asx1 * E1 + Asy1 * E2 + Asz1 * E3;
+}
+
+// Transport only (when the viscous terms are off the reference still
+// evaluates Amu/Amk at every stage; kept for the per-stage ABI).
+__global__ __launch_bounds__(256) void k_transport(DevState S) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= S.ni) return;
+  const int i = (int)(c % S.nx) + 1;
+  const int j = (int)((c / S.nx) % S.ny) + 1;
+  const int k = (int)(c / ((long)S.nx * S.ny)) + 1;
+  const long q = hq(S, i, j, k);
+  const double TA = 288.15, TC = 110.4;
+  const double T0 = S.T[q];
+  const double tr = T0 / TA;
+  const double amu = S.amu0 * sqrt(pw3(tr)) * (TA + TC) / (TC + T0);
+  const double Cp = (0.0 + 3.5 * S.Ri0 * S.di[q]) / S.d[q];
+  S.Amu[c] = amu;
+  S.Amk[c] = amu * Cp / S.pr;
+}
+
+// ===========================================================================
+// Per-cell update kernel, fused for the stepping path:
+//   du(2:5) += (dEv1/dxi + dEv2/deta + dEv3/dzeta)*Ajac  (OCFD_viscous.f90:111-129)
+//   du += mass force                                     (OCFD_inviscous.f90:187-199)
+//   f = alpha*fn + beta*(f + dt*du)                      (OCFD_Time_Adv.f90:98-130)
+//   d,u,v,w,Et from f; Rescaling_di; T(E); p             (Comput_flow_variables.f90:3-46,101-121)
+// The flags select the pieces so the per-stage entry points share this code.
+// A cell with Et <= 0 is recorded for handle_NegativeEt (k_negt_repair).
+// ===========================================================================
+enum { RK_MASS = 1, RK_VISC = 2, RK_WRITE_DU = 4, RK_UPDATE_F = 8, RK_PRIMS = 16, RK_DI_FROM_D = 32 };
+
+__global__ __launch_bounds__(256) void k_rk(DevState S, int krk, double dt, int flags) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= S.ni) return;
+  const int i = (int)(c % S.nx) + 1;
+  const int j = (int)((c / S.nx) % S.ny) + 1;
+  const int k = (int)(c / ((long)S.nx * S.ny)) + 1;
+  const long q = hq(S, i, j, k);
+  const long ni = S.ni;
+  const double d_old = S.d[q];
+  double du[5];
+  if (flags & (RK_MASS | RK_VISC | RK_UPDATE_F)) {
+#pragma unroll
+    for (int m = 0; m < 5; m++) du[m] = S.du[c + m * ni];
+  }
+  if (flags & RK_MASS) {
+    du[1] = du[1] + d_old * S.mf[0];
+    du[2] = du[2] + d_old * S.mf[1];
+    du[3] = du[3] + d_old * S.mf[2];
+    du[4] = du[4] + d_old * (S.u[q] * S.mf[0] + S.v[q] * S.mf[1] + S.w[q] * S.mf[2]);
+  }
+  if (flags & RK_VISC) {
+    const double ajac = S.Ajac[q];
+    auto dv = [&](const double* F, int dir, int s) {
+      return deriv1(S, dir, s, [&](int o) {
+        const int t = wrap(s + o, S.n[dir], S.per[dir]);
+        return F[c + (long)(t - s) * S.si[dir]];
+      });
+    };
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      const double uk = dv(S.Ev + (long)m * ni, 0, i);
+      const double ui = dv(S.Ev + (long)(4 + m) * ni, 1, j);
+      const double us = dv(S.Ev + (long)(8 + m) * ni, 2, k);
+      du[m + 1] = du[m + 1] + (uk + ui + us) * ajac;
+    }
+  }
+  if (flags & RK_WRITE_DU) {
+#pragma unroll
+    for (int m = 0; m < 5; m++) S.du[c + m * ni] = du[m];
+  }
+  double f[5];
+  if (flags & RK_UPDATE_F) {
+    if (krk == 1) {
+#pragma unroll
+      for (int m = 0; m < 5; m++) f[m] = S.fn[c + m * ni] + dt * du[m];
+    } else {
+      const double ra = krk == 2 ? 3.0 / 4.0 : 1.0 / 3.0;
+      const double rb = krk == 2 ? 1.0 / 4.0 : 2.0 / 3.0;
+#pragma unroll
+      for (int m = 0; m < 5; m++) f[m] = ra * S.fn[c + m * ni] + rb * (S.f[c + m * ni] + dt * du[m]);
+    }
+#pragma unroll
+    for (int m = 0; m < 5; m++) S.f[c + m * ni] = f[m];
+  } else {
+#pragma unroll
+    for (int m = 0; m < 5; m++) f[m] = S.f[c + m * ni];
+  }
+  if (!(flags & RK_PRIMS)) return;
+  const double dd = f[0];
+  const double uu = f[1] / f[0], vv = f[2] / f[0], ww = f[3] / f[0];
+  const double Et = f[4] - (f[1] * uu + f[2] * vv + f[3] * ww) * 0.5;
+  S.Et_prev[c] = S.Et[q];
+  if (Et <= 0.0) {
+    const int slot = atomicAdd(&S.negt[0], 1);
+    if (slot < 16) S.negt[1 + slot] = (int)c;
+  }
+  // N_SPEC == 1 stepping path: di was set to d at the stage start
+  // (OCFD_Time_Adv.f90:45-52), so the old d is the di being rescaled.
+  double di = (flags & RK_DI_FROM_D) ? d_old : S.di[q];
+  if (di < 0.0) di = 0.0;
+  const double d0 = 0.0 + di;
+  const double tmp = dd / d0;
+  di = di * tmp;
+  const double T = Et / (0.0 + 2.5 * S.Ri0 * di);   // comput_T_pgas
+  const double p = 0.0 + S.Ri0 * di * T;            // comput_P
+  S.d[q] = dd;
+  S.u[q] = uu;
+  S.v[q] = vv;
+  S.w[q] = ww;
+  S.Et[q] = Et;
+  S.di[q] = di;
+  S.T[q] = T;
+  S.p[q] = p;
+}
+
+// di = d over the whole halo box for N_SPEC == 1 (OCFD_Time_Adv.f90:45-52)
+__global__ __launch_bounds__(256) void k_di_from_d(DevState S) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < S.nh) S.di[q] = S.d[q];
+}
+
+// start of a stage's negative-Et bookkeeping (Num_NegT=0, Comput_flow_variables.f90:8)
+__global__ void k_negt_reset(DevState S, int all) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    S.negt[0] = 0;
+    if (all) S.negt[17] = 0;
+  }
+}
+
+// handle_NegativeEt (OCFD_handle_NegativeT.f90:1-20), replayed in the
+// reference's (k,j,i) order by one thread: a neighbour already visited by
+// the reference loop holds its new Et, a later one (and every ghost) its old.
+__global__ void k_negt_repair(DevState S) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int cnt = S.negt[0];
+  if (cnt > S.negt[17]) S.negt[17] = cnt;
+  if (cnt == 0) return;
+  int n = cnt < 16 ? cnt : 16;
+  int idx[16];
+  for (int a = 0; a < n; a++) idx[a] = S.negt[1 + a];
+  for (int a = 1; a < n; a++) {
+    int x = idx[a], b = a - 1;
+    while (b >= 0 && idx[b] > x) { idx[b + 1] = idx[b]; b--; }
+    idx[b + 1] = x;
+  }
+  for (int a = 0; a < n; a++) {
+    const long c = idx[a];
+    const int i = (int)(c % S.nx) + 1;
+    const int j = (int)((c / S.nx) % S.ny) + 1;
+    const int k = (int)(c / ((long)S.nx * S.ny)) + 1;
+    const long q = hq(S, i, j, k);
+    auto nb = [&](int ii, int jj, int kk) -> double {
+      const bool inside = ii >= 1 && ii <= S.nx && jj >= 1 && jj <= S.ny && kk >= 1 && kk <= S.nz;
+      if (inside) {
+        const long cn = iq(S, ii, jj, kk);
+        return cn < c ? S.Et[hq(S, ii, jj, kk)] : S.Et_prev[cn];
+      }
+      const int wi = wrap(ii, S.nx, S.per[0]), wj = wrap(jj, S.ny, S.per[1]), wk = wrap(kk, S.nz, S.per[2]);
+      const bool img = wi >= 1 && wi <= S.nx && wj >= 1 && wj <= S.ny && wk >= 1 && wk <= S.nz;
+      return img ? S.Et_prev[iq(S, wi, wj, wk)] : S.Et[hq(S, ii, jj, kk)];
+    };
+    const double Et0 = S.Et[q];
+    const double En = (nb(i + 1, j, k) + nb(i - 1, j, k) + nb(i, j + 1, k) + nb(i, j - 1, k) + nb(i, j, k + 1) +
+                       nb(i, j, k - 1)) / 6.0;
+    S.Et[q] = En;
+    S.f[c + 4 * S.ni] = S.f[c + 4 * S.ni] + (En - Et0);
+    const double di = S.di[q];
+    const double T = En / (0.0 + 2.5 * S.Ri0 * di);
+    S.T[q] = T;
+    S.p[q] = 0.0 + S.Ri0 * di * T;
+  }
+}
+
+// ===========================================================================
+// Boundary conditions (OCFD_bc, OCFD_BC.f90:2-188; boundary layer case
+// ocfd_bc_BoundaryLayer, OCFD_BC_Boundarylayers.f90:20-243).
+// ===========================================================================
+__global__ void k_bl_inlet(DevState S) {  // :63-80 (BcData_inlet = 1)
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)S.ny * S.nz) return;
+  const int j = (int)(t % S.ny) + 1, k = (int)(t / S.ny) + 1;
+  const long q = hq(S, 1, j, k);
+  const double* r = S.bl_inlet1d;
+  const long ny = S.ny;
+  S.d[q] = r[(j - 1) + 0 * ny];
+  S.u[q] = r[(j - 1) + 1 * ny];
+  S.v[q] = r[(j - 1) + 2 * ny];
+  S.w[q] = r[(j - 1) + 3 * ny];
+  S.T[q] = r[(j - 1) + 4 * ny];
+  S.di[q] = r[(j - 1) + 5 * ny];
+}
+
+__global__ void k_bl_upper(DevState S) {  // :100-140 (Dirichlet where not non-reflecting)
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)S.nx * S.nz) return;
+  if (S.bl_nonref[t] != 0) return;
+  const int i = (int)(t % S.nx) + 1, k = (int)(t / S.nx) + 1;
+  const long q = hq(S, i, S.ny, k);
+  const double* u0 = S.bl_upper0d;
+  S.d[q] = u0[0];
+  S.u[q] = u0[1];
+  S.v[q] = u0[2];
+  S.w[q] = u0[3];
+  S.T[q] = u0[4];
+  S.di[q] = u0[5];
+}
+
+__global__ void k_bl_wall(DevState S) {  // :143-201 (isothermal, non-catalytic, blow/suction)
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)S.nx * S.nz) return;
+  const int i = (int)(t % S.nx) + 1, k = (int)(t / S.nx) + 1;
+  const long q1 = hq(S, i, 1, k), q2 = hq(S, i, 2, k), q3 = hq(S, i, 3, k);
+  if (S.Axx[q1] < S.bl_xinit) {
+    S.d[q1] = S.d[q2];
+    S.u[q1] = S.u[q2];
+    S.v[q1] = S.v[q2];
+    S.w[q1] = S.w[q2];
+    S.T[q1] = S.T[q2];
+    S.di[q1] = S.di[q2];
+    return;
+  }
+  const double tmp = 1.0 / sqrt(sqr(S.Aix[q1]) + sqr(S.Aiy[q1]) + sqr(S.Aiz[q1]));
+  const double A0 = S.bl_disA * S.bl_ht * S.bl_pertb[t];
+  S.u[q1] = A0 * S.Aix[q1] * tmp;
+  S.v[q1] = A0 * S.Aiy[q1] * tmp;
+  S.w[q1] = A0 * S.Aiz[q1] * tmp;
+  double T1;
+  if (S.bl_tw > 0) T1 = S.bl_tw;
+  else T1 = (4.0 * S.T[q2] - S.T[q3]) / 3.0;
+  S.T[q1] = T1;
+  const double di2 = S.di[q2], di3 = S.di[q3];
+  const double p2 = 0.0 + S.Ri0 * di2 * S.T[q2];
+  const double p3 = 0.0 + S.Ri0 * di3 * S.T[q3];
+  const double p1 = (4.0 * p2 - p3) / 3.0;
+  const double di1 = di2 / S.d[q2];
+  const double R1 = 0.0 + S.Ri0 * di1;              // comput_d, CHEM_thermdyn.f90:399-409
+  const double d1 = p1 / (T1 * R1);
+  S.d[q1] = d1;
+  S.di[q1] = di1 * d1;
+}
+
+__global__ void k_bl_outlet(DevState S) {  // :203-232 (types 1 and 2)
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)S.ny * S.nz) return;
+  const int j = (int)(t % S.ny) + 1, k = (int)(t / S.ny) + 1;
+  const long q = hq(S, S.nx, j, k), qa = hq(S, S.nx - 1, j, k), qb = hq(S, S.nx - 2, j, k);
+  double* arr[6] = {S.d, S.u, S.v, S.w, S.T, S.di};
+  for (int a = 0; a < 6; a++) arr[a][q] = S.bl_outlet == 1 ? arr[a][qa] : 2.0 * arr[a][qa] - arr[a][qb];
+}
+
+// Face re-synthesis of conserved variables (OCFD_BC.f90:33-184), applied on
+// every face in the reference's face order x-, x+, y-, y+, z-, z+; an edge or
+// corner cell gets each of its faces' updates in that order.
+__device__ __forceinline__ void face_update(const DevState& S, long q, long c) {
+  double di = S.di[q];
+  const double d0 = 0.0 + di;
+  const double dtmp = S.d[q] / d0;
+  di = di * dtmp;
+  S.di[q] = di;
+  const double T = S.T[q];
+  const double Et = 0.0 + 2.5 * S.Ri0 * di * T;  // comput_E_pgas
+  const double p = 0.0 + S.Ri0 * di * T;          // comput_P
+  S.Et[q] = Et;
+  S.p[q] = p;
+  const double dd = S.d[q], uu = S.u[q], vv = S.v[q], ww = S.w[q];
+  S.f[c] = dd;
+  S.f[c + S.ni] = dd * uu;
+  S.f[c + 2 * S.ni] = dd * vv;
+  S.f[c + 3 * S.ni] = dd * ww;
+  S.f[c + 4 * S.ni] = Et + dd * (uu * uu + vv * vv + ww * ww) * 0.5;
+}
+
+__global__ void k_bc_faces(DevState S) {
+  const int nx = S.nx, ny = S.ny, nz = S.nz;
+  const long nxi = nx > 2 ? nx - 2 : 0, nyi = ny > 2 ? ny - 2 : 0;
+  const long n0 = (long)ny * nz, n1 = nx > 1 ? (long)ny * nz : 0;
+  const long n2 = nxi * nz, n3 = ny > 1 ? nxi * nz : 0;
+  const long n4 = nxi * nyi, n5 = nz > 1 ? nxi * nyi : 0;
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (t < n0) { i = 1; j = (int)(t % ny) + 1; k = (int)(t / ny) + 1; }
+  else if ((t -= n0) < n1) { i = nx; j = (int)(t % ny) + 1; k = (int)(t / ny) + 1; }
+  else if ((t -= n1) < n2) { i = (int)(t % nxi) + 2; j = 1; k = (int)(t / nxi) + 1; }
+  else if ((t -= n2) < n3) { i = (int)(t % nxi) + 2; j = ny; k = (int)(t / nxi) + 1; }
+  else if ((t -= n3) < n4) { i = (int)(t % nxi) + 2; j = (int)(t / nxi) + 2; k = 1; }
+  else if ((t -= n4) < n5) { i = (int)(t % nxi) + 2; j = (int)(t / nxi) + 2; k = nz; }
+  else return;
+  const long q = hq(S, i, j, k), c = iq(S, i, j, k);
+  if (i == 1) face_update(S, q, c);
+  if (i == nx) face_update(S, q, c);
+  if (j == 1) face_update(S, q, c);
+  if (j == ny) face_update(S, q, c);
+  if (k == 1) face_update(S, q, c);
+  if (k == nz) face_update(S, q, c);
+}
+
+// fn = f, din = di  (OCFD_Time_Adv.f90:10-29)
+__global__ __launch_bounds__(256) void k_save0(DevState S) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= S.ni) return;
+  for (int m = 0; m < 5; m++) S.fn[c + m * S.ni] = S.f[c + m * S.ni];
+  const int i = (int)(c % S.nx) + 1;
+  const int j = (int)((c / S.nx) % S.ny) + 1;
+  const int k = (int)(c / ((long)S.nx * S.ny)) + 1;
+  S.din[c] = S.di[hq(S, i, j, k)];
+}
+
+// ===========================================================================
+// Hybrid shock sensor (comput_Rhybrid, OCFD_SetHybrid.f90:17-143) and the
+// per-interface scheme masks (set_hybrid_scheme, :2-14; OCFD_inviscous.f90:63-68).
+// ===========================================================================
+__global__ __launch_bounds__(256) void k_sensor1(DevState S) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= S.ni) return;
+  const int i = (int)(c % S.nx) + 1;
+  const int j = (int)((c / S.nx) % S.ny) + 1;
+  const int k = (int)(c / ((long)S.nx * S.ny)) + 1;
+  const long q = hq(S, i, j, k);
+  const double* P = S.p;
+  const double p0 = P[q];
+  const double xm = P[hq(S, wrap(i - 1, S.nx, S.per[0]), j, k)], xp = P[hq(S, wrap(i + 1, S.nx, S.per[0]), j, k)];
+  const double ym = P[hq(S, i, wrap(j - 1, S.ny, S.per[1]), k)], yp = P[hq(S, i, wrap(j + 1, S.ny, S.per[1]), k)];
+  const double zm = P[hq(S, i, j, wrap(k - 1, S.nz, S.per[2]))], zp = P[hq(S, i, j, wrap(k + 1, S.nz, S.per[2]))];
+  S.Rh[q] = fabs((xm - 2.0 * p0 + xp) / (xm + 2.0 * p0 + xp)) + fabs((ym - 2.0 * p0 + yp) / (ym + 2.0 * p0 + yp)) +
+            fabs((zm - 2.0 * p0 + zp) / (zm + 2.0 * p0 + zp));
+}
+
+__global__ __launch_bounds__(256) void k_sensor2(DevState S) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= S.ni) return;
+  const int i = (int)(c % S.nx) + 1;
+  const int j = (int)((c / S.nx) % S.ny) + 1;
+  const int k = (int)(c / ((long)S.nx * S.ny)) + 1;
+  const long q = hq(S, i, j, k);
+  auto dh = [&](int dir, int s) {
+    return deriv1(S, dir, s, [&](int o) {
+      const int t = wrap(s + o, S.n[dir], S.per[dir]);
+      return S.p[q + (long)(t - s) * S.sh[dir]];
+    });
+  };
+  const double pk = dh(0, i), pi = dh(1, j), ps = dh(2, k);
+  const double px = pk * S.Akx[q] + pi * S.Aix[q] + ps * S.Asx[q];
+  const double py = pk * S.Aky[q] + pi * S.Aiy[q] + ps * S.Asy[q];
+  const double pz = pk * S.Akz[q] + pi * S.Aiz[q] + ps * S.Asz[q];
+  S.Rh[q] = sqrt(px * px + py * py + pz * pz);
+}
+
+// Ordered mean: one lane sums in the reference's (k,j,i) order, so the
+// result is bit-identical to the single-rank MPI_ALLREDUCE of :64.
+__global__ void k_sum_ordered(DevState S) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double dp0 = 0.0;
+  for (int k = 1; k <= S.nz; k++)
+    for (int j = 1; j <= S.ny; j++) {
+      const double* row = S.Rh + hq(S, 1, j, k);
+      int i = 0;
+      for (; i + 8 <= S.nx; i += 8) {
+        double r[8];
+#pragma unroll
+        for (int a = 0; a < 8; a++) r[a] = row[i + a];
+#pragma unroll
+        for (int a = 0; a < 8; a++) dp0 = dp0 + r[a];
+      }
+      for (; i < S.nx; i++) dp0 = dp0 + row[i];
+    }
+  const double dp_av = dp0 / (1.0 * S.nx * S.ny * S.nz);
+  S.red[0] = 1.0 / (dp_av + 1.0e-30);
+  S.red[1] = dp_av;
+}
+
+// Fast mean: per-block tree sums, then one block sums the partials.
+__global__ __launch_bounds__(256) void k_sum_partial(DevState S, double* part) {
+  __shared__ double sm[256];
+  double acc = 0.0;
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < S.ni; c += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(c % S.nx) + 1;
+    const int j = (int)((c / S.nx) % S.ny) + 1;
+    const int k = (int)(c / ((long)S.nx * S.ny)) + 1;
+    acc += S.Rh[hq(S, i, j, k)];
+  }
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+__global__ __launch_bounds__(256) void k_sum_final(DevState S, const double* part, int nparts) {
+  __shared__ double sm[256];
+  double acc = 0.0;
+  for (int a = threadIdx.x; a < nparts; a += blockDim.x) acc += part[a];
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double dp_av = sm[0] / (1.0 * S.nx * S.ny * S.nz);
+    S.red[0] = 1.0 / (dp_av + 1.0e-30);
+    S.red[1] = dp_av;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sensor_norm(DevState S) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= S.ni) return;
+  const int i = (int)(c % S.nx) + 1;
+  const int j = (int)((c / S.nx) % S.ny) + 1;
+  const int k = (int)(c / ((long)S.nx * S.ny)) + 1;
+  const long q = hq(S, i, j, k);
+  S.Rh[q] = S.Rh[q] * S.red[0];
+}
+
+template <int DIR>
+__global__ __launch_bounds__(256) void k_masks(DevState S) {
+  const int n = S.n[DIR];
+  const long npen = DIR == 0 ? (long)S.ny * S.nz : DIR == 1 ? (long)S.nx * S.nz : (long)S.nx * S.ny;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npen * (n + 1)) return;
+  long pen;
+  int s;
+  if (DIR == 0) { s = (int)(t % (n + 1)); pen = t / (n + 1); }
+  else { pen = t % npen; s = (int)(t / npen); }
+  int a0, a1;
+  pencil_ijk<DIR>(S, pen, a0, a1);
+  // exchanged Rhybrid: periodic faces wrap; non-periodic ghosts keep 0
+  const double r0 = S.Rh[sweep_hq<DIR>(S, a0, a1, s)];
+  const double r1 = S.Rh[sweep_hq<DIR>(S, a0, a1, s + 1)];
+  const double R0 = fmax(r0, r1);
+  const uint8_t m = R0 <= S.hyb_t1 ? 1 : (R0 <= S.hyb_t2 ? 2 : 3);
+  S.mask[DIR][mask_index<DIR>(S, pen, s)] = m;
+}
+
+// ===========================================================================
+// Mesh metrics (init_mesh / comput_Jacobian3d, OCFD_mesh.f90:1-107)
+// ===========================================================================
+// Periodic face fill of a halo array with an additive span on the ghosts
+// (exchange_boundary_* + updata_Axyz_periodic, OCFD_mesh.f90:109-185).
+template <int DIR>
+__global__ void k_fill_faces(DevState S, double* A, int shift, double span) {
+  const int n = S.n[DIR];
+  const long npen = DIR == 0 ? (long)S.ny * S.nz : DIR == 1 ? (long)S.nx * S.nz : (long)S.nx * S.ny;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npen * 2 * OCFD_LAP) return;
+  const long pen = t / (2 * OCFD_LAP);
+  const int g = (int)(t % (2 * OCFD_LAP));
+  int a0, a1;
+  pencil_ijk<DIR>(S, pen, a0, a1);
+  const int l = g % OCFD_LAP + 1;
+  auto at = [&](int s) -> long {
+    if (DIR == 0) return hq(S, s, a0, a1);
+    if (DIR == 1) return hq(S, a0, s, a1);
+    return hq(S, a0, a1, s);
+  };
+  // exchange = exact copy (keeps -0.0); coordinates then get the span shift
+  if (g < OCFD_LAP) A[at(1 - l)] = shift ? A[at(n + 1 - l)] - span : A[at(n + 1 - l)];
+  else A[at(n + l)] = shift ? A[at(l)] + span : A[at(l)];
+}
+
+__global__ __launch_bounds__(256) void k_metrics(DevState S) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= S.ni) return;
+  const int i = (int)(c % S.nx) + 1;
+  const int j = (int)((c / S.nx) % S.ny) + 1;
+  const int k = (int)(c / ((long)S.nx * S.ny)) + 1;
+  const long q = hq(S, i, j, k);
+  // coordinates already carry their (shifted) ghosts: read them directly
+  auto dh = [&](const double* F, int dir, int s) {
+    return deriv1(S, dir, s, [&](int o) { return F[q + (long)o * S.sh[dir]]; });
+  };
+  const double xi1 = dh(S.Axx, 0, i), yi1 = dh(S.Ayy, 0, i), zi1 = dh(S.Azz, 0, i);
+  const double xj1 = dh(S.Axx, 1, j), yj1 = dh(S.Ayy, 1, j), zj1 = dh(S.Azz, 1, j);
+  const double xk1 = dh(S.Axx, 2, k), yk1 = dh(S.Ayy, 2, k), zk1 = dh(S.Azz, 2, k);
+  const double Jac1 = 1.0 / (xi1 * yj1 * zk1 + yi1 * zj1 * xk1 + zi1 * xj1 * yk1 - zi1 * yj1 * xk1 - yi1 * xj1 * zk1 -
+                             xi1 * zj1 * yk1);
+  S.Ajac[q] = Jac1;
+  S.Akx[q] = Jac1 * (yj1 * zk1 - zj1 * yk1);
+  S.Aky[q] = Jac1 * (zj1 * xk1 - xj1 * zk1);
+  S.Akz[q] = Jac1 * (xj1 * yk1 - yj1 * xk1);
+  S.Aix[q] = Jac1 * (yk1 * zi1 - zk1 * yi1);
+  S.Aiy[q] = Jac1 * (zk1 * xi1 - xk1 * zi1);
+  S.Aiz[q] = Jac1 * (xk1 * yi1 - yk1 * xi1);
+  S.Asx[q] = Jac1 * (yi1 * zj1 - zi1 * yj1);
+  S.Asy[q] = Jac1 * (zi1 * xj1 - xi1 * zj1);
+  S.Asz[q] = Jac1 * (xi1 * yj1 - yi1 * xj1);
+}
+
+__global__ __launch_bounds__(256) void k_metrics_over_jac(DevState S) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= S.nh) return;
+  const double J = S.Ajac[q];
+  S.Akx1[q] = S.Akx[q] / J;
+  S.Aky1[q] = S.Aky[q] / J;
+  S.Akz1[q] = S.Akz[q] / J;
+  S.Aix1[q] = S.Aix[q] / J;
+  S.Aiy1[q] = S.Aiy[q] / J;
+  S.Aiz1[q] = S.Aiz[q] / J;
+  S.Asx1[q] = S.Asx[q] / J;
+  S.Asy1[q] = S.Asy[q] / J;
+  S.Asz1[q] = S.Asz[q] / J;
+}
+
+__global__ __launch_bounds__(256) void k_copy_interior(DevState S, const double* src_halo, double* dst_halo) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= S.ni) return;
+  const int i = (int)(c % S.nx) + 1;
+  const int j = (int)((c / S.nx) % S.ny) + 1;
+  const int k = (int)(c / ((long)S.nx * S.ny)) + 1;
+  const long q = hq(S, i, j, k);
+  dst_halo[q] = src_halo[q];
+}
+
+__global__ void k_fill(double* a, long n, double v) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = v;
+}
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+namespace {
+
+struct KStat {
+  double ms = 0.0;
+  long long launches = 0;
+};
+
+struct Ctx {
+  bool inited = false;
+  ocfd_cfg cfg{};
+  DevState S{};
+  hipStream_t stream = nullptr;
+  std::vector<void*> allocs;
+  double* part = nullptr;  // fast-sum partials
+  int nparts = 0;
+  int hybrid_sum_mode = 0;
+  int use_graph = 0;
+  int timing = 0;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  double graph_dt = 0.0;
+  int* h_negt = nullptr;  // pinned copy of the negative-Et counter
+  std::vector<std::string> knames;
+  std::map<std::string, int> kindex;
+  std::vector<KStat> kstats;
+  struct Pending { int id; hipEvent_t a, b; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> evpool;
+  std::string err;
+};
+Ctx G;
+
+int fail(const std::string& m) {
+  G.err = m;
+  return 1;
+}
+#define HIPCHK(x)                                                                     \
+  do {                                                                                \
+    hipError_t e_ = (x);                                                              \
+    if (e_ != hipSuccess) return fail(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #x); \
+  } while (0)
+
+double* dalloc(long n) {
+  void* p = nullptr;
+  if (hipMalloc(&p, sizeof(double) * (size_t)(n > 0 ? n : 1)) != hipSuccess) return nullptr;
+  G.allocs.push_back(p);
+  return (double*)p;
+}
+
+hipEvent_t get_event() {
+  if (!G.evpool.empty()) {
+    hipEvent_t e = G.evpool.back();
+    G.evpool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  hipEventCreate(&e);
+  return e;
+}
+
+void collect_timing() {
+  if (G.pending.empty()) return;
+  hipStreamSynchronize(G.stream);
+  for (auto& p : G.pending) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, p.a, p.b);
+    G.kstats[p.id].ms += ms;
+    G.kstats[p.id].launches += 1;
+    G.evpool.push_back(p.a);
+    G.evpool.push_back(p.b);
+  }
+  G.pending.clear();
+}
+
+int kid(const char* name) {
+  auto it = G.kindex.find(name);
+  if (it != G.kindex.end()) return it->second;
+  int id = (int)G.knames.size();
+  G.knames.push_back(name);
+  G.kstats.push_back(KStat{});
+  G.kindex[name] = id;
+  return id;
+}
+
+template <typename K, typename... A>
+void launch(const char* name, K kern, dim3 g, dim3 b, A... args) {
+  if (G.timing) {
+    hipEvent_t a = get_event(), e = get_event();
+    hipEventRecord(a, G.stream);
+    hipLaunchKernelGGL(kern, g, b, 0, G.stream, args...);
+    hipEventRecord(e, G.stream);
+    G.pending.push_back({kid(name), a, e});
+    if (G.pending.size() > 4096) collect_timing();
+  } else {
+    hipLaunchKernelGGL(kern, g, b, 0, G.stream, args...);
+  }
+}
+
+inline dim3 grid1(long n) { return dim3((unsigned)((n + 255) / 256)); }
+
+template <int DIR>
+void launch_sweep() {
+  const DevState& S = G.S;
+  constexpr int NP = SweepTile<DIR>::NP, TS = SweepTile<DIR>::TS;
+  const long npen = DIR == 0 ? (long)S.ny * S.nz : DIR == 1 ? (long)S.nx * S.nz : (long)S.nx * S.ny;
+  dim3 g((unsigned)((S.n[DIR] + TS - 1) / TS), (unsigned)((npen + NP - 1) / NP));
+  static const char* names[3] = {"k_sweep_x", "k_sweep_y", "k_sweep_z"};
+  launch(names[DIR], k_sweep<DIR>, g, dim3(256), S);
+}
+
+template <int DIR>
+void launch_masks() {
+  const DevState& S = G.S;
+  const long npen = DIR == 0 ? (long)S.ny * S.nz : DIR == 1 ? (long)S.nx * S.nz : (long)S.nx * S.ny;
+  launch("k_masks", k_masks<DIR>, grid1(npen * (S.n[DIR] + 1)), dim3(256), S);
+}
+
+void enq_rhybrid() {
+  const DevState& S = G.S;
+  const int sensor = (int)std::lround(G.cfg.hybrid_para[0]);
+  if (sensor == 1) {
+    launch("k_sensor1", k_sensor1, grid1(S.ni), dim3(256), S);
+  } else {
+    launch("k_sensor2", k_sensor2, grid1(S.ni), dim3(256), S);
+    if (G.hybrid_sum_mode == 0) {
+      launch("k_sum_ordered", k_sum_ordered, dim3(1), dim3(64), S);
+    } else {
+      launch("k_sum_partial", k_sum_partial, dim3(G.nparts), dim3(256), S, G.part);
+      launch("k_sum_final", k_sum_final, dim3(1), dim3(256), S, (const double*)G.part, G.nparts);
+    }
+    launch("k_sensor_norm", k_sensor_norm, grid1(S.ni), dim3(256), S);
+  }
+  launch_masks<0>();
+  launch_masks<1>();
+  launch_masks<2>();
+}
+
+void enq_bc() {
+  const DevState& S = G.S;
+  if (G.cfg.ibc == OCFD_BC_BOUNDARYLAYER) {
+    if (S.bl_inlet == 1) launch("k_bl_inlet", k_bl_inlet, grid1((long)S.ny * S.nz), dim3(256), S);
+    launch("k_bl_upper", k_bl_upper, grid1((long)S.nx * S.nz), dim3(256), S);
+    launch("k_bl_wall", k_bl_wall, grid1((long)S.nx * S.nz), dim3(256), S);
+    if (S.bl_outlet == 1 || S.bl_outlet == 2) launch("k_bl_outlet", k_bl_outlet, grid1((long)S.ny * S.nz), dim3(256), S);
+  }
+  const long nxi = S.nx > 2 ? S.nx - 2 : 0, nyi = S.ny > 2 ? S.ny - 2 : 0;
+  const long nf = 2L * S.ny * S.nz + 2L * nxi * S.nz + 2L * nxi * nyi;
+  launch("k_bc_faces", k_bc_faces, grid1(nf), dim3(256), S);
+}
+
+void enq_stage(int krk, double dt1) {
+  const DevState& S = G.S;
+  if (krk == 1 && G.cfg.scheme_invis == OCFD_SCHEME_HYBRID) enq_rhybrid();
+  launch_sweep<0>();
+  launch_sweep<1>();
+  launch_sweep<2>();
+  if (S.viscous) launch("k_visc_prep", k_visc_prep, grid1(S.ni), dim3(256), S);
+  else launch("k_transport", k_transport, grid1(S.ni), dim3(256), S);
+  const int flags = (S.mass_force ? RK_MASS : 0) | (S.viscous ? RK_VISC : 0) | RK_WRITE_DU | RK_UPDATE_F | RK_PRIMS |
+                    RK_DI_FROM_D;
+  launch("k_negt_reset", k_negt_reset, dim3(1), dim3(64), S, 0);
+  launch("k_rk", k_rk, grid1(S.ni), dim3(256), S, krk, dt1, flags);
+  launch("k_negt_repair", k_negt_repair, dim3(1), dim3(64), S);
+  enq_bc();
+}
+
+void enq_step(double dt1) {
+  const DevState& S = G.S;
+  launch("k_negt_reset", k_negt_reset, dim3(1), dim3(64), S, 1);
+  launch("k_save0", k_save0, grid1(S.ni), dim3(256), S);
+  for (int krk = 1; krk <= 3; krk++) enq_stage(krk, dt1);
+  hipMemcpyAsync(G.h_negt, S.negt + 17, sizeof(int), hipMemcpyDeviceToHost, G.stream);
+}
+
+void drop_graph() {
+  if (G.gexec) hipGraphExecDestroy(G.gexec);
+  if (G.graph) hipGraphDestroy(G.graph);
+  G.gexec = nullptr;
+  G.graph = nullptr;
+}
+
+int check_ready() {
+  if (!G.inited) return fail("ocfd: library not initialised (call ocfd_init)");
+  return 0;
+}
+
+double* field_ptr(int id, long* n) {
+  DevState& S = G.S;
+  const long nh = S.nh, ni = S.ni;
+  switch (id) {
+    case OCFD_FIELD_F: *n = 5 * ni; return S.f;
+    case OCFD_FIELD_FN: *n = 5 * ni; return S.fn;
+    case OCFD_FIELD_DU: *n = 5 * ni; return S.du;
+    case OCFD_FIELD_DI: *n = nh * S.ns; return S.di;
+    case OCFD_FIELD_DIN: *n = ni * S.ns; return S.din;
+    case OCFD_FIELD_D: *n = nh; return S.d;
+    case OCFD_FIELD_U: *n = nh; return S.u;
+    case OCFD_FIELD_V: *n = nh; return S.v;
+    case OCFD_FIELD_W: *n = nh; return S.w;
+    case OCFD_FIELD_T: *n = nh; return S.T;
+    case OCFD_FIELD_P: *n = nh; return S.p;
+    case OCFD_FIELD_ET: *n = nh; return S.Et;
+    case OCFD_FIELD_AMU: *n = ni; return S.Amu;
+    case OCFD_FIELD_AMK: *n = ni; return S.Amk;
+    case OCFD_FIELD_RHYBRID: *n = nh; return S.Rh;
+  }
+  double* M[19] = {S.Akx, S.Aky, S.Akz, S.Aix, S.Aiy, S.Aiz, S.Asx, S.Asy, S.Asz, S.Ajac,
+                   S.Akx1, S.Aky1, S.Akz1, S.Aix1, S.Aiy1, S.Aiz1, S.Asx1, S.Asy1, S.Asz1};
+  if (id >= OCFD_FIELD_AKX && id <= OCFD_FIELD_ASZ1) { *n = nh; return M[id - OCFD_FIELD_AKX]; }
+  *n = 0;
+  return nullptr;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+const char* ocfd_last_error(void) { return G.err.c_str(); }
+
+int32_t ocfd_finalize(void) {
+  if (!G.inited) return 0;
+  hipStreamSynchronize(G.stream);
+  drop_graph();
+  for (auto& p : G.pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
+  G.pending.clear();
+  for (auto e : G.evpool) hipEventDestroy(e);
+  G.evpool.clear();
+  for (void* p : G.allocs) hipFree(p);
+  G.allocs.clear();
+  if (G.h_negt) hipHostFree(G.h_negt);
+  G.h_negt = nullptr;
+  hipStreamDestroy(G.stream);
+  G.stream = nullptr;
+  G.inited = false;
+  return 0;
+}
+
+int32_t ocfd_init(const ocfd_cfg* cfg) {
+  ocfd_finalize();
+  G.err.clear();
+  if (!cfg) return fail("ocfd_init: null config");
+  const ocfd_cfg& c = *cfg;
+  if (c.lap != OCFD_LAP) return fail("ocfd_init: LAP must be 4");
+  if (c.nx < 8 || c.ny < 8 || c.nz < 8) return fail("ocfd_init: each dimension needs at least 8 points");
+  if (c.n_spec != 1) return fail("ocfd_init: the flow path supports N_SPEC == 1 (non-reacting)");
+  if (c.if_scheme_character != 0) return fail("ocfd_init: IF_Scheme_Character=1 is not supported");
+  if (c.if_real_gas != 0) return fail("ocfd_init: the N_SPEC == 1 flow path is the perfect gas (IF_RealGas=0)");
+  if (c.flag_vis != 0 || c.flag_cond != 0) return fail("ocfd_init: only Sutherland/Prandtl transport on this path");
+  if (c.scheme_vis != OCFD_SCHEME_CD6 && c.scheme_vis != OCFD_SCHEME_CD8) return fail("ocfd_init: Scheme_Vis must be CD6 or CD8");
+  if (c.flux_splitting != OCFD_SPLIT_SW && c.flux_splitting != OCFD_SPLIT_LLF) return fail("ocfd_init: bad Flux_Splitting");
+  switch (c.scheme_invis) {
+    case OCFD_SCHEME_WENO5: case OCFD_SCHEME_WENO7: case OCFD_SCHEME_OMP6: case OCFD_SCHEME_UD7L: case OCFD_SCHEME_HYBRID: break;
+    default: return fail("ocfd_init: unsupported Scheme_Invis");
+  }
+  for (int a = 0; a < 6; a++) {
+    if (c.scheme_boundary[a] < -1 || c.scheme_boundary[a] > 1) return fail("ocfd_init: Scheme_boundary must be -1, 0 or 1");
+    if (c.ghost_cell[a] != 0) return fail("ocfd_init: Ghost_Cell /= 0 is not supported");
+  }
+  if (c.ibc != OCFD_BC_NONE && c.ibc != OCFD_BC_BOUNDARYLAYER) return fail("ocfd_init: unsupported boundary condition");
+  if (c.scheme_invis == OCFD_SCHEME_HYBRID && std::lround(c.hybrid_para[3]) != 0)
+    return fail("ocfd_init: hybrid patches (Hybrid_para(4) > 0) are not supported");
+  if (c.wall_catalysis != 0) return fail("ocfd_init: only non-catalytic walls are supported");
+
+  G.cfg = c;
+  DevState& S = G.S;
+  S = DevState{};
+  S.nx = c.nx; S.ny = c.ny; S.nz = c.nz; S.ns = c.n_spec;
+  S.nxh = c.nx + 2 * OCFD_LAP; S.nyh = c.ny + 2 * OCFD_LAP; S.nzh = c.nz + 2 * OCFD_LAP;
+  S.nh = (long)S.nxh * S.nyh * S.nzh;
+  S.ni = (long)S.nx * S.ny * S.nz;
+  S.sh[0] = 1; S.sh[1] = S.nxh; S.sh[2] = (long)S.nxh * S.nyh;
+  S.si[0] = 1; S.si[1] = S.nx; S.si[2] = (long)S.nx * S.ny;
+  S.n[0] = c.nx; S.n[1] = c.ny; S.n[2] = c.nz;
+  for (int d = 0; d < 3; d++) {
+    S.per[d] = c.periodic[d] ? 1 : 0;
+    S.bidx[d][0] = S.bidx[d][1] = c.periodic[d] ? 0 : 1;   // set_parameters, OCFD_readpara.f90:360-369
+  }
+  for (int a = 0; a < 6; a++) S.sb[a] = c.scheme_boundary[a];
+  S.scheme = c.scheme_invis;
+  S.split = c.flux_splitting;
+  S.cd8 = c.scheme_vis == OCFD_SCHEME_CD8;
+  S.viscous = c.if_viscous;
+  S.mass_force = c.if_mass_force;
+  const double hh[3] = {1.0 / (c.nx - 1.0), 1.0 / (c.ny - 1.0), 1.0 / (c.nz - 1.0)};  // OCFD_readpara.f90:351-353
+  for (int d = 0; d < 3; d++) {
+    const double h = hh[d];
+    S.dh[d] = h;
+    S.h_1[d] = 1.0 / h;
+    S.cd6c[d][0] = 1.0 / (60.0 * h);
+    S.cd6c[d][1] = -3.0 / (20.0 * h);
+    S.cd6c[d][2] = 3.0 / (4.0 * h);
+    S.cd8c[d][0] = 0.8 / h;
+    S.cd8c[d][1] = -0.2 / h;
+    S.cd8c[d][2] = 3.80952380952380952e-2 / h;
+    S.cd8c[d][3] = -3.571428571428571428e-3 / h;
+  }
+  {  // set_scheme_para, OCFD_readpara.f90:424-431
+    const double UD7[8] = {-3.0, 25.0, -101.0, 319.0, 214.0, -38.0, 4.0, 0.0};
+    const double CD8[8] = {-3.0, 29.0, -139.0, 533.0, 533.0, -139.0, 29.0, -3.0};
+    for (int a = 0; a < 8; a++) S.ud7l[a] = c.ud7l_diss * UD7[a] / 420.0 + (1.0 - c.ud7l_diss) * CD8[a] / 840.0;
+  }
+  S.hyb_t1 = c.hybrid_para[1];
+  S.hyb_t2 = c.hybrid_para[2];
+  S.Mi0 = 0.02897;
+  S.Ri0 = 8.314 / S.Mi0;
+  S.amu0 = c.amu0; S.pr = c.pr; S.sc = c.sc;
+  for (int a = 0; a < 3; a++) S.mf[a] = c.mass_force[a];
+  S.bl_inlet = (int)std::lround(c.bc_para[0]);
+  S.bl_outlet = (int)std::lround(c.bc_para[3]);
+  S.bl_tw = c.bc_para[4];
+  S.bl_xinit = c.bc_para[5];
+  S.bl_disA = c.bc_para[7];
+  S.bl_ht = 1.0;
+  if (c.ibc == OCFD_BC_BOUNDARYLAYER) {
+    if (S.bl_inlet != 1) return fail("ocfd_init: boundary layer inlet must be 1-D data (BC_Para(1)=1)");
+    if (std::lround(c.bc_para[1]) != 0) return fail("ocfd_init: boundary layer upper data must be 0-D (BC_Para(2)=0)");
+  }
+
+  if (hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking) != hipSuccess) return fail("ocfd_init: no HIP device/stream");
+  const long nh = S.nh, ni = S.ni;
+  struct { double** p; long n; double v; } arrs[] = {
+      {&S.f, 5 * ni, 0.0}, {&S.fn, 5 * ni, 0.0}, {&S.du, 5 * ni, 0.0}, {&S.di, nh, 1.0}, {&S.din, ni, 1.0},
+      {&S.d, nh, 1.0}, {&S.u, nh, 1.0}, {&S.v, nh, 1.0}, {&S.w, nh, 1.0}, {&S.T, nh, 300.0}, {&S.p, nh, 1.0},
+      {&S.Et, nh, 1.0}, {&S.Et_prev, ni, 1.0}, {&S.Amu, ni, 0.0}, {&S.Amk, ni, 0.0},
+      {&S.Axx, nh, 1.0}, {&S.Ayy, nh, 1.0}, {&S.Azz, nh, 1.0},
+      {&S.Akx, nh, 1.0}, {&S.Aky, nh, 1.0}, {&S.Akz, nh, 1.0}, {&S.Aix, nh, 1.0}, {&S.Aiy, nh, 1.0},
+      {&S.Aiz, nh, 1.0}, {&S.Asx, nh, 1.0}, {&S.Asy, nh, 1.0}, {&S.Asz, nh, 1.0}, {&S.Ajac, nh, 1.0},
+      {&S.Akx1, nh, 1.0}, {&S.Aky1, nh, 1.0}, {&S.Akz1, nh, 1.0}, {&S.Aix1, nh, 1.0}, {&S.Aiy1, nh, 1.0},
+      {&S.Aiz1, nh, 1.0}, {&S.Asx1, nh, 1.0}, {&S.Asy1, nh, 1.0}, {&S.Asz1, nh, 1.0},
+      {&S.Rh, nh, 0.0}, {&S.Ev, 12 * ni, 0.0}, {&S.red, 8, 0.0}};
+  G.inited = true;  // so that finalize frees partial allocations on error
+  for (auto& a : arrs) {
+    *a.p = dalloc(a.n);
+    if (!*a.p) return fail("ocfd_init: hipMalloc failed");
+    hipLaunchKernelGGL(k_fill, grid1(a.n), dim3(256), 0, G.stream, *a.p, a.n, a.v);
+  }
+  for (int d = 0; d < 3; d++) {
+    const long nm = (long)(S.nx + (d == 0)) * (S.ny + (d == 1)) * (S.nz + (d == 2));
+    void* m = nullptr;
+    if (hipMalloc(&m, (size_t)nm) != hipSuccess) return fail("ocfd_init: hipMalloc failed");
+    G.allocs.push_back(m);
+    hipMemsetAsync(m, 0, (size_t)nm, G.stream);
+    S.mask[d] = (uint8_t*)m;
+  }
+  {
+    void* m = nullptr;
+    if (hipMalloc(&m, sizeof(int) * 32) != hipSuccess) return fail("ocfd_init: hipMalloc failed");
+    G.allocs.push_back(m);
+    hipMemsetAsync(m, 0, sizeof(int) * 32, G.stream);
+    S.negt = (int*)m;
+  }
+  G.nparts = 1024;
+  G.part = dalloc(G.nparts);
+  double* bl = dalloc((long)S.ny * 6 + 6 + (long)S.nx * S.nz);
+  void* nr = nullptr;
+  if (!G.part || !bl || hipMalloc(&nr, sizeof(int) * (size_t)S.nx * S.nz) != hipSuccess) return fail("ocfd_init: hipMalloc failed");
+  G.allocs.push_back(nr);
+  hipMemsetAsync(nr, 0, sizeof(int) * (size_t)S.nx * S.nz, G.stream);
+  S.bl_inlet1d = bl;
+  S.bl_upper0d = bl + (long)S.ny * 6;
+  S.bl_pertb = bl + (long)S.ny * 6 + 6;
+  S.bl_nonref = (const int*)nr;
+  hipLaunchKernelGGL(k_fill, grid1((long)S.ny * 6 + 6 + (long)S.nx * S.nz), dim3(256), 0, G.stream, bl,
+                     (long)S.ny * 6 + 6 + (long)S.nx * S.nz, 0.0);
+  if (hipHostMalloc((void**)&G.h_negt, sizeof(int)) != hipSuccess) return fail("ocfd_init: hipHostMalloc failed");
+  *G.h_negt = 0;
+  HIPCHK(hipStreamSynchronize(G.stream));
+  return 0;
+}
+
+int32_t ocfd_set_option(int32_t key, int32_t value) {
+  switch (key) {
+    case OCFD_OPT_HYBRID_SUM_MODE: G.hybrid_sum_mode = value; drop_graph(); return 0;
+    case OCFD_OPT_USE_GRAPH: G.use_graph = value; drop_graph(); return 0;
+    case OCFD_OPT_KERNEL_TIMING: collect_timing(); G.timing = value; drop_graph(); return 0;
+  }
+  return fail("ocfd_set_option: unknown key");
+}
+
+int32_t ocfd_set_species(int32_t n_tct, const double* tct, double tct_max, const double* mi, const double* coef) {
+  (void)n_tct; (void)tct; (void)tct_max; (void)coef;
+  if (check_ready()) return 1;
+  if (!mi || !(mi[0] > 0.0)) return fail("ocfd_set_species: bad molar mass");
+  G.S.Mi0 = mi[0];
+  G.S.Ri0 = 8.314 / mi[0];  // Sp%Ri = R0/Sp%Mi, CHEM_thermdyn.f90:41
+  drop_graph();
+  return 0;
+}
+
+int32_t ocfd_set_mesh(const double* axx, const double* ayy, const double* azz, const double* span) {
+  if (check_ready()) return 1;
+  DevState& S = G.S;
+  const size_t bytes = sizeof(double) * (size_t)S.nh;
+  double* tmp = dalloc(S.nh);
+  if (!tmp) return fail("ocfd_set_mesh: hipMalloc failed");
+  const double* src[3] = {axx, ayy, azz};
+  double* dst[3] = {S.Axx, S.Ayy, S.Azz};
+  for (int a = 0; a < 3; a++) {
+    HIPCHK(hipMemcpyAsync(tmp, src[a], bytes, hipMemcpyHostToDevice, G.stream));
+    hipLaunchKernelGGL(k_copy_interior, grid1(S.ni), dim3(256), 0, G.stream, S, (const double*)tmp, dst[a]);
+    HIPCHK(hipStreamSynchronize(G.stream));
+  }
+  for (int a = 0; a < 3; a++) {
+    const long npx = (long)S.ny * S.nz, npy = (long)S.nx * S.nz, npz = (long)S.nx * S.ny;
+    if (S.per[0]) hipLaunchKernelGGL(k_fill_faces<0>, grid1(npx * 8), dim3(256), 0, G.stream, S, dst[a], 1, span ? span[a] : 0.0);
+    if (S.per[1]) hipLaunchKernelGGL(k_fill_faces<1>, grid1(npy * 8), dim3(256), 0, G.stream, S, dst[a], 1, span ? span[3 + a] : 0.0);
+    if (S.per[2]) hipLaunchKernelGGL(k_fill_faces<2>, grid1(npz * 8), dim3(256), 0, G.stream, S, dst[a], 1, span ? span[6 + a] : 0.0);
+  }
+  hipLaunchKernelGGL(k_metrics, grid1(S.ni), dim3(256), 0, G.stream, S);
+  double* M[10] = {S.Akx, S.Aky, S.Akz, S.Aix, S.Aiy, S.Aiz, S.Asx, S.Asy, S.Asz, S.Ajac};
+  for (int a = 0; a < 10; a++) {
+    const long npx = (long)S.ny * S.nz, npy = (long)S.nx * S.nz, npz = (long)S.nx * S.ny;
+    if (S.per[0]) hipLaunchKernelGGL(k_fill_faces<0>, grid1(npx * 8), dim3(256), 0, G.stream, S, M[a], 0, 0.0);
+    if (S.per[1]) hipLaunchKernelGGL(k_fill_faces<1>, grid1(npy * 8), dim3(256), 0, G.stream, S, M[a], 0, 0.0);
+    if (S.per[2]) hipLaunchKernelGGL(k_fill_faces<2>, grid1(npz * 8), dim3(256), 0, G.stream, S, M[a], 0, 0.0);
+  }
+  hipLaunchKernelGGL(k_metrics_over_jac, grid1(S.nh), dim3(256), 0, G.stream, S);
+  HIPCHK(hipStreamSynchronize(G.stream));
+  HIPCHK(hipGetLastError());
+  drop_graph();
+  return 0;
+}
+
+int32_t ocfd_upload_field(int32_t field, const double* host) {
+  if (check_ready()) return 1;
+  long n;
+  double* p = field_ptr(field, &n);
+  if (!p || !host) return fail("ocfd_upload_field: bad field id or null pointer");
+  HIPCHK(hipMemcpyAsync(p, host, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, G.stream));
+  HIPCHK(hipStreamSynchronize(G.stream));
+  return 0;
+}
+
+int32_t ocfd_download_field(int32_t field, double* host) {
+  if (check_ready()) return 1;
+  long n;
+  double* p = field_ptr(field, &n);
+  if (!p || !host) return fail("ocfd_download_field: bad field id or null pointer");
+  HIPCHK(hipMemcpyAsync(host, p, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, G.stream));
+  HIPCHK(hipStreamSynchronize(G.stream));
+  return 0;
+}
+
+int32_t ocfd_download_mask(int32_t dir, uint8_t* host) {
+  if (check_ready()) return 1;
+  if (dir < 0 || dir > 2 || !host) return fail("ocfd_download_mask: bad direction or null pointer");
+  const DevState& S = G.S;
+  const size_t nm = (size_t)(S.nx + (dir == 0)) * (S.ny + (dir == 1)) * (S.nz + (dir == 2));
+  HIPCHK(hipMemcpyAsync(host, S.mask[dir], nm, hipMemcpyDeviceToHost, G.stream));
+  HIPCHK(hipStreamSynchronize(G.stream));
+  return 0;
+}
+
+int32_t ocfd_set_bc_boundarylayer(const double* inlet1d, const double* upper0d, const int32_t* nonreflect_upper,
+                                  const double* wall_pertb) {
+  if (check_ready()) return 1;
+  const DevState& S = G.S;
+  if (!inlet1d || !upper0d || !nonreflect_upper || !wall_pertb) return fail("ocfd_set_bc_boundarylayer: null pointer");
+  HIPCHK(hipMemcpyAsync((void*)S.bl_inlet1d, inlet1d, sizeof(double) * (size_t)S.ny * 6, hipMemcpyHostToDevice, G.stream));
+  HIPCHK(hipMemcpyAsync((void*)S.bl_upper0d, upper0d, sizeof(double) * 6, hipMemcpyHostToDevice, G.stream));
+  HIPCHK(hipMemcpyAsync((void*)S.bl_nonref, nonreflect_upper, sizeof(int) * (size_t)S.nx * S.nz, hipMemcpyHostToDevice, G.stream));
+  HIPCHK(hipMemcpyAsync((void*)S.bl_pertb, wall_pertb, sizeof(double) * (size_t)S.nx * S.nz, hipMemcpyHostToDevice, G.stream));
+  HIPCHK(hipStreamSynchronize(G.stream));
+  return 0;
+}
+
+int32_t ocfd_set_bc_time(double ht) {
+  if (check_ready()) return 1;
+  if (G.S.bl_ht != ht) drop_graph();
+  G.S.bl_ht = ht;
+  return 0;
+}
+
+static int check_negt() {
+  if (*G.h_negt > 10) return fail("ocfd: number of negative-Et points > 10 (handle_NegativeEt stops the run)");
+  return 0;
+}
+
+int32_t ocfd_flow_time_advance(double dt1) {
+  if (check_ready()) return 1;
+  if (check_negt()) return 1;
+  if (G.use_graph && !G.timing) {
+    if (!G.gexec || G.graph_dt != dt1) {
+      drop_graph();
+      HIPCHK(hipStreamBeginCapture(G.stream, hipStreamCaptureModeThreadLocal));
+      enq_step(dt1);
+      HIPCHK(hipStreamEndCapture(G.stream, &G.graph));
+      HIPCHK(hipGraphInstantiate(&G.gexec, G.graph, nullptr, nullptr, 0));
+      G.graph_dt = dt1;
+    }
+    HIPCHK(hipGraphLaunch(G.gexec, G.stream));
+  } else {
+    enq_step(dt1);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int32_t ocfd_save_stage0(void) {
+  if (check_ready()) return 1;
+  launch("k_negt_reset", k_negt_reset, dim3(1), dim3(64), G.S, 1);
+  launch("k_save0", k_save0, grid1(G.S.ni), dim3(256), G.S);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+int32_t ocfd_comput_amu_amk(void) {
+  if (check_ready()) return 1;
+  launch("k_transport", k_transport, grid1(G.S.ni), dim3(256), G.S);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+int32_t ocfd_stage_begin(int32_t krk) {
+  if (check_ready()) return 1;
+  launch("k_transport", k_transport, grid1(G.S.ni), dim3(256), G.S);
+  launch("k_di_from_d", k_di_from_d, grid1(G.S.nh), dim3(256), G.S);
+  if (krk == 1 && G.cfg.scheme_invis == OCFD_SCHEME_HYBRID) enq_rhybrid();
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+int32_t ocfd_comput_rhybrid(void) {
+  if (check_ready()) return 1;
+  enq_rhybrid();
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+int32_t ocfd_du_inviscous(void) {
+  if (check_ready()) return 1;
+  launch_sweep<0>();
+  launch_sweep<1>();
+  launch_sweep<2>();
+  if (G.S.mass_force) launch("k_rk", k_rk, grid1(G.S.ni), dim3(256), G.S, 1, 0.0, (int)(RK_MASS | RK_WRITE_DU));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+int32_t ocfd_du_viscous(void) {
+  if (check_ready()) return 1;
+  launch("k_visc_prep", k_visc_prep, grid1(G.S.ni), dim3(256), G.S);
+  launch("k_rk", k_rk, grid1(G.S.ni), dim3(256), G.S, 1, 0.0, (int)(RK_VISC | RK_WRITE_DU));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+int32_t ocfd_time_adv_rk3(int32_t krk, double dt) {
+  if (check_ready()) return 1;
+  if (krk < 1 || krk > 3) return fail("ocfd_time_adv_rk3: KRK must be 1, 2 or 3");
+  launch("k_rk", k_rk, grid1(G.S.ni), dim3(256), G.S, (int)krk, dt, (int)RK_UPDATE_F);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+int32_t ocfd_comput_flow_variables(void) {
+  if (check_ready()) return 1;
+  launch("k_negt_reset", k_negt_reset, dim3(1), dim3(64), G.S, 0);
+  launch("k_rk", k_rk, grid1(G.S.ni), dim3(256), G.S, 1, 0.0, (int)RK_PRIMS);
+  launch("k_negt_repair", k_negt_repair, dim3(1), dim3(64), G.S);
+  hipMemcpyAsync(G.h_negt, G.S.negt + 17, sizeof(int), hipMemcpyDeviceToHost, G.stream);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+int32_t ocfd_bc(void) {
+  if (check_ready()) return 1;
+  enq_bc();
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int32_t ocfd_negative_et_count(void) {
+  if (!G.inited) return 0;
+  hipStreamSynchronize(G.stream);
+  return *G.h_negt;
+}
+
+int32_t ocfd_synchronize(void) {
+  if (check_ready()) return 1;
+  HIPCHK(hipStreamSynchronize(G.stream));
+  HIPCHK(hipGetLastError());
+  collect_timing();
+  return check_negt();
+}
+
+void* ocfd_stream(void) { return (void*)G.stream; }
+
+int32_t ocfd_kernel_count(void) {
+  collect_timing();
+  return (int32_t)G.knames.size();
+}
+const char* ocfd_kernel_name(int32_t i) {
+  if (i < 0 || i >= (int)G.knames.size()) return "";
+  return G.knames[i].c_str();
+}
+int32_t ocfd_kernel_stats(int32_t i, double* total_ms, int64_t* launches) {
+  collect_timing();
+  if (i < 0 || i >= (int)G.kstats.size()) return fail("ocfd_kernel_stats: bad index");
+  if (total_ms) *total_ms = G.kstats[i].ms;
+  if (launches) *launches = G.kstats[i].launches;
+  return 0;
+}
+int32_t ocfd_kernel_stats_reset(void) {
+  collect_timing();
+  for (auto& k : G.kstats) k = KStat{};
+  return 0;
+}
+
+}  // extern "C"

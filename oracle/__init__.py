@@ -1,0 +1,176 @@
+"""CPU oracle for the flow and chemistry paths (TEST INFRASTRUCTURE ONLY).
+
+A ctypes wrapper around liboracle.so (built from ocfd_oracle*.c by the
+Makefile in this directory).  ``Oracle`` mirrors the method names of the
+product's ``FlowSolver`` so a test can run the same sequence on both.  Only
+tests/, the smoke check and the CPU-baseline leg of bench.py import this.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+_LIB = None
+
+
+def load(ocfd_module):
+    """Load liboracle.so; ``ocfd_module`` is the product package (for the
+    shared ocfd_cfg layout)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = ctypes.CDLL(LIB_PATH)
+    i32, dbl, vp = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+    P = ctypes.POINTER
+    sig = {
+        "orc_init": (i32, [P(ocfd_module.OcfdCfg)]), "orc_finalize": (i32, []),
+        "orc_last_error": (ctypes.c_char_p, []),
+        "orc_set_species": (i32, [i32, vp, dbl, vp, vp]), "orc_set_mesh": (i32, [vp, vp, vp, vp]),
+        "orc_upload_field": (i32, [i32, vp]), "orc_download_field": (i32, [i32, vp]),
+        "orc_download_mask": (i32, [i32, vp]),
+        "orc_set_bc_boundarylayer": (i32, [vp, vp, vp, vp]), "orc_set_bc_time": (i32, [dbl]),
+        "orc_flow_time_advance": (i32, [dbl]), "orc_comput_amu_amk": (i32, []),
+        "orc_du_inviscous": (i32, []), "orc_du_viscous": (i32, []), "orc_time_adv_rk3": (i32, [i32, dbl]),
+        "orc_comput_flow_variables": (i32, []), "orc_bc": (i32, []), "orc_comput_rhybrid": (i32, []),
+        "orc_exchange_all": (i32, []), "orc_negative_et_count": (i32, []), "orc_sensor_mean": (dbl, []),
+        "orc_dx0": (None, [vp, vp]), "orc_dy0": (None, [vp, vp]), "orc_dz0": (None, [vp, vp]),
+        "orc_recon_flux1": (i32, [i32, vp, vp, i32, vp, vp, vp]),
+        "orc_recon_flux2": (i32, [i32, vp, vp, i32, vp, vp, vp]),
+        "orc_set_ud7l_diss": (i32, [dbl]),
+        "orc_split": (i32, [i32, i32] + [vp] * 12),
+        "orc_comput_T": (dbl, [vp, dbl]), "orc_comput_E": (dbl, [vp, dbl]), "orc_comput_Cp": (dbl, [dbl, vp, dbl]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype, fn.argtypes = res, args
+    _LIB = lib
+    return lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Oracle:
+    """Same interface as the product's FlowSolver, computed by the C restatement."""
+
+    def __init__(self, ocfd_module, cfg):
+        self.m = ocfd_module
+        self.lib = load(ocfd_module)
+        self.cfg = cfg
+        self.nx, self.ny, self.nz = cfg.nx, cfg.ny, cfg.nz
+        L = ocfd_module.LAP
+        self.halo_shape = (cfg.nx + 2 * L, cfg.ny + 2 * L, cfg.nz + 2 * L)
+        self._check(self.lib.orc_init(ctypes.byref(cfg)))
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(self.lib.orc_last_error().decode())
+
+    field_shape = None  # bound below
+
+    def upload(self, name, arr):
+        a = np.asfortranarray(arr, dtype=np.float64)
+        assert a.shape == self.field_shape(name), (name, a.shape)
+        self._check(self.lib.orc_upload_field(self.m.FIELD[name], _ptr(a)))
+
+    def download(self, name):
+        a = np.zeros(self.field_shape(name), dtype=np.float64, order="F")
+        self._check(self.lib.orc_download_field(self.m.FIELD[name], _ptr(a)))
+        return a
+
+    def download_mask(self, direction):
+        shp = [self.nx, self.ny, self.nz]
+        shp[direction] += 1
+        a = np.zeros(tuple(shp), dtype=np.uint8, order="F")
+        self._check(self.lib.orc_download_mask(direction, _ptr(a)))
+        return a
+
+    def set_species(self, mi, n_tct=1, tct=None, tct_max=5000.0, coef=None):
+        mi = np.ascontiguousarray(mi, dtype=np.float64)
+        t = None if tct is None else np.ascontiguousarray(tct, dtype=np.float64)
+        cf = None if coef is None else np.ascontiguousarray(coef, dtype=np.float64)
+        self._check(self.lib.orc_set_species(n_tct, None if t is None else _ptr(t), tct_max, _ptr(mi),
+                                             None if cf is None else _ptr(cf)))
+
+    def set_mesh(self, axx, ayy, azz, span=None):
+        arrs = [np.asfortranarray(a, dtype=np.float64) for a in (axx, ayy, azz)]
+        sp = np.zeros(9) if span is None else np.ascontiguousarray(span, dtype=np.float64)
+        self._check(self.lib.orc_set_mesh(_ptr(arrs[0]), _ptr(arrs[1]), _ptr(arrs[2]), _ptr(sp)))
+
+    def set_bc_boundarylayer(self, inlet1d, upper0d, nonreflect_upper, wall_pertb):
+        inl = np.asfortranarray(inlet1d, dtype=np.float64)
+        up = np.ascontiguousarray(upper0d, dtype=np.float64)
+        nr = np.asfortranarray(nonreflect_upper, dtype=np.int32)
+        wp = np.asfortranarray(wall_pertb, dtype=np.float64)
+        self._check(self.lib.orc_set_bc_boundarylayer(_ptr(inl), _ptr(up), _ptr(nr), _ptr(wp)))
+
+    def set_bc_time(self, ht):
+        self._check(self.lib.orc_set_bc_time(float(ht)))
+
+    def upload_state(self, state):
+        for name in ("F", "DI", "D", "U", "V", "W", "T", "P", "ET"):
+            self.upload(name, state[name])
+
+    def flow_time_advance(self, dt1):
+        self._check(self.lib.orc_flow_time_advance(float(dt1)))
+
+    def comput_flow_variables(self):
+        self._check(self.lib.orc_comput_flow_variables())
+
+    def OCFD_bc(self):
+        self._check(self.lib.orc_bc())
+
+    def comput_Amu_Amk_AMD(self):
+        self._check(self.lib.orc_comput_amu_amk())
+
+    def du_inviscous(self):
+        self._check(self.lib.orc_du_inviscous())
+
+    def du_viscous(self):
+        self._check(self.lib.orc_du_viscous())
+
+    def OCFD_time_adv_RK3(self, krk, dt):
+        self._check(self.lib.orc_time_adv_rk3(int(krk), float(dt)))
+
+    def comput_Rhybrid(self):
+        self._check(self.lib.orc_comput_rhybrid())
+
+    def exchange_all(self):
+        self._check(self.lib.orc_exchange_all())
+
+    def negative_et_count(self):
+        return int(self.lib.orc_negative_et_count())
+
+    def sensor_mean(self):
+        return float(self.lib.orc_sensor_mean())
+
+    def close(self):
+        self.lib.orc_finalize()
+
+
+def _field_shape(self, name):
+    if name in ("F", "FN", "DU"):
+        return (self.nx, self.ny, self.nz, 5)
+    if name in ("AMU", "AMK"):
+        return (self.nx, self.ny, self.nz)
+    if name == "DIN":
+        return (self.nx, self.ny, self.nz, self.cfg.n_spec)
+    if name == "DI":
+        return self.halo_shape + (self.cfg.n_spec,)
+    return self.halo_shape
+
+
+Oracle.field_shape = _field_shape
