@@ -1,0 +1,191 @@
+#!/usr/bin/env python3
+"""Benchmark of the FP64 RK3 flow step on one MI355X.
+
+A step is one TVD-RK3 step (flow_time_advance, code/flow-solver/OCFD_Time_Adv.f90:7-92)
+over the whole synthetic Mach 2.25 flat-plate boundary layer, 640 x 120 x 64
+cells, hybrid UD7L/WENO7/WENO5 inviscid scheme switched by the pressure
+gradient sensor, Steger-Warming splitting, CD6 viscous terms, boundary-layer
+BCs.  The state is resident in HBM before the timed region; the step is
+replayed from a captured hipGraph.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+METRIC = "Mcell-steps/s per RK3 step (FP64); chem cells/s surrogate vs plain at ≤1% ΔT"
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBPS = 8000.0
+
+
+def load_pkg():
+    if "ocfd_amd" in sys.modules:
+        return sys.modules["ocfd_amd"]
+    pkg_dir = os.path.join(ROOT, "opencfd-comb_amd")
+    spec = importlib.util.spec_from_file_location("ocfd_amd", os.path.join(pkg_dir, "__init__.py"),
+                                                  submodule_search_locations=[pkg_dir])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["ocfd_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def setup(engine, case):
+    cfg, mesh, span, st, bc, info = case
+    engine.set_mesh(*mesh, span)
+    engine.set_bc_boundarylayer(bc["inlet1d"], bc["upper0d"], bc["nonreflect_upper"], bc["wall_pertb"])
+    engine.set_bc_time(info["ht"](0.0))
+    engine.upload_state(st)
+    engine.comput_flow_variables()
+    engine.OCFD_bc()
+
+
+def run_steps(engine, info, nsteps, t0_step):
+    """nsteps RK3 steps; the reference advances tt by dt after every second
+    RK3 step (OCFD_NS_Solver.f90:28-46), which sets the wall forcing ht(tt)."""
+    dt = info["dt"]
+    for s in range(t0_step, t0_step + nsteps):
+        if s % 2 == 0:
+            engine.set_bc_time(info["ht"]((s // 2) * dt))
+        engine.flow_time_advance(0.5 * dt)
+    return t0_step + nsteps
+
+
+def cpu_baseline(m, nsteps=20):
+    """The oracle (single-threaded C restatement) on a bounded sample of the
+    same workload: the flat plate at 160 x 120 x 32 (same wall-normal grid),
+    nsteps RK3 steps."""
+    sys.path.insert(0, ROOT)
+    import oracle
+    from ocfd_amd import cases
+    case = cases.flat_plate(160, 120, 32, lx=0.2 * 160 / 640, lz=0.004445 * 0.5)
+    o = oracle.Oracle(m, case[0])
+    setup(o, case)
+    t = time.perf_counter()
+    run_steps(o, case[5], nsteps, 0)
+    el = time.perf_counter() - t
+    o.close()
+    ncell = 160 * 120 * 32
+    return dict(value=ncell * nsteps / el / 1e6, unit="Mcell-steps/s", cores=1, kind="port",
+                sample=f"oracle (plain C, 1 thread) on the flat plate at 160x120x32, {nsteps} RK3 steps, {el:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--nx", type=int, default=640)
+    ap.add_argument("--ny", type=int, default=120)
+    ap.add_argument("--nz", type=int, default=64)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-kernel-timing", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU sharding is a design note, not an implemented path.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    m = load_pkg()
+    from ocfd_amd import cases, flops
+    case = cases.flat_plate(args.nx, args.ny, args.nz)
+    cfg, info = case[0], case[5]
+    ncell = cfg.nx * cfg.ny * cfg.nz
+    g = m.FlowSolver(cfg)
+    setup(g, case)
+    g.set_option(m.OPT_HYBRID_SUM_MODE, 1)
+    g.set_option(m.OPT_USE_GRAPH, 1)
+    step = run_steps(g, info, args.warmup, 0)
+    g.synchronize()
+    torch.cuda.synchronize()
+
+    t0 = time.perf_counter()
+    step = run_steps(g, info, args.steps, step)
+    g.synchronize()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    value = ncell * args.steps / elapsed / 1e6
+    f = g.download("F")
+    if not np.all(np.isfinite(f)):
+        raise RuntimeError("non-finite state after the timed steps")
+
+    # hybrid mask fractions (interior interfaces) of the last step
+    frac = {}
+    for d in range(3):
+        mk = g.download_mask(d)
+        sl = [slice(None)] * 3
+        sl[d] = slice(1, -1)
+        mk = mk[tuple(sl)]
+        frac[d] = {v: float(np.mean(mk == v)) for v in (1, 2, 3)}
+
+    roofline = None
+    kstats = {}
+    if not args.no_kernel_timing:
+        # per-launch HIP events on the library stream, over a second timed
+        # region of the same steps (graph replay off while events are on)
+        g.set_option(m.OPT_KERNEL_TIMING, 1)
+        g.kernel_stats_reset()
+        nprof = min(args.steps, 20)
+        step = run_steps(g, info, nprof, step)
+        g.synchronize()
+        kstats = g.kernel_stats()
+        g.set_option(m.OPT_KERNEL_TIMING, 0)
+        name, (ms, n) = max(kstats.items(), key=lambda kv: kv[1][0])
+        avg_s = ms / n / 1e3
+        per_cell = {
+            "k_sweep_x": flops.sweep_flops_per_cell(0, cfg.scheme_invis, cfg.flux_splitting, frac[0]),
+            "k_sweep_y": flops.sweep_flops_per_cell(1, cfg.scheme_invis, cfg.flux_splitting, frac[1]),
+            "k_sweep_z": flops.sweep_flops_per_cell(2, cfg.scheme_invis, cfg.flux_splitting, frac[2]),
+            "k_rk": flops.VISC_DIV + flops.RK + flops.PRIMS,
+            "k_visc_prep": flops.VISC_PREP,
+        }.get(name)
+        # algorithmic HBM bytes per cell of the kernel (one read of each input
+        # field, one write of each output field)
+        bytes_cell = {"k_sweep_x": (9 * 8 + 5 * 8 + 1), "k_sweep_y": (9 * 8 + 10 * 8 + 1),
+                      "k_sweep_z": (10 * 8 + 10 * 8 + 1), "k_rk": (5 * 8 * 4 + 12 * 8 + 8 * 8 + 9 * 8),
+                      "k_visc_prep": (8 * 8 + 19 * 8 + 14 * 8)}.get(name)
+        if per_cell is not None:
+            achieved = per_cell * ncell / avg_s / 1e12
+            roofline = dict(bound="mfma", pipe="fp64 vector (VALU); FP64 peak of MI355X", kernel=name,
+                            achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
+                            frac=achieved / PEAK_FP64_TFLOPS, traffic=None,
+                            flops_per_cell=per_cell, cells_per_launch=ncell, avg_launch_ms=avg_s * 1e3,
+                            hbm_achieved_GBps=(bytes_cell * ncell / avg_s / 1e9) if bytes_cell else None,
+                            hbm_frac=(bytes_cell * ncell / avg_s / 1e9 / PEAK_HBM_GBPS) if bytes_cell else None)
+
+    # whole-step roofline: measured rate / min(HBM / B_alg, FP64 / F_alg)
+    mean_frac = {v: sum(frac[d][v] for d in range(3)) / 3 for v in (1, 2, 3)}
+    f_alg = flops.step_flops_per_cell(cfg.scheme_invis, cfg.flux_splitting, mean_frac)
+    b_alg = flops.step_bytes_per_cell(5)
+    bound_hbm = PEAK_HBM_GBPS * 1e9 / b_alg / 1e6
+    bound_fp64 = PEAK_FP64_TFLOPS * 1e12 / f_alg / 1e6
+    step_roofline = dict(F_alg_per_cell=f_alg, B_alg_per_cell=b_alg, hbm_bound=bound_hbm, fp64_bound=bound_fp64,
+                         unit="Mcell-steps/s", frac=value / min(bound_hbm, bound_fp64))
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline(m)
+    g.close()
+    out = {
+        "metric": METRIC, "value": value, "unit": "Mcell-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"Mach 2.25 flat-plate boundary layer {cfg.nx}x{cfg.ny}x{cfg.nz}, one TVD-RK3 step",
+                   "scheme": "HYBRID UD7L/WENO7/WENO5, pressure-gradient sensor (fast mean)",
+                   "splitting": "Steger-Warming", "viscous": "CD6", "gas": "perfect gas, Sutherland",
+                   "graph": "hipGraph replay", "parallelism": "single GPU",
+                   "mask_fractions": {"x": frac[0], "y": frac[1], "z": frac[2]}},
+        "roofline": roofline, "step_roofline": step_roofline, "cpu_baseline": cpu,
+        "kernels_ms_per_step": {k: v[0] / max(1, min(args.steps, 20)) for k, v in kstats.items()},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -50,7 +50,8 @@ struct DevState {
   double mf[3];
   // BC (boundary layer)
   int bl_inlet, bl_outlet;
-  double bl_tw, bl_xinit, bl_disA, bl_ht;
+  double bl_tw, bl_xinit, bl_disA;
+  const double* bl_ht;   // device scalar: ht(tt) of the current time step
   const double* bl_inlet1d;
   const double* bl_upper0d;
   const int* bl_nonref;

@@ -507,7 +507,7 @@ __global__ void k_bl_wall(DevState S) {  // :143-201 (isothermal, non-catalytic,
     return;
   }
   const double tmp = 1.0 / sqrt(sqr(S.Aix[q1]) + sqr(S.Aiy[q1]) + sqr(S.Aiz[q1]));
-  const double A0 = S.bl_disA * S.bl_ht * S.bl_pertb[t];
+  const double A0 = S.bl_disA * S.bl_ht[0] * S.bl_pertb[t];
   S.u[q1] = A0 * S.Aix[q1] * tmp;
   S.v[q1] = A0 * S.Aiy[q1] * tmp;
   S.w[q1] = A0 * S.Aiz[q1] * tmp;
@@ -1122,7 +1122,6 @@ int32_t ocfd_init(const ocfd_cfg* cfg) {
   S.bl_tw = c.bc_para[4];
   S.bl_xinit = c.bc_para[5];
   S.bl_disA = c.bc_para[7];
-  S.bl_ht = 1.0;
   if (c.ibc == OCFD_BC_BOUNDARYLAYER) {
     if (S.bl_inlet != 1) return fail("ocfd_init: boundary layer inlet must be 1-D data (BC_Para(1)=1)");
     if (std::lround(c.bc_para[1]) != 0) return fail("ocfd_init: boundary layer upper data must be 0-D (BC_Para(2)=0)");
@@ -1172,6 +1171,12 @@ int32_t ocfd_init(const ocfd_cfg* cfg) {
   S.bl_upper0d = bl + (long)S.ny * 6;
   S.bl_pertb = bl + (long)S.ny * 6 + 6;
   S.bl_nonref = (const int*)nr;
+  {
+    double* htp = dalloc(1);
+    if (!htp) return fail("ocfd_init: hipMalloc failed");
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, G.stream, htp, 1L, 1.0);
+    S.bl_ht = htp;
+  }
   hipLaunchKernelGGL(k_fill, grid1((long)S.ny * 6 + 6 + (long)S.nx * S.nz), dim3(256), 0, G.stream, bl,
                      (long)S.ny * 6 + 6 + (long)S.nx * S.nz, 0.0);
   if (hipHostMalloc((void**)&G.h_negt, sizeof(int)) != hipSuccess) return fail("ocfd_init: hipHostMalloc failed");
@@ -1277,9 +1282,10 @@ int32_t ocfd_set_bc_boundarylayer(const double* inlet1d, const double* upper0d, 
 }
 
 int32_t ocfd_set_bc_time(double ht) {
+  // a device scalar, so a captured step graph stays valid when ht changes
   if (check_ready()) return 1;
-  if (G.S.bl_ht != ht) drop_graph();
-  G.S.bl_ht = ht;
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, G.stream, (double*)G.S.bl_ht, 1L, ht);
+  HIPCHK(hipGetLastError());
   return 0;
 }
 
