@@ -102,6 +102,7 @@ def main():
     g = m.FlowSolver(cfg)
     setup(g, case)
     g.set_option(m.OPT_HYBRID_SUM_MODE, 1)
+    tiles = g.autotune(3)
     g.set_option(m.OPT_USE_GRAPH, 1)
     step = run_steps(g, info, args.warmup, 0)
     g.synchronize()
@@ -180,6 +181,7 @@ def main():
                    "scheme": "HYBRID UD7L/WENO7/WENO5, pressure-gradient sensor (fast mean)",
                    "splitting": "Steger-Warming", "viscous": "CD6", "gas": "perfect gas, Sutherland",
                    "graph": "hipGraph replay", "parallelism": "single GPU",
+                   "sweep_tiles": {"xyz"[d]: {"NP": tiles[d][1], "TS": tiles[d][2]} for d in range(3)},
                    "mask_fractions": {"x": frac[0], "y": frac[1], "z": frac[2]}},
         "roofline": roofline, "step_roofline": step_roofline, "cpu_baseline": cpu,
         "kernels_ms_per_step": {k: v[0] / max(1, min(args.steps, 20)) for k, v in kstats.items()},

@@ -191,6 +191,17 @@ int32_t ocfd_bc(void);                      /* OCFD_BC.f90:2-188 */
  * ocfd_synchronize, a download or the next step. */
 int32_t ocfd_negative_et_count(void);
 
+/* ---- tile autotuner (no reference counterpart) ---------------------------
+ * The inviscid sweeps come in several workgroup tile shapes (pencils per
+ * workgroup x cells per pencil).  ocfd_autotune times each shape per
+ * direction on the current state with HIP events (reps launches each) and
+ * keeps the fastest; ocfd_set_tile forces one.  Results are bit-identical
+ * for every shape.  ocfd_tile_choice returns the chosen variant index and
+ * its shape, and the measured ms per launch of every variant (4 floats). */
+int32_t ocfd_autotune(int32_t reps);
+int32_t ocfd_set_tile(int32_t dir, int32_t variant);
+int32_t ocfd_tile_choice(int32_t dir, int32_t* np, int32_t* ts, float* ms_per_variant);
+
 /* ---- timing ------------------------------------------------------------ */
 /* Synchronises the library stream. */
 int32_t ocfd_synchronize(void);

@@ -123,6 +123,8 @@ def load_library(path=None):
         "ocfd_bc": (i32, []), "ocfd_negative_et_count": (i32, []), "ocfd_synchronize": (i32, []),
         "ocfd_stream": (vp, []), "ocfd_kernel_count": (i32, []), "ocfd_kernel_name": (ctypes.c_char_p, [i32]),
         "ocfd_kernel_stats": (i32, [i32, P(dbl), P(ctypes.c_int64)]), "ocfd_kernel_stats_reset": (i32, []),
+        "ocfd_autotune": (i32, [i32]), "ocfd_set_tile": (i32, [i32, i32]),
+        "ocfd_tile_choice": (i32, [i32, P(i32), P(i32), P(ctypes.c_float)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -257,6 +259,22 @@ class FlowSolver:
 
     def synchronize(self):
         self._check(self.lib.ocfd_synchronize())
+
+    # -- tile autotuner -------------------------------------------------
+    def autotune(self, reps=3):
+        """Time every sweep tile shape on the current state and keep the
+        fastest per direction; returns {dir: (variant, NP, TS, [ms...])}."""
+        self._check(self.lib.ocfd_autotune(int(reps)))
+        return {d: self.tile_choice(d) for d in range(3)}
+
+    def tile_choice(self, direction):
+        np_, ts = ctypes.c_int32(), ctypes.c_int32()
+        ms = (ctypes.c_float * 4)()
+        v = self.lib.ocfd_tile_choice(direction, ctypes.byref(np_), ctypes.byref(ts), ms)
+        return v, np_.value, ts.value, [float(x) for x in ms]
+
+    def set_tile(self, direction, variant):
+        self._check(self.lib.ocfd_set_tile(direction, variant))
 
     # -- timing ---------------------------------------------------------
     def kernel_stats(self):

@@ -84,6 +84,20 @@ __device__ __forceinline__ int wrap(int s, int n, int per) {
 
 __device__ __forceinline__ double sqr(double x) { return x * x; }
 
+// XCD-aware block order.  The hardware hands workgroup b to XCD b % 8; each
+// XCD has its own L2.  Remapping so that XCD x works on one contiguous range
+// of logical blocks keeps the stencil neighbours (k +- 3 planes) of a block in
+// the L2 of the XCD that reads them.  Bijective on [0, nb).
+__device__ __forceinline__ long xcd_block(long b, long nb) {
+  const long q = nb >> 3, rem = nb & 7;
+  const long x = b & 7, r = b >> 3;
+  return x * q + (x < rem ? x : rem) + r;
+}
+// linear cell index of this thread for a 1-D launch of 256-thread blocks
+__device__ __forceinline__ long cell_index() {
+  return xcd_block((long)blockIdx.x, (long)gridDim.x) * blockDim.x + threadIdx.x;
+}
+
 // ---------------------------------------------------------------------------
 // reconstructions on a window w[0..7] = v(-3..4)
 __device__ __forceinline__ double recon_weno5(const double* w, int drop) {

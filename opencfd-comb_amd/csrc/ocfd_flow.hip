@@ -30,11 +30,23 @@
 // are i-rows (coalesced along the sweep); for DIR 1/2 the NP pencils are
 // adjacent in i, so loads coalesce across pencils.
 // ===========================================================================
-template <int DIR>
-struct SweepTile;
-template <> struct SweepTile<0> { static constexpr int NP = 4, TS = 64; };
-template <> struct SweepTile<1> { static constexpr int NP = 16, TS = 16; };
-template <> struct SweepTile<2> { static constexpr int NP = 16, TS = 16; };
+// Tile shapes (pencils per workgroup NP x cells per pencil TS) offered to
+// the autotuner; variant 0 is the default.  Every variant computes every
+// interface with the same arithmetic, so results do not depend on the choice.
+template <int DIR, int V> struct SweepTile;
+template <> struct SweepTile<0, 0> { static constexpr int NP = 4, TS = 64; };
+template <> struct SweepTile<0, 1> { static constexpr int NP = 8, TS = 32; };
+template <> struct SweepTile<0, 2> { static constexpr int NP = 2, TS = 128; };
+template <> struct SweepTile<0, 3> { static constexpr int NP = 8, TS = 64; };
+template <> struct SweepTile<1, 0> { static constexpr int NP = 16, TS = 16; };
+template <> struct SweepTile<1, 1> { static constexpr int NP = 8, TS = 32; };
+template <> struct SweepTile<1, 2> { static constexpr int NP = 16, TS = 32; };
+template <> struct SweepTile<1, 3> { static constexpr int NP = 32, TS = 16; };
+template <> struct SweepTile<2, 0> { static constexpr int NP = 16, TS = 16; };
+template <> struct SweepTile<2, 1> { static constexpr int NP = 8, TS = 32; };
+template <> struct SweepTile<2, 2> { static constexpr int NP = 16, TS = 32; };
+template <> struct SweepTile<2, 3> { static constexpr int NP = 32, TS = 16; };
+#define OCFD_SWEEP_VARIANTS 4
 
 template <int DIR>
 __device__ __forceinline__ void pencil_ijk(const DevState& S, long pen, int& a0, int& a1) {
@@ -122,25 +134,38 @@ __device__ __forceinline__ void split_point(int kind, double Et1, double p1, dou
   }
 }
 
-template <int DIR>
+template <int DIR, int V>
 __global__ __launch_bounds__(256) void k_sweep(DevState S) {
-  constexpr int NP = SweepTile<DIR>::NP, TS = SweepTile<DIR>::TS;
-  constexpr int NPT = TS + 8, NIF = TS + 1;
+  constexpr int NP = SweepTile<DIR, V>::NP, TS = SweepTile<DIR, V>::TS;
+  constexpr int NPT = TS + 8, NIF = TS + 1, NE = 2 * NP * NIF;
+  constexpr int MAXT = (5 * NE + 255) / 256;
+  static_assert(10 * NP * NIF <= 10 * NP * NPT, "interface values must fit in the split buffer");
+  // one LDS buffer: split fluxes (phase 1-2), then interface values (phase 3)
   __shared__ double sF[10 * NP * NPT];
-  __shared__ double sH[10 * NP * NIF];
+  double* sH = sF;
+  __shared__ unsigned short sList[NE];
+  __shared__ unsigned char sKind[NE];
+  __shared__ int sCnt[4][10];
+  __shared__ int sTotal;
   const int n = S.n[DIR];
-  const int c0 = (int)blockIdx.x * TS + 1;
+  // XCD-aware tile order over the flattened 2-D grid
+  const long nb = (long)gridDim.x * gridDim.y;
+  const long lb = xcd_block((long)blockIdx.y * gridDim.x + blockIdx.x, nb);
+  const int bx = (int)(lb % gridDim.x);
+  const long by = lb / gridDim.x;
+  const int c0 = bx * TS + 1;
   const int ncell = min(TS, n - c0 + 1);
   const long npen = DIR == 0 ? (long)S.ny * S.nz : DIR == 1 ? (long)S.nx * S.nz : (long)S.nx * S.ny;
-  const long pen0 = (long)blockIdx.y * NP;
+  const long pen0 = by * NP;
   auto lp = [](int pl, int s) { return DIR == 0 ? pl * NPT + s : s * NP + pl; };
   auto lh = [](int pl, int l) { return DIR == 0 ? pl * NIF + l : l * NP + pl; };
   const double* M1 = DIR == 0 ? S.Akx1 : DIR == 1 ? S.Aix1 : S.Asx1;
   const double* M2 = DIR == 0 ? S.Aky1 : DIR == 1 ? S.Aiy1 : S.Asy1;
   const double* M3 = DIR == 0 ? S.Akz1 : DIR == 1 ? S.Aiz1 : S.Asz1;
+  const int tid = threadIdx.x;
 
   // phase 1: split fluxes at the tile's points s = c0-4 .. c0+ncell+3
-  for (int t = threadIdx.x; t < NP * NPT; t += blockDim.x) {
+  for (int t = tid; t < NP * NPT; t += blockDim.x) {
     const int pl = DIR == 0 ? t / NPT : t % NP;
     const int sl = DIR == 0 ? t % NPT : t / NP;
     const long pen = pen0 + pl;
@@ -157,32 +182,108 @@ __global__ __launch_bounds__(256) void k_sweep(DevState S) {
       sF[(5 + m) * NP * NPT + lp(pl, sl)] = fm[m];
     }
   }
-  __syncthreads();
 
-  // phase 2: reconstruct at interfaces s_if = c0-1 .. c0+ncell-1
+  // phase 2a: classify the (interface, side) entries and sort them by
+  // reconstruction kind (stable: spatial order kept inside a kind), so the
+  // lanes of a wavefront run one scheme; entry e = side*NP*NIF + lh(pl, l).
   const bool hyb = S.scheme == OCFD_SCHEME_HYBRID;
-  for (int t = threadIdx.x; t < 10 * NP * NIF; t += blockDim.x) {
-    const int cs = t / (NP * NIF);
-    const int r = t % (NP * NIF);
+  const bool nearb = !S.per[DIR] && (c0 - 1 <= 4 || c0 + ncell - 1 >= n - 4);
+  const bool uniform = !hyb && !nearb;
+  const int wave = tid >> 6, lane = tid & 63;
+  constexpr int PERW = (NE + 3) / 4;
+  const int e0 = wave * PERW, e1 = min(NE, e0 + PERW);
+  int total;
+  if (!uniform) {
+    int cnt[10];
+#pragma unroll
+    for (int k = 0; k < 10; k++) cnt[k] = 0;
+    for (int base = e0; base < e1; base += 64) {
+      const int e = base + lane;
+      int kind = 0;
+      if (e < e1) {
+        const int side = e / (NP * NIF);
+        const int r = e % (NP * NIF);
+        const int pl = DIR == 0 ? r / NIF : r % NP;
+        const int l = DIR == 0 ? r % NIF : r / NP;
+        const long pen = pen0 + pl;
+        if (pen < npen && l <= ncell) {
+          const int s_if = c0 - 1 + l;
+          const int mask = hyb ? (int)S.mask[DIR][mask_index<DIR>(S, pen, s_if)] : 0;
+          kind = recon_kind(S, DIR, s_if, side, mask);
+        }
+        sKind[e] = (unsigned char)kind;
+      }
+#pragma unroll
+      for (int k = 1; k < 10; k++) cnt[k] += __popcll(__ballot(kind == k));
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < 10; k++) sCnt[wave][k] = cnt[k];
+    }
+    __syncthreads();
+    int off[10];
+    int acc = 0;
+#pragma unroll
+    for (int k = 1; k < 10; k++) {
+      int before = 0;
+      for (int w = 0; w < 4; w++) {
+        if (w < wave) before += sCnt[w][k];
+        acc += sCnt[w][k];
+      }
+      off[k] = acc - (sCnt[0][k] + sCnt[1][k] + sCnt[2][k] + sCnt[3][k]) + before;
+    }
+    if (tid == 0) sTotal = acc;
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    for (int base = e0; base < e1; base += 64) {
+      const int e = base + lane;
+      const int kind = e < e1 ? (int)sKind[e] : 0;
+#pragma unroll
+      for (int k = 1; k < 10; k++) {
+        const unsigned long long bal = __ballot(kind == k);
+        if (kind == k) sList[off[k] + __popcll(bal & lt)] = (unsigned short)e;
+        off[k] += __popcll(bal);
+      }
+    }
+  }
+  __syncthreads();
+  total = uniform ? NE : sTotal;
+
+  // phase 2b: reconstruct; consecutive lanes take consecutive sorted entries.
+  // Results are held in registers until every lane has read its windows,
+  // then written over the split buffer.
+  const int inner = S.scheme == 1 ? K_WENO5 : S.scheme == 2 ? K_WENO7 : S.scheme == 3 ? K_OMP6 : K_UD7L;
+  double res[MAXT];
+  int dst[MAXT];
+#pragma unroll
+  for (int it = 0; it < MAXT; it++) {
+    dst[it] = -1;
+    const int t = tid + it * 256;
+    if (t >= 5 * total) continue;
+    const int comp = t / total;
+    const int idx = t - comp * total;
+    const int e = uniform ? idx : (int)sList[idx];
+    const int side = e / (NP * NIF);
+    const int r = e - side * (NP * NIF);
     const int pl = DIR == 0 ? r / NIF : r % NP;
     const int l = DIR == 0 ? r % NIF : r / NP;
-    const long pen = pen0 + pl;
-    if (pen >= npen || l > ncell) continue;
-    const int s_if = c0 - 1 + l;
-    const int side = cs / 5;
-    const int mask = hyb ? (int)S.mask[DIR][mask_index<DIR>(S, pen, s_if)] : 0;
-    const int kind = recon_kind(S, DIR, s_if, side, mask);
-    const double* F = sF + cs * NP * NPT;
+    if (uniform && (pen0 + pl >= npen || l > ncell)) continue;
+    const int kind = uniform ? inner : (int)sKind[e];
+    const double* F = sF + (side * 5 + comp) * NP * NPT;
     double w[8];
 #pragma unroll
     for (int qq = 0; qq < 8; qq++) w[qq] = F[lp(pl, side ? l + 7 - qq : l + qq)];
-    sH[cs * NP * NIF + lh(pl, l)] = recon(kind, w, S.ud7l);
+    res[it] = recon(kind, w, S.ud7l);
+    dst[it] = (side * 5 + comp) * NP * NIF + r;
   }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < MAXT; it++)
+    if (dst[it] >= 0) sH[dst[it]] = res[it];
   __syncthreads();
 
   // phase 3: du from interface differences
   const double h1 = S.h_1[DIR];
-  for (int t = threadIdx.x; t < 5 * NP * TS; t += blockDim.x) {
+  for (int t = tid; t < 5 * NP * TS; t += blockDim.x) {
     const int m = t / (NP * TS);
     const int r = t % (NP * TS);
     const int pl = DIR == 0 ? r / TS : r % NP;
@@ -215,7 +316,7 @@ __global__ __launch_bounds__(256) void k_sweep(DevState S) {
 __device__ __forceinline__ double pw3(double x) { double x2 = x * x; return x2 * x; }
 
 __global__ __launch_bounds__(256) void k_visc_prep(DevState S) {
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long c = cell_index();
   if (c >= S.ni) return;
   const int i = (int)(c % S.nx) + 1;
   const int j = (int)((c / S.nx) % S.ny) + 1;
@@ -289,7 +390,7 @@ __global__ __launch_bounds__(256) void k_visc_prep(DevState S) {
 // Transport only (when the viscous terms are off the reference still
 // evaluates Amu/Amk at every stage; kept for the per-stage ABI).
 __global__ __launch_bounds__(256) void k_transport(DevState S) {
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long c = cell_index();
   if (c >= S.ni) return;
   const int i = (int)(c % S.nx) + 1;
   const int j = (int)((c / S.nx) % S.ny) + 1;
@@ -316,7 +417,7 @@ __global__ __launch_bounds__(256) void k_transport(DevState S) {
 enum { RK_MASS = 1, RK_VISC = 2, RK_WRITE_DU = 4, RK_UPDATE_F = 8, RK_PRIMS = 16, RK_DI_FROM_D = 32 };
 
 __global__ __launch_bounds__(256) void k_rk(DevState S, int krk, double dt, int flags) {
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long c = cell_index();
   if (c >= S.ni) return;
   const int i = (int)(c % S.nx) + 1;
   const int j = (int)((c / S.nx) % S.ny) + 1;
@@ -597,7 +698,7 @@ __global__ __launch_bounds__(256) void k_save0(DevState S) {
 // per-interface scheme masks (set_hybrid_scheme, :2-14; OCFD_inviscous.f90:63-68).
 // ===========================================================================
 __global__ __launch_bounds__(256) void k_sensor1(DevState S) {
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long c = cell_index();
   if (c >= S.ni) return;
   const int i = (int)(c % S.nx) + 1;
   const int j = (int)((c / S.nx) % S.ny) + 1;
@@ -613,7 +714,7 @@ __global__ __launch_bounds__(256) void k_sensor1(DevState S) {
 }
 
 __global__ __launch_bounds__(256) void k_sensor2(DevState S) {
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long c = cell_index();
   if (c >= S.ni) return;
   const int i = (int)(c % S.nx) + 1;
   const int j = (int)((c / S.nx) % S.ny) + 1;
@@ -747,7 +848,7 @@ __global__ void k_fill_faces(DevState S, double* A, int shift, double span) {
 }
 
 __global__ __launch_bounds__(256) void k_metrics(DevState S) {
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long c = cell_index();
   if (c >= S.ni) return;
   const int i = (int)(c % S.nx) + 1;
   const int j = (int)((c / S.nx) % S.ny) + 1;
@@ -823,6 +924,8 @@ struct Ctx {
   double* part = nullptr;  // fast-sum partials
   int nparts = 0;
   int hybrid_sum_mode = 0;
+  int sweep_variant[3] = {0, 0, 0};
+  float tune_ms[3][8] = {};
   int use_graph = 0;
   int timing = 0;
   hipGraphExec_t gexec = nullptr;
@@ -907,15 +1010,28 @@ void launch(const char* name, K kern, dim3 g, dim3 b, A... args) {
 
 inline dim3 grid1(long n) { return dim3((unsigned)((n + 255) / 256)); }
 
-template <int DIR>
-void launch_sweep() {
+template <int DIR, int V>
+void launch_sweep_v() {
   const DevState& S = G.S;
-  constexpr int NP = SweepTile<DIR>::NP, TS = SweepTile<DIR>::TS;
+  constexpr int NP = SweepTile<DIR, V>::NP, TS = SweepTile<DIR, V>::TS;
   const long npen = DIR == 0 ? (long)S.ny * S.nz : DIR == 1 ? (long)S.nx * S.nz : (long)S.nx * S.ny;
   dim3 g((unsigned)((S.n[DIR] + TS - 1) / TS), (unsigned)((npen + NP - 1) / NP));
   static const char* names[3] = {"k_sweep_x", "k_sweep_y", "k_sweep_z"};
-  launch(names[DIR], k_sweep<DIR>, g, dim3(256), S);
+  launch(names[DIR], k_sweep<DIR, V>, g, dim3(256), S);
 }
+
+template <int DIR>
+void launch_sweep_variant(int v) {
+  switch (v) {
+    case 1: launch_sweep_v<DIR, 1>(); break;
+    case 2: launch_sweep_v<DIR, 2>(); break;
+    case 3: launch_sweep_v<DIR, 3>(); break;
+    default: launch_sweep_v<DIR, 0>(); break;
+  }
+}
+
+template <int DIR>
+void launch_sweep() { launch_sweep_variant<DIR>(G.sweep_variant[DIR]); }
 
 template <int DIR>
 void launch_masks() {
@@ -965,8 +1081,7 @@ void enq_stage(int krk, double dt1) {
   launch_sweep<2>();
   if (S.viscous) launch("k_visc_prep", k_visc_prep, grid1(S.ni), dim3(256), S);
   else launch("k_transport", k_transport, grid1(S.ni), dim3(256), S);
-  const int flags = (S.mass_force ? RK_MASS : 0) | (S.viscous ? RK_VISC : 0) | RK_WRITE_DU | RK_UPDATE_F | RK_PRIMS |
-                    RK_DI_FROM_D;
+  const int flags = (S.mass_force ? RK_MASS : 0) | (S.viscous ? RK_VISC : 0) | RK_UPDATE_F | RK_PRIMS | RK_DI_FROM_D;
   launch("k_negt_reset", k_negt_reset, dim3(1), dim3(64), S, 0);
   launch("k_rk", k_rk, grid1(S.ni), dim3(256), S, krk, dt1, flags);
   launch("k_negt_repair", k_negt_repair, dim3(1), dim3(64), S);
@@ -1377,6 +1492,71 @@ int32_t ocfd_bc(void) {
   if (check_ready()) return 1;
   enq_bc();
   HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// Autotuner for the inviscid sweep tiles: times every tile variant of each
+// direction on the current state (the sweeps only overwrite du, which every
+// stage recomputes) and keeps the fastest.  Results are bitwise independent
+// of the choice.
+int32_t ocfd_autotune(int32_t reps) {
+  if (check_ready()) return 1;
+  if (reps < 1) reps = 3;
+  drop_graph();
+  const int saved_timing = G.timing;
+  G.timing = 0;
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  for (int dir = 0; dir < 3; dir++) {
+    float best = 1e30f;
+    int bestv = 0;
+    for (int v = 0; v < OCFD_SWEEP_VARIANTS; v++) {
+      auto run = [&]() {
+        if (dir == 0) launch_sweep_variant<0>(v);
+        else if (dir == 1) launch_sweep_variant<1>(v);
+        else launch_sweep_variant<2>(v);
+      };
+      run();  // warm
+      HIPCHK(hipEventRecord(a, G.stream));
+      for (int r = 0; r < reps; r++) run();
+      HIPCHK(hipEventRecord(b, G.stream));
+      HIPCHK(hipEventSynchronize(b));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, a, b));
+      G.tune_ms[dir][v] = ms / reps;
+      if (ms < best) { best = ms; bestv = v; }
+    }
+    G.sweep_variant[dir] = bestv;
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  G.timing = saved_timing;
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int32_t ocfd_tile_choice(int32_t dir, int32_t* np, int32_t* ts, float* ms_per_variant) {
+  if (dir < 0 || dir > 2) return fail("ocfd_tile_choice: bad direction");
+  static const int tiles[3][OCFD_SWEEP_VARIANTS][2] = {
+      {{SweepTile<0, 0>::NP, SweepTile<0, 0>::TS}, {SweepTile<0, 1>::NP, SweepTile<0, 1>::TS},
+       {SweepTile<0, 2>::NP, SweepTile<0, 2>::TS}, {SweepTile<0, 3>::NP, SweepTile<0, 3>::TS}},
+      {{SweepTile<1, 0>::NP, SweepTile<1, 0>::TS}, {SweepTile<1, 1>::NP, SweepTile<1, 1>::TS},
+       {SweepTile<1, 2>::NP, SweepTile<1, 2>::TS}, {SweepTile<1, 3>::NP, SweepTile<1, 3>::TS}},
+      {{SweepTile<2, 0>::NP, SweepTile<2, 0>::TS}, {SweepTile<2, 1>::NP, SweepTile<2, 1>::TS},
+       {SweepTile<2, 2>::NP, SweepTile<2, 2>::TS}, {SweepTile<2, 3>::NP, SweepTile<2, 3>::TS}}};
+  const int v = G.sweep_variant[dir];
+  if (np) *np = tiles[dir][v][0];
+  if (ts) *ts = tiles[dir][v][1];
+  if (ms_per_variant)
+    for (int k = 0; k < OCFD_SWEEP_VARIANTS; k++) ms_per_variant[k] = G.tune_ms[dir][k];
+  return v;
+}
+
+int32_t ocfd_set_tile(int32_t dir, int32_t variant) {
+  if (dir < 0 || dir > 2 || variant < 0 || variant >= OCFD_SWEEP_VARIANTS) return fail("ocfd_set_tile: bad argument");
+  G.sweep_variant[dir] = variant;
+  drop_graph();
   return 0;
 }
 
