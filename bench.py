@@ -75,6 +75,22 @@ def cpu_baseline(m, nsteps=20):
                 sample=f"oracle (plain C, 1 thread) on the flat plate at 160x120x32, {nsteps} RK3 steps, {el:.1f} s")
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy), or None."""
+    path = os.path.join(ROOT, "profiles", "flat_plate_640x120x64_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.split("|")]
+        if len(cells) > 5 and cells[1] == kernel:
+            try:
+                return float(cells[5]) * 2 ** 20
+            except ValueError:
+                return None
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,7 +101,19 @@ def main():
     ap.add_argument("--nz", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="only run a plain 512 MiB device copy (3 times): the reference kernel for scaling "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.calib_copy:
+        import torch
+        a = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calib_copy_MiB": 512, "repeats": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU sharding is a design note, not an implemented path.", file=sys.stderr)
@@ -150,17 +178,29 @@ def main():
         }.get(name)
         # algorithmic HBM bytes per cell of the kernel (one read of each input
         # field, one write of each output field)
-        bytes_cell = {"k_sweep_x": (9 * 8 + 5 * 8 + 1), "k_sweep_y": (9 * 8 + 10 * 8 + 1),
-                      "k_sweep_z": (10 * 8 + 10 * 8 + 1), "k_rk": (5 * 8 * 4 + 12 * 8 + 8 * 8 + 9 * 8),
-                      "k_visc_prep": (8 * 8 + 19 * 8 + 14 * 8)}.get(name)
+        # algorithmic HBM bytes per cell (each input field read once, each
+        # output written once): sweeps read d,u,v,w,p,Et + 3 metrics + mask and
+        # write du (y, z also read du, z reads Ajac); k_rk reads f,fn,du,Ev(12),
+        # d,Et,Ajac and writes f,d,u,v,w,Et,di,T,p,Et_prev; k_visc_prep reads
+        # u,v,w,T,d,di + 18 metrics and writes Amu,Amk,Ev(12)
+        bytes_cell = {"k_sweep_x": 73 + 40, "k_sweep_y": 73 + 80, "k_sweep_z": 73 + 88,
+                      "k_rk": 240 + 112, "k_visc_prep": 192 + 112}.get(name)
+        traffic = measured_traffic(name)
         if per_cell is not None:
-            achieved = per_cell * ncell / avg_s / 1e12
-            roofline = dict(bound="mfma", pipe="fp64 vector (VALU); FP64 peak of MI355X", kernel=name,
-                            achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
-                            frac=achieved / PEAK_FP64_TFLOPS, traffic=None,
-                            flops_per_cell=per_cell, cells_per_launch=ncell, avg_launch_ms=avg_s * 1e3,
-                            hbm_achieved_GBps=(bytes_cell * ncell / avg_s / 1e9) if bytes_cell else None,
-                            hbm_frac=(bytes_cell * ncell / avg_s / 1e9 / PEAK_HBM_GBPS) if bytes_cell else None)
+            fp64 = per_cell * ncell / avg_s / 1e12
+            hbm = bytes_cell * ncell / avg_s / 1e9
+            memory_bound = name in ("k_rk", "k_visc_prep")
+            if memory_bound:
+                roofline = dict(bound="hbm", kernel=name, achieved=hbm, peak=PEAK_HBM_GBPS, unit="GB/s",
+                                frac=hbm / PEAK_HBM_GBPS, traffic=traffic, bytes_per_cell=bytes_cell,
+                                fp64_achieved_TFLOPs=fp64)
+            else:
+                roofline = dict(bound="mfma", pipe="fp64 vector (VALU); MI355X FP64 peak", kernel=name,
+                                achieved=fp64, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s", frac=fp64 / PEAK_FP64_TFLOPS,
+                                traffic=traffic, flops_per_cell=per_cell, hbm_achieved_GBps=hbm)
+            roofline.update(cells_per_launch=ncell, avg_launch_ms=avg_s * 1e3,
+                            timing="HIP events around every launch on the library stream, second pass of "
+                                   f"{nprof} steps (graph replay off)")
 
     # whole-step roofline: measured rate / min(HBM / B_alg, FP64 / F_alg)
     mean_frac = {v: sum(frac[d][v] for d in range(3)) / 3 for v in (1, 2, 3)}
