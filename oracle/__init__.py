@@ -174,3 +174,93 @@ def _field_shape(self, name):
 
 
 Oracle.field_shape = _field_shape
+
+
+class ChemOracle:
+    """Chemistry restatement (CHEM_reactions.f90 / CHEM_thermdyn.f90)."""
+
+    def __init__(self, ocfd_module, mech):
+        self.lib = load(ocfd_module)
+        L = self.lib
+        i32, dbl, vp = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+        sig = {
+            "orc_chem_set_mechanism": (i32, [i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, i32, vp, dbl, vp, vp]),
+            "orc_chem_E": (dbl, [vp, dbl]), "orc_chem_T": (dbl, [vp, dbl]), "orc_chem_gibbs": (dbl, [i32, dbl]),
+            "orc_chem_rate_coef": (i32, [vp, dbl, vp, vp, vp]), "orc_chem_rates": (i32, [vp, dbl, vp, vp]),
+            "orc_gauss": (i32, [i32, vp, vp, vp]), "orc_chem_timeadv": (i32, [i32, vp, vp, dbl]),
+            "orc_chem_batch": (i32, [i32, ctypes.c_long, vp, vp, vp, i32]),
+            "orc_chem_adaptive": (i32, [vp, vp, dbl, dbl, dbl, i32, vp]),
+            "orc_chem_adaptive_batch": (i32, [ctypes.c_long, vp, vp, vp, dbl, dbl, i32, vp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype, fn.argtypes = res, args
+        self.m = mech
+        self._keep = [np.ascontiguousarray(a) for a in (
+            mech.af_type.astype(np.int32), mech.third_body.astype(np.int32), mech.fc_troe.astype(np.float64),
+            mech.arrhenius.astype(np.float64), mech.vf.astype(np.int32), mech.vr.astype(np.int32),
+            mech.eff.astype(np.float64), np.asarray(mech.tct, dtype=np.float64), mech.mi.astype(np.float64),
+            mech.coef.astype(np.float64))]
+        k = self._keep
+        rc = L.orc_chem_set_mechanism(mech.n_spec, mech.n_reac, mech.rereac, _ptr(k[0]), _ptr(k[1]), _ptr(k[2]),
+                                      _ptr(k[3]), _ptr(k[4]), _ptr(k[5]), _ptr(k[6]), mech.n_tct, _ptr(k[7]),
+                                      float(mech.tct_max), _ptr(k[8]), _ptr(k[9]))
+        if rc:
+            raise RuntimeError("orc_chem_set_mechanism failed")
+
+    def E(self, di, T):
+        return self.lib.orc_chem_E(_ptr(np.ascontiguousarray(di, dtype=np.float64)), float(T))
+
+    def T(self, di, E):
+        return self.lib.orc_chem_T(_ptr(np.ascontiguousarray(di, dtype=np.float64)), float(E))
+
+    def gibbs(self, k, T):
+        return self.lib.orc_chem_gibbs(int(k), float(T))
+
+    def rate_coef(self, ci, T):
+        nr = self.m.n_reac
+        wf, wr, kj = np.zeros(nr), np.zeros(nr), np.zeros(nr)
+        c = np.ascontiguousarray(ci, dtype=np.float64)
+        self.lib.orc_chem_rate_coef(_ptr(c), float(T), _ptr(wf), _ptr(wr), _ptr(kj))
+        return wf, wr, kj
+
+    def rates(self, ci, T, jacobian=True):
+        ns = self.m.n_spec
+        ri = np.zeros(ns)
+        A = np.zeros((ns, ns)) if jacobian else None
+        c = np.ascontiguousarray(ci, dtype=np.float64)
+        self.lib.orc_chem_rates(_ptr(c), float(T), _ptr(ri), _ptr(A) if jacobian else None)
+        return ri, A
+
+    def gauss(self, A, b):
+        A = np.ascontiguousarray(A, dtype=np.float64).copy()
+        b = np.ascontiguousarray(b, dtype=np.float64).copy()
+        x = np.zeros(len(b))
+        self.lib.orc_gauss(len(b), _ptr(A), _ptr(b), _ptr(x))
+        return x
+
+    def timeadv(self, mode, di, E, dt):
+        d = np.ascontiguousarray(di, dtype=np.float64).copy()
+        e = np.array([E], dtype=np.float64)
+        if self.lib.orc_chem_timeadv(int(mode), _ptr(d), _ptr(e), float(dt)):
+            raise RuntimeError("orc_chem_timeadv failed")
+        return d, float(e[0])
+
+    def batch(self, mode, rho, E, dt, nsub):
+        """rho [n][ns] (C order), E [n], dt [n]; returns new copies."""
+        r = np.ascontiguousarray(rho, dtype=np.float64).copy()
+        e = np.ascontiguousarray(E, dtype=np.float64).copy()
+        d = np.ascontiguousarray(dt, dtype=np.float64)
+        if self.lib.orc_chem_batch(int(mode), len(e), _ptr(r), _ptr(e), _ptr(d), int(nsub)):
+            raise RuntimeError("orc_chem_batch failed")
+        return r, e
+
+    def adaptive_batch(self, rho, E, dt, frac_y, frac_t, max_sub):
+        r = np.ascontiguousarray(rho, dtype=np.float64).copy()
+        e = np.ascontiguousarray(E, dtype=np.float64).copy()
+        d = np.ascontiguousarray(dt, dtype=np.float64)
+        ns = np.zeros(len(e), dtype=np.int32)
+        if self.lib.orc_chem_adaptive_batch(len(e), _ptr(r), _ptr(e), _ptr(d), float(frac_y), float(frac_t),
+                                            int(max_sub), _ptr(ns)):
+            raise RuntimeError("orc_chem_adaptive_batch failed")
+        return r, e, ns
