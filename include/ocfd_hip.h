@@ -215,6 +215,62 @@ const char* ocfd_kernel_name(int32_t i);
 int32_t ocfd_kernel_stats(int32_t i, double* total_ms, int64_t* launches);
 int32_t ocfd_kernel_stats_reset(void);
 
+/* ==== finite-rate chemistry: batched 0-D cells ============================
+ * Independent of the flow context above (its own stream and error string).
+ * A "cell" is the 0-D state chemical_timeAdv integrates at constant volume:
+ * partial densities rho_k (kg/m3) and the energy E (J/m3).  All arrays are
+ * DEVICE pointers: rho is species-major [n_spec][n] (rho[k*n + c]), E and dt
+ * are [n].  Calls are asynchronous on ocfd_chem_stream(); the state is
+ * updated in place. */
+
+/* Mechanism and thermo data in the layout read_Reac / read_Spec fill
+ * (CHEM_reactions.f90:15-71, CHEM_thermdyn.f90:16-51): af_type, third_body,
+ * fc_troe [nr]; arrhenius [nr][2][3] (A, b, E for the low / second set);
+ * vf, vr, eff [nr][ns]; tct [n_tct-1], tct_max; mi [ns];
+ * coef [ns][n_tct][7] (NASA-7 A..G per temperature range).  This build
+ * takes ns = 9, nr <= 32, n_tct <= 4 and ReReac = 0 (reverse rates from
+ * Gibbs equilibrium constants, comput_KjX CHEM_reactions.f90:76-103). */
+int32_t ocfd_chem_set_mechanism(int32_t ns, int32_t nr, int32_t rereac, const int32_t* af_type,
+                                const int32_t* third_body, const double* fc_troe, const double* arrhenius,
+                                const int32_t* vf, const int32_t* vr, const double* eff, int32_t n_tct,
+                                const double* tct, double tct_max, const double* mi, const double* coef);
+
+/* Chemical_time_advance over a batch (OCFD_Time_Adv.f90:169-196 calling
+ * chemical_timeAdv, CHEM_reactions.f90:374-390, per cell): nsub substeps of
+ * dt[c]/nsub each; mode = CHEM_TimeAdv (1 Euler, 2 RK2, 3 RK3, -1
+ * point-implicit Euler).  One cell per thread. */
+int32_t ocfd_chem_timeadv_batch(int64_t n, double* rho, double* E, const double* dt, int32_t mode, int32_t nsub);
+
+/* Adaptive substepping (no reference counterpart; the reference uses a fixed
+ * Chemical_sub_step).  Point-implicit Euler substeps (chemical_Euler_implicit)
+ * cover dt[c].  Each attempt evaluates the rates and Jacobian (negative
+ * partial densities counted as zero) and proposes
+ *   h = min(dt - t, h_cap, min_k s_k/|w_k|, s_T/|q|),
+ * with s_k = rtol_y (max(rho_k,0) + y_floor rho), s_T = rtol_t T rcv,
+ * rcv = sum_k 2.5 R_k rho_k, w_k the species mass rates and q = sum_k w_k R_k F_k
+ * (F_k the formation coefficient of update_Et).  The increment dd of the
+ * linear solve is accepted when |dd_k| <= 2 s_k and |sum_k dd_k R_k F_k| <= 2 s_T;
+ * then h_cap = 2h, else the attempt is discarded and h_cap = h/2.  A cell
+ * stops after max_sub accepted or max_sub rejected attempts; one that has
+ * not reached dt by then counts as unfinished (ocfd_chem_unfinished).
+ * nsub_out [n] (device, may be NULL) receives the accepted substeps per
+ * cell.  scheduler 0: one cell per thread; 1: a persistent kernel whose
+ * lanes take cells from a device work queue (same per-cell arithmetic and
+ * results, better balance when substep counts are uneven). */
+int32_t ocfd_chem_adaptive_batch(int64_t n, double* rho, double* E, const double* dt, double rtol_y, double y_floor,
+                                 double rtol_t, int32_t max_sub, int32_t* nsub_out, int32_t scheduler);
+
+/* Cells left unfinished by the last ocfd_chem_adaptive_batch (synchronises
+ * the chemistry stream); -1 on error. */
+int64_t ocfd_chem_unfinished(void);
+int32_t ocfd_chem_synchronize(void);
+void*   ocfd_chem_stream(void);
+/* Launch the chemistry kernels on this hipStream_t from now on (NULL is the
+ * null stream).  ocfd_chem_stream() returns the stream in use; the library's
+ * own stream is the one in use after ocfd_chem_set_mechanism. */
+int32_t ocfd_chem_set_stream(void* stream);
+const char* ocfd_chem_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -125,6 +125,11 @@ def load_library(path=None):
         "ocfd_kernel_stats": (i32, [i32, P(dbl), P(ctypes.c_int64)]), "ocfd_kernel_stats_reset": (i32, []),
         "ocfd_autotune": (i32, [i32]), "ocfd_set_tile": (i32, [i32, i32]),
         "ocfd_tile_choice": (i32, [i32, P(i32), P(i32), P(ctypes.c_float)]),
+        "ocfd_chem_set_mechanism": (i32, [i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, i32, vp, dbl, vp, vp]),
+        "ocfd_chem_timeadv_batch": (i32, [ctypes.c_int64, vp, vp, vp, i32, i32]),
+        "ocfd_chem_adaptive_batch": (i32, [ctypes.c_int64, vp, vp, vp, dbl, dbl, dbl, i32, vp, i32]),
+        "ocfd_chem_synchronize": (i32, []), "ocfd_chem_stream": (vp, []), "ocfd_chem_set_stream": (i32, [vp]),
+        "ocfd_chem_last_error": (ctypes.c_char_p, []), "ocfd_chem_unfinished": (ctypes.c_int64, []),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -207,3 +207,51 @@ class Mechanism:
 
     def index(self, name):
         return self.names.index(name)
+
+    # -- host thermo for setting up states (comput_E_rgas, CHEM_thermdyn.f90:203-231) --
+    def ect(self):
+        """Enthalpy-continuity offsets Ect[k][j] (read_Spec, CHEM_thermdyn.f90:43-47)."""
+        e = np.zeros((self.n_spec, self.n_tct))
+        c = self.coef
+        for j in range(self.n_tct - 1):
+            t = self.tct[j]
+            d = c[:, j + 1] - c[:, j]
+            e[:, j + 1] = -(d[:, 0] * t + d[:, 1] * (t * t) / 2.0 + d[:, 2] * ((t * t) * t) / 3.0
+                            + d[:, 3] * ((t * t) * (t * t)) / 4.0 + d[:, 4] * (((t * t) * (t * t)) * t) / 5.0) + e[:, j]
+        return e
+
+    def energy(self, rho, T):
+        """Internal energy per volume E (J/m3) of partial densities rho
+        [n_spec, n] at temperatures T [n]."""
+        rho = np.asarray(rho, dtype=np.float64)
+        T = np.asarray(T, dtype=np.float64)
+        jc = np.zeros(T.shape, dtype=int)
+        for j in range(self.n_tct - 1):
+            jc = np.where(T > self.tct[j], j + 1, jc)
+        R = 8.314 / self.mi
+        ect = self.ect()
+        A1, B1, C1, D1, E1, Ect = (np.zeros(T.shape) for _ in range(6))
+        for k in range(self.n_spec):
+            w = R[k] * rho[k]
+            cf = self.coef[k][jc]                               # [n, 7]
+            A1 = A1 + (cf[..., 0] - 1.0) * w
+            B1 = B1 + cf[..., 1] * w
+            C1 = C1 + cf[..., 2] * w
+            D1 = D1 + cf[..., 3] * w
+            E1 = E1 + cf[..., 4] * w
+            Ect = Ect + ect[k][jc] * w
+        tm = self.tct_max
+        Tc = np.minimum(T, tm)
+        e = A1 * Tc + B1 * Tc ** 2 / 2.0 + C1 * Tc ** 3 / 3.0 + D1 * Tc ** 4 / 4.0 + E1 * Tc ** 5 / 5.0 + Ect
+        cv_max = A1 + B1 * tm + C1 * tm ** 2 + D1 * tm ** 3 + E1 * tm ** 4
+        return np.where(T <= tm, e, e + cv_max * (T - tm))
+
+    def density(self, Y, T, p):
+        """Mixture density from mass fractions Y [n_spec, n], T and p (Pa)."""
+        return p / (T * np.sum(np.asarray(Y) * (8.314 / self.mi)[:, None], axis=0))
+
+    def mass_fractions(self, X):
+        """Mass fractions from mole fractions X [n_spec, ...]."""
+        X = np.asarray(X, dtype=np.float64)
+        w = X * self.mi.reshape((-1,) + (1,) * (X.ndim - 1))
+        return w / np.sum(w, axis=0)

@@ -189,8 +189,8 @@ class ChemOracle:
             "orc_chem_rate_coef": (i32, [vp, dbl, vp, vp, vp]), "orc_chem_rates": (i32, [vp, dbl, vp, vp]),
             "orc_gauss": (i32, [i32, vp, vp, vp]), "orc_chem_timeadv": (i32, [i32, vp, vp, dbl]),
             "orc_chem_batch": (i32, [i32, ctypes.c_long, vp, vp, vp, i32]),
-            "orc_chem_adaptive": (i32, [vp, vp, dbl, dbl, dbl, i32, vp]),
-            "orc_chem_adaptive_batch": (i32, [ctypes.c_long, vp, vp, vp, dbl, dbl, i32, vp]),
+            "orc_chem_adaptive": (i32, [vp, vp, dbl, dbl, dbl, dbl, i32, vp, vp]),
+            "orc_chem_adaptive_batch": (i32, [ctypes.c_long, vp, vp, vp, dbl, dbl, dbl, i32, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -255,12 +255,14 @@ class ChemOracle:
             raise RuntimeError("orc_chem_batch failed")
         return r, e
 
-    def adaptive_batch(self, rho, E, dt, frac_y, frac_t, max_sub):
+    def adaptive_batch(self, rho, E, dt, rtol_y, y_floor, rtol_t, max_sub, with_unfinished=False):
+        """rho [n][ns]; returns new (rho, E, nsub[, unfinished])."""
         r = np.ascontiguousarray(rho, dtype=np.float64).copy()
         e = np.ascontiguousarray(E, dtype=np.float64).copy()
         d = np.ascontiguousarray(dt, dtype=np.float64)
         ns = np.zeros(len(e), dtype=np.int32)
-        if self.lib.orc_chem_adaptive_batch(len(e), _ptr(r), _ptr(e), _ptr(d), float(frac_y), float(frac_t),
-                                            int(max_sub), _ptr(ns)):
+        un = np.zeros(len(e), dtype=np.int32)
+        if self.lib.orc_chem_adaptive_batch(len(e), _ptr(r), _ptr(e), _ptr(d), float(rtol_y), float(y_floor),
+                                            float(rtol_t), int(max_sub), _ptr(ns), _ptr(un)):
             raise RuntimeError("orc_chem_adaptive_batch failed")
-        return r, e, ns
+        return (r, e, ns, un) if with_unfinished else (r, e, ns)

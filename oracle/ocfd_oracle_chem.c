@@ -371,51 +371,92 @@ int orc_chem_batch(int mode, long n, double* rho, double* E, const double* dt, i
 /* ------------------------------------------------------------------------
  * Adaptive per-cell substepping (build addition, no reference counterpart).
  * The chemistry step dt of one cell is covered by point-implicit Euler
- * substeps h_k (the reference's integrator).  Before each substep the rates
- * are evaluated and the substep is limited so that no species' partial
- * density changes by more than frac_y of the cell density at the explicit
- * rate, and T by no more than frac_t of itself:
- *     h = min(remaining, frac_y * rho / max_i |w_i M_i 1e6|,
- *             frac_t * T * rho*cv / |sum_i w_i M_i 1e6 R_i F_i|)
- * with h >= h_min*dt; the number of substeps is returned per cell.  Validated
- * against the fixed-substep integrator at refined substep counts.
+ * substeps (the reference's integrator, chemical_Euler_implicit).  Each
+ * attempt evaluates T, the rates w_k = ri_k M_k 1e6 (kg/m3/s) and the
+ * Jacobian at the current state.  With the per-species scale
+ * s_k = rtol_y (max(rho_k, 0) + y_floor rho), the thermal scale
+ * s_T = rtol_t T rcv (rcv = sum_k 2.5 R_k rho_k) and the heat release
+ * q = sum_k w_k R_k F_k (F_k the formation coefficient of update_Et), it
+ * proposes
+ *     h = min(dt - t, h_cap, min_k s_k / |w_k|, s_T / |q|).
+ * The relative species bound limits h to about rtol_y / lambda while
+ * radicals grow like exp(lambda t) during ignition.  Negative partial
+ * densities (undershoot of a consumed species) enter the rates and the
+ * Jacobian as zero: with the reference's regularised Jacobian
+ * d(ri)/d(cj) ~ 1/(cj + 1e-20), the point-implicit update of a consumed
+ * species has a fixed point at cj = -1e-20 mol/cm3, where that denominator
+ * vanishes.  The linear solve gives
+ * dd; the attempt is accepted when
+ *     |dd_k| <= 2 s_k for every k  and  |sum_k dd_k R_k F_k| <= 2 s_T.
+ * A rejected attempt leaves the state unchanged and halves h_cap; an
+ * accepted one applies the update (dinew = di + dd, update_Et) and sets
+ * h_cap = 2h.  At most max_sub substeps are accepted and at most max_sub
+ * attempts rejected, so the loop ends; a cell that has not reached dt by
+ * then is "unfinished" (returned through *unfinished; its state is where
+ * the integration stopped).  nsub counts accepted substeps.
  * ------------------------------------------------------------------------ */
-int orc_chem_adaptive(double* di, double* E, double dt, double frac_y, double frac_t, int max_sub, int* nsub_out) {
-  int ns = C.ns, nsub = 0;
-  double t = 0.0;
-  while (t < dt && nsub < max_sub) {
-    double ci[NS_MAX], ri[NS_MAX];
+int orc_chem_adaptive(double* di, double* E, double dt, double rtol_y, double y_floor, double rtol_t, int max_sub,
+                      int* nsub_out, int* unfinished) {
+  int ns = C.ns, nsub = 0, nrej = 0;
+  double t = 0.0, h_cap = dt;
+  while (t < dt && nsub < max_sub && nrej < max_sub) {
+    double ci[NS_MAX], ri[NS_MAX], Ar[NS_MAX * NS_MAX], A1[NS_MAX * NS_MAX], rhs[NS_MAX], dd[NS_MAX], dinew[NS_MAX];
+    double sk[NS_MAX];
     double T = orc_chem_T(di, *E);
     double rho = 0.0;
-    for (int i = 0; i < ns; i++) { ci[i] = di[i] / C.mi[i] * 1.0e-6; rho = rho + di[i]; }
-    if (orc_chem_rates(ci, T, ri, NULL)) return 1;
-    double wmax = 0.0, qdot = 0.0;
+    for (int i = 0; i < ns; i++) {
+      ci[i] = (di[i] > 0.0 ? di[i] : 0.0) / C.mi[i] * 1.0e-6;
+      rho = rho + di[i];
+    }
+    if (orc_chem_rates(ci, T, ri, Ar)) return 1;
+    double h = dt - t;
+    if (h_cap < h) h = h_cap;
+    double q = 0.0, rcv = 0.0;
     for (int i = 0; i < ns; i++) {
       double w = ri[i] * C.mi[i] * 1.0e6;
-      if (fabs(w) > wmax) wmax = fabs(w);
-      qdot = qdot + w * C.ri[i] * C.coef[i][0][5];
+      sk[i] = rtol_y * ((di[i] > 0.0 ? di[i] : 0.0) + y_floor * rho);
+      if (fabs(w) * h > sk[i]) h = sk[i] / fabs(w);
+      q = q + w * C.ri[i] * C.coef[i][0][5];
+      rcv = rcv + 2.5 * C.ri[i] * di[i];
     }
-    double rcv = 0.0;
-    for (int i = 0; i < ns; i++) rcv = rcv + 2.5 * C.ri[i] * di[i];
-    double h = dt - t;
-    if (wmax > 0.0) { double hy = frac_y * rho / wmax; if (hy < h) h = hy; }
-    if (qdot != 0.0) { double ht = frac_t * T * rcv / fabs(qdot); if (ht < h) h = ht; }
-    double hmin = dt / (double)max_sub;
-    if (h < hmin) h = hmin;
-    if (t + h > dt) h = dt - t;
-    if (euler_implicit(di, E, h)) return 1;
+    double sT = rtol_t * T * rcv;
+    if (fabs(q) * h > sT) h = sT / fabs(q);
+    /* chemical_Euler_implicit's linear system with step h */
+    for (int j = 0; j < ns; j++)
+      for (int i = 0; i < ns; i++) A1[i * ns + j] = ((i == j) ? 1.0 : 0.0) - Ar[i * ns + j] * C.mi[i] / C.mi[j] * h;
+    for (int i = 0; i < ns; i++) rhs[i] = ri[i] * C.mi[i] * 1.0e6 * h;
+    orc_gauss(ns, A1, rhs, dd);
+    int ok = 1;
+    double de = 0.0;
+    for (int i = 0; i < ns; i++) {
+      ok = ok && fabs(dd[i]) <= 2.0 * sk[i];
+      de = de + dd[i] * C.ri[i] * C.coef[i][0][5];
+    }
+    ok = ok && fabs(de) <= 2.0 * sT;
+    if (!ok) {
+      h_cap = 0.5 * h;
+      nrej++;
+      continue;
+    }
+    for (int i = 0; i < ns; i++) dinew[i] = di[i] + dd[i];
+    update_et(di, dinew, E);
+    for (int i = 0; i < ns; i++) di[i] = dinew[i];
     t = t + h;
     nsub++;
+    h_cap = 2.0 * h;
     if (dt - t <= 1e-12 * dt) break;
   }
   if (nsub_out) *nsub_out = nsub;
+  if (unfinished) *unfinished = !(t >= dt || dt - t <= 1e-12 * dt);
   return 0;
 }
 
-int orc_chem_adaptive_batch(long n, double* rho, double* E, const double* dt, double frac_y, double frac_t, int max_sub,
-                            int* nsub) {
+int orc_chem_adaptive_batch(long n, double* rho, double* E, const double* dt, double rtol_y, double y_floor,
+                            double rtol_t, int max_sub, int* nsub, int* unfinished) {
   int bad = 0;
-#pragma omp parallel for schedule(dynamic, 64) reduction(| : bad)
-  for (long c = 0; c < n; c++) bad |= orc_chem_adaptive(rho + c * C.ns, E + c, dt[c], frac_y, frac_t, max_sub, nsub + c);
+#pragma omp parallel for schedule(dynamic, 16) reduction(| : bad)
+  for (long c = 0; c < n; c++)
+    bad |= orc_chem_adaptive(rho + c * C.ns, E + c, dt[c], rtol_y, y_floor, rtol_t, max_sub, nsub + c,
+                             unfinished ? unfinished + c : NULL);
   return bad;
 }

@@ -179,7 +179,7 @@ def test_adaptive_converges_to_refined_fixed_substeps(chem):
         d0, e0 = c.timeadv(-1, d0, e0, 1e-7)
     dt = 2e-5
     ref, eref = c.batch(-1, d0[None, :], np.array([e0]), np.array([dt]), 20000)
-    ada, eada, nsub = c.adaptive_batch(d0[None, :], np.array([e0]), np.array([dt]), 0.01, 0.002, 4000)
+    ada, eada, nsub = c.adaptive_batch(d0[None, :], np.array([e0]), np.array([dt]), 0.05, 1e-6, 0.01, 4000)
     coarse, ec = c.batch(-1, d0[None, :], np.array([e0]), np.array([dt]), 1)
     Tr, Ta, Tc = c.T(ref[0], eref[0]), c.T(ada[0], eada[0]), c.T(coarse[0], ec[0])
     assert nsub[0] > 10
