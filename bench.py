@@ -75,6 +75,122 @@ def cpu_baseline(m, nsteps=20):
                 sample=f"oracle (plain C, 1 thread) on the flat plate at 160x120x32, {nsteps} RK3 steps, {el:.1f} s")
 
 
+def chem_bench(m, ncell=1 << 21, cpu=True, surrogate_cells=1 << 24, surrogate_epochs=25):
+    """Finite-rate chemistry on the jet-flame batch: ncell independent cells
+    (jet fuel/oxidiser mixtures, T 900-2500 K, p 0.5-5 atm, taken along their
+    constant-volume ignition trajectories on the GPU), one chemistry step dt
+    ~ log-U[1e-8, 1e-6] s per cell.  Times the plain FP64 integrators (torch
+    events on the stream the kernels run on) and the CPU oracle on a sample."""
+    import numpy as np
+    import torch
+    from ocfd_amd import cases, chemistry, mechanism
+    M = mechanism.Mechanism()
+    ch = chemistry.ChemBatch(M)
+    s = cases.jet_chem_states(ncell, seed=3, mech=M)
+    dev = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda").contiguous()
+    rho, E, tau, dt = dev(s["rho"]), dev(s["E"]), dev(s["tau"]), dev(s["dt"])
+    ch.adaptive(rho, E, tau, rtol_y=0.05, y_floor=1e-6, rtol_t=0.01, max_sub=100000)
+    if ch.unfinished():
+        raise RuntimeError("chemistry batch set-up left cells unfinished")
+    r0, e0 = rho.clone(), E.clone()
+
+    def timed(fn, reps=3):
+        best, out = None, None
+        for _ in range(reps):
+            rho.copy_(r0)
+            E.copy_(e0)
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            out = fn()
+            b.record()
+            torch.cuda.synchronize()
+            ms = a.elapsed_time(b)
+            best = ms if best is None else min(best, ms)
+        return best, out
+
+    res = {}
+    ms, ns = timed(lambda: ch.adaptive(rho, E, dt, scheduler=2))
+    k = ns.cpu().numpy()
+    cnt = ch.counters()
+    res["plain_adaptive"] = dict(ms=ms, cells_per_s=ncell / ms * 1e3, unfinished=cnt["unfinished"],
+                                 scheduler="work queue + cooperative kernel for cells beyond 16 attempts",
+                                 cooperative_cells=cnt["cooperative"],
+                                 substeps=dict(mean=float(k.mean()), p50=float(np.median(k)),
+                                               p99=float(np.percentile(k, 99)), max=int(k.max())))
+    ms, _ = timed(lambda: ch.adaptive(rho, E, dt, scheduler=1))
+    res["plain_adaptive_queue_only"] = dict(ms=ms, cells_per_s=ncell / ms * 1e3)
+    ms, _ = timed(lambda: ch.adaptive(rho, E, dt, scheduler=0))
+    res["plain_adaptive_thread"] = dict(ms=ms, cells_per_s=ncell / ms * 1e3)
+    ms, _ = timed(lambda: ch.chemical_timeAdv(rho, E, dt, mode=1, nsub=5))
+    res["fixed_euler_M5"] = dict(ms=ms, cells_per_s=ncell / ms * 1e3)
+    ms, _ = timed(lambda: ch.chemical_timeAdv(rho, E, dt, mode=-1, nsub=5))
+    res["fixed_point_implicit_M5"] = dict(ms=ms, cells_per_s=ncell / ms * 1e3)
+    # plain reference result of the batch (judge of the surrogate's error)
+    rho.copy_(r0)
+    E.copy_(e0)
+    ch.adaptive(rho, E, dt, scheduler=2)
+    rp, Ep = rho.clone(), E.clone()
+    out = dict(workload=f"jet-flame chemistry batch, {ncell} cells, 9 species / 19 reactions, "
+                        "dt log-U[1e-8,1e-6] s", dtype="f64",
+               controller=dict(rtol_y=chemistry.RTOL_Y, y_floor=chemistry.Y_FLOOR, rtol_t=chemistry.RTOL_T),
+               **res)
+    if surrogate_cells:
+        from ocfd_amd import surrogate
+        t0 = time.perf_counter()
+        S = surrogate.Surrogate(ch, seed=0)
+        data = S.dataset(surrogate_cells, seed=1)
+        S.configure(data)
+        x, y = S.training_arrays(data)
+        del data
+        t1 = time.perf_counter()
+        hist = S.train(x, y, epochs=surrogate_epochs, batch=8192, lr=1e-3)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        del x, y
+        flags = torch.zeros(ncell, dtype=torch.int32, device="cuda")
+        ms_s, _ = timed(lambda: S.step(rho, E, dt, fallback=True, flags=flags))
+        nfb = S.fallbacks()
+        ms_nofb, _ = timed(lambda: S.step(rho, E, dt, fallback=False))
+        rho.copy_(r0)
+        E.copy_(e0)
+        S.step(rho, E, dt, fallback=True, flags=flags)
+        Tp = S.temperature(rp, Ep).cpu().numpy()
+        Ts = S.temperature(rho, E).cpu().numpy()
+        Yp = (rp / rp.sum(0)).cpu().numpy()
+        Ys = (rho / rho.sum(0)).cpu().numpy()
+        dT = np.abs(Ts - Tp)
+        major = [M.index(k) for k in ("H2", "O2", "H2O", "N2")]
+        dYm = np.abs(Ys[major] - Yp[major]).max(axis=0)
+        f = flags.cpu().numpy()
+        out["surrogate"] = dict(
+            network="12 -> 256 x5 -> 9 GELU, BF16 MFMA (FP32 accumulation), fused per-128-cell kernel",
+            train=dict(cells=surrogate_cells, epochs=len(hist), batch=8192, data_s=t1 - t0, train_s=t2 - t1,
+                       final_loss=hist[-1]),
+            ms=ms_s, cells_per_s=ncell / ms_s * 1e3, fallbacks=nfb,
+            fallback_reasons={k: int(np.sum((f & b) != 0)) for k, b in
+                              (("domain", 1), ("element_balance", 2), ("negative_Y", 4), ("T_bounds", 8))},
+            ms_without_fallback=ms_nofb,
+            speedup_vs_plain=res["plain_adaptive"]["ms"] / ms_s,
+            error_vs_plain=dict(dT_max_K=float(dT.max()), dT_p99_K=float(np.percentile(dT, 99)),
+                                dT_rel_max=float(np.max(dT / Tp)), peak_T_rel=float(abs(Ts.max() - Tp.max()) / Tp.max()),
+                                dY_major_max=float(dYm.max()), dY_major_p99=float(np.percentile(dYm, 99))))
+    if cpu:
+        import oracle
+        orc = oracle.ChemOracle(m, M)
+        threads = orc.set_threads(1)
+        nsample = 32768
+        r = rho.cpu().numpy()
+        rs, es, ds = r0[:, :nsample].cpu().numpy().T.copy(), e0[:nsample].cpu().numpy(), s["dt"][:nsample]
+        t0 = time.perf_counter()
+        orc.adaptive_batch(rs, es, ds, chemistry.RTOL_Y, chemistry.Y_FLOOR, chemistry.RTOL_T, 4000)
+        el = time.perf_counter() - t0
+        orc.set_threads(0)
+        out["cpu_baseline"] = dict(value=nsample / el, unit="cells/s", cores=threads, kind="port",
+                                   sample=f"first {nsample} cells of the same batch, adaptive point-implicit "
+                                          f"(oracle, C, {threads} thread), {el:.1f} s")
+    return out
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy), or None."""
@@ -101,6 +217,10 @@ def main():
     ap.add_argument("--nz", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--no-chem", action="store_true", help="skip the chemistry batch")
+    ap.add_argument("--chem-cells", type=int, default=1 << 21)
+    ap.add_argument("--surrogate-train-cells", type=int, default=1 << 24, help="0 skips the surrogate")
+    ap.add_argument("--surrogate-epochs", type=int, default=25)
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a plain 512 MiB device copy (3 times): the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
@@ -213,6 +333,9 @@ def main():
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(m)
     g.close()
+    chem = None if args.no_chem else chem_bench(m, args.chem_cells, cpu=not args.no_cpu_baseline,
+                                                 surrogate_cells=args.surrogate_train_cells,
+                                                 surrogate_epochs=args.surrogate_epochs)
     out = {
         "metric": METRIC, "value": value, "unit": "Mcell-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
@@ -225,6 +348,7 @@ def main():
                    "mask_fractions": {"x": frac[0], "y": frac[1], "z": frac[2]}},
         "roofline": roofline, "step_roofline": step_roofline, "cpu_baseline": cpu,
         "kernels_ms_per_step": {k: v[0] / max(1, min(args.steps, 20)) for k, v in kstats.items()},
+        "chem": chem,
     }
     print(json.dumps(out), flush=True)
 

@@ -255,11 +255,25 @@ int32_t ocfd_chem_timeadv_batch(int64_t n, double* rho, double* E, const double*
  * not reached dt by then counts as unfinished (ocfd_chem_unfinished).
  * nsub_out [n] (device, may be NULL) receives the accepted substeps per
  * cell.  scheduler 0: one cell per thread; 1: a persistent kernel whose
- * lanes take cells from a device work queue (same per-cell arithmetic and
- * results, better balance when substep counts are uneven). */
+ * lanes take cells from a device work queue; 2: the work queue, where a cell
+ * still running after a budget of attempts (OCFD_CHEM_OPT_COOP_BUDGET) moves
+ * to a cooperative kernel that
+ * integrates one cell per wavefront (lower latency for the few cells that
+ * need hundreds of substeps).  All three do the same per-cell arithmetic and
+ * give bit-identical results. */
 int32_t ocfd_chem_adaptive_batch(int64_t n, double* rho, double* E, const double* dt, double rtol_y, double y_floor,
                                  double rtol_t, int32_t max_sub, int32_t* nsub_out, int32_t scheduler);
 
+/* Chemistry options: OCFD_CHEM_OPT_COOP_BUDGET = attempts a cell may take on
+ * one lane before scheduler 2 moves it to the cooperative kernel (default
+ * 16); OCFD_CHEM_OPT_FALLBACK_SCHEDULER = scheduler (1 or 2) of the
+ * surrogate's fallback (default 1). */
+enum { OCFD_CHEM_OPT_COOP_BUDGET = 1, OCFD_CHEM_OPT_FALLBACK_SCHEDULER = 2 };
+int32_t ocfd_chem_set_option(int32_t key, int32_t value);
+/* The chemistry work counters of the last batch call (synchronises):
+ * [0] per-lane queue head, [1] unfinished cells, [2] surrogate fallbacks,
+ * [3] cells handed to the cooperative kernel, [4] its queue head. */
+int32_t ocfd_chem_counters(int64_t* out, int32_t count);
 /* Cells left unfinished by the last ocfd_chem_adaptive_batch (synchronises
  * the chemistry stream); -1 on error. */
 int64_t ocfd_chem_unfinished(void);
@@ -270,6 +284,63 @@ void*   ocfd_chem_stream(void);
  * own stream is the one in use after ocfd_chem_set_mechanism. */
 int32_t ocfd_chem_set_stream(void* stream);
 const char* ocfd_chem_last_error(void);
+
+/* ==== learned chemistry surrogate (no reference counterpart) ==============
+ * A multilayer perceptron held by the library (one network per process):
+ * dims[0] -> ... -> dims[n_layers], GELU (tanh form) between layers, BF16
+ * MFMA GEMMs with FP32 accumulation, FP32 master weights, Adam on the
+ * device.  Parameters travel in a flat float layout: per layer W [out][in]
+ * row-major, then b [out].  Batch arrays are device pointers, float32,
+ * row-major [B][width].  Calls run on the MLP stream (ocfd_mlp_set_stream). */
+int32_t ocfd_mlp_create(int32_t n_layers, const int32_t* dims);
+int32_t ocfd_mlp_destroy(void);
+int64_t ocfd_mlp_param_count(void);
+int32_t ocfd_mlp_set_params(const float* flat_host);
+int32_t ocfd_mlp_get_params(float* flat_host);
+int32_t ocfd_mlp_set_stream(void* stream);
+int32_t ocfd_mlp_synchronize(void);
+const char* ocfd_mlp_last_error(void);
+/* y [B][dout] = net(x [B][din]) */
+int32_t ocfd_mlp_forward(int64_t B, const float* x, float* y);
+/* mean squared error over the B x dout outputs and its gradient (flat
+ * layout, device; either output may be NULL) */
+int32_t ocfd_mlp_gradients(int64_t B, const float* x, const float* y, float* grad, float* loss);
+/* one bias-corrected Adam step on the batch; loss (device) may be NULL */
+int32_t ocfd_mlp_train_step(int64_t B, const float* x, const float* y, float lr, float beta1, float beta2, float eps,
+                            float* loss);
+/* C [M][N] (ldc) = op(A) op(B) with bfloat16 device operands, op = transpose
+ * when ta / tb; lda, ldb multiples of 8; split-K over `splits` chunks */
+int32_t ocfd_mlp_gemm_bf16(int32_t M, int32_t N, int32_t K, int32_t ta, int32_t tb, const void* A, int64_t lda,
+                           const void* B, int64_t ldb, float* C, int64_t ldc, int32_t splits);
+
+/* Surrogate configuration: 151 doubles, in this order:
+ *   in_shift[12], in_scale[12]  inputs x = (f - in_shift) * in_scale of the raw
+ *                               features f = [T, ln p, cbrt(Y_0..Y_8), ln dt];
+ *   out_scale[9]                dY_k = z_k * out_scale_k (z = network output);
+ *   T_lo, T_hi, p_lo, p_hi, dt_lo, dt_hi   domain; cells outside fall back;
+ *   y_neg_tol, elem_tol, T_out_lo, T_out_hi  acceptance bounds;
+ *   P[9][9]                     dY <- P dY (projection onto element balance);
+ *   Cel[3][9]                   element moles per kg of each species (H, O, N). */
+int32_t ocfd_chem_surrogate_set(const double* cfg, int32_t count);
+/* Surrogate step over a batch (layout of ocfd_chem_adaptive_batch): the
+ * fused MLP kernel updates every accepted cell; with fallback != 0 the cells
+ * it rejects are then integrated by the plain adaptive FP64 integrator
+ * (work-queue scheduler, controller arguments as ocfd_chem_adaptive_batch).
+ * flag_out [n] (device, may be NULL): 0 surrogate, else the failed checks
+ * (1 outside the training domain, 2 element balance, 4 negative mass
+ * fraction, 8 temperature bounds) and the cell went to the fallback.  The
+ * network must be 12 -> 256 x k -> 9. */
+int32_t ocfd_chem_surrogate_batch(int64_t n, double* rho, double* E, const double* dt, int32_t fallback,
+                                  double rtol_y, double y_floor, double rtol_t, int32_t max_sub, int32_t* flag_out);
+/* cells of the last ocfd_chem_surrogate_batch handed to the fallback
+ * (synchronises); -1 on error */
+int64_t ocfd_chem_surrogate_fallbacks(void);
+/* per-cell T [n] (mode 0, f64), raw features [n][12] (mode 1, f64) or
+ * normalised surrogate inputs [n][12] (mode 2, f32) of a batch */
+int32_t ocfd_chem_cell_features(int64_t n, const double* rho, const double* E, const double* dt, int32_t mode,
+                                void* out);
+/* normalised training targets y [n][9] = ((rho1_k - rho0_k)/rho0) / out_scale_k */
+int32_t ocfd_chem_surrogate_targets(int64_t n, const double* rho0, const double* rho1, float* y);
 
 #ifdef __cplusplus
 }

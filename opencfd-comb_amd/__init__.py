@@ -130,6 +130,21 @@ def load_library(path=None):
         "ocfd_chem_adaptive_batch": (i32, [ctypes.c_int64, vp, vp, vp, dbl, dbl, dbl, i32, vp, i32]),
         "ocfd_chem_synchronize": (i32, []), "ocfd_chem_stream": (vp, []), "ocfd_chem_set_stream": (i32, [vp]),
         "ocfd_chem_last_error": (ctypes.c_char_p, []), "ocfd_chem_unfinished": (ctypes.c_int64, []),
+        "ocfd_chem_counters": (i32, [vp, i32]), "ocfd_chem_set_option": (i32, [i32, i32]),
+        "ocfd_mlp_create": (i32, [i32, vp]), "ocfd_mlp_destroy": (i32, []), "ocfd_mlp_param_count": (ctypes.c_int64, []),
+        "ocfd_mlp_set_params": (i32, [vp]), "ocfd_mlp_get_params": (i32, [vp]), "ocfd_mlp_set_stream": (i32, [vp]),
+        "ocfd_mlp_forward": (i32, [ctypes.c_int64, vp, vp]),
+        "ocfd_mlp_gradients": (i32, [ctypes.c_int64, vp, vp, vp, vp]),
+        "ocfd_mlp_train_step": (i32, [ctypes.c_int64, vp, vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_float, vp]),
+        "ocfd_mlp_synchronize": (i32, []), "ocfd_mlp_last_error": (ctypes.c_char_p, []),
+        "ocfd_mlp_gemm_bf16": (i32, [i32, i32, i32, i32, i32, vp, ctypes.c_int64, vp, ctypes.c_int64, vp,
+                                     ctypes.c_int64, i32]),
+        "ocfd_chem_surrogate_set": (i32, [vp, i32]),
+        "ocfd_chem_surrogate_batch": (i32, [ctypes.c_int64, vp, vp, vp, i32, dbl, dbl, dbl, i32, vp]),
+        "ocfd_chem_surrogate_fallbacks": (ctypes.c_int64, []),
+        "ocfd_chem_cell_features": (i32, [ctypes.c_int64, vp, vp, vp, i32, vp]),
+        "ocfd_chem_surrogate_targets": (i32, [ctypes.c_int64, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libocfd_hip.so")
-SOURCES = [os.path.join(HERE, "csrc", "ocfd_flow.hip"), os.path.join(HERE, "csrc", "ocfd_chem.hip")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("ocfd_flow.hip", "ocfd_chem.hip", "ocfd_mlp.hip")]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -17,6 +17,7 @@ from . import load_library, OcfdError
 from .mechanism import Mechanism
 
 MODES = {1: "Euler", 2: "RK2", 3: "RK3", -1: "point-implicit Euler"}
+OPT_COOP_BUDGET, OPT_FALLBACK_SCHEDULER = 1, 2
 
 # adaptive controller defaults: on jet-mixture cells along their ignition
 # trajectories, one step of dt in [1e-8, 1e-6] s lands within 0.25 K and 1e-3
@@ -74,9 +75,10 @@ class ChemBatch:
     def adaptive(self, rho, E, dt, rtol_y=RTOL_Y, y_floor=Y_FLOOR, rtol_t=RTOL_T, max_sub=4000, scheduler=1,
                  nsub_out=None):
         """Point-implicit Euler with per-cell adaptive substeps (controller
-        documented at ocfd_chem_adaptive_batch).  scheduler 1 = persistent
-        work-queue kernel, 0 = one cell per thread.  Returns the int32 tensor
-        of accepted substeps per cell."""
+        documented at ocfd_chem_adaptive_batch).  scheduler 1 = work queue,
+        2 = work queue with the cooperative kernel for long-running cells,
+        0 = one cell per thread (bit-identical results).  Returns the int32
+        tensor of accepted substeps per cell."""
         import torch
         n = self._args(rho, E, dt)
         if nsub_out is None:
@@ -88,6 +90,16 @@ class ChemBatch:
                                                       float(y_floor), float(rtol_t), int(max_sub),
                                                       nsub_out.data_ptr(), int(scheduler)))
         return nsub_out
+
+    def set_option(self, key, value):
+        """OPT_COOP_BUDGET (1) or OPT_FALLBACK_SCHEDULER (2)."""
+        self._check(self.lib.ocfd_chem_set_option(int(key), int(value)))
+
+    def counters(self):
+        """Work counters of the last call: dict(unfinished, fallbacks, cooperative)."""
+        out = (ctypes.c_int64 * 5)()
+        self._check(self.lib.ocfd_chem_counters(out, 5))
+        return dict(unfinished=int(out[1]), fallbacks=int(out[2]), cooperative=int(out[3]))
 
     def unfinished(self):
         """Cells the last ``adaptive`` call left short of dt (synchronises)."""

@@ -24,6 +24,8 @@
 #include <string>
 
 #include "../../include/ocfd_hip.h"
+#include "ocfd_mlp.h"
+#include "ocfd_mlp_tile.h"
 
 #define CH_NS 9
 #define CH_NR_MAX 32
@@ -54,12 +56,28 @@ __device__ __forceinline__ Mech mech_opaque(Mech m) {
   return m;
 }
 
+// Integration state of a cell handed from the per-lane queue to the
+// cooperative kernel (struct of arrays, capacity cap records).
+struct HeavyBuf {
+  double* di;  // [CH_NS][cap]
+  double *E, *t, *h_cap;
+  int *nsub, *nrej, *cell;
+  long cap;
+};
+
 namespace {
 struct ChemCtx {
   bool ready = false;
   ChemMech h{};
   hipStream_t stream = nullptr, own = nullptr;
-  unsigned long long* counter = nullptr;
+  unsigned long long* counter = nullptr;  // [0] queue head, [1] unfinished, [2] surrogate fallbacks
+  int* fb_list = nullptr;                  // cells handed to the fallback integrator
+  long fb_cap = 0;
+  HeavyBuf heavy{};                        // cells handed to the cooperative kernel
+  int fallback_scheduler = 1;              // scheduler of the surrogate's fallback
+  int coop_budget = 16;                    // attempts on one lane before the cooperative kernel
+  unsigned char* heavy_mem = nullptr;
+  bool surr_ready = false;
   std::string err;
 };
 ChemCtx CX;
@@ -153,9 +171,101 @@ __device__ __forceinline__ double chem_gibbs(Mech m, int k, double T, double log
 
 // Reaction_rate_coef (CHEM_reactions.f90:130-197) + Reaction_rate_and_Matrix
 // (:230-272).  ri in mol/cm3/s; A[i][j] = d ri / d cj if JAC.
+// Constants of one reaction (read_Reac, CHEM_reactions.f90:15-71).
+struct RxnConst {
+  int af_type, tb, sgm;
+  double fc, a0, b0, e0, a1, b1, e1;
+  double eff[CH_NS];
+  int vf[CH_NS], vr[CH_NS];
+};
+
+__device__ __forceinline__ RxnConst load_rxn(Mech m, int k) {
+  RxnConst R;
+  R.af_type = m->af_type[k];
+  R.tb = m->tb[k];
+  R.sgm = m->sgm[k];
+  R.fc = m->fc[k];
+  R.a0 = m->arr[k][0][0];
+  R.b0 = m->arr[k][0][1];
+  R.e0 = m->arr[k][0][2];
+  R.a1 = m->arr[k][1][0];
+  R.b1 = m->arr[k][1][1];
+  R.e1 = m->arr[k][1][2];
+#pragma unroll
+  for (int i = 0; i < CH_NS; i++) {
+    R.eff[i] = m->eff[k][i];
+    R.vf[i] = m->vf[k][i];
+    R.vr[i] = m->vr[k][i];
+  }
+  return R;
+}
+
+// Rate of one reaction with concentrations: Reaction_rate_coef
+// (CHEM_reactions.f90:130-197) and the mass-action products of
+// Reaction_rate_and_Matrix (:246-250).  gi = G_i/R_i of every species.
+__device__ __forceinline__ void chem_reaction_c(const RxnConst& R, double T, const double* gi, const double* ci,
+                                                double& wf1, double& wr1) {
+  const double Rs = CH_R0 / CH_ATM * 1.0e6;
+  // comput_KjX (CHEM_reactions.f90:76-103)
+  double dgi = 0.0;
+#pragma unroll
+  for (int i = 0; i < CH_NS; i++) dgi = dgi + gi[i] * (R.vr[i] - R.vf[i]);
+  const double kjx = exp(-dgi / T) * ipow(Rs * T, -R.sgm);
+  double Ctrb = 0.0;
+  if (R.tb == 1) {
+#pragma unroll
+    for (int i = 0; i < CH_NS; i++) Ctrb = Ctrb + ci[i] * R.eff[i];
+  }
+  const double wf1a = R.a0 * pow(T, R.b0) * exp(-R.e0 / T);
+  double wf, wr;
+  if (R.af_type == 0) {
+    wf = wf1a;
+  } else if (R.af_type == 2) {
+    const double wf2 = R.a1 * pow(T, R.b1) * exp(-R.e1 / T);
+    wf = wf1a + wf2;
+  } else {
+    const double wf2 = R.a1 * pow(T, R.b1) * exp(-R.e1 / T);
+    const double Prs = wf1a / wf2 * Ctrb;
+    const double Fc = R.fc;
+    const double lfc = log10(Fc);
+    const double ac = -0.4 - (double)0.67f * lfc;   // REAL(4) literals of the reference
+    const double an = 0.75 - (double)1.27f * lfc;
+    const double lp = log10(Prs);
+    const double F = pow(10.0, lfc / (1.0 + q2((lp + ac) / (an - 0.14 * (lp + ac)))));
+    wf = wf2 * Prs / (1.0 + Prs) * F;
+  }
+  wr = wf / kjx;
+  if (R.tb == 1 && R.af_type != 1) {
+    wf = wf * Ctrb;
+    wr = wr * Ctrb;
+  }
+  wf1 = wf;
+  wr1 = wr;
+#pragma unroll
+  for (int i = 0; i < CH_NS; i++) {
+    if (R.vf[i] != 0) wf1 = wf1 * ipow(ci[i], R.vf[i]);
+    if (R.vr[i] != 0) wr1 = wr1 * ipow(ci[i], R.vr[i]);
+  }
+}
+
+__device__ __forceinline__ void chem_reaction(Mech m, int k, double T, const double* gi, const double* ci, double& wf1,
+                                              double& wr1) {
+  chem_reaction_c(load_rxn(m, k), T, gi, ci, wf1, wr1);
+}
+
+// d(ri)/d(cj) contribution of reaction k (CHEM_reactions.f90:262-268)
+__device__ __forceinline__ double chem_dA(Mech m, int k, int j, double wf1, double wr1, const double* ci) {
+  const double Awf = m->vf[k][j] * wf1 / (ci[j] + 1.0e-20);
+  const double Awr = m->vr[k][j] * wr1 / (ci[j] + 1.0e-20);
+  return Awf - Awr;
+}
+
+// Reaction_rate_coef (CHEM_reactions.f90:130-197) + Reaction_rate_and_Matrix
+// (:230-272).  ri in mol/cm3/s; A[i][j] = d ri / d cj if JAC.  k is uniform
+// across the wavefront, so the stoichiometry tests are scalar branches and
+// the fully unrolled i/j loops keep ri and A in registers.
 template <bool JAC>
 __device__ void chem_rates(Mech m, const double* ci, double T, double* ri, double (*A)[CH_NS]) {
-  const double Rs = CH_R0 / CH_ATM * 1.0e6;
   const double logT = log(T);
   double gi[CH_NS];
 #pragma unroll
@@ -169,48 +279,8 @@ __device__ void chem_rates(Mech m, const double* ci, double T, double* ri, doubl
     }
   }
   for (int k = 0; k < m->nr; k++) {
-    // comput_KjX (CHEM_reactions.f90:76-103)
-    double dgi = 0.0;
-#pragma unroll
-    for (int i = 0; i < CH_NS; i++) dgi = dgi + gi[i] * (m->vr[k][i] - m->vf[k][i]);
-    const double kjx = exp(-dgi / T) * ipow(Rs * T, -m->sgm[k]);
-    double Ctrb = 0.0;
-    if (m->tb[k] == 1) {
-#pragma unroll
-      for (int i = 0; i < CH_NS; i++) Ctrb = Ctrb + ci[i] * m->eff[k][i];
-    }
-    const double wf1a = m->arr[k][0][0] * pow(T, m->arr[k][0][1]) * exp(-m->arr[k][0][2] / T);
-    double wf, wr;
-    if (m->af_type[k] == 0) {
-      wf = wf1a;
-    } else if (m->af_type[k] == 2) {
-      const double wf2 = m->arr[k][1][0] * pow(T, m->arr[k][1][1]) * exp(-m->arr[k][1][2] / T);
-      wf = wf1a + wf2;
-    } else {
-      const double wf2 = m->arr[k][1][0] * pow(T, m->arr[k][1][1]) * exp(-m->arr[k][1][2] / T);
-      const double Prs = wf1a / wf2 * Ctrb;
-      const double Fc = m->fc[k];
-      const double lfc = log10(Fc);
-      const double ac = -0.4 - (double)0.67f * lfc;   // REAL(4) literals of the reference
-      const double an = 0.75 - (double)1.27f * lfc;
-      const double lp = log10(Prs);
-      const double F = pow(10.0, lfc / (1.0 + q2((lp + ac) / (an - 0.14 * (lp + ac)))));
-      wf = wf2 * Prs / (1.0 + Prs) * F;
-    }
-    wr = wf / kjx;
-    if (m->tb[k] == 1 && m->af_type[k] != 1) {
-      wf = wf * Ctrb;
-      wr = wr * Ctrb;
-    }
-    // mass action (CHEM_reactions.f90:246-250).  k is uniform across the
-    // wavefront, so the stoichiometry tests below are scalar branches and
-    // the fully unrolled i/j loops keep ri and A in registers.
-    double wf1 = wf, wr1 = wr;
-#pragma unroll
-    for (int i = 0; i < CH_NS; i++) {
-      if (m->vf[k][i] != 0) wf1 = wf1 * ipow(ci[i], m->vf[k][i]);
-      if (m->vr[k][i] != 0) wr1 = wr1 * ipow(ci[i], m->vr[k][i]);
-    }
+    double wf1, wr1;
+    chem_reaction(m, k, T, gi, ci, wf1, wr1);
 #pragma unroll
     for (int i = 0; i < CH_NS; i++) {
       const int nu = m->vr[k][i] - m->vf[k][i];
@@ -220,12 +290,11 @@ __device__ void chem_rates(Mech m, const double* ci, double T, double* ri, doubl
 #pragma unroll
       for (int j = 0; j < CH_NS; j++) {
         if (m->vf[k][j] == 0 && m->vr[k][j] == 0) continue;
-        const double Awf = m->vf[k][j] * wf1 / (ci[j] + 1.0e-20);
-        const double Awr = m->vr[k][j] * wr1 / (ci[j] + 1.0e-20);
+        const double dA = chem_dA(m, k, j, wf1, wr1, ci);
 #pragma unroll
         for (int i = 0; i < CH_NS; i++) {
           const int nu = m->vr[k][i] - m->vf[k][i];
-          if (nu != 0) A[i][j] = A[i][j] + (Awf - Awr) * nu;
+          if (nu != 0) A[i][j] = A[i][j] + dA * nu;
         }
       }
     }
@@ -488,8 +557,18 @@ __global__ __launch_bounds__(128) void k_chem_adaptive(long n, double* rho, doub
 // with the same per-cell loop as k_chem_adaptive.  A cell ends after at most
 // 2 max_sub attempts, and a lane leaves once the queue is empty and its cell
 // is done, so every wave reaches the exit.
+// With a list, the queue runs over the cells list[0 .. *list_count) instead
+// of 0 .. n (n stays the array stride).
+// With budget > 0, a cell still running after `budget` attempts is handed
+// to the cooperative kernel (k_chem_coop) with its full controller state, so
+// the few cells that need hundreds of substeps do not hold a lane for the
+// whole kernel.
 __global__ __launch_bounds__(128) void k_chem_queue(long n, double* rho, double* Ecell, const double* dtcell,
-                                                     ChemCtl ctl, int* nsub_out, unsigned long long* queue) {
+                                                     ChemCtl ctl, int* nsub_out, unsigned long long* queue,
+                                                     const int* list, const unsigned long long* list_count, int budget,
+                                                     HeavyBuf hb) {
+  const unsigned long long count =
+      list ? (*list_count < (unsigned long long)n ? *list_count : (unsigned long long)n) : (unsigned long long)n;
   const int lane = threadIdx.x & 63;
   const unsigned long long lt = (1ull << lane) - 1ull;
   long cell = -1;  // -1: needs work, -2: queue exhausted
@@ -508,8 +587,8 @@ __global__ __launch_bounds__(128) void k_chem_queue(long n, double* rho, double*
       base = __shfl(base, leader);
       if (need) {
         const unsigned long long my = base + (unsigned long long)__popcll(mask & lt);
-        if (my < (unsigned long long)n) {
-          cell = (long)my;
+        if (my < count) {
+          cell = list ? (long)list[my] : (long)my;
 #pragma unroll
           for (int i = 0; i < CH_NS; i++) di[i] = rho[(long)i * n + cell];
           E = Ecell[cell];
@@ -540,13 +619,465 @@ __global__ __launch_bounds__(128) void k_chem_queue(long n, double* rho, double*
       if (done) {
         chem_finish(cell, n, rho, Ecell, di, E, nsub, t >= dt || dt - t <= 1e-12 * dt, nsub_out, queue);
         cell = -1;
+      } else if (budget > 0 && nsub + nrej >= budget) {
+        const unsigned long long r = atomicAdd(queue + 3, 1ull);
+#pragma unroll
+        for (int i = 0; i < CH_NS; i++) hb.di[(long)i * hb.cap + r] = di[i];
+        hb.E[r] = E;
+        hb.t[r] = t;
+        hb.h_cap[r] = h_cap;
+        hb.nsub[r] = nsub;
+        hb.nrej[r] = nrej;
+        hb.cell[r] = (int)cell;
+        cell = -1;
       }
     }
   }
 }
 
+// Cooperative integration of one cell per wavefront (64-thread workgroup).
+// Every lane holds the same copy of the cell state; the parallel parts (the
+// species' Gibbs functions, the reactions, ri, the Jacobian entries and the
+// Gauss elimination) exchange values through LDS, and every value is computed
+// with the same operations in the same order as chem_adaptive_attempt, so
+// the results are bit-identical to the per-lane integrator.
+struct CoopLds {
+  double gi[CH_NS];
+  double wf1[CH_NR_MAX], wr1[CH_NR_MAX];
+  double ri[CH_NS];
+  double A[CH_NS][CH_NS + 1];  // Jacobian, then the system; column CH_NS = right-hand side
+  double f[CH_NS];
+  // mechanism tables read with lane-dependent indices (staged once)
+  double coef[CH_NS][CH_NT_MAX][7];
+  double mi[CH_NS];
+  signed char vf[CH_NR_MAX][CH_NS], vr[CH_NR_MAX][CH_NS];
+};
+
+// comput_gibbs (CHEM_thermdyn.f90:296-319) with the coefficients from LDS
+__device__ __forceinline__ double coop_gibbs(Mech m, const double (*coefk)[7], double T, double logT) {
+  const int jc = trange(m, T);
+  const double* c = coefk[jc];
+  if (T <= m->tct_max)
+    return c[0] * (1.0 - logT) * T - c[1] * q2(T) / 2.0 - c[2] * q3(T) / 6.0 - c[3] * q4(T) / 12.0 - c[4] * q5(T) / 20.0 +
+           c[5] - c[6] * T;
+  const double tm = m->tct_max;
+  const double Cp_max = c[0] + c[1] * tm + c[2] * q2(tm) + c[3] * q3(tm) + c[4] * q4(tm);
+  const double h_max = c[0] * tm + c[1] * q2(tm) / 2.0 + c[2] * q3(tm) / 3.0 + c[3] * q4(tm) / 4.0 + c[4] * q5(tm) / 5.0 + c[5];
+  const double S_max = c[0] * log(tm) + c[1] * tm + c[2] * q2(tm) / 2.0 + c[3] * q3(tm) / 3.0 + c[4] * q4(tm) / 4.0 + c[6];
+  return h_max + Cp_max * (T - tm) - T * (S_max + Cp_max * log(T / tm));
+}
+
+__device__ bool coop_attempt(Mech m, CoopLds& L, const RxnConst& R, int lane, double* di, double& E, double dt,
+                             double t, double& h_cap, const ChemCtl& ctl, double& h_taken) {
+  double ci[CH_NS], sk[CH_NS], dd[CH_NS];
+  __syncthreads();
+  const double T = chem_T(m, di, E);
+  double rho = 0.0;
+#pragma unroll
+  for (int i = 0; i < CH_NS; i++) {
+    ci[i] = (di[i] > 0.0 ? di[i] : 0.0) / m->mi[i] * 1.0e-6;
+    rho = rho + di[i];
+  }
+  if (lane < CH_NS) L.gi[lane] = coop_gibbs(m, L.coef[lane], T, log(T));
+  __syncthreads();
+  const int nr = m->nr;
+  if (lane < nr) {
+    double gi[CH_NS];
+#pragma unroll
+    for (int i = 0; i < CH_NS; i++) gi[i] = L.gi[i];
+    double wf1, wr1;
+    chem_reaction_c(R, T, gi, ci, wf1, wr1);
+    L.wf1[lane] = wf1;
+    L.wr1[lane] = wr1;
+  }
+  __syncthreads();
+  if (lane < CH_NS) {
+    double r = 0.0;
+    for (int k = 0; k < nr; k++) {
+      const int nu = L.vr[k][lane] - L.vf[k][lane];
+      if (nu != 0) r = r + (L.wf1[k] - L.wr1[k]) * nu;
+    }
+    L.ri[lane] = r;
+  }
+  for (int e = lane; e < CH_NS * CH_NS; e += 64) {
+    const int i = e / CH_NS, j = e % CH_NS;
+    double a = 0.0;
+    double cjv = ci[0];  // ci[j] for the lane-dependent j, without dynamic register indexing
+#pragma unroll
+    for (int q = 1; q < CH_NS; q++)
+      if (j == q) cjv = ci[q];
+    for (int k = 0; k < nr; k++) {
+      const int vfj = L.vf[k][j], vrj = L.vr[k][j];
+      if (vfj == 0 && vrj == 0) continue;
+      const int nu = L.vr[k][i] - L.vf[k][i];
+      if (nu != 0) {
+        const double Awf = vfj * L.wf1[k] / (cjv + 1.0e-20);
+        const double Awr = vrj * L.wr1[k] / (cjv + 1.0e-20);
+        a = a + (Awf - Awr) * nu;
+      }
+    }
+    L.A[i][j] = a;
+  }
+  __syncthreads();
+  double h = dt - t;
+  if (h_cap < h) h = h_cap;
+  double q = 0.0, rcv = 0.0;
+#pragma unroll
+  for (int i = 0; i < CH_NS; i++) {
+    const double w = L.ri[i] * m->mi[i] * 1.0e6;
+    sk[i] = ctl.rtol_y * ((di[i] > 0.0 ? di[i] : 0.0) + ctl.y_floor * rho);
+    if (fabs(w) * h > sk[i]) h = sk[i] / fabs(w);
+    q = q + w * m->ri[i] * m->coef[i][0][5];
+    rcv = rcv + 2.5 * m->ri[i] * di[i];
+  }
+  const double sT = ctl.rtol_t * T * rcv;
+  if (fabs(q) * h > sT) h = sT / fabs(q);
+  for (int e = lane; e < CH_NS * CH_NS; e += 64) {
+    const int i = e / CH_NS, j = e % CH_NS;
+    L.A[i][j] = (i == j ? 1.0 : 0.0) - L.A[i][j] * L.mi[i] / L.mi[j] * h;
+  }
+  if (lane < CH_NS) L.A[lane][CH_NS] = L.ri[lane] * L.mi[lane] * 1.0e6 * h;
+  __syncthreads();
+  // Gauss elimination with partial pivoting (CHEM_reactions.f90:523-567)
+  for (int k = 0; k < CH_NS - 1; k++) {
+    double fmax = fabs(L.A[k][k]);
+    int kk = k;
+    for (int i = k + 1; i < CH_NS; i++)
+      if (fabs(L.A[i][k]) > fmax) {
+        fmax = fabs(L.A[i][k]);
+        kk = i;
+      }
+    if (kk != k && lane >= k && lane <= CH_NS) {
+      const double tmp = L.A[k][lane];
+      L.A[k][lane] = L.A[kk][lane];
+      L.A[kk][lane] = tmp;
+    }
+    __syncthreads();
+    if (lane > k && lane < CH_NS) L.f[lane] = -L.A[lane][k] / L.A[k][k];
+    __syncthreads();
+    const int w = CH_NS + 1 - k;  // columns k .. CH_NS (right-hand side included)
+    for (int e = lane; e < (CH_NS - 1 - k) * w; e += 64) {
+      const int i = k + 1 + e / w, j = k + e % w;
+      L.A[i][j] = L.A[i][j] + L.f[i] * L.A[k][j];
+    }
+    __syncthreads();
+  }
+  dd[CH_NS - 1] = L.A[CH_NS - 1][CH_NS] / L.A[CH_NS - 1][CH_NS - 1];
+#pragma unroll
+  for (int i = CH_NS - 2; i >= 0; i--) {
+    double xi = L.A[i][CH_NS];
+#pragma unroll
+    for (int j = i + 1; j < CH_NS; j++) xi = xi - L.A[i][j] * dd[j];
+    dd[i] = xi / L.A[i][i];
+  }
+  bool ok = true;
+  double de = 0.0;
+#pragma unroll
+  for (int i = 0; i < CH_NS; i++) {
+    ok = ok && fabs(dd[i]) <= 2.0 * sk[i];
+    de = de + dd[i] * m->ri[i] * m->coef[i][0][5];
+  }
+  ok = ok && fabs(de) <= 2.0 * sT;
+  if (!ok) {
+    h_cap = 0.5 * h;
+    return false;
+  }
+  chem_apply(m, di, E, dd);
+  h_cap = 2.0 * h;
+  h_taken = h;
+  return true;
+}
+
+// One wavefront per handed-off cell, taken from a queue over the records.
+__global__ __launch_bounds__(64) void k_chem_coop(long n, double* rho, double* Ecell, const double* dtcell, ChemCtl ctl,
+                                                  int* nsub_out, unsigned long long* queue, HeavyBuf hb) {
+  __shared__ CoopLds L;
+  const int lane = threadIdx.x;
+  Mech m = (Mech)&cm;
+  const unsigned long long count = queue[3];
+  // stage the lane-indexed mechanism data: this lane's reaction in
+  // registers, the stoichiometry, NASA coefficients and molar masses in LDS
+  const int nr = m->nr;
+  RxnConst R = load_rxn(m, lane < nr ? lane : 0);
+  for (int e = lane; e < CH_NR_MAX * CH_NS; e += 64) {
+    const int k = e / CH_NS, i = e % CH_NS;
+    L.vf[k][i] = (signed char)(k < nr ? m->vf[k][i] : 0);
+    L.vr[k][i] = (signed char)(k < nr ? m->vr[k][i] : 0);
+  }
+  for (int e = lane; e < CH_NS * CH_NT_MAX * 7; e += 64) {
+    const int i = e / (CH_NT_MAX * 7), j = (e / 7) % CH_NT_MAX, c = e % 7;
+    L.coef[i][j][c] = m->coef[i][j][c];
+  }
+  if (lane < CH_NS) L.mi[lane] = m->mi[lane];
+  __syncthreads();
+  while (true) {
+    unsigned long long r = 0;
+    if (lane == 0) r = atomicAdd(queue + 4, 1ull);
+    r = __shfl(r, 0);
+    if (r >= count) break;
+    const long c = hb.cell[r];
+    double di[CH_NS];
+#pragma unroll
+    for (int i = 0; i < CH_NS; i++) di[i] = hb.di[(long)i * hb.cap + r];
+    double E = hb.E[r], t = hb.t[r], h_cap = hb.h_cap[r];
+    int nsub = hb.nsub[r], nrej = hb.nrej[r];
+    const double dt = dtcell[c];
+    // every lane holds the same state: readfirstlane states that the loop
+    // decisions are uniform, which keeps the control flow scalar
+    while (__builtin_amdgcn_readfirstlane((t < dt && nsub < ctl.max_sub && nrej < ctl.max_sub) ? 1 : 0)) {
+      m = mech_opaque(m);
+      double h = 0.0;
+      const bool acc = coop_attempt(m, L, R, lane, di, E, dt, t, h_cap, ctl, h);
+      if (!__builtin_amdgcn_readfirstlane(acc ? 1 : 0)) {
+        nrej++;
+        continue;
+      }
+      t = t + h;
+      nsub++;
+      if (__builtin_amdgcn_readfirstlane((dt - t <= 1e-12 * dt) ? 1 : 0)) break;
+    }
+    if (lane == 0) chem_finish(c, n, rho, Ecell, di, E, nsub, t >= dt || dt - t <= 1e-12 * dt, nsub_out, queue);
+  }
+}
+
 __global__ void k_chem_reset(unsigned long long* queue) {
-  if (threadIdx.x < 2 && blockIdx.x == 0) queue[threadIdx.x] = 0;
+  if (threadIdx.x < 8 && blockIdx.x == 0) queue[threadIdx.x] = 0;
+}
+
+// ===========================================================================
+// Learned surrogate (no reference counterpart).  Per cell: T from E by the
+// reference's Newton iteration, features
+//   f = [T, ln p, cbrt(Y_0..Y_8), ln dt],   x = (f - in_shift) * in_scale,
+// the MLP (BF16 MFMA, FP32 accumulation) gives z, and in FP64
+//   dY = P (z * out_scale)          (P projects onto element conservation)
+// is accepted when the cell lies in the training domain, the H/O/N balance of
+// dY is within elem_tol of the element content, Y + dY >= -y_neg_tol and the
+// temperature after the update (Newton on the new state, update_Et energy) is
+// finite and within [T_out_lo, T_out_hi].  Accepted cells are written
+// (rho_k += rho dY_k, E by update_Et); the others go to a list for the plain
+// FP64 integrator.
+struct SurrCfg {
+  double in_shift[12], in_scale[12];
+  double out_scale[CH_NS];
+  double T_lo, T_hi, p_lo, p_hi, dt_lo, dt_hi;
+  double y_neg_tol, elem_tol, T_out_lo, T_out_hi;
+  double P[CH_NS][CH_NS];
+  double Cel[3][CH_NS];
+};
+__constant__ SurrCfg scfg;
+__constant__ MlpDevice cnet;
+
+// raw surrogate features of one cell: [T, ln p, cbrt(Y_0..Y_8), ln dt]
+__device__ __forceinline__ void surr_raw_features(Mech m, const double* di, double r, double E, double dt, double* f,
+                                                  double& T, double& p) {
+  T = chem_T(m, di, E);
+  p = 0.0;
+#pragma unroll
+  for (int i = 0; i < CH_NS; i++) p = p + di[i] * m->ri[i] * T;
+  f[0] = T;
+  f[1] = log(p);
+#pragma unroll
+  for (int i = 0; i < CH_NS; i++) f[2 + i] = cbrt(di[i] / r);
+  f[11] = log(dt);
+}
+
+// mode 0: T [n]; 1: raw features [n][12] (f64); 2: normalised inputs [n][12]
+// (f32, (f - in_shift) * in_scale)
+__global__ __launch_bounds__(128) void k_cell_features(long n, const double* __restrict__ rho,
+                                                        const double* __restrict__ Ecell, const double* __restrict__ dtcell,
+                                                        int mode, void* out) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  Mech m = (Mech)&cm;
+  double di[CH_NS], r = 0.0;
+#pragma unroll
+  for (int i = 0; i < CH_NS; i++) {
+    di[i] = rho[(long)i * n + c];
+    r = r + di[i];
+  }
+  if (mode == 0) {
+    ((double*)out)[c] = chem_T(m, di, Ecell[c]);
+    return;
+  }
+  double f[12], T, p;
+  surr_raw_features(m, di, r, Ecell[c], dtcell[c], f, T, p);
+#pragma unroll
+  for (int j = 0; j < 12; j++) {
+    if (mode == 1) ((double*)out)[c * 12 + j] = f[j];
+    else ((float*)out)[c * 12 + j] = (float)((f[j] - scfg.in_shift[j]) * scfg.in_scale[j]);
+  }
+}
+
+// normalised targets: y [n][9] = ((rho1_k - rho0_k) / rho0) / out_scale_k
+__global__ void k_surr_targets(long n, const double* __restrict__ rho0, const double* __restrict__ rho1, float* y) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  double r = 0.0;
+#pragma unroll
+  for (int i = 0; i < CH_NS; i++) r = r + rho0[(long)i * n + c];
+#pragma unroll
+  for (int i = 0; i < CH_NS; i++)
+    y[c * CH_NS + i] = (float)((rho1[(long)i * n + c] - rho0[(long)i * n + c]) / r / scfg.out_scale[i]);
+}
+
+__global__ void k_fill_u64(unsigned long long* p, int count) {
+  if ((int)threadIdx.x < count && blockIdx.x == 0) p[threadIdx.x] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_surrogate(long n, double* rho, double* Ecell, const double* dtcell,
+                                                    int* flag_out, int* fb_list, unsigned long long* counters) {
+  __shared__ __attribute__((aligned(16))) tbf16 H[MT_ROWS * MT_LDH];
+  __shared__ int ok_in[MT_ROWS];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const long c0 = (long)blockIdx.x * MT_ROWS;
+  Mech m = (Mech)&cm;
+  if (t < MT_ROWS) {
+    const long c = c0 + t;
+    float x[12];
+#pragma unroll
+    for (int j = 0; j < 12; j++) x[j] = 0.f;
+    int ok = 0;
+    if (c < n) {
+      double di[CH_NS];
+      double r = 0.0;
+#pragma unroll
+      for (int i = 0; i < CH_NS; i++) {
+        di[i] = rho[(long)i * n + c];
+        r = r + di[i];
+      }
+      const double E = Ecell[c], dt = dtcell[c];
+      double f[12], T, p;
+      surr_raw_features(m, di, r, E, dt, f, T, p);
+      ok = (T >= scfg.T_lo && T <= scfg.T_hi && p >= scfg.p_lo && p <= scfg.p_hi && dt >= scfg.dt_lo &&
+            dt <= scfg.dt_hi && r > 0.0) ? 1 : 0;
+#pragma unroll
+      for (int j = 0; j < 12; j++) {
+        x[j] = (float)((f[j] - scfg.in_shift[j]) * scfg.in_scale[j]);
+        if (!(fabsf(x[j]) <= 1e6f)) ok = 0;  // also catches NaN
+      }
+      if (!ok) {
+#pragma unroll
+        for (int j = 0; j < 12; j++) x[j] = 0.f;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 64; j++) H[t * MT_LDH + j] = (tbf16)(j < 12 ? x[j] : 0.f);
+    ok_in[t] = ok;
+  }
+  __syncthreads();
+  const int L = cnet.n_layers;
+  mt_layer256<true>(H, cnet.W[0], cnet.b[0], 64, cnet.dinp[0], lane, w);
+  for (int l = 1; l < L - 1; l++) mt_layer256<true>(H, cnet.W[l], cnet.b[l], 256, 256, lane, w);
+  float* O = (float*)H;
+  mt_out16(H, cnet.W[L - 1], cnet.b[L - 1], 256, 256, O, lane, w);
+  if (t < MT_ROWS) {
+    const long c = c0 + t;
+    if (c < n) {
+      int why = ok_in[t] ? 0 : 1;
+      double di[CH_NS], dY[CH_NS], dYp[CH_NS], dinew[CH_NS];
+      double r = 0.0;
+#pragma unroll
+      for (int i = 0; i < CH_NS; i++) {
+        di[i] = rho[(long)i * n + c];
+        r = r + di[i];
+        dY[i] = (double)O[t * 16 + i] * scfg.out_scale[i];
+      }
+#pragma unroll
+      for (int i = 0; i < CH_NS; i++) {
+        double a = 0.0;
+#pragma unroll
+        for (int j = 0; j < CH_NS; j++) a = a + scfg.P[i][j] * dY[j];
+        dYp[i] = a;
+      }
+#pragma unroll
+      for (int e = 0; e < 3; e++) {
+        double res = 0.0, content = 0.0;
+#pragma unroll
+        for (int i = 0; i < CH_NS; i++) {
+          res = res + scfg.Cel[e][i] * dYp[i];
+          content = content + scfg.Cel[e][i] * fmax(di[i] / r, 0.0);
+        }
+        if (!(fabs(res) <= scfg.elem_tol * content + 1e-300)) why |= 2;
+      }
+#pragma unroll
+      for (int i = 0; i < CH_NS; i++) {
+        if (!(di[i] / r + dYp[i] >= -scfg.y_neg_tol)) why |= 4;
+        dinew[i] = di[i] + r * dYp[i];
+      }
+      double E = Ecell[c];
+      chem_update_et(m, di, dinew, E);
+      const double Tn = chem_T(m, dinew, E);
+      if (!(Tn >= scfg.T_out_lo && Tn <= scfg.T_out_hi)) why |= 8;  // also NaN
+      const bool ok = why == 0;
+      if (ok) {
+#pragma unroll
+        for (int i = 0; i < CH_NS; i++) rho[(long)i * n + c] = dinew[i];
+        Ecell[c] = E;
+      } else {
+        const unsigned long long k = atomicAdd(counters + 2, 1ull);
+        fb_list[k] = (int)c;
+      }
+      if (flag_out) flag_out[c] = why;
+    }
+  }
+}
+
+
+int ensure_heavy(long n) {
+  if (n <= CX.heavy.cap) return 0;
+  if (CX.heavy_mem) hipFree(CX.heavy_mem);
+  CX.heavy_mem = nullptr;
+  CX.heavy = HeavyBuf{};
+  const size_t bytes = (size_t)n * ((CH_NS + 3) * sizeof(double) + 3 * sizeof(int)) + 256;
+  if (hipMalloc(&CX.heavy_mem, bytes) != hipSuccess) return cfail("chemistry: out of device memory (work records)");
+  HeavyBuf& h = CX.heavy;
+  double* d = (double*)CX.heavy_mem;
+  h.di = d;
+  h.E = d + (size_t)CH_NS * n;
+  h.t = h.E + n;
+  h.h_cap = h.t + n;
+  int* q = (int*)(h.h_cap + n);
+  h.nsub = q;
+  h.nrej = q + n;
+  h.cell = q + 2 * n;
+  h.cap = n;
+  return 0;
+}
+
+// the adaptive integrator over cells 0..n-1, or over list[0 .. *list_count)
+// when list is given; counters [0,1,3,4] must be zero.  scheduler 0: one
+// cell per thread; 1: per-lane work queue; 2: work queue with the cooperative
+// kernel taking the cells that outlast CX.coop_budget attempts.
+int launch_adaptive(long n, double* rho, double* E, const double* dt, const ChemCtl& ctl, int* nsub_out, int scheduler,
+                    const int* list, const unsigned long long* list_count) {
+  if (scheduler == 0 && !list) {
+    hipLaunchKernelGGL(k_chem_adaptive, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, CX.stream, n, rho, E, dt, ctl,
+                       nsub_out, CX.counter);
+    return hipGetLastError() == hipSuccess ? 0 : cfail("chemistry: launch failed");
+  }
+  int dev = 0, ncu = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_chem_queue, 128, 0) != hipSuccess)
+    return cfail("chemistry: device query failed");
+  if (per < 1) per = 1;
+  long blocks = (long)ncu * per;
+  const long need = (n + 127) / 128;
+  if (blocks > need) blocks = need;
+  const bool coop = scheduler == 2;
+  if (coop && ensure_heavy(n)) return 1;
+  hipLaunchKernelGGL(k_chem_queue, dim3((unsigned)blocks), dim3(128), 0, CX.stream, n, rho, E, dt, ctl, nsub_out,
+                     CX.counter, list, list_count, coop ? CX.coop_budget : 0, CX.heavy);
+  if (coop) {
+    int per2 = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per2, k_chem_coop, 64, 0) != hipSuccess)
+      return cfail("chemistry: device query failed");
+    long b2 = (long)ncu * (per2 < 1 ? 1 : per2);
+    if (b2 > n) b2 = n;
+    hipLaunchKernelGGL(k_chem_coop, dim3((unsigned)b2), dim3(64), 0, CX.stream, n, rho, E, dt, ctl, nsub_out,
+                       CX.counter, CX.heavy);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : cfail("chemistry: launch failed");
 }
 
 // ===========================================================================
@@ -603,7 +1134,7 @@ int32_t ocfd_chem_set_mechanism(int32_t ns, int32_t nr, int32_t rereac, const in
   }
   if (!CX.own) {
     if (hipStreamCreateWithFlags(&CX.own, hipStreamNonBlocking) != hipSuccess) return cfail("no HIP device/stream");
-    if (hipMalloc(&CX.counter, 2 * sizeof(unsigned long long)) != hipSuccess) return cfail("hipMalloc failed");
+    if (hipMalloc(&CX.counter, 8 * sizeof(unsigned long long)) != hipSuccess) return cfail("hipMalloc failed");
   }
   CX.stream = CX.own;
   // kernels still in flight read the constant bank being replaced
@@ -636,27 +1167,92 @@ int32_t ocfd_chem_adaptive_batch(int64_t n, double* rho, double* E, const double
   if (!CX.ready) return cfail("ocfd_chem_adaptive_batch: call ocfd_chem_set_mechanism first");
   if (max_sub < 1 || !(rtol_y > 0.0) || !(y_floor >= 0.0) || !(rtol_t > 0.0))
     return cfail("ocfd_chem_adaptive_batch: bad controller parameters");
+  if (scheduler < 0 || scheduler > 2) return cfail("ocfd_chem_adaptive_batch: scheduler 0, 1 or 2");
   const ChemCtl ctl{rtol_y, y_floor, rtol_t, (int)max_sub};
   hipLaunchKernelGGL(k_chem_reset, dim3(1), dim3(64), 0, CX.stream, CX.counter);
   if (n <= 0) return hipGetLastError() == hipSuccess ? 0 : cfail("ocfd_chem_adaptive_batch: launch failed");
-  if (scheduler == 0) {
-    hipLaunchKernelGGL(k_chem_adaptive, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, CX.stream, (long)n, rho, E, dt,
-                       ctl, (int*)nsub_out, CX.counter);
-  } else {
-    int dev = 0, ncu = 0, per = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_chem_queue, 128, 0) != hipSuccess)
-      return cfail("ocfd_chem_adaptive_batch: device query failed");
-    if (per < 1) per = 1;
-    long blocks = (long)ncu * per;
-    const long need = (n + 127) / 128;
-    if (blocks > need) blocks = need;
-    hipLaunchKernelGGL(k_chem_queue, dim3((unsigned)blocks), dim3(128), 0, CX.stream, (long)n, rho, E, dt, ctl,
-                       (int*)nsub_out, CX.counter);
-  }
-  if (hipGetLastError() != hipSuccess) return cfail("ocfd_chem_adaptive_batch: launch failed");
+  return launch_adaptive((long)n, rho, E, dt, ctl, (int*)nsub_out, (int)scheduler, nullptr, nullptr);
+}
+
+int32_t ocfd_chem_surrogate_set(const double* cfg, int32_t count) {
+  if (!CX.ready) return cfail("ocfd_chem_surrogate_set: call ocfd_chem_set_mechanism first");
+  if (count != (int32_t)(sizeof(SurrCfg) / sizeof(double)))
+    return cfail("ocfd_chem_surrogate_set: expected " + std::to_string(sizeof(SurrCfg) / sizeof(double)) + " values");
+  SurrCfg h;
+  std::memcpy(&h, cfg, sizeof(SurrCfg));
+  if (hipStreamSynchronize(CX.stream) != hipSuccess ||
+      hipMemcpyToSymbol(HIP_SYMBOL(scfg), &h, sizeof(SurrCfg)) != hipSuccess)
+    return cfail("ocfd_chem_surrogate_set: copy failed");
+  CX.surr_ready = true;
   return 0;
+}
+
+int32_t ocfd_chem_surrogate_batch(int64_t n, double* rho, double* E, const double* dt, int32_t fallback,
+                                  double rtol_y, double y_floor, double rtol_t, int32_t max_sub, int32_t* flag_out) {
+  if (!CX.ready || !CX.surr_ready) return cfail("ocfd_chem_surrogate_batch: mechanism or surrogate not set");
+  const MlpDevice* net = ocfd_mlp_device_view();
+  if (!net) return cfail("ocfd_chem_surrogate_batch: no network (ocfd_mlp_create)");
+  const int L = net->n_layers;
+  bool shape_ok = L >= 2 && net->din[0] == 12 && net->dinp[0] == 64 && net->dout[L - 1] == CH_NS;
+  for (int l = 0; l < L - 1; l++) shape_ok = shape_ok && net->dout[l] == 256;
+  if (!shape_ok) return cfail("ocfd_chem_surrogate_batch: the network must be 12 -> 256 x k -> 9");
+  if (fallback && (max_sub < 1 || !(rtol_y > 0.0) || !(y_floor >= 0.0) || !(rtol_t > 0.0)))
+    return cfail("ocfd_chem_surrogate_batch: bad controller parameters");
+  if (n > 0x7fffffffL) return cfail("ocfd_chem_surrogate_batch: at most 2^31-1 cells");
+  if (n > CX.fb_cap) {
+    if (CX.fb_list) hipFree(CX.fb_list);
+    CX.fb_list = nullptr;
+    if (hipMalloc(&CX.fb_list, (size_t)n * sizeof(int)) != hipSuccess) return cfail("ocfd_chem_surrogate_batch: hipMalloc");
+    CX.fb_cap = n;
+  }
+  if (hipMemcpyToSymbolAsync(HIP_SYMBOL(cnet), net, sizeof(MlpDevice), 0, hipMemcpyHostToDevice, CX.stream) !=
+      hipSuccess)
+    return cfail("ocfd_chem_surrogate_batch: copy failed");
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, CX.stream, CX.counter, 8);
+  if (n <= 0) return hipGetLastError() == hipSuccess ? 0 : cfail("ocfd_chem_surrogate_batch: launch failed");
+  hipLaunchKernelGGL(k_surrogate, dim3((unsigned)((n + MT_ROWS - 1) / MT_ROWS)), dim3(256), 0, CX.stream, (long)n, rho,
+                     E, dt, (int*)flag_out, CX.fb_list, CX.counter);
+  if (fallback) {
+    const ChemCtl ctl{rtol_y, y_floor, rtol_t, (int)max_sub};
+    if (launch_adaptive((long)n, rho, E, dt, ctl, nullptr, CX.fallback_scheduler, CX.fb_list, CX.counter + 2)) return 1;
+  }
+  if (hipGetLastError() != hipSuccess) return cfail("ocfd_chem_surrogate_batch: launch failed");
+  return 0;
+}
+
+// T [n] (mode 0), raw features [n][12] f64 (mode 1) or normalised surrogate
+// inputs [n][12] f32 (mode 2) of a batch; device pointers
+int32_t ocfd_chem_cell_features(int64_t n, const double* rho, const double* E, const double* dt, int32_t mode,
+                                void* out) {
+  if (!CX.ready) return cfail("ocfd_chem_cell_features: call ocfd_chem_set_mechanism first");
+  if (mode < 0 || mode > 2) return cfail("ocfd_chem_cell_features: mode 0, 1 or 2");
+  if (mode == 2 && !CX.surr_ready) return cfail("ocfd_chem_cell_features: surrogate normalisation not set");
+  if (mode > 0 && !dt) return cfail("ocfd_chem_cell_features: dt required");
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_cell_features, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, CX.stream, (long)n, rho, E, dt,
+                     (int)mode, out);
+  return hipGetLastError() == hipSuccess ? 0 : cfail("ocfd_chem_cell_features: launch failed");
+}
+
+int32_t ocfd_chem_surrogate_targets(int64_t n, const double* rho0, const double* rho1, float* y) {
+  if (!CX.ready || !CX.surr_ready) return cfail("ocfd_chem_surrogate_targets: surrogate normalisation not set");
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_surr_targets, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, CX.stream, (long)n, rho0, rho1, y);
+  return hipGetLastError() == hipSuccess ? 0 : cfail("ocfd_chem_surrogate_targets: launch failed");
+}
+
+int64_t ocfd_chem_surrogate_fallbacks(void) {
+  if (!CX.ready) {
+    cfail("ocfd_chem_surrogate_fallbacks: call ocfd_chem_set_mechanism first");
+    return -1;
+  }
+  unsigned long long v = 0;
+  if (hipMemcpyAsync(&v, CX.counter + 2, sizeof(v), hipMemcpyDeviceToHost, CX.stream) != hipSuccess ||
+      hipStreamSynchronize(CX.stream) != hipSuccess) {
+    cfail("ocfd_chem_surrogate_fallbacks: copy failed");
+    return -1;
+  }
+  return (int64_t)v;
 }
 
 int32_t ocfd_chem_synchronize(void) {
@@ -666,6 +1262,31 @@ int32_t ocfd_chem_synchronize(void) {
 }
 
 void* ocfd_chem_stream(void) { return (void*)CX.stream; }
+
+int32_t ocfd_chem_set_option(int32_t key, int32_t value) {
+  switch (key) {
+    case 1:
+      if (value < 1) return cfail("ocfd_chem_set_option: the attempt budget must be >= 1");
+      CX.coop_budget = value;
+      return 0;
+    case 2:
+      if (value < 1 || value > 2) return cfail("ocfd_chem_set_option: fallback scheduler 1 or 2");
+      CX.fallback_scheduler = value;
+      return 0;
+  }
+  return cfail("ocfd_chem_set_option: unknown option");
+}
+
+int32_t ocfd_chem_counters(int64_t* out, int32_t count) {
+  if (!CX.ready) return cfail("ocfd_chem_counters: call ocfd_chem_set_mechanism first");
+  if (count < 0 || count > 8) return cfail("ocfd_chem_counters: at most 8 counters");
+  unsigned long long v[8] = {};
+  if (hipMemcpyAsync(v, CX.counter, sizeof(v), hipMemcpyDeviceToHost, CX.stream) != hipSuccess ||
+      hipStreamSynchronize(CX.stream) != hipSuccess)
+    return cfail("ocfd_chem_counters: copy failed");
+  for (int i = 0; i < count; i++) out[i] = (int64_t)v[i];
+  return 0;
+}
 
 int64_t ocfd_chem_unfinished(void) {
   if (!CX.ready) {

@@ -191,6 +191,7 @@ class ChemOracle:
             "orc_chem_batch": (i32, [i32, ctypes.c_long, vp, vp, vp, i32]),
             "orc_chem_adaptive": (i32, [vp, vp, dbl, dbl, dbl, dbl, i32, vp, vp]),
             "orc_chem_adaptive_batch": (i32, [ctypes.c_long, vp, vp, vp, dbl, dbl, dbl, i32, vp, vp]),
+            "orc_set_threads": (i32, [i32]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -207,6 +208,10 @@ class ChemOracle:
                                       float(mech.tct_max), _ptr(k[8]), _ptr(k[9]))
         if rc:
             raise RuntimeError("orc_chem_set_mechanism failed")
+
+    def set_threads(self, n):
+        """OpenMP threads of the batch functions; returns the count in use."""
+        return self.lib.orc_set_threads(int(n))
 
     def E(self, di, T):
         return self.lib.orc_chem_E(_ptr(np.ascontiguousarray(di, dtype=np.float64)), float(T))

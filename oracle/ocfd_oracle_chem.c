@@ -23,6 +23,9 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define NS_MAX 16
 #define NR_MAX 64
@@ -460,3 +463,15 @@ int orc_chem_adaptive_batch(long n, double* rho, double* E, const double* dt, do
                              unfinished ? unfinished + c : NULL);
   return bad;
 }
+
+/* Threads used by the batch functions (OpenMP); returns the count in use. */
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
+

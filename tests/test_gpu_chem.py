@@ -60,7 +60,7 @@ def test_fixed_substeps_match_oracle(env, mode, nsub, dt_scale):
     assert dr <= RTOL_RHO and de <= RTOL_E
 
 
-@pytest.mark.parametrize("scheduler", [0, 1])
+@pytest.mark.parametrize("scheduler", [0, 1, 2])
 def test_adaptive_matches_oracle(env, scheduler):
     pkg, chem, b, M, orc = env
     rho, E, d = to_dev(b["rho"], b["E"], b["dt"] * 20.0)
@@ -78,17 +78,23 @@ def test_adaptive_matches_oracle(env, scheduler):
     assert dr <= 1e3 * RTOL_RHO and de <= 1e3 * RTOL_E
 
 
-def test_queue_equals_thread_scheduler_bitwise(env):
-    """Both schedulers run the same per-cell arithmetic: results identical."""
+def test_schedulers_bitwise_identical(env):
+    """One cell per thread, the work queue and the work queue with the
+    cooperative kernel run the same per-cell arithmetic: identical results.
+    The long steps make some cells outlast the per-lane budget."""
     pkg, chem, b, M, orc = env
     out = []
-    for sched in (0, 1):
+    for sched in (0, 1, 2):
         rho, E, d = to_dev(b["rho"], b["E"], b["dt"] * 50.0)
         ns = chem.adaptive(rho, E, d, max_sub=3000, scheduler=sched)
-        chem.synchronize()
+        cnt = chem.counters()
         out.append((rho.cpu().numpy(), E.cpu().numpy(), ns.cpu().numpy()))
-    for a, c in zip(out[0], out[1]):
-        assert np.array_equal(a, c)
+        if sched == 2:
+            print("cells moved to the cooperative kernel:", cnt["cooperative"])
+            assert cnt["cooperative"] > 0
+    for o in out[1:]:
+        for a, c in zip(out[0], o):
+            assert np.array_equal(a, c)
 
 
 def test_unfinished_cells_are_counted(env):
@@ -98,7 +104,7 @@ def test_unfinished_cells_are_counted(env):
     dt = b["dt"] * 100.0
     ro, eo, no, uo = orc.adaptive_batch(b["rho"].T, b["E"], dt, 0.05, 1e-6, 0.01, 3, with_unfinished=True)
     assert 0 < uo.sum() < len(uo)
-    for sched in (0, 1):
+    for sched in (0, 1, 2):
         rho, E, d = to_dev(b["rho"], b["E"], dt)
         ns = chem.adaptive(rho, E, d, max_sub=3, scheduler=sched)
         assert chem.unfinished() == int(uo.sum())
@@ -123,7 +129,7 @@ def test_ragged_and_edge_sizes(env):
     chem.adaptive(empty, e0, e0.clone())
     dt = b["dt"][:200].copy()
     dt[::3] = 0.0
-    for sched in (0, 1):
+    for sched in (0, 1, 2):
         rho, E, d = to_dev(b["rho"][:, :200], b["E"][:200], dt)
         ns = chem.adaptive(rho, E, d, scheduler=sched)
         chem.synchronize()
