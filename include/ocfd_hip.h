@@ -89,6 +89,9 @@ enum {
   OCFD_FIELD_AIX1, OCFD_FIELD_AIY1, OCFD_FIELD_AIZ1,
   OCFD_FIELD_ASX1, OCFD_FIELD_ASY1, OCFD_FIELD_ASZ1,
   OCFD_FIELD_DIN,     /* din(nx,ny,nz,N_SPEC)     interior x N_SPEC */
+  OCFD_FIELD_AMD,     /* Amd(nx,ny,nz,N_SPEC)     interior x N_SPEC (N_SPEC > 1) */
+  OCFD_FIELD_DUS,     /* species right-hand sides interior x N_SPEC (N_SPEC > 1) */
+  OCFD_FIELD_QP,      /* Qpx, Qpy, Qpz            interior x 3 */
   OCFD_FIELD_COUNT
 };
 
@@ -125,9 +128,23 @@ int32_t ocfd_set_option(int32_t key, int32_t value);
 /* Species thermo (read_Spec, code/Chemical/CHEM_thermdyn.f90:16-51):
  * mi[n_spec] (kg/mol); coef[k][j][7] = (A,B,C,D,E,F,G) of range j of
  * species k (n_spec x n_tct x 7); tct[n_tct-1]; tct_max.  For a perfect
- * gas only mi is used (coef may be NULL). */
+ * gas only mi is used (coef may be NULL).  The energy offsets Ect that keep
+ * the enthalpy continuous across ranges are derived here (:42-46). */
 int32_t ocfd_set_species(int32_t n_tct, const double* tct, double tct_max,
                          const double* mi, const double* coef);
+
+/* Transport fits (read_transport / Trans%mu_an, k_an, D_an,
+ * code/Chemical/TRANSPORT_modules.f90:80-253), used when Flag_Vis,
+ * Flag_Cond or Flag_Diff is 1: mu_an[n_spec][4], k_an[n_spec][4] and
+ * D_an[n_spec][n_spec][4] (pair (k,j) read for k <= j) of
+ * exp(a1 + a2 lnT + a3 lnT^2 + a4 lnT^3).  A set whose flag is 0 may be NULL. */
+int32_t ocfd_set_transport(const double* mu_an, const double* k_an, const double* D_an);
+
+/* Jet inflow data flow_inlet2d (ocfd_bc_jet, code/boundary-condition/
+ * OCFD_BC_jet.f90:2-96): column-major (ny, nz, 5+N_SPEC) with the variables
+ * d, u, v, w, T, di(1..N_SPEC).  BC_Para(1) = 1 takes the pressure of i=2
+ * where the inflow is subsonic. */
+int32_t ocfd_set_bc_jet(const double* inlet2d);
 
 /* Mesh: coordinates Axx, Ayy, Azz (halo arrays; only the interior is read).
  * Computes the halo coordinate fill, periodic span shift, the 9 metrics and
@@ -160,15 +177,34 @@ int32_t ocfd_set_bc_time(double ht);
 
 /* ---- the hot path ------------------------------------------------------ */
 /* One TVD-RK3 step of size dt1 (flow_time_advance,
- * code/flow-solver/OCFD_Time_Adv.f90:7-92).  Non-reacting, N_SPEC==1 path. */
+ * code/flow-solver/OCFD_Time_Adv.f90:7-92).  N_SPEC == 1, or N_SPEC > 1 with
+ * the species convection/diffusion and Scalar_time_adv_RK3 (:64-71,134-166). */
 int32_t ocfd_flow_time_advance(double dt1);
+
+/* One step of NS_solver (code/flow-solver/OCFD_NS_Solver.f90:26-37): flow
+ * dt/2, then (reaction /= 0) nsub chemistry substeps of dt/nsub
+ * (Chemical_time_advance with CHEM_TimeAdv = mode) and comput_T_P, then
+ * flow dt/2.  The chemistry needs ocfd_chem_set_mechanism and N_SPEC = 9. */
+int32_t ocfd_ns_step(double dt, int32_t nsub, int32_t mode, int32_t reaction);
+/* Chemical_time_advance (OCFD_Time_Adv.f90:169-196) on every interior cell. */
+int32_t ocfd_chemical_time_advance(double dt2, int32_t mode);
+/* comput_T_P (code/flow-solver/Comput_flow_variables.f90:32-46), N_SPEC > 1. */
+int32_t ocfd_comput_T_P(void);
+/* Species pieces (N_SPEC > 1): the right-hand sides du_scaler_invis +
+ * du_scaler_viscous of every species into OCFD_FIELD_DUS and Qp into
+ * OCFD_FIELD_QP (code/flow-solver/OCFD_scaler_inviscous.f90:1-84,
+ * OCFD_scaler_viscous.f90:4-79), and the same followed by
+ * Scalar_time_adv_RK3 of stage krk. */
+int32_t ocfd_du_species(void);
+int32_t ocfd_species_time_adv(int32_t krk, double dt);
 
 /* Per-stage pieces, for a host that keeps the reference's own stage loop
  * (OCFD_Time_Adv.f90:33-91).  Called in this order they reproduce
  * ocfd_flow_time_advance exactly:
  *   ocfd_save_stage0();
  *   for krk = 1..3:
- *     ocfd_stage_begin(krk); ocfd_du_inviscous(); ocfd_du_viscous();
+ *     ocfd_stage_begin(krk); [N_SPEC > 1: ocfd_species_time_adv(krk, dt1);]
+ *     ocfd_du_inviscous(); ocfd_du_viscous();
  *     ocfd_time_adv_rk3(krk, dt1); ocfd_comput_flow_variables(); ocfd_bc(); */
 int32_t ocfd_save_stage0(void);             /* fn=f, din=di (OCFD_Time_Adv.f90:10-29) */
 /* Stage prologue, OCFD_Time_Adv.f90:35-62: comput_Amu_Amk_AMD, the halo

@@ -34,7 +34,8 @@ BC_NONE, BC_BOUNDARYLAYER, BC_JET = 0, 2, 4
 # field identifiers (include/ocfd_hip.h)
 FIELDS = ["F", "FN", "DU", "DI", "D", "U", "V", "W", "T", "P", "ET", "AMU", "AMK", "RHYBRID",
           "AKX", "AKY", "AKZ", "AIX", "AIY", "AIZ", "ASX", "ASY", "ASZ", "AJAC",
-          "AKX1", "AKY1", "AKZ1", "AIX1", "AIY1", "AIZ1", "ASX1", "ASY1", "ASZ1", "DIN"]
+          "AKX1", "AKY1", "AKZ1", "AIX1", "AIY1", "AIZ1", "ASX1", "ASY1", "ASZ1", "DIN",
+          "AMD", "DUS", "QP"]
 FIELD = {name: i for i, name in enumerate(FIELDS)}
 METRICS = FIELDS[FIELD["AKX"]:FIELD["ASZ1"] + 1]
 
@@ -64,7 +65,7 @@ class OcfdCfg(ctypes.Structure):
 def make_cfg(nx, ny, nz, *, periodic=(1, 1, 1), scheme_invis=SCHEME_WENO7, scheme_vis=SCHEME_CD6,
              flux_splitting=SPLIT_SW, if_viscous=1, ibc=BC_NONE, ud7l_diss=1.0, hybrid_para=None,
              amu0=1.789e-5, pr=0.72, sc=1.0, mass_force=None, bc_para=None, scheme_boundary=None,
-             n_spec=1, if_real_gas=0):
+             n_spec=1, if_real_gas=0, flag_vis=0, flag_cond=0, flag_diff=0):
     """Build an ``ocfd_cfg`` with the namelist defaults of the reference
     (code/mpi-io/OCFD_readpara.f90:65-110) for fields not given."""
     c = OcfdCfg()
@@ -78,7 +79,7 @@ def make_cfg(nx, ny, nz, *, periodic=(1, 1, 1), scheme_invis=SCHEME_WENO7, schem
         c.ghost_cell[a] = 0
     c.if_mass_force = 1 if mass_force is not None else 0
     c.if_real_gas = if_real_gas
-    c.flag_vis = c.flag_cond = c.flag_diff = 0
+    c.flag_vis, c.flag_cond, c.flag_diff = flag_vis, flag_cond, flag_diff
     c.wall_catalysis = 0
     c.ud7l_diss = ud7l_diss
     hp = [1.0] * 16 if hybrid_para is None else list(hybrid_para) + [1.0] * (16 - len(hybrid_para))
@@ -125,6 +126,9 @@ def load_library(path=None):
         "ocfd_kernel_stats": (i32, [i32, P(dbl), P(ctypes.c_int64)]), "ocfd_kernel_stats_reset": (i32, []),
         "ocfd_autotune": (i32, [i32]), "ocfd_set_tile": (i32, [i32, i32]),
         "ocfd_tile_choice": (i32, [i32, P(i32), P(i32), P(ctypes.c_float)]),
+        "ocfd_set_transport": (i32, [vp, vp, vp]), "ocfd_set_bc_jet": (i32, [vp]),
+        "ocfd_ns_step": (i32, [dbl, i32, i32, i32]), "ocfd_chemical_time_advance": (i32, [dbl, i32]),
+        "ocfd_comput_T_P": (i32, []), "ocfd_du_species": (i32, []), "ocfd_species_time_adv": (i32, [i32, dbl]),
         "ocfd_chem_set_mechanism": (i32, [i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, i32, vp, dbl, vp, vp]),
         "ocfd_chem_timeadv_batch": (i32, [ctypes.c_int64, vp, vp, vp, i32, i32]),
         "ocfd_chem_adaptive_batch": (i32, [ctypes.c_int64, vp, vp, vp, dbl, dbl, dbl, i32, vp, i32]),
@@ -191,8 +195,10 @@ class FlowSolver:
             return (self.nx, self.ny, self.nz, 5)
         if name in ("AMU", "AMK"):
             return (self.nx, self.ny, self.nz)
-        if name == "DIN":
+        if name in ("DIN", "AMD", "DUS"):
             return (self.nx, self.ny, self.nz, self.cfg.n_spec)
+        if name == "QP":
+            return (self.nx, self.ny, self.nz, 3)
         if name == "DI":
             return self.halo_shape + (self.cfg.n_spec,)
         return self.halo_shape
@@ -238,6 +244,18 @@ class FlowSolver:
     def set_bc_time(self, ht):
         self._check(self.lib.ocfd_set_bc_time(float(ht)))
 
+    def set_transport(self, mu_an=None, k_an=None, D_an=None):
+        """Chemkin fits (n_spec, 4), (n_spec, 4), (n_spec, n_spec, 4) (pair k <= j)."""
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (mu_an, k_an, D_an)]
+        self._check(self.lib.ocfd_set_transport(*[None if a is None else _ptr(a) for a in arrs]))
+
+    def set_bc_jet(self, inlet2d):
+        """flow_inlet2d of shape (ny, nz, 5 + n_spec): d, u, v, w, T, di."""
+        a = np.asfortranarray(inlet2d, dtype=np.float64)
+        if a.shape != (self.ny, self.nz, 5 + self.cfg.n_spec):
+            raise ValueError(f"inlet2d: expected shape {(self.ny, self.nz, 5 + self.cfg.n_spec)}, got {a.shape}")
+        self._check(self.lib.ocfd_set_bc_jet(_ptr(a)))
+
     def upload_state(self, state):
         """state: dict with F, DI, D, U, V, W, T, P, ET (reference layouts)."""
         for name in ("F", "DI", "D", "U", "V", "W", "T", "P", "ET"):
@@ -246,6 +264,24 @@ class FlowSolver:
     # -- hot path (reference names) -------------------------------------
     def flow_time_advance(self, dt1):
         self._check(self.lib.ocfd_flow_time_advance(float(dt1)))
+
+    def NS_step(self, dt, nsub=1, mode=1, reaction=True):
+        """One time step of NS_solver (OCFD_NS_Solver.f90:26-37)."""
+        self._check(self.lib.ocfd_ns_step(float(dt), int(nsub), int(mode), int(bool(reaction))))
+
+    def Chemical_time_advance(self, dt2, mode=1):
+        self._check(self.lib.ocfd_chemical_time_advance(float(dt2), int(mode)))
+
+    def comput_T_P(self):
+        self._check(self.lib.ocfd_comput_T_P())
+
+    def du_species(self):
+        """du_scaler_invis + du_scaler_viscous of every species (into DUS, Qp into QP)."""
+        self._check(self.lib.ocfd_du_species())
+
+    def Scalar_time_adv_RK3(self, krk, dt):
+        """Species right-hand sides and update of stage krk (OCFD_Time_Adv.f90:64-71)."""
+        self._check(self.lib.ocfd_species_time_adv(int(krk), float(dt)))
 
     def save_stage0(self):
         self._check(self.lib.ocfd_save_stage0())

@@ -312,3 +312,114 @@ def jet_chem_states(n, seed=4, mech=None, T_range=(900.0, 2500.0), p_range_atm=(
     dt = np.exp(rng.uniform(math.log(dt_range[0]), math.log(dt_range[1]), n))
     tau = np.exp(rng.uniform(math.log(tau_range[0]), math.log(tau_range[1]), n))
     return dict(rho=np.ascontiguousarray(rho), E=E, T=T, p=p, Z=Z, dt=dt, tau=tau)
+
+
+# ---------------------------------------------------------------------------
+# Reacting jet (N_SPEC = 9, real gas, Chemkin/Wilke transport)
+# ---------------------------------------------------------------------------
+def jet(nx=64, ny=48, nz=48, seed=3, lx=0.004, ly=0.003, lz=0.003, rd=0.0005, rbuf=0.0001, xh=0.001, Th=1800.0,
+        epsl=0.05, rdist=0.0002, omeg=8.0, mech=None, cfl=0.3, iflag=1):
+    """Round H2/N2 jet into a hot air coflow: the set-up of cases/jet
+    (opencfd-comb2.in, inlet.in and the profile construction of
+    jet-comb-inlet.f90) on a uniform box [0, lx] x [-ly/2, ly/2] x
+    [-lz/2, lz/2].
+
+    Fuel stream X_H2 = 0.85, X_N2 = 0.15, 900 m/s, 305 K; coflow air at
+    20 m/s, 1150 K; 1 atm.  Radial profiles blend the two streams linearly
+    over rbuf around r = rd; a seeded azimuthal perturbation of amplitude
+    epsl (modes omeg, 2 omeg, 4 omeg) on the shear layer
+    (rd +- rdist/2) feeds u, v, w.  The initial field is the inflow plane
+    repeated downstream with the jet core at Th for x > xh.  OMP6 / CD6,
+    Steger-Warming, real gas, Flag_Viscosity = Flag_Conductivity =
+    Flag_Diffusion = 1, BC_JET with BC_Para(1) = iflag.
+
+    Returns cfg, mesh, span, state, bc (inlet2d), info (dt, mech)."""
+    from .mechanism import Mechanism
+    from . import BC_JET, SCHEME_OMP6
+    mech = mech or Mechanism()
+    ns = mech.n_spec
+    xf = np.zeros(ns)
+    xo = np.zeros(ns)
+    xf[mech.index("H2")], xf[mech.index("N2")] = 0.85, 0.15
+    xo[mech.index("O2")], xo[mech.index("N2")] = 0.21, 0.79
+    u1, T1, p1 = 900.0, 305.0, 101300.0
+    u2, T2, p2 = 20.0, 1150.0, 101300.0
+    x = np.linspace(0.0, lx, nx)
+    y = np.linspace(-0.5 * ly, 0.5 * ly, ny)
+    z = np.linspace(-0.5 * lz, 0.5 * lz, nz)
+    Y2, Z2 = np.meshgrid(y, z, indexing="ij")
+    rr = np.sqrt(Y2 * Y2 + Z2 * Z2)
+    tmp = np.clip((rr - (rd - 0.5 * rbuf)) / rbuf, 0.0, 1.0)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    fai = rng.random(3)
+    Ri = R0 / mech.mi
+
+    def plane(t_core):
+        T0 = t_core + (T2 - t_core) * tmp
+        p0 = p1 + (p2 - p1) * tmp
+        u0 = u1 + (u2 - u1) * tmp
+        X0 = xf[:, None, None] + (xo - xf)[:, None, None] * tmp[None]
+        di = p0[None] * X0 / (Ri[:, None, None] * T0[None])
+        seta = np.arccos(np.divide(Y2, rr, out=np.ones_like(rr), where=rr > 0))
+        seta = np.where(Z2 < 0, 2.0 * math.pi - seta, seta)
+        r1, r2 = rd - 0.5 * rdist, rd + 0.5 * rdist
+        band = (rr >= r1) & (rr <= r2)
+        fr = np.where(band, u0 * epsl / 1.75 * np.sin(math.pi * (rr - r1) / (r2 - r1)) *
+                      (np.sin(2.0 * math.pi * (omeg * seta + fai[0])) +
+                       0.5 * np.sin(2.0 * math.pi * (2.0 * omeg * seta + fai[1])) +
+                       0.25 * np.sin(2.0 * math.pi * (4.0 * omeg * seta + fai[2]))), 0.0)
+        safe = np.where(rr > 0, rr, 1.0)
+        return di, u0 + fr, fr * Y2 / safe, fr * Z2 / safe, T0
+
+    di_in, u_in, v_in, w_in, T_in = plane(T1)
+    di_h, u_h, v_h, w_h, T_h = plane(Th)
+    hot = x > xh
+    di = np.where(hot[None, :, None, None], di_h[:, None], di_in[:, None])      # [ns, nx, ny, nz]
+    u = np.where(hot[:, None, None], u_h[None], u_in[None])
+    v = np.where(hot[:, None, None], v_h[None], v_in[None])
+    w = np.where(hot[:, None, None], w_h[None], w_in[None])
+    T = np.where(hot[:, None, None], T_h[None], T_in[None])
+    d = di.sum(axis=0)
+    st = {}
+    for name, val, src in (("D", 1.0, d), ("U", 1.0, u), ("V", 1.0, v), ("W", 1.0, w), ("T", 300.0, T)):
+        a = halo_zeros(nx, ny, nz, val)
+        interior(a)[...] = src
+        st[name] = a
+    dih = np.ones((nx + 2 * LAP, ny + 2 * LAP, nz + 2 * LAP, ns), order="F")
+    for m in range(ns):
+        interior(dih[..., m])[...] = di[m]
+    st["DI"] = dih
+    E = mech.energy(di.reshape(ns, -1), T.reshape(-1)).reshape(d.shape)
+    f = np.zeros((nx, ny, nz, 5), order="F")
+    f[..., 0] = d
+    f[..., 1] = d * u
+    f[..., 2] = d * v
+    f[..., 3] = d * w
+    f[..., 4] = E + d * (u * u + v * v + w * w) * 0.5
+    st["F"] = f
+    st["P"] = halo_zeros(nx, ny, nz, 1.0)
+    st["ET"] = halo_zeros(nx, ny, nz, 1.0)
+    inlet = np.zeros((ny, nz, 5 + ns), order="F")
+    inlet[..., 0] = di_in.sum(axis=0)
+    inlet[..., 1], inlet[..., 2], inlet[..., 3], inlet[..., 4] = u_in, v_in, w_in, T_in
+    for m in range(ns):
+        inlet[..., 5 + m] = di_in[m]
+    mesh = mesh_1d(x, y, z)
+    span = np.zeros(9)
+    cfg = make_cfg(nx, ny, nz, periodic=(0, 0, 0), scheme_invis=SCHEME_OMP6, scheme_vis=SCHEME_CD6,
+                   flux_splitting=SPLIT_SW, if_viscous=1, ibc=BC_JET, bc_para=[iflag], n_spec=ns, if_real_gas=1,
+                   flag_vis=1, flag_cond=1, flag_diff=1)
+    # acoustic CFL with the frozen sound speed sqrt(gamma p / d), gamma = p/E + 1
+    p = (di * (Ri[:, None, None, None] * T[None])).sum(axis=0)
+    c = np.sqrt((p / E + 1.0) * p / d)
+    lam = (np.abs(u) + c) / (lx / (nx - 1)) + (np.abs(v) + c) / (ly / (ny - 1)) + (np.abs(w) + c) / (lz / (nz - 1))
+    dt = cfl / float(np.max(lam))
+    info = dict(name="jet", dt=dt, mech=mech)
+    return cfg, mesh, span, st, dict(inlet2d=inlet), info
+
+
+def jet_gas(engine, mech):
+    """Species thermo and transport fits of the mechanism into a solver (or
+    the oracle): read_Spec and read_vis/conduct/diffusion_coefficients."""
+    engine.set_species(mech.mi, n_tct=mech.n_tct, tct=mech.tct, tct_max=mech.tct_max, coef=mech.coef)
+    engine.set_transport(mech.viscosity, mech.conductivity, mech.diffusion)

@@ -647,6 +647,7 @@ struct CoopLds {
   double ri[CH_NS];
   double A[CH_NS][CH_NS + 1];  // Jacobian, then the system; column CH_NS = right-hand side
   double f[CH_NS];
+  double dA[CH_NR_MAX][CH_NS];  // d(rate of reaction k)/d(c_j), shared by every row i
   // mechanism tables read with lane-dependent indices (staged once)
   double coef[CH_NS][CH_NT_MAX][7];
   double mi[CH_NS];
@@ -699,22 +700,27 @@ __device__ bool coop_attempt(Mech m, CoopLds& L, const RxnConst& R, int lane, do
     }
     L.ri[lane] = r;
   }
-  for (int e = lane; e < CH_NS * CH_NS; e += 64) {
-    const int i = e / CH_NS, j = e % CH_NS;
-    double a = 0.0;
+  // the factor (Awf - Awr) of chem_dA once per (reaction, species) ...
+  for (int e = lane; e < nr * CH_NS; e += 64) {
+    const int k = e / CH_NS, j = e % CH_NS;
+    const int vfj = L.vf[k][j], vrj = L.vr[k][j];
     double cjv = ci[0];  // ci[j] for the lane-dependent j, without dynamic register indexing
 #pragma unroll
     for (int q = 1; q < CH_NS; q++)
       if (j == q) cjv = ci[q];
+    const double Awf = vfj * L.wf1[k] / (cjv + 1.0e-20);
+    const double Awr = vrj * L.wr1[k] / (cjv + 1.0e-20);
+    L.dA[k][j] = Awf - Awr;
+  }
+  __syncthreads();
+  // ... then every Jacobian entry sums its reactions in order
+  for (int e = lane; e < CH_NS * CH_NS; e += 64) {
+    const int i = e / CH_NS, j = e % CH_NS;
+    double a = 0.0;
     for (int k = 0; k < nr; k++) {
-      const int vfj = L.vf[k][j], vrj = L.vr[k][j];
-      if (vfj == 0 && vrj == 0) continue;
+      if (L.vf[k][j] == 0 && L.vr[k][j] == 0) continue;
       const int nu = L.vr[k][i] - L.vf[k][i];
-      if (nu != 0) {
-        const double Awf = vfj * L.wf1[k] / (cjv + 1.0e-20);
-        const double Awr = vrj * L.wr1[k] / (cjv + 1.0e-20);
-        a = a + (Awf - Awr) * nu;
-      }
+      if (nu != 0) a = a + L.dA[k][j] * nu;
     }
     L.A[i][j] = a;
   }

@@ -9,6 +9,7 @@
 #include <stdint.h>
 
 #define OCFD_LAP 4
+#define OCFD_NS_MAX 9   // species of the multi-species flow path
 
 // Reconstruction kinds.  A reconstruction reads an 8-point window w[0..7]
 // holding v(s-3..s+4) for flux+ at interface s (between cells s and s+1).
@@ -66,6 +67,13 @@ struct DevState {
   double* Rh;
   uint8_t* mask[3];
   double* Ev;          // 12 interior arrays: Ev1(1:4), Ev2(1:4), Ev3(1:4)
+  // multi-species path (N_SPEC > 1)
+  double* AmD;         // [ns][interior] diffusion coefficients
+  double* dus;         // [ns][interior] species right-hand sides
+  double* Evs;         // [3*ns][interior] species diffusive fluxes (xi, eta, zeta)
+  double* Qp;          // [3][interior] enthalpy flux of species diffusion
+  const double* jet_in;  // jet inflow flow_inlet2d (ny, nz, 5+ns)
+  int jet_iflag;       // BC_Para(1) of the jet
   double* red;         // reduction scratch
   int* negt;           // [0] = count, [1..16] = flagged cell linear indices
 };

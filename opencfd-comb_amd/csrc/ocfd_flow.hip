@@ -307,6 +307,8 @@ __global__ __launch_bounds__(256) void k_sweep(DevState S) {
   }
 }
 
+#include "ocfd_species.h"
+
 // ===========================================================================
 // Viscous terms, part 1 (du_viscous, OCFD_viscous.f90:27-95, with the
 // transport coefficients of comput_Amu_Amk_AMD, Comput_flow_variables.f90:69-98):
@@ -315,6 +317,9 @@ __global__ __launch_bounds__(256) void k_sweep(DevState S) {
 // ===========================================================================
 __device__ __forceinline__ double pw3(double x) { double x2 = x * x; return x2 * x; }
 
+// MS (N_SPEC > 1): the transport coefficients come from k_transport_ms and
+// the species-diffusion enthalpy flux Qp from k_species_visc_prep.
+template <bool MS>
 __global__ __launch_bounds__(256) void k_visc_prep(DevState S) {
   const long c = cell_index();
   if (c >= S.ni) return;
@@ -322,16 +327,22 @@ __global__ __launch_bounds__(256) void k_visc_prep(DevState S) {
   const int j = (int)((c / S.nx) % S.ny) + 1;
   const int k = (int)(c / ((long)S.nx * S.ny)) + 1;
   const long q = hq(S, i, j, k);
-  // transport (N_SPEC = 1): comput_vis_coeff Sutherland (TRANSPORT_modules.f90:265-269),
-  // comput_Cp_pgas (CHEM_thermdyn.f90:125-136), Amk = Amu*Cp/Pr (:326-327)
-  const double TA = 288.15, TC = 110.4;
-  const double T0 = S.T[q];
-  const double tr = T0 / TA;
-  const double amu = S.amu0 * sqrt(pw3(tr)) * (TA + TC) / (TC + T0);
-  const double Cp = (0.0 + 3.5 * S.Ri0 * S.di[q]) / S.d[q];
-  const double amk = amu * Cp / S.pr;
-  S.Amu[c] = amu;
-  S.Amk[c] = amk;
+  double amu, amk;
+  if (MS) {
+    amu = S.Amu[c];
+    amk = S.Amk[c];
+  } else {
+    // transport (N_SPEC = 1): comput_vis_coeff Sutherland (TRANSPORT_modules.f90:265-269),
+    // comput_Cp_pgas (CHEM_thermdyn.f90:125-136), Amk = Amu*Cp/Pr (:326-327)
+    const double TA = 288.15, TC = 110.4;
+    const double T0 = S.T[q];
+    const double tr = T0 / TA;
+    amu = S.amu0 * sqrt(pw3(tr)) * (TA + TC) / (TC + T0);
+    const double Cp = (0.0 + 3.5 * S.Ri0 * S.di[q]) / S.d[q];
+    amk = amu * Cp / S.pr;
+    S.Amu[c] = amu;
+    S.Amk[c] = amk;
+  }
 
   auto dh = [&](const double* F, int dir, int s) {
     return deriv1(S, dir, s, [&](int o) {
@@ -365,9 +376,10 @@ __global__ __launch_bounds__(256) void k_visc_prep(DevState S) {
   const double s12 = (uy + vx) * amu, s13 = (uz + wx) * amu, s23 = (vz + wy) * amu;
   const double uu = S.u[q], vv = S.v[q], ww = S.w[q];
   // Qpx/Qpy/Qpz are zero without species diffusion (OCFD_viscous.f90:13-15)
-  const double E1 = uu * s11 + vv * s12 + ww * s13 + amk * Tx + 0.0;
-  const double E2 = uu * s12 + vv * s22 + ww * s23 + amk * Ty + 0.0;
-  const double E3 = uu * s13 + vv * s23 + ww * s33 + amk * Tz + 0.0;
+  const double qx = MS ? S.Qp[c] : 0.0, qy = MS ? S.Qp[c + S.ni] : 0.0, qz = MS ? S.Qp[c + 2 * S.ni] : 0.0;
+  const double E1 = uu * s11 + vv * s12 + ww * s13 + amk * Tx + qx;
+  const double E2 = uu * s12 + vv * s22 + ww * s23 + amk * Ty + qy;
+  const double E3 = uu * s13 + vv * s23 + ww * s33 + amk * Tz + qz;
   const double Akx1 = S.Akx1[q], Aky1 = S.Aky1[q], Akz1 = S.Akz1[q];
   const double Aix1 = S.Aix1[q], Aiy1 = S.Aiy1[q], Aiz1 = S.Aiz1[q];
   const double Asx1 = S.Asx1[q], Asy1 = S.Asy1[q], Asz1 = S.Asz1[q];
@@ -414,7 +426,7 @@ __global__ __launch_bounds__(256) void k_transport(DevState S) {
 // The flags select the pieces so the per-stage entry points share this code.
 // A cell with Et <= 0 is recorded for handle_NegativeEt (k_negt_repair).
 // ===========================================================================
-enum { RK_MASS = 1, RK_VISC = 2, RK_WRITE_DU = 4, RK_UPDATE_F = 8, RK_PRIMS = 16, RK_DI_FROM_D = 32 };
+enum { RK_MASS = 1, RK_VISC = 2, RK_WRITE_DU = 4, RK_UPDATE_F = 8, RK_PRIMS = 16, RK_DI_FROM_D = 32, RK_MS = 64 };
 
 __global__ __launch_bounds__(256) void k_rk(DevState S, int krk, double dt, int flags) {
   const long c = cell_index();
@@ -481,6 +493,18 @@ __global__ __launch_bounds__(256) void k_rk(DevState S, int krk, double dt, int 
   if (Et <= 0.0) {
     const int slot = atomicAdd(&S.negt[0], 1);
     if (slot < 16) S.negt[1 + slot] = (int)c;
+  }
+  if (flags & RK_MS) {
+    double T, p;
+    ms_prims(S, q, dd, Et, T, p);
+    S.d[q] = dd;
+    S.u[q] = uu;
+    S.v[q] = vv;
+    S.w[q] = ww;
+    S.Et[q] = Et;
+    S.T[q] = T;
+    S.p[q] = p;
+    return;
   }
   // N_SPEC == 1 stepping path: di was set to d at the stage start
   // (OCFD_Time_Adv.f90:45-52), so the old d is the di being rescaled.
@@ -552,6 +576,14 @@ __global__ void k_negt_repair(DevState S) {
                        nb(i, j, k - 1)) / 6.0;
     S.Et[q] = En;
     S.f[c + 4 * S.ni] = S.f[c + 4 * S.ni] + (En - Et0);
+    if (S.ns > 1) {
+      double dim[OCFD_NS_MAX];
+      for (int m = 0; m < S.ns; m++) dim[m] = S.di[q + (long)m * S.nh];
+      const double T = gas_T(dim, En);
+      S.T[q] = T;
+      S.p[q] = gas_P(dim, T);
+      continue;
+    }
     const double di = S.di[q];
     const double T = En / (0.0 + 2.5 * S.Ri0 * di);
     S.T[q] = T;
@@ -674,12 +706,11 @@ __global__ void k_bc_faces(DevState S) {
   else if ((t -= n4) < n5) { i = (int)(t % nxi) + 2; j = (int)(t / nxi) + 2; k = nz; }
   else return;
   const long q = hq(S, i, j, k), c = iq(S, i, j, k);
-  if (i == 1) face_update(S, q, c);
-  if (i == nx) face_update(S, q, c);
-  if (j == 1) face_update(S, q, c);
-  if (j == ny) face_update(S, q, c);
-  if (k == 1) face_update(S, q, c);
-  if (k == nz) face_update(S, q, c);
+  const int nf = (i == 1) + (i == nx) + (j == 1) + (j == ny) + (k == 1) + (k == nz);
+  for (int a = 0; a < nf; a++) {
+    if (S.ns > 1) face_update_ms(S, q, c);
+    else face_update(S, q, c);
+  }
 }
 
 // fn = f, din = di  (OCFD_Time_Adv.f90:10-29)
@@ -690,7 +721,8 @@ __global__ __launch_bounds__(256) void k_save0(DevState S) {
   const int i = (int)(c % S.nx) + 1;
   const int j = (int)((c / S.nx) % S.ny) + 1;
   const int k = (int)(c / ((long)S.nx * S.ny)) + 1;
-  S.din[c] = S.di[hq(S, i, j, k)];
+  const long q = hq(S, i, j, k);
+  for (int m = 0; m < S.ns; m++) S.din[c + (long)m * S.ni] = S.di[q + (long)m * S.nh];
 }
 
 // ===========================================================================
@@ -932,6 +964,8 @@ struct Ctx {
   hipGraph_t graph = nullptr;
   double graph_dt = 0.0;
   int* h_negt = nullptr;  // pinned copy of the negative-Et counter
+  FlowGas hg{};           // host copy of the species constant bank
+  double *chem_rho = nullptr, *chem_E = nullptr, *chem_dt = nullptr;  // chemistry batch buffers
   std::vector<std::string> knames;
   std::map<std::string, int> kindex;
   std::vector<KStat> kstats;
@@ -1067,6 +1101,9 @@ void enq_bc() {
     launch("k_bl_upper", k_bl_upper, grid1((long)S.nx * S.nz), dim3(256), S);
     launch("k_bl_wall", k_bl_wall, grid1((long)S.nx * S.nz), dim3(256), S);
     if (S.bl_outlet == 1 || S.bl_outlet == 2) launch("k_bl_outlet", k_bl_outlet, grid1((long)S.ny * S.nz), dim3(256), S);
+  } else if (G.cfg.ibc == OCFD_BC_JET) {
+    launch("k_jet_inlet", k_jet_inlet, grid1((long)S.ny * S.nz), dim3(256), S);
+    launch("k_jet_outlet", k_jet_outlet, grid1((long)S.ny * S.nz), dim3(256), S);
   }
   const long nxi = S.nx > 2 ? S.nx - 2 : 0, nyi = S.ny > 2 ? S.ny - 2 : 0;
   const long nf = 2L * S.ny * S.nz + 2L * nxi * S.nz + 2L * nxi * nyi;
@@ -1079,9 +1116,49 @@ void enq_stage(int krk, double dt1) {
   launch_sweep<0>();
   launch_sweep<1>();
   launch_sweep<2>();
-  if (S.viscous) launch("k_visc_prep", k_visc_prep, grid1(S.ni), dim3(256), S);
+  if (S.viscous) launch("k_visc_prep", k_visc_prep<false>, grid1(S.ni), dim3(256), S);
   else launch("k_transport", k_transport, grid1(S.ni), dim3(256), S);
   const int flags = (S.mass_force ? RK_MASS : 0) | (S.viscous ? RK_VISC : 0) | RK_UPDATE_F | RK_PRIMS | RK_DI_FROM_D;
+  launch("k_negt_reset", k_negt_reset, dim3(1), dim3(64), S, 0);
+  launch("k_rk", k_rk, grid1(S.ni), dim3(256), S, krk, dt1, flags);
+  launch("k_negt_repair", k_negt_repair, dim3(1), dim3(64), S);
+  enq_bc();
+}
+
+template <int DIR>
+void launch_species_sweep() {
+  const DevState& S = G.S;
+  constexpr int NP = SpTile<DIR>::NP, TS = SpTile<DIR>::TS;
+  const long npen = DIR == 0 ? (long)S.ny * S.nz : DIR == 1 ? (long)S.nx * S.nz : (long)S.nx * S.ny;
+  dim3 g((unsigned)((S.n[DIR] + TS - 1) / TS), (unsigned)((npen + NP - 1) / NP));
+  static const char* names[3] = {"k_species_sweep_x", "k_species_sweep_y", "k_species_sweep_z"};
+  launch(names[DIR], k_species_sweep<DIR>, g, dim3(256), S);
+}
+
+// species right-hand sides (krk = 0: leave them in dus) or the species RK3
+// stage (OCFD_Time_Adv.f90:64-71)
+void enq_species(int krk, double dt1) {
+  const DevState& S = G.S;
+  launch_species_sweep<0>();
+  launch_species_sweep<1>();
+  launch_species_sweep<2>();
+  if (S.viscous) launch("k_species_visc_prep", k_species_visc_prep, grid1(S.ni), dim3(256), S);
+  launch("k_species_rk", k_species_rk, grid1(S.ni), dim3(256), S, krk, dt1, S.viscous);
+}
+
+// one RK3 stage for N_SPEC > 1 (OCFD_Time_Adv.f90:33-91): transport, sensor,
+// species convection/diffusion and update, flow fluxes and update, primitive
+// recovery with Rescaling_di, negative-Et repair, boundary conditions
+void enq_stage_ms(int krk, double dt1) {
+  const DevState& S = G.S;
+  launch("k_transport_ms", k_transport_ms, grid1(S.ni), dim3(256), S);
+  if (krk == 1 && G.cfg.scheme_invis == OCFD_SCHEME_HYBRID) enq_rhybrid();
+  enq_species(krk, dt1);
+  launch_sweep<0>();
+  launch_sweep<1>();
+  launch_sweep<2>();
+  if (S.viscous) launch("k_visc_prep", k_visc_prep<true>, grid1(S.ni), dim3(256), S);
+  const int flags = (S.mass_force ? RK_MASS : 0) | (S.viscous ? RK_VISC : 0) | RK_UPDATE_F | RK_PRIMS | RK_MS;
   launch("k_negt_reset", k_negt_reset, dim3(1), dim3(64), S, 0);
   launch("k_rk", k_rk, grid1(S.ni), dim3(256), S, krk, dt1, flags);
   launch("k_negt_repair", k_negt_repair, dim3(1), dim3(64), S);
@@ -1092,7 +1169,10 @@ void enq_step(double dt1) {
   const DevState& S = G.S;
   launch("k_negt_reset", k_negt_reset, dim3(1), dim3(64), S, 1);
   launch("k_save0", k_save0, grid1(S.ni), dim3(256), S);
-  for (int krk = 1; krk <= 3; krk++) enq_stage(krk, dt1);
+  for (int krk = 1; krk <= 3; krk++) {
+    if (S.ns > 1) enq_stage_ms(krk, dt1);
+    else enq_stage(krk, dt1);
+  }
   hipMemcpyAsync(G.h_negt, S.negt + 17, sizeof(int), hipMemcpyDeviceToHost, G.stream);
 }
 
@@ -1127,6 +1207,9 @@ double* field_ptr(int id, long* n) {
     case OCFD_FIELD_AMU: *n = ni; return S.Amu;
     case OCFD_FIELD_AMK: *n = ni; return S.Amk;
     case OCFD_FIELD_RHYBRID: *n = nh; return S.Rh;
+    case OCFD_FIELD_AMD: *n = ni * S.ns; return S.AmD;
+    case OCFD_FIELD_DUS: *n = ni * S.ns; return S.dus;
+    case OCFD_FIELD_QP: *n = 3 * ni; return S.Qp;
   }
   double* M[19] = {S.Akx, S.Aky, S.Akz, S.Aix, S.Aiy, S.Aiz, S.Asx, S.Asy, S.Asz, S.Ajac,
                    S.Akx1, S.Aky1, S.Akz1, S.Aix1, S.Aiy1, S.Aiz1, S.Asx1, S.Asy1, S.Asz1};
@@ -1169,10 +1252,14 @@ int32_t ocfd_init(const ocfd_cfg* cfg) {
   const ocfd_cfg& c = *cfg;
   if (c.lap != OCFD_LAP) return fail("ocfd_init: LAP must be 4");
   if (c.nx < 8 || c.ny < 8 || c.nz < 8) return fail("ocfd_init: each dimension needs at least 8 points");
-  if (c.n_spec != 1) return fail("ocfd_init: the flow path supports N_SPEC == 1 (non-reacting)");
+  if (c.n_spec < 1 || c.n_spec > OCFD_NS_MAX) return fail("ocfd_init: N_SPEC must be 1..9");
   if (c.if_scheme_character != 0) return fail("ocfd_init: IF_Scheme_Character=1 is not supported");
-  if (c.if_real_gas != 0) return fail("ocfd_init: the N_SPEC == 1 flow path is the perfect gas (IF_RealGas=0)");
-  if (c.flag_vis != 0 || c.flag_cond != 0) return fail("ocfd_init: only Sutherland/Prandtl transport on this path");
+  if (c.n_spec == 1 && c.if_real_gas != 0) return fail("ocfd_init: the N_SPEC == 1 flow path is the perfect gas (IF_RealGas=0)");
+  if (c.n_spec == 1 && (c.flag_vis != 0 || c.flag_cond != 0))
+    return fail("ocfd_init: N_SPEC == 1 uses Sutherland/Prandtl transport");
+  if (c.if_real_gas < 0 || c.if_real_gas > 1) return fail("ocfd_init: IF_RealGas must be 0 or 1");
+  if (c.flag_vis < 0 || c.flag_vis > 1 || c.flag_cond < 0 || c.flag_cond > 1 || c.flag_diff < 0 || c.flag_diff > 1)
+    return fail("ocfd_init: transport flags must be 0 or 1");
   if (c.scheme_vis != OCFD_SCHEME_CD6 && c.scheme_vis != OCFD_SCHEME_CD8) return fail("ocfd_init: Scheme_Vis must be CD6 or CD8");
   if (c.flux_splitting != OCFD_SPLIT_SW && c.flux_splitting != OCFD_SPLIT_LLF) return fail("ocfd_init: bad Flux_Splitting");
   switch (c.scheme_invis) {
@@ -1183,7 +1270,9 @@ int32_t ocfd_init(const ocfd_cfg* cfg) {
     if (c.scheme_boundary[a] < -1 || c.scheme_boundary[a] > 1) return fail("ocfd_init: Scheme_boundary must be -1, 0 or 1");
     if (c.ghost_cell[a] != 0) return fail("ocfd_init: Ghost_Cell /= 0 is not supported");
   }
-  if (c.ibc != OCFD_BC_NONE && c.ibc != OCFD_BC_BOUNDARYLAYER) return fail("ocfd_init: unsupported boundary condition");
+  if (c.ibc != OCFD_BC_NONE && c.ibc != OCFD_BC_BOUNDARYLAYER && c.ibc != OCFD_BC_JET)
+    return fail("ocfd_init: unsupported boundary condition");
+  if (c.ibc == OCFD_BC_JET && c.n_spec < 2) return fail("ocfd_init: the jet boundary condition needs N_SPEC > 1");
   if (c.scheme_invis == OCFD_SCHEME_HYBRID && std::lround(c.hybrid_para[3]) != 0)
     return fail("ocfd_init: hybrid patches (Hybrid_para(4) > 0) are not supported");
   if (c.wall_catalysis != 0) return fail("ocfd_init: only non-catalytic walls are supported");
@@ -1237,15 +1326,18 @@ int32_t ocfd_init(const ocfd_cfg* cfg) {
   S.bl_tw = c.bc_para[4];
   S.bl_xinit = c.bc_para[5];
   S.bl_disA = c.bc_para[7];
+  S.jet_iflag = c.ibc == OCFD_BC_JET ? (int)std::lround(c.bc_para[0]) : 0;
   if (c.ibc == OCFD_BC_BOUNDARYLAYER) {
     if (S.bl_inlet != 1) return fail("ocfd_init: boundary layer inlet must be 1-D data (BC_Para(1)=1)");
     if (std::lround(c.bc_para[1]) != 0) return fail("ocfd_init: boundary layer upper data must be 0-D (BC_Para(2)=0)");
   }
 
   if (hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking) != hipSuccess) return fail("ocfd_init: no HIP device/stream");
-  const long nh = S.nh, ni = S.ni;
+  const long nh = S.nh, ni = S.ni, ns = S.ns, nms = S.ns > 1 ? S.ns : 0;
   struct { double** p; long n; double v; } arrs[] = {
-      {&S.f, 5 * ni, 0.0}, {&S.fn, 5 * ni, 0.0}, {&S.du, 5 * ni, 0.0}, {&S.di, nh, 1.0}, {&S.din, ni, 1.0},
+      {&S.f, 5 * ni, 0.0}, {&S.fn, 5 * ni, 0.0}, {&S.du, 5 * ni, 0.0}, {&S.di, nh * ns, 1.0}, {&S.din, ni * ns, 1.0},
+      {&S.AmD, ni * nms, 0.0}, {&S.dus, ni * nms, 0.0}, {&S.Evs, 3 * ni * nms, 0.0}, {&S.Qp, 3 * ni, 0.0},
+      {(double**)&S.jet_in, (long)S.ny * S.nz * (5 + ns), 1.0},
       {&S.d, nh, 1.0}, {&S.u, nh, 1.0}, {&S.v, nh, 1.0}, {&S.w, nh, 1.0}, {&S.T, nh, 300.0}, {&S.p, nh, 1.0},
       {&S.Et, nh, 1.0}, {&S.Et_prev, ni, 1.0}, {&S.Amu, ni, 0.0}, {&S.Amk, ni, 0.0},
       {&S.Axx, nh, 1.0}, {&S.Ayy, nh, 1.0}, {&S.Azz, nh, 1.0},
@@ -1309,13 +1401,78 @@ int32_t ocfd_set_option(int32_t key, int32_t value) {
   return fail("ocfd_set_option: unknown key");
 }
 
+static int upload_gas() {
+  HIPCHK(hipStreamSynchronize(G.stream));  // kernels in flight read the bank
+  HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(fg), &G.hg, sizeof(FlowGas)));
+  drop_graph();
+  return 0;
+}
+
 int32_t ocfd_set_species(int32_t n_tct, const double* tct, double tct_max, const double* mi, const double* coef) {
-  (void)n_tct; (void)tct; (void)tct_max; (void)coef;
   if (check_ready()) return 1;
-  if (!mi || !(mi[0] > 0.0)) return fail("ocfd_set_species: bad molar mass");
+  const int ns = G.S.ns;
+  if (!mi) return fail("ocfd_set_species: null molar masses");
+  for (int k = 0; k < ns; k++)
+    if (!(mi[k] > 0.0)) return fail("ocfd_set_species: bad molar mass");
+  if (G.cfg.if_real_gas && (n_tct < 1 || n_tct > GAS_NT_MAX || !coef || (n_tct > 1 && !tct)))
+    return fail("ocfd_set_species: the real gas needs 1..4 temperature ranges and their coefficients");
   G.S.Mi0 = mi[0];
   G.S.Ri0 = 8.314 / mi[0];  // Sp%Ri = R0/Sp%Mi, CHEM_thermdyn.f90:41
-  drop_graph();
+  FlowGas& h = G.hg;
+  h.ns = ns;
+  h.real_gas = G.cfg.if_real_gas;
+  h.flag_vis = G.cfg.flag_vis;
+  h.flag_cond = G.cfg.flag_cond;
+  h.flag_diff = G.cfg.flag_diff;
+  h.n_tct = n_tct < 1 ? 1 : (n_tct > GAS_NT_MAX ? GAS_NT_MAX : n_tct);
+  for (int j = 0; j < GAS_NT_MAX; j++) h.tct[j] = (tct && j < h.n_tct - 1) ? tct[j] : 0.0;
+  h.tct_max = tct_max;
+  auto P3 = [](double x) { double x2 = x * x; return x2 * x; };
+  auto P4 = [](double x) { double x2 = x * x; return x2 * x2; };
+  auto P5 = [](double x) { double x2 = x * x; return (x2 * x2) * x; };
+  for (int k = 0; k < ns; k++) {
+    h.Mi[k] = mi[k];
+    h.Ri[k] = GAS_R0 / mi[k];
+    for (int j = 0; j < GAS_NT_MAX; j++)
+      for (int a = 0; a < 7; a++) h.coef[k][j][a] = (coef && j < h.n_tct) ? coef[((long)k * h.n_tct + j) * 7 + a] : 0.0;
+    // energy offsets that make the internal energy continuous across the
+    // range boundaries (CHEM_thermdyn.f90:42-46)
+    h.ect[k][0] = 0.0;
+    for (int j = 0; j + 1 < h.n_tct; j++) {
+      const double t = h.tct[j];
+      h.ect[k][j + 1] = -((h.coef[k][j + 1][0] - h.coef[k][j][0]) * t + (h.coef[k][j + 1][1] - h.coef[k][j][1]) * (t * t) / 2.0 +
+                          (h.coef[k][j + 1][2] - h.coef[k][j][2]) * P3(t) / 3.0 +
+                          (h.coef[k][j + 1][3] - h.coef[k][j][3]) * P4(t) / 4.0 +
+                          (h.coef[k][j + 1][4] - h.coef[k][j][4]) * P5(t) / 5.0) +
+                        h.ect[k][j];
+    }
+  }
+  return upload_gas();
+}
+
+int32_t ocfd_set_transport(const double* mu_an, const double* k_an, const double* D_an) {
+  if (check_ready()) return 1;
+  const int ns = G.S.ns;
+  FlowGas& h = G.hg;
+  if ((G.cfg.flag_vis && !mu_an) || (G.cfg.flag_cond && !k_an) || (G.cfg.flag_diff && !D_an))
+    return fail("ocfd_set_transport: a fitted coefficient set is missing");
+  for (int k = 0; k < ns; k++)
+    for (int a = 0; a < 4; a++) {
+      h.mu_an[k][a] = mu_an ? mu_an[4 * k + a] : 0.0;
+      h.k_an[k][a] = k_an ? k_an[4 * k + a] : 0.0;
+      for (int j = 0; j < ns; j++) h.D_an[k][j][a] = D_an ? D_an[4 * ((long)k * ns + j) + a] : 0.0;
+    }
+  return upload_gas();
+}
+
+int32_t ocfd_set_bc_jet(const double* inlet2d) {
+  if (check_ready()) return 1;
+  if (G.cfg.ibc != OCFD_BC_JET) return fail("ocfd_set_bc_jet: the run is not configured with the jet boundary condition");
+  if (!inlet2d) return fail("ocfd_set_bc_jet: null pointer");
+  const DevState& S = G.S;
+  HIPCHK(hipMemcpyAsync((void*)S.jet_in, inlet2d, sizeof(double) * (size_t)S.ny * S.nz * (5 + S.ns), hipMemcpyHostToDevice,
+                        G.stream));
+  HIPCHK(hipStreamSynchronize(G.stream));
   return 0;
 }
 
@@ -1438,14 +1595,19 @@ int32_t ocfd_save_stage0(void) {
 }
 int32_t ocfd_comput_amu_amk(void) {
   if (check_ready()) return 1;
-  launch("k_transport", k_transport, grid1(G.S.ni), dim3(256), G.S);
+  if (G.S.ns > 1) launch("k_transport_ms", k_transport_ms, grid1(G.S.ni), dim3(256), G.S);
+  else launch("k_transport", k_transport, grid1(G.S.ni), dim3(256), G.S);
   HIPCHK(hipGetLastError());
   return 0;
 }
 int32_t ocfd_stage_begin(int32_t krk) {
   if (check_ready()) return 1;
-  launch("k_transport", k_transport, grid1(G.S.ni), dim3(256), G.S);
-  launch("k_di_from_d", k_di_from_d, grid1(G.S.nh), dim3(256), G.S);
+  if (G.S.ns > 1) {
+    launch("k_transport_ms", k_transport_ms, grid1(G.S.ni), dim3(256), G.S);
+  } else {
+    launch("k_transport", k_transport, grid1(G.S.ni), dim3(256), G.S);
+    launch("k_di_from_d", k_di_from_d, grid1(G.S.nh), dim3(256), G.S);
+  }
   if (krk == 1 && G.cfg.scheme_invis == OCFD_SCHEME_HYBRID) enq_rhybrid();
   HIPCHK(hipGetLastError());
   return 0;
@@ -1467,7 +1629,8 @@ int32_t ocfd_du_inviscous(void) {
 }
 int32_t ocfd_du_viscous(void) {
   if (check_ready()) return 1;
-  launch("k_visc_prep", k_visc_prep, grid1(G.S.ni), dim3(256), G.S);
+  if (G.S.ns > 1) launch("k_visc_prep", k_visc_prep<true>, grid1(G.S.ni), dim3(256), G.S);
+  else launch("k_visc_prep", k_visc_prep<false>, grid1(G.S.ni), dim3(256), G.S);
   launch("k_rk", k_rk, grid1(G.S.ni), dim3(256), G.S, 1, 0.0, (int)(RK_VISC | RK_WRITE_DU));
   HIPCHK(hipGetLastError());
   return 0;
@@ -1482,7 +1645,7 @@ int32_t ocfd_time_adv_rk3(int32_t krk, double dt) {
 int32_t ocfd_comput_flow_variables(void) {
   if (check_ready()) return 1;
   launch("k_negt_reset", k_negt_reset, dim3(1), dim3(64), G.S, 0);
-  launch("k_rk", k_rk, grid1(G.S.ni), dim3(256), G.S, 1, 0.0, (int)RK_PRIMS);
+  launch("k_rk", k_rk, grid1(G.S.ni), dim3(256), G.S, 1, 0.0, (int)(G.S.ns > 1 ? RK_PRIMS | RK_MS : RK_PRIMS));
   launch("k_negt_repair", k_negt_repair, dim3(1), dim3(64), G.S);
   hipMemcpyAsync(G.h_negt, G.S.negt + 17, sizeof(int), hipMemcpyDeviceToHost, G.stream);
   HIPCHK(hipGetLastError());
@@ -1493,6 +1656,69 @@ int32_t ocfd_bc(void) {
   enq_bc();
   HIPCHK(hipGetLastError());
   return 0;
+}
+
+// ---- multi-species pieces ---------------------------------------------------
+int32_t ocfd_du_species(void) {
+  if (check_ready()) return 1;
+  if (G.S.ns < 2) return fail("ocfd_du_species: needs N_SPEC > 1");
+  enq_species(0, 0.0);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+int32_t ocfd_species_time_adv(int32_t krk, double dt) {
+  if (check_ready()) return 1;
+  if (G.S.ns < 2) return fail("ocfd_species_time_adv: needs N_SPEC > 1");
+  if (krk < 1 || krk > 3) return fail("ocfd_species_time_adv: KRK must be 1, 2 or 3");
+  enq_species(krk, dt);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+int32_t ocfd_comput_T_P(void) {
+  if (check_ready()) return 1;
+  if (G.S.ns < 2) return fail("ocfd_comput_T_P: needs N_SPEC > 1");
+  launch("k_comput_T_P", k_comput_T_P, grid1(G.S.ni), dim3(256), G.S);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// Chemical_time_advance (OCFD_Time_Adv.f90:169-196): the interior cells go
+// through the chemistry batch on this library's stream.  The species count
+// must be the chemistry build's (9).
+int32_t ocfd_chemical_time_advance(double dt2, int32_t mode) {
+  if (check_ready()) return 1;
+  const DevState& S = G.S;
+  if (S.ns != OCFD_NS_MAX) return fail("ocfd_chemical_time_advance: the chemistry batch is built for 9 species");
+  if (!G.chem_rho) {
+    G.chem_rho = dalloc(S.ni * S.ns);
+    G.chem_E = dalloc(S.ni);
+    G.chem_dt = dalloc(S.ni);
+    if (!G.chem_rho || !G.chem_E || !G.chem_dt) return fail("ocfd_chemical_time_advance: hipMalloc failed");
+  }
+  launch("k_chem_pack", k_chem_pack, grid1(S.ni), dim3(256), S, G.chem_rho, G.chem_E, G.chem_dt, dt2);
+  void* prev = ocfd_chem_stream();
+  if (ocfd_chem_set_stream((void*)G.stream)) return fail(std::string("ocfd_chemical_time_advance: ") + ocfd_chem_last_error());
+  const int rc = ocfd_chem_timeadv_batch(S.ni, G.chem_rho, G.chem_E, G.chem_dt, mode, 1);
+  ocfd_chem_set_stream(prev);
+  if (rc) return fail(std::string("ocfd_chemical_time_advance: ") + ocfd_chem_last_error());
+  launch("k_chem_unpack", k_chem_unpack, grid1(S.ni), dim3(256), S, (const double*)G.chem_rho, (const double*)G.chem_E);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// One time step of NS_solver (OCFD_NS_Solver.f90:26-37): flow dt/2, nsub
+// chemistry substeps of dt/nsub and comput_T_P, flow dt/2
+int32_t ocfd_ns_step(double dt, int32_t nsub, int32_t mode, int32_t reaction) {
+  if (check_ready()) return 1;
+  if (nsub < 1) return fail("ocfd_ns_step: nsub must be >= 1");
+  const double dt1 = 0.5 * dt, dt2 = dt / nsub;
+  if (ocfd_flow_time_advance(dt1)) return 1;
+  if (reaction) {
+    for (int m = 0; m < nsub; m++)
+      if (ocfd_chemical_time_advance(dt2, mode)) return 1;
+    if (ocfd_comput_T_P()) return 1;
+  }
+  return ocfd_flow_time_advance(dt1);
 }
 
 // Autotuner for the inviscid sweep tiles: times every tile variant of each

@@ -37,14 +37,16 @@ constexpr int TM = 64, TN = 64, TK = 32, LDT = TK + 8;  // LDS row: 40 bf16 = 80
 
 enum { EPI_STORE = 0, EPI_BIAS_GELU = 1, EPI_DGELU = 2 };
 
+// GELU in its tanh form, 0.5 z (1 + tanh u) with u = c (z + 0.044715 z^3),
+// evaluated as z * sigmoid(2u) = z / (1 + exp(-2u)) with the hardware exp
 __device__ __forceinline__ float gelu_f(float z) {
   const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
-  return 0.5f * z * (1.0f + tanhf(u));
+  return z / (1.0f + __expf(-2.0f * u));
 }
 __device__ __forceinline__ float dgelu_f(float z) {
   const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
-  const float t = tanhf(u);
-  return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * 0.7978845608028654f * (1.0f + 3.0f * 0.044715f * z * z);
+  const float s = 1.0f / (1.0f + __expf(-2.0f * u));
+  return s + z * s * (1.0f - s) * 2.0f * 0.7978845608028654f * (1.0f + 3.0f * 0.044715f * z * z);
 }
 
 template <bool TA, bool TB>

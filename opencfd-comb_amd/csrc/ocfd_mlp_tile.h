@@ -17,9 +17,10 @@ typedef float tfloatx4 __attribute__((ext_vector_type(4)));
 constexpr int MT_ROWS = 128;  // cells per workgroup
 constexpr int MT_LDH = 264;   // LDS row pitch (bf16): 256 + 8 against bank conflicts
 
+// GELU (tanh form) as z * sigmoid(2u), the same evaluation as training
 __device__ __forceinline__ float mt_gelu(float z) {
   const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
-  return 0.5f * z * (1.0f + tanhf(u));
+  return z / (1.0f + __expf(-2.0f * u));
 }
 
 // H[128][0:K] (LDS) -> act(H . W^T + b) written back into H[128][0:256].

@@ -50,6 +50,9 @@ def load(ocfd_module):
         "orc_recon_flux2": (i32, [i32, vp, vp, i32, vp, vp, vp]),
         "orc_set_ud7l_diss": (i32, [dbl]),
         "orc_split": (i32, [i32, i32] + [vp] * 12),
+        "orc_set_transport": (i32, [vp, vp, vp]), "orc_set_bc_jet": (i32, [vp]),
+        "orc_ns_step": (i32, [dbl, i32, i32, i32]), "orc_chemical_time_advance": (i32, [dbl, i32]),
+        "orc_comput_T_P": (i32, []), "orc_du_species": (i32, [vp]), "orc_download_qp": (i32, [vp]),
         "orc_comput_T": (dbl, [vp, dbl]), "orc_comput_E": (dbl, [vp, dbl]), "orc_comput_Cp": (dbl, [dbl, vp, dbl]),
     }
     for name, (res, args) in sig.items():
@@ -157,6 +160,32 @@ class Oracle:
     def sensor_mean(self):
         return float(self.lib.orc_sensor_mean())
 
+    def set_transport(self, mu_an=None, k_an=None, D_an=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (mu_an, k_an, D_an)]
+        self._check(self.lib.orc_set_transport(*[None if a is None else _ptr(a) for a in arrs]))
+
+    def set_bc_jet(self, inlet2d):
+        a = np.asfortranarray(inlet2d, dtype=np.float64)
+        assert a.shape == (self.ny, self.nz, 5 + self.cfg.n_spec)
+        self._check(self.lib.orc_set_bc_jet(_ptr(a)))
+
+    def NS_step(self, dt, nsub=1, mode=1, reaction=True):
+        self._check(self.lib.orc_ns_step(float(dt), int(nsub), int(mode), int(bool(reaction))))
+
+    def Chemical_time_advance(self, dt2, mode=1):
+        self._check(self.lib.orc_chemical_time_advance(float(dt2), int(mode)))
+
+    def comput_T_P(self):
+        self._check(self.lib.orc_comput_T_P())
+
+    def du_species(self):
+        """(DUS (nx, ny, nz, n_spec), QP (nx, ny, nz, 3)) of the current state."""
+        dus = np.zeros((self.nx, self.ny, self.nz, self.cfg.n_spec), order="F")
+        qp = np.zeros((self.nx, self.ny, self.nz, 3), order="F")
+        self._check(self.lib.orc_du_species(_ptr(dus)))
+        self._check(self.lib.orc_download_qp(_ptr(qp)))
+        return dus, qp
+
     def close(self):
         self.lib.orc_finalize()
 
@@ -166,7 +195,7 @@ def _field_shape(self, name):
         return (self.nx, self.ny, self.nz, 5)
     if name in ("AMU", "AMK"):
         return (self.nx, self.ny, self.nz)
-    if name == "DIN":
+    if name in ("DIN", "AMD"):
         return (self.nx, self.ny, self.nz, self.cfg.n_spec)
     if name == "DI":
         return self.halo_shape + (self.cfg.n_spec,)

@@ -59,6 +59,11 @@ static struct {
   double *Ev1, *Ev2, *Ev3;
   double *uk, *ui, *us, *vk, *vi, *vs, *wk, *wi, *ws, *Tk, *Ti, *Ts;
   double *Qpx, *Qpy, *Qpz;
+  double *dus;                       /* species RHS scratch (the reference's du(:,:,:,1)) */
+  /* transport fits (TRANSPORT_modules.f90): mu_an, k_an [k][4], D_an [k][j][4] */
+  double *mu_an, *k_an, *D_an;
+  /* jet BC: flow_inlet2d(j,k,m), m = d,u,v,w,T,di(1..N_SPEC) */
+  double *inlet2d;
   /* species */
   int n_tct;
   double tct[NTCT_MAX], tct_max;
@@ -765,38 +770,88 @@ static void du_inviscous(void) {
 }
 
 /* ------------------------------------------------------------------ */
+/* Chemkin log fit exp(a1 + a2 lnT + a3 lnT^2 + a4 lnT^3) (TRANSPORT_modules.f90:276-277) */
+static double chemkin_fit(const double* a, double lT) {
+  return exp(a[0] + a[1] * lT + a[2] * (lT * lT) + a[3] * pw3(lT));
+}
+
 /* comput_Amu_Amk_AMD, Comput_flow_variables.f90:69-98, with
- * comput_vis_coeff (TRANSPORT_modules.f90:259-306, Sutherland branch),
- * comput_thermal_conductivity (:312-327, Pr branch) and
- * comput_diffusion_coeff (:378-393, Sc branch). */
+ * comput_vis_coeff (TRANSPORT_modules.f90:259-306: Sutherland, or Chemkin
+ * fits + Wilke), comput_thermal_conductivity (:312-372: Pr, or fits with
+ * the 1/2 (sum X k + 1/sum X/k) mixing) and comput_diffusion_coeff
+ * (:378-424: Sc, or binary fits with (1-X_k)/sum_j X_j/D_kj atm/p). */
 static void comput_Amu_Amk_AMD(void) {
   const double TA = 288.15, TC = 110.4;
-  double* di0 = dalloc(G.nspec, 0);
-  double* Xi = dalloc(G.nspec, 0);
+  int ns = G.nspec;
+  double* di0 = dalloc(ns, 0);
+  double* Xi = dalloc(ns, 0);
+  double* mui = dalloc(ns, 0);
+  double* ki = dalloc(ns, 0);
+  double* D2 = dalloc((long)ns * ns, 0);
   for (int k = 1; k <= G.nz; k++)
     for (int j = 1; j <= G.ny; j++)
       for (int i = 1; i <= G.nx; i++) {
         long q = H(i, j, k), c = I(i, j, k);
         double X0 = 0.0;
-        for (int m = 0; m < G.nspec; m++) {
+        for (int m = 0; m < ns; m++) {
           di0[m] = G.di[q + m * G.nh];
           Xi[m] = di0[m] / G.Mi[m];
           X0 = X0 + Xi[m];
         }
-        for (int m = 0; m < G.nspec; m++) Xi[m] = Xi[m] / X0;
+        for (int m = 0; m < ns; m++) Xi[m] = Xi[m] / X0;
         double T = G.T[q];
-        double tr = T / TA;
-        double amu = G.cfg.amu0 * sqrt(pw3(tr)) * (TA + TC) / (TC + T);
+        double amu;
+        if (G.cfg.flag_vis == 0) {
+          double tr = T / TA;
+          amu = G.cfg.amu0 * sqrt(pw3(tr)) * (TA + TC) / (TC + T);
+        } else {
+          for (int m = 0; m < ns; m++) mui[m] = chemkin_fit(G.mu_an + 4 * m, log(T)) * 0.1;
+          double mu = 0.0;
+          for (int kk = 0; kk < ns; kk++) {
+            double Pk0 = 0.0;
+            for (int jj = 0; jj < ns; jj++) {
+              double a = 1.0 + sqrt(mui[kk] / mui[jj] * sqrt(G.Mi[jj] / G.Mi[kk]));
+              double Pkj = (a * a) / sqrt(8.0 * (1.0 + G.Mi[kk] / G.Mi[jj]));
+              Pk0 = Pk0 + Xi[jj] * Pkj;
+            }
+            mu = mu + Xi[kk] * mui[kk] / Pk0;
+          }
+          amu = mu;
+        }
         double Cp = comput_Cp(G.d[q], di0, T);
-        double amk = amu * Cp / G.cfg.pr;
+        double amk;
+        if (G.cfg.flag_cond == 0) {
+          amk = amu * Cp / G.cfg.pr;
+        } else {
+          for (int m = 0; m < ns; m++) ki[m] = chemkin_fit(G.k_an + 4 * m, log(T)) * 1.0e-5;
+          double Amk1 = 0.0, Amk2 = 0.0;
+          for (int m = 0; m < ns; m++) {
+            Amk1 = Amk1 + Xi[m] * ki[m];
+            Amk2 = Amk2 + Xi[m] / ki[m];
+          }
+          amk = 0.5 * (Amk1 + 1.0 / Amk2);
+        }
         G.Amu[c] = amu;
         G.Amk[c] = amk;
-        double d0 = 0.0;
-        for (int m = 0; m < G.nspec; m++) d0 = d0 + Xi[m] * G.Mi[m];
-        for (int m = 0; m < G.nspec; m++)
-          G.AmD[c + m * G.ni] = amu * (1.0 - Xi[m] * G.Mi[m] / d0) / ((1.0 - Xi[m]) * G.cfg.sc * G.d[q] + 1.0e-20);
+        if (G.cfg.flag_diff == 0) {
+          double d0 = 0.0;
+          for (int m = 0; m < ns; m++) d0 = d0 + Xi[m] * G.Mi[m];
+          for (int m = 0; m < ns; m++)
+            G.AmD[c + m * G.ni] = amu * (1.0 - Xi[m] * G.Mi[m] / d0) / ((1.0 - Xi[m]) * G.cfg.sc * G.d[q] + 1.0e-20);
+        } else {
+          for (int jj = 0; jj < ns; jj++)
+            for (int kk = 0; kk <= jj; kk++) D2[kk * ns + jj] = chemkin_fit(G.D_an + 4 * ((long)kk * ns + jj), log(T)) * 1.0e-4;
+          for (int jj = 0; jj < ns; jj++)
+            for (int kk = jj + 1; kk < ns; kk++) D2[kk * ns + jj] = D2[jj * ns + kk];
+          for (int kk = 0; kk < ns; kk++) {
+            double ds = 0.0;
+            for (int jj = 0; jj < ns; jj++)
+              if (jj != kk) ds = ds + Xi[jj] / D2[kk * ns + jj];
+            G.AmD[c + kk * G.ni] = (1.0 - Xi[kk]) / (ds + 1.0e-20) * ATM_PA / G.p[q];
+          }
+        }
       }
-  free(di0); free(Xi);
+  free(di0); free(Xi); free(mui); free(ki); free(D2);
 }
 
 /* du_viscous, OCFD_viscous.f90:27-131 */
@@ -860,6 +915,121 @@ static void du_viscous(void) {
           G.du[cm] = G.du[cm] + (G.uk[c] + G.ui[c] + G.us[c]) * G.Ajac[H(i, j, k)];
         }
   }
+}
+
+/* comput_hi_pgas (CHEM_thermdyn.f90:176-182), comput_hi_rgas (:271-293) */
+static double comput_hi(int k, double T) {
+  if (!G.cfg.if_real_gas) return 3.5 * G.Ri[k] * T;
+  int jc = 0;
+  for (int j = 0; j < G.n_tct - 1; j++) if (T > G.tct[j]) jc = j + 1;
+  double c0 = sp_c(k, jc, 0), c1 = sp_c(k, jc, 1), c2 = sp_c(k, jc, 2), c3 = sp_c(k, jc, 3), c4 = sp_c(k, jc, 4);
+  double ect = G.Ect[k * NTCT_MAX + jc];
+  if (T <= G.tct_max)
+    return (c0 * T + c1 * (T * T) / 2.0 + c2 * pw3(T) / 3.0 + c3 * pw4(T) / 4.0 + c4 * pw5(T) / 5.0 + ect) * G.Ri[k];
+  double tm = G.tct_max;
+  double Cp_max = c0 + c1 * tm + c2 * (tm * tm) + c3 * pw3(tm) + c4 * pw4(tm);
+  return (c0 * tm + c1 * (tm * tm) / 2.0 + c2 * pw3(tm) / 3.0 + c3 * pw4(tm) / 4.0 + c4 * pw5(tm) / 5.0 + ect +
+          Cp_max * (T - tm)) * G.Ri[k];
+}
+
+/* du_scaler_invis, OCFD_scaler_inviscous.f90:1-84: scalar Steger-Warming
+ * split F+- = (vs +- |vs|)/2 fs with vs = A1 u + A2 v + A3 w, the same
+ * reconstructions and closures as the flow, Dus = -(sum of differences) Ajac */
+static void du_scaler_invis(const double* fs, double* Dus) {
+  int nmax = G.nx > G.ny ? G.nx : G.ny;
+  if (G.nz > nmax) nmax = G.nz;
+  int lap = G.lap, npm = nmax + 2 * lap;
+  double *fp = dalloc(npm, 0), *fm = dalloc(npm, 0), *hh1 = dalloc(nmax + 1, 0), *hh2 = dalloc(nmax + 1, 0);
+  unsigned char* scm = (unsigned char*)calloc((size_t)nmax + 1, 1);
+  int hyb = G.cfg.scheme_invis == OCFD_SCHEME_HYBRID;
+  double hx_1 = 1.0 / G.hx, hy_1 = 1.0 / G.hy, hz_1 = 1.0 / G.hz;
+  for (int dir = 0; dir < 3; dir++) {
+    int n = dir == 0 ? G.nx : dir == 1 ? G.ny : G.nz;
+    long st = dir == 0 ? 1 : dir == 1 ? G.nxh : (long)G.nxh * G.nyh;
+    const double *M1 = dir == 0 ? G.Akx1 : dir == 1 ? G.Aix1 : G.Asx1;
+    const double *M2 = dir == 0 ? G.Aky1 : dir == 1 ? G.Aiy1 : G.Asy1;
+    const double *M3 = dir == 0 ? G.Akz1 : dir == 1 ? G.Aiz1 : G.Asz1;
+    const int* sb = &G.cfg.scheme_boundary[2 * dir];
+    int na = dir == 0 ? G.ny : G.nx, nb = dir == 2 ? G.ny : G.nz;
+    for (int b = 1; b <= nb; b++)
+      for (int a = 1; a <= na; a++) {
+        int i0 = dir == 0 ? 0 : a, j0 = dir == 0 ? a : dir == 1 ? 0 : b, k0 = dir == 2 ? 0 : b;
+        long base = H(i0, j0, k0);
+        if (hyb)
+          for (int s = 0; s <= n; s++) scm[s] = set_hybrid_scheme(fmax(G.Rhybrid[base + s * st], G.Rhybrid[base + (s + 1) * st]));
+        for (int s = 1 - lap; s <= n + lap; s++) {
+          long q = base + s * st;
+          int o = s + lap - 1;
+          double vs = M1[q] * G.u[q] + M2[q] * G.v[q] + M3[q] * G.w[q];
+          fp[o] = 0.5 * (vs + fabs(vs)) * fs[q];
+          fm[o] = 0.5 * (vs - fabs(vs)) * fs[q];
+        }
+        flux1(fp, hh1, n, lap, G.bound_index[dir], scm, sb);
+        flux2(fm, hh2, n, lap, G.bound_index[dir], scm, sb);
+        for (int s = 1; s <= n; s++) {
+          int i = dir == 0 ? s : a, j = dir == 1 ? s : (dir == 0 ? a : b), k = dir == 2 ? s : b;
+          long c = I(i, j, k);
+          if (dir == 0) Dus[c] = (hh1[s] - hh1[s - 1] + hh2[s] - hh2[s - 1]) * hx_1;
+          else if (dir == 1) Dus[c] = Dus[c] + (hh1[s] - hh1[s - 1] + hh2[s] - hh2[s - 1]) * hy_1;
+          else Dus[c] = -(Dus[c] + (hh1[s] - hh1[s - 1] + hh2[s] - hh2[s - 1]) * hz_1) * G.Ajac[H(i, j, k)];
+        }
+      }
+  }
+  free(fp); free(fm); free(hh1); free(hh2); free(scm);
+}
+
+/* du_scaler_viscous, OCFD_scaler_viscous.f90:4-79: diffusive flux
+ * rho D_k grad(c_k) of species k (c_k = fs/d), its enthalpy flux added to
+ * Qp (zeroed at the first species), Dus += div(flux) Ajac */
+static void du_scaler_viscous(int ks, const double* fs, double* Dus) {
+  long nh = G.nh;
+  if (ks == 0)
+    for (long c = 0; c < G.ni; c++) { G.Qpx[c] = 0.0; G.Qpy[c] = 0.0; G.Qpz[c] = 0.0; }
+  for (long q = 0; q < nh; q++) G.Ev1[q] = fs[q] / G.d[q];
+  orc_dx0(G.Ev1, G.uk);
+  orc_dy0(G.Ev1, G.ui);
+  orc_dz0(G.Ev1, G.us);
+  for (int k = 1; k <= G.nz; k++)
+    for (int j = 1; j <= G.ny; j++)
+      for (int i = 1; i <= G.nx; i++) {
+        long c = I(i, j, k), q = H(i, j, k);
+        double hi = comput_hi(ks, G.T[q]);
+        double cix = G.uk[c] * G.Akx[q] + G.ui[c] * G.Aix[q] + G.us[c] * G.Asx[q];
+        double ciy = G.uk[c] * G.Aky[q] + G.ui[c] * G.Aiy[q] + G.us[c] * G.Asy[q];
+        double ciz = G.uk[c] * G.Akz[q] + G.ui[c] * G.Aiz[q] + G.us[c] * G.Asz[q];
+        double dD = G.d[q] * G.AmD[c + ks * G.ni];
+        double E1 = dD * cix, E2 = dD * ciy, E3 = dD * ciz;
+        G.Qpx[c] = G.Qpx[c] + E1 * hi;
+        G.Qpy[c] = G.Qpy[c] + E2 * hi;
+        G.Qpz[c] = G.Qpz[c] + E3 * hi;
+        G.Ev1[q] = E1 * G.Akx1[q] + E2 * G.Aky1[q] + E3 * G.Akz1[q];
+        G.Ev1[q + nh] = E1 * G.Aix1[q] + E2 * G.Aiy1[q] + E3 * G.Aiz1[q];
+        G.Ev1[q + 2 * nh] = E1 * G.Asx1[q] + E2 * G.Asy1[q] + E3 * G.Asz1[q];
+      }
+  exchange_x(G.Ev1);
+  exchange_y(G.Ev1 + nh);
+  exchange_z(G.Ev1 + 2 * nh);
+  orc_dx0(G.Ev1, G.uk);
+  orc_dy0(G.Ev1 + nh, G.ui);
+  orc_dz0(G.Ev1 + 2 * nh, G.us);
+  for (int k = 1; k <= G.nz; k++)
+    for (int j = 1; j <= G.ny; j++)
+      for (int i = 1; i <= G.nx; i++) {
+        long c = I(i, j, k);
+        Dus[c] = Dus[c] + (G.uk[c] + G.ui[c] + G.us[c]) * G.Ajac[H(i, j, k)];
+      }
+}
+
+/* Scalar_time_adv_RK3, OCFD_Time_Adv.f90:134-166 */
+static void scalar_time_adv_rk3(int krk, int ks, const double* Dus, double dt) {
+  const double Ralfa[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0}, Rbeta[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
+  for (int k = 1; k <= G.nz; k++)
+    for (int j = 1; j <= G.ny; j++)
+      for (int i = 1; i <= G.nx; i++) {
+        long c = I(i, j, k), q = H(i, j, k) + (long)ks * G.nh, cn = c + (long)ks * G.ni;
+        if (krk == 1) G.di[q] = G.din[cn] + dt * Dus[c];
+        else G.di[q] = Ralfa[krk - 1] * G.din[cn] + Rbeta[krk - 1] * (G.di[q] + dt * Dus[c]);
+      }
 }
 
 /* OCFD_time_adv_RK3, OCFD_Time_Adv.f90:98-130 */
@@ -995,6 +1165,53 @@ static void bc_boundarylayer(void) {
   free(di1); free(di2); free(di3);
 }
 
+/* ocfd_bc_jet, OCFD_BC_jet.f90:2-96: 2-D inflow data on i=1 (Dirichlet,
+ * or with BC_Para(1)=1 the pressure taken from i=2 where the inflow is
+ * subsonic), zeroth-order outflow on i=nx */
+static void bc_jet(void) {
+  int ny = G.ny, nz = G.nz, ns = G.nspec;
+  long nh = G.nh, pl = (long)ny * nz;
+  int iflag = (int)lround(G.cfg.bc_para[0]);
+  double* di1 = dalloc(ns, 0);
+  double* ai1 = dalloc(ns, 0);
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++) {
+      long q = H(1, j, k), r = (j - 1) + (long)ny * (k - 1);
+      const double* in = G.inlet2d;
+      if (iflag == 0) {
+        G.d[q] = in[r]; G.u[q] = in[r + pl]; G.v[q] = in[r + 2 * pl]; G.w[q] = in[r + 3 * pl]; G.T[q] = in[r + 4 * pl];
+        for (int m = 0; m < ns; m++) G.di[q + m * nh] = in[r + (5L + m) * pl];
+      } else {
+        G.u[q] = in[r + pl]; G.v[q] = in[r + 2 * pl]; G.w[q] = in[r + 3 * pl]; G.T[q] = in[r + 4 * pl];
+        double d1 = in[r];
+        double d0 = 0.0;
+        for (int m = 0; m < ns; m++) { di1[m] = in[r + (5L + m) * pl]; d0 = d0 + di1[m]; }
+        for (int m = 0; m < ns; m++) ai1[m] = di1[m] / d0;
+        double p1 = comput_P(di1, G.T[q]);
+        double Et1 = comput_E(di1, G.T[q]);
+        double gamma = p1 / Et1 + 1.0;            /* comput_soundspeed, CHEM_thermdyn.f90:412-418 */
+        double c1 = sqrt(gamma * p1 / d1);
+        if (G.u[q] < c1) {
+          G.p[q] = G.p[H(2, j, k)];
+          double d1new = comput_d(ai1, G.T[q], G.p[q]);
+          G.d[q] = d1new;
+          for (int m = 0; m < ns; m++) G.di[q + m * nh] = d1new * ai1[m];
+        } else {
+          G.d[q] = d1;
+          for (int m = 0; m < ns; m++) G.di[q + m * nh] = di1[m];
+        }
+      }
+    }
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++) {
+      long q = H(G.nx, j, k), qa = H(G.nx - 1, j, k);
+      G.d[q] = G.d[qa]; G.u[q] = G.u[qa]; G.v[q] = G.v[qa]; G.w[q] = G.w[qa];
+      G.T[q] = G.T[qa]; G.p[q] = G.p[qa]; G.Et[q] = G.Et[qa];
+      for (int m = 0; m < ns; m++) G.di[q + m * nh] = G.di[qa + m * nh];
+    }
+  free(di1); free(ai1);
+}
+
 /* face re-synthesis of one face cell, OCFD_BC.f90:33-56 (same body per face) */
 static void face_update(int i, int j, int k, double* di1) {
   long q = H(i, j, k), c = I(i, j, k);
@@ -1015,7 +1232,10 @@ static void face_update(int i, int j, int k, double* di1) {
 /* OCFD_bc, OCFD_BC.f90:2-188 (one rank: every face is a rank face) */
 static int bc_all(void) {
   if (G.cfg.ibc == OCFD_BC_BOUNDARYLAYER) bc_boundarylayer();
-  else if (G.cfg.ibc != OCFD_BC_NONE) return set_err("oracle: unsupported boundary condition");
+  else if (G.cfg.ibc == OCFD_BC_JET) {
+    if (!G.inlet2d) return set_err("oracle: jet inflow data not set");
+    bc_jet();
+  } else if (G.cfg.ibc != OCFD_BC_NONE) return set_err("oracle: unsupported boundary condition");
   double* di1 = dalloc(G.nspec, 0);
   for (int k = 1; k <= G.nz; k++) for (int j = 1; j <= G.ny; j++) face_update(1, j, k, di1);
   for (int k = 1; k <= G.nz; k++) for (int j = 1; j <= G.ny; j++) face_update(G.nx, j, k, di1);
@@ -1071,26 +1291,90 @@ static int comput_rhybrid(int ordered) {
 }
 
 /* ------------------------------------------------------------------ */
-/* flow_time_advance, OCFD_Time_Adv.f90:7-92 (N_SPEC==1, non-characteristic) */
+/* flow_time_advance, OCFD_Time_Adv.f90:7-92 (non-characteristic) */
 int orc_flow_time_advance(double dt1) {
-  if (G.nspec != 1) return set_err("oracle: flow_time_advance restated for N_SPEC==1 only");
+  int ns = G.nspec;
   memcpy(G.fn, G.f, sizeof(double) * 5 * G.ni);
-  for (int k = 1; k <= G.nz; k++)
-    for (int j = 1; j <= G.ny; j++)
-      for (int i = 1; i <= G.nx; i++) G.din[I(i, j, k)] = G.di[H(i, j, k)];
+  for (int m = 0; m < ns; m++)
+    for (int k = 1; k <= G.nz; k++)
+      for (int j = 1; j <= G.ny; j++)
+        for (int i = 1; i <= G.nx; i++) G.din[I(i, j, k) + (long)m * G.ni] = G.di[H(i, j, k) + (long)m * G.nh];
   for (int krk = 1; krk <= 3; krk++) {
     comput_Amu_Amk_AMD();
     exchange_xyz(G.d); exchange_xyz(G.u); exchange_xyz(G.v); exchange_xyz(G.w);
     exchange_xyz(G.T); exchange_xyz(G.p); exchange_xyz(G.Et);
-    memcpy(G.di, G.d, sizeof(double) * G.nh);
+    if (ns == 1) memcpy(G.di, G.d, sizeof(double) * G.nh);
+    else for (int m = 0; m < ns; m++) exchange_xyz(G.di + (long)m * G.nh);
     if (krk == 1 && G.cfg.scheme_invis == OCFD_SCHEME_HYBRID)
       if (comput_rhybrid(1)) return 1;
+    if (ns != 1)
+      for (int m = 0; m < ns; m++) {
+        du_scaler_invis(G.di + (long)m * G.nh, G.dus);
+        if (G.cfg.if_viscous) du_scaler_viscous(m, G.di + (long)m * G.nh, G.dus);
+        scalar_time_adv_rk3(krk, m, G.dus, dt1);
+      }
     du_inviscous();
     if (G.cfg.if_viscous) du_viscous();
     time_adv_rk3(krk, dt1);
     if (comput_flow_variables()) return set_err("oracle: more than 10 negative-Et cells");
     if (bc_all()) return 1;
   }
+  return 0;
+}
+
+/* Chemical_time_advance, OCFD_Time_Adv.f90:169-196: chemical_timeAdv on every
+ * interior cell (mode = CHEM_TimeAdv), f5 += Et1 - Et0, Et = Et1.  The
+ * mechanism is the chemistry restatement's (orc_chem_set_mechanism). */
+int orc_chem_timeadv(int mode, double* di, double* E, double dt);
+int orc_chemical_time_advance(double dt2, int mode) {
+  int ns = G.nspec;
+  double* di1 = dalloc(ns, 0);
+  for (int k = 1; k <= G.nz; k++)
+    for (int j = 1; j <= G.ny; j++)
+      for (int i = 1; i <= G.nx; i++) {
+        long q = H(i, j, k), c = I(i, j, k);
+        for (int m = 0; m < ns; m++) di1[m] = G.di[q + (long)m * G.nh];
+        double Et1 = G.Et[q], Et0 = Et1;
+        if (orc_chem_timeadv(mode, di1, &Et1, dt2)) { free(di1); return set_err("oracle: chemical_timeAdv failed"); }
+        for (int m = 0; m < ns; m++) G.di[q + (long)m * G.nh] = di1[m];
+        G.f[c + 4 * G.ni] = G.f[c + 4 * G.ni] + (Et1 - Et0);
+        G.Et[q] = Et1;
+      }
+  free(di1);
+  return 0;
+}
+
+int orc_comput_T_P(void) { comput_T_P(); return 0; }
+
+/* one time step of NS_solver, OCFD_NS_Solver.f90:26-37: flow dt/2, M chemistry
+ * substeps of dt/M and comput_T_P, flow dt/2 */
+int orc_ns_step(double dt, int nsub, int mode, int reaction) {
+  double dt1 = 0.5 * dt, dt2 = dt / nsub;
+  if (orc_flow_time_advance(dt1)) return 1;
+  if (reaction) {
+    for (int m = 0; m < nsub; m++)
+      if (orc_chemical_time_advance(dt2, mode)) return 1;
+    comput_T_P();
+  }
+  return orc_flow_time_advance(dt1);
+}
+
+/* the species right-hand sides of one stage without the update (for
+ * comparisons of pieces): dus [N_SPEC][interior], and Qp after the last species */
+int orc_du_species(double* dus_out) {
+  for (int m = 0; m < G.nspec; m++) {
+    du_scaler_invis(G.di + (long)m * G.nh, G.dus);
+    if (G.cfg.if_viscous) du_scaler_viscous(m, G.di + (long)m * G.nh, G.dus);
+    memcpy(dus_out + (long)m * G.ni, G.dus, sizeof(double) * G.ni);
+  }
+  return 0;
+}
+
+/* Qpx, Qpy, Qpz after the last species of orc_du_species: [3][interior] */
+int orc_download_qp(double* out) {
+  memcpy(out, G.Qpx, sizeof(double) * G.ni);
+  memcpy(out + G.ni, G.Qpy, sizeof(double) * G.ni);
+  memcpy(out + 2 * G.ni, G.Qpz, sizeof(double) * G.ni);
   return 0;
 }
 
@@ -1189,7 +1473,8 @@ static void free_all(void) {
                      &G.Aiz, &G.Asx, &G.Asy, &G.Asz, &G.Ajac, &G.Akx1, &G.Aky1, &G.Akz1, &G.Aix1, &G.Aiy1,
                      &G.Aiz1, &G.Asx1, &G.Asy1, &G.Asz1, &G.Rhybrid, &G.Ev1, &G.Ev2, &G.Ev3, &G.uk, &G.ui,
                      &G.us, &G.vk, &G.vi, &G.vs, &G.wk, &G.wi, &G.ws, &G.Tk, &G.Ti, &G.Ts, &G.Qpx, &G.Qpy,
-                     &G.Qpz, &G.Mi, &G.Ri, &G.coef, &G.Ect, &G.bl_inlet1d, &G.bl_upper0d, &G.bl_wall_pertb};
+                     &G.Qpz, &G.Mi, &G.Ri, &G.coef, &G.Ect, &G.bl_inlet1d, &G.bl_upper0d, &G.bl_wall_pertb,
+                     &G.dus, &G.mu_an, &G.k_an, &G.D_an, &G.inlet2d};
   for (size_t a = 0; a < sizeof arrs / sizeof arrs[0]; a++) { free(*arrs[a]); *arrs[a] = NULL; }
   free(G.maskx); free(G.masky); free(G.maskz); free(G.bl_nonref);
   G.maskx = G.masky = G.maskz = NULL; G.bl_nonref = NULL;
@@ -1242,6 +1527,7 @@ int orc_init(const ocfd_cfg* cfg) {
   G.vi = dalloc(ni, 0); G.vs = dalloc(ni, 0); G.wk = dalloc(ni, 0); G.wi = dalloc(ni, 0);
   G.ws = dalloc(ni, 0); G.Tk = dalloc(ni, 0); G.Ti = dalloc(ni, 0); G.Ts = dalloc(ni, 0);
   G.Qpx = dalloc(ni, 0); G.Qpy = dalloc(ni, 0); G.Qpz = dalloc(ni, 0);
+  G.dus = dalloc(ni, 0);
   G.Mi = dalloc(G.nspec, 0.02897); G.Ri = dalloc(G.nspec, 0);
   for (int k = 0; k < G.nspec; k++) G.Ri[k] = R0_GAS / G.Mi[k];
   G.coef = dalloc((long)G.nspec * NTCT_MAX * 7, 0); G.Ect = dalloc((long)G.nspec * NTCT_MAX, 0);
@@ -1250,6 +1536,29 @@ int orc_init(const ocfd_cfg* cfg) {
   G.bl_nonref = (int*)calloc((size_t)G.nx * G.nz, sizeof(int));
   G.bl_ht = 1.0;
   G.err[0] = 0;
+  return 0;
+}
+
+/* Chemkin transport fits (read_vis/conduct/diffusion_coefficients,
+ * TRANSPORT_modules.f90:80-253): mu_an, k_an [N_SPEC][4], D_an
+ * [N_SPEC][N_SPEC][4]; any may be NULL when its flag is 0 */
+int orc_set_transport(const double* mu_an, const double* k_an, const double* D_an) {
+  int ns = G.nspec;
+  free(G.mu_an); free(G.k_an); free(G.D_an);
+  G.mu_an = dalloc(4L * ns, 0); G.k_an = dalloc(4L * ns, 0); G.D_an = dalloc(4L * ns * ns, 0);
+  if (mu_an) memcpy(G.mu_an, mu_an, sizeof(double) * 4 * ns);
+  if (k_an) memcpy(G.k_an, k_an, sizeof(double) * 4 * ns);
+  if (D_an) memcpy(G.D_an, D_an, sizeof(double) * 4 * ns * ns);
+  return 0;
+}
+
+/* jet inflow data flow_inlet2d(j,k,m) (init_bcdata_boundary with 2-D data):
+ * column-major (ny, nz, 5+N_SPEC), m = d, u, v, w, T, di(1..N_SPEC) */
+int orc_set_bc_jet(const double* inlet2d) {
+  long n = (long)G.ny * G.nz * (5 + G.nspec);
+  free(G.inlet2d);
+  G.inlet2d = dalloc(n, 0);
+  memcpy(G.inlet2d, inlet2d, sizeof(double) * n);
   return 0;
 }
 
@@ -1307,6 +1616,7 @@ static double* field_ptr(int id, long* n) {
     case OCFD_FIELD_AMU: *n = ni; return G.Amu;
     case OCFD_FIELD_AMK: *n = ni; return G.Amk;
     case OCFD_FIELD_RHYBRID: *n = nh; return G.Rhybrid;
+    case OCFD_FIELD_AMD: *n = ni * G.nspec; return G.AmD;
   }
   double* M[19] = {G.Akx, G.Aky, G.Akz, G.Aix, G.Aiy, G.Aiz, G.Asx, G.Asy, G.Asz, G.Ajac,
                    G.Akx1, G.Aky1, G.Akz1, G.Aix1, G.Aiy1, G.Aiz1, G.Asx1, G.Asy1, G.Asz1};

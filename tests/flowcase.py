@@ -66,3 +66,27 @@ def compare_state(gpu, orc, names=("F",) + PRIMS, tol=1e-12):
     bad = {k: v for k, v in rep.items() if not (v[0] <= tol)}
     assert not bad, f"parity failure (max rel diff, bit-identical fraction): {bad}"
     return rep
+
+
+def small_jet(nx=32, ny=24, nz=24, **kw):
+    load_package()
+    from ocfd_amd import cases
+    return cases.jet(nx, ny, nz, **kw)
+
+
+def setup_jet(engine, case):
+    """Species thermo/transport, jet inflow data, state, then the NS_solver
+    prologue (comput_flow_variables + OCFD_bc)."""
+    from ocfd_amd import cases
+    cfg, mesh, span, st, bc, info = case
+    engine.set_mesh(*mesh, span)
+    cases.jet_gas(engine, info["mech"])
+    if bc is not None and "inlet2d" in bc:
+        engine.set_bc_jet(bc["inlet2d"])
+    engine.upload_state(st)
+    engine.comput_flow_variables()
+    engine.OCFD_bc()
+    return engine
+
+
+SPECIES_PRIMS = PRIMS + ("DI",)

@@ -1,0 +1,159 @@
+"""GPU parity of the multi-species flow path (N_SPEC = 9, real gas, Chemkin
+fits with Wilke mixing, BC_JET) and of its coupling to the chemistry.
+
+Bar: <= 1e-12 relative (to each field's largest magnitude) per conserved
+variable, primitive and partial density after one RK3 step and after an
+NS_solver step with chemistry.  Differences come only from the device
+exp/log/sqrt used by the transport fits, the Wilke weights and the
+chemistry (a few ulps per call); the fraction of bit-identical values is
+reported.
+"""
+import numpy as np
+import pytest
+
+from conftest import load_package, gpu_available
+import flowcase as fc
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def pkg():
+    if not gpu_available():
+        pytest.skip("no GPU")
+    return load_package()
+
+
+def pair(pkg, case):
+    import oracle
+    cfg = case[0]
+    g = pkg.FlowSolver(cfg)
+    o = oracle.Oracle(pkg, cfg)
+    fc.setup_jet(g, case)
+    fc.setup_jet(o, case)
+    return g, o
+
+
+def compare_all(g, o, names=("F", "D", "U", "V", "W", "T", "P", "ET", "DI"), tol=TOL):
+    rep = {}
+    for name in names:
+        a, b = g.download(name), o.download(name)
+        if name in fc.PRIMS or name == "DI":
+            a, b = fc.interior(a), fc.interior(b)
+        if a.ndim == 4:
+            for m in range(a.shape[3]):
+                rep[f"{name}{m + 1}"] = fc.compare(a[..., m], b[..., m], name)
+        else:
+            rep[name] = fc.compare(a, b, name)
+    print({k: (f"{v[0]:.1e}", f"{v[1]:.3f}") for k, v in rep.items()})
+    bad = {k: v for k, v in rep.items() if not (v[0] <= tol)}
+    assert not bad, f"parity failure (max rel diff, bit-identical fraction): {bad}"
+    return rep
+
+
+def test_setup_state(pkg):
+    g, o = pair(pkg, fc.small_jet())
+    compare_all(g, o)
+    g.close()
+
+
+def test_transport_coefficients(pkg):
+    g, o = pair(pkg, fc.small_jet())
+    g.comput_Amu_Amk_AMD()
+    o.comput_Amu_Amk_AMD()
+    compare_all(g, o, names=("AMU", "AMK", "AMD"))
+    g.close()
+
+
+def test_species_right_hand_sides(pkg):
+    g, o = pair(pkg, fc.small_jet())
+    g.comput_Amu_Amk_AMD()
+    o.comput_Amu_Amk_AMD()
+    g.du_species()
+    dus, qp = o.du_species()
+    for name, a, b in (("DUS", g.download("DUS"), dus), ("QP", g.download("QP"), qp)):
+        for m in range(a.shape[3]):
+            d, same = fc.compare(a[..., m], b[..., m], name)
+            print(name, m, d, same)
+            assert d <= TOL, (name, m, d)
+    g.close()
+
+
+@pytest.mark.parametrize("iflag", [0, 1])
+def test_rk3_step(pkg, iflag):
+    case = fc.small_jet(iflag=iflag)
+    g, o = pair(pkg, case)
+    dt = case[5]["dt"]
+    g.flow_time_advance(dt)
+    o.flow_time_advance(dt)
+    compare_all(g, o)
+    assert g.negative_et_count() == o.negative_et_count()
+    g.close()
+
+
+def test_stage_pieces_equal_fused_step(pkg):
+    case = fc.small_jet()
+    dt = case[5]["dt"]
+    a = pkg.FlowSolver(case[0])          # one library context at a time
+    fc.setup_jet(a, case)
+    a.flow_time_advance(dt)
+    fa = {n: a.download(n) for n in ("F", "DI", "T", "P")}
+    a.close()
+    b = pkg.FlowSolver(case[0])
+    fc.setup_jet(b, case)
+    b.save_stage0()
+    for krk in (1, 2, 3):
+        b.stage_begin(krk)
+        b.Scalar_time_adv_RK3(krk, dt)
+        b.du_inviscous()
+        b.du_viscous()
+        b.OCFD_time_adv_RK3(krk, dt)
+        b.comput_flow_variables()
+        b.OCFD_bc()
+    for n, v in fa.items():
+        assert np.array_equal(v, b.download(n)), n
+    b.close()
+
+
+def test_perfect_gas_sutherland_mixture(pkg):
+    """N_SPEC = 9 with the perfect-gas thermo and Sutherland/Pr/Sc transport."""
+    case = list(fc.small_jet())
+    cfg = case[0]
+    cfg.if_real_gas = 0
+    cfg.flag_vis = cfg.flag_cond = cfg.flag_diff = 0
+    g, o = pair(pkg, tuple(case))
+    dt = case[5]["dt"]
+    for _ in range(2):
+        g.flow_time_advance(dt)
+        o.flow_time_advance(dt)
+    compare_all(g, o)
+    g.close()
+
+
+def test_three_steps_graph(pkg):
+    case = fc.small_jet()
+    g, o = pair(pkg, case)
+    g.set_option(pkg.OPT_USE_GRAPH, 1)
+    dt = case[5]["dt"]
+    for _ in range(3):
+        g.flow_time_advance(dt)
+        o.flow_time_advance(dt)
+    compare_all(g, o, tol=1e-11)
+    g.close()
+
+
+@pytest.mark.parametrize("mode", [1, -1])
+def test_ns_step_with_chemistry(pkg, mode):
+    import oracle
+    from ocfd_amd import chemistry
+    case = fc.small_jet()
+    mech = case[5]["mech"]
+    chemistry.ChemBatch(mech)            # mechanism into the HIP library
+    oracle.ChemOracle(pkg, mech)         # and into the oracle
+    g, o = pair(pkg, case)
+    dt = case[5]["dt"]
+    g.NS_step(dt, nsub=5, mode=mode)
+    o.NS_step(dt, nsub=5, mode=mode)
+    compare_all(g, o, tol=1e-11)
+    g.close()
