@@ -1237,6 +1237,9 @@ int32_t ocfd_finalize(void) {
   G.evpool.clear();
   for (void* p : G.allocs) hipFree(p);
   G.allocs.clear();
+  G.chem_rho = G.chem_E = G.chem_dt = nullptr;
+  G.part = nullptr;
+  G.hg = FlowGas{};
   if (G.h_negt) hipHostFree(G.h_negt);
   G.h_negt = nullptr;
   hipStreamDestroy(G.stream);
@@ -1350,7 +1353,7 @@ int32_t ocfd_init(const ocfd_cfg* cfg) {
   for (auto& a : arrs) {
     *a.p = dalloc(a.n);
     if (!*a.p) return fail("ocfd_init: hipMalloc failed");
-    hipLaunchKernelGGL(k_fill, grid1(a.n), dim3(256), 0, G.stream, *a.p, a.n, a.v);
+    if (a.n > 0) hipLaunchKernelGGL(k_fill, grid1(a.n), dim3(256), 0, G.stream, *a.p, a.n, a.v);
   }
   for (int d = 0; d < 3; d++) {
     const long nm = (long)(S.nx + (d == 0)) * (S.ny + (d == 1)) * (S.nz + (d == 2));

@@ -1389,6 +1389,7 @@ int orc_comput_rhybrid(void) { return comput_rhybrid(1); }
 int orc_exchange_all(void) {
   exchange_xyz(G.d); exchange_xyz(G.u); exchange_xyz(G.v); exchange_xyz(G.w);
   exchange_xyz(G.T); exchange_xyz(G.p); exchange_xyz(G.Et);
+  for (int m = 0; m < G.nspec; m++) exchange_xyz(G.di + (long)m * G.nh);
   return 0;
 }
 int orc_negative_et_count(void) { return G.num_negt; }

@@ -70,6 +70,7 @@ def test_species_right_hand_sides(pkg):
     g, o = pair(pkg, fc.small_jet())
     g.comput_Amu_Amk_AMD()
     o.comput_Amu_Amk_AMD()
+    o.exchange_all()
     g.du_species()
     dus, qp = o.du_species()
     for name, a, b in (("DUS", g.download("DUS"), dus), ("QP", g.download("QP"), qp)):
