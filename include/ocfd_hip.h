@@ -188,6 +188,16 @@ int32_t ocfd_flow_time_advance(double dt1);
 int32_t ocfd_ns_step(double dt, int32_t nsub, int32_t mode, int32_t reaction);
 /* Chemical_time_advance (OCFD_Time_Adv.f90:169-196) on every interior cell. */
 int32_t ocfd_chemical_time_advance(double dt2, int32_t mode);
+/* Integrator of ocfd_chemical_time_advance / ocfd_ns_step (no reference
+ * counterpart beyond OCFD_CHEM_FIXED): OCFD_CHEM_FIXED = chemical_timeAdv
+ * with CHEM_TimeAdv = mode (default); OCFD_CHEM_ADAPTIVE = the adaptive
+ * point-implicit controller of ocfd_chem_adaptive_batch with these
+ * controls and scheduler (mode ignored); OCFD_CHEM_SURROGATE =
+ * ocfd_chem_surrogate_batch with the adaptive integrator as fallback
+ * (needs a configured surrogate network). */
+enum { OCFD_CHEM_FIXED = 0, OCFD_CHEM_ADAPTIVE = 1, OCFD_CHEM_SURROGATE = 2 };
+int32_t ocfd_set_chem_method(int32_t method, double rtol_y, double y_floor, double rtol_t, int32_t max_sub,
+                             int32_t scheduler);
 /* comput_T_P (code/flow-solver/Comput_flow_variables.f90:32-46), N_SPEC > 1. */
 int32_t ocfd_comput_T_P(void);
 /* Species pieces (N_SPEC > 1): the right-hand sides du_scaler_invis +

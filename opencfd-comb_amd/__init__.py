@@ -40,6 +40,7 @@ FIELD = {name: i for i, name in enumerate(FIELDS)}
 METRICS = FIELDS[FIELD["AKX"]:FIELD["ASZ1"] + 1]
 
 OPT_HYBRID_SUM_MODE, OPT_USE_GRAPH, OPT_KERNEL_TIMING = 1, 2, 3
+CHEM_FIXED, CHEM_ADAPTIVE, CHEM_SURROGATE = 0, 1, 2
 
 
 class OcfdCfg(ctypes.Structure):
@@ -129,6 +130,7 @@ def load_library(path=None):
         "ocfd_set_transport": (i32, [vp, vp, vp]), "ocfd_set_bc_jet": (i32, [vp]),
         "ocfd_ns_step": (i32, [dbl, i32, i32, i32]), "ocfd_chemical_time_advance": (i32, [dbl, i32]),
         "ocfd_comput_T_P": (i32, []), "ocfd_du_species": (i32, []), "ocfd_species_time_adv": (i32, [i32, dbl]),
+        "ocfd_set_chem_method": (i32, [i32, dbl, dbl, dbl, i32, i32]),
         "ocfd_chem_set_mechanism": (i32, [i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, i32, vp, dbl, vp, vp]),
         "ocfd_chem_timeadv_batch": (i32, [ctypes.c_int64, vp, vp, vp, i32, i32]),
         "ocfd_chem_adaptive_batch": (i32, [ctypes.c_int64, vp, vp, vp, dbl, dbl, dbl, i32, vp, i32]),
@@ -268,6 +270,12 @@ class FlowSolver:
     def NS_step(self, dt, nsub=1, mode=1, reaction=True):
         """One time step of NS_solver (OCFD_NS_Solver.f90:26-37)."""
         self._check(self.lib.ocfd_ns_step(float(dt), int(nsub), int(mode), int(bool(reaction))))
+
+    def set_chem_method(self, method, rtol_y=0.05, y_floor=1e-6, rtol_t=0.01, max_sub=4000, scheduler=1):
+        """CHEM_FIXED (chemical_timeAdv, CHEM_TimeAdv = mode), CHEM_ADAPTIVE or
+        CHEM_SURROGATE (with the adaptive integrator as fallback)."""
+        self._check(self.lib.ocfd_set_chem_method(int(method), float(rtol_y), float(y_floor), float(rtol_t),
+                                                  int(max_sub), int(scheduler)))
 
     def Chemical_time_advance(self, dt2, mode=1):
         self._check(self.lib.ocfd_chemical_time_advance(float(dt2), int(mode)))

@@ -966,6 +966,9 @@ struct Ctx {
   int* h_negt = nullptr;  // pinned copy of the negative-Et counter
   FlowGas hg{};           // host copy of the species constant bank
   double *chem_rho = nullptr, *chem_E = nullptr, *chem_dt = nullptr;  // chemistry batch buffers
+  int chem_method = OCFD_CHEM_FIXED;
+  double chem_rtol_y = 0.05, chem_y_floor = 1e-6, chem_rtol_t = 0.01;
+  int chem_max_sub = 4000, chem_scheduler = 1;
   std::vector<std::string> knames;
   std::map<std::string, int> kindex;
   std::vector<KStat> kstats;
@@ -1240,6 +1243,7 @@ int32_t ocfd_finalize(void) {
   G.chem_rho = G.chem_E = G.chem_dt = nullptr;
   G.part = nullptr;
   G.hg = FlowGas{};
+  G.chem_method = OCFD_CHEM_FIXED;
   if (G.h_negt) hipHostFree(G.h_negt);
   G.h_negt = nullptr;
   hipStreamDestroy(G.stream);
@@ -1701,11 +1705,34 @@ int32_t ocfd_chemical_time_advance(double dt2, int32_t mode) {
   launch("k_chem_pack", k_chem_pack, grid1(S.ni), dim3(256), S, G.chem_rho, G.chem_E, G.chem_dt, dt2);
   void* prev = ocfd_chem_stream();
   if (ocfd_chem_set_stream((void*)G.stream)) return fail(std::string("ocfd_chemical_time_advance: ") + ocfd_chem_last_error());
-  const int rc = ocfd_chem_timeadv_batch(S.ni, G.chem_rho, G.chem_E, G.chem_dt, mode, 1);
+  int rc;
+  if (G.chem_method == OCFD_CHEM_ADAPTIVE)
+    rc = ocfd_chem_adaptive_batch(S.ni, G.chem_rho, G.chem_E, G.chem_dt, G.chem_rtol_y, G.chem_y_floor, G.chem_rtol_t,
+                                  G.chem_max_sub, nullptr, G.chem_scheduler);
+  else if (G.chem_method == OCFD_CHEM_SURROGATE)
+    rc = ocfd_chem_surrogate_batch(S.ni, G.chem_rho, G.chem_E, G.chem_dt, 1, G.chem_rtol_y, G.chem_y_floor, G.chem_rtol_t,
+                                   G.chem_max_sub, nullptr);
+  else
+    rc = ocfd_chem_timeadv_batch(S.ni, G.chem_rho, G.chem_E, G.chem_dt, mode, 1);
   ocfd_chem_set_stream(prev);
   if (rc) return fail(std::string("ocfd_chemical_time_advance: ") + ocfd_chem_last_error());
   launch("k_chem_unpack", k_chem_unpack, grid1(S.ni), dim3(256), S, (const double*)G.chem_rho, (const double*)G.chem_E);
   HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int32_t ocfd_set_chem_method(int32_t method, double rtol_y, double y_floor, double rtol_t, int32_t max_sub,
+                             int32_t scheduler) {
+  if (method < OCFD_CHEM_FIXED || method > OCFD_CHEM_SURROGATE) return fail("ocfd_set_chem_method: unknown method");
+  if (method != OCFD_CHEM_FIXED &&
+      (max_sub < 1 || !(rtol_y > 0.0) || !(y_floor >= 0.0) || !(rtol_t > 0.0) || scheduler < 0 || scheduler > 2))
+    return fail("ocfd_set_chem_method: bad controller parameters");
+  G.chem_method = method;
+  G.chem_rtol_y = rtol_y;
+  G.chem_y_floor = y_floor;
+  G.chem_rtol_t = rtol_t;
+  G.chem_max_sub = max_sub;
+  G.chem_scheduler = scheduler;
   return 0;
 }
 

@@ -158,3 +158,41 @@ def test_ns_step_with_chemistry(pkg, mode):
     o.NS_step(dt, nsub=5, mode=mode)
     compare_all(g, o, tol=1e-11)
     g.close()
+
+
+def test_ns_step_adaptive_chemistry(pkg):
+    """NS_solver step with the adaptive point-implicit integrator in the
+    loop against the oracle composed from its pieces: flow dt/2, the adaptive
+    batch on the interior cells (f5 += dE), comput_T_P, flow dt/2."""
+    import oracle
+    from ocfd_amd import chemistry
+    case = fc.small_jet()
+    mech = case[5]["mech"]
+    chemistry.ChemBatch(mech)
+    co = oracle.ChemOracle(pkg, mech)
+    g, o = pair(pkg, case)
+    dt = case[5]["dt"] * 2.0             # flow halves at the case's CFL
+    ctl = (chemistry.RTOL_Y, chemistry.Y_FLOOR, chemistry.RTOL_T, 4000)
+    g.set_chem_method(pkg.CHEM_ADAPTIVE, *ctl, scheduler=2)
+    g.NS_step(dt, nsub=1)
+    o.flow_time_advance(0.5 * dt)
+    di = o.download("DI")
+    et = o.download("ET")
+    f = o.download("F")
+    ns = case[0].n_spec
+    rho = np.stack([fc.interior(di[..., m]).reshape(-1, order="F") for m in range(ns)], axis=1)
+    e0 = fc.interior(et).reshape(-1, order="F")
+    r1, e1, nsub = co.adaptive_batch(rho, e0, np.full(e0.shape, dt), *ctl)
+    print("substeps per cell: max", nsub.max(), "mean", nsub.mean())
+    shp = fc.interior(et).shape
+    for m in range(ns):
+        fc.interior(di[..., m])[...] = r1[:, m].reshape(shp, order="F")
+    f[..., 4] = f[..., 4] + (e1 - e0).reshape(shp, order="F")
+    fc.interior(et)[...] = e1.reshape(shp, order="F")
+    o.upload("DI", di)
+    o.upload("ET", et)
+    o.upload("F", f)
+    o.comput_T_P()
+    o.flow_time_advance(0.5 * dt)
+    compare_all(g, o, tol=1e-10)
+    g.close()
