@@ -423,3 +423,100 @@ def jet_gas(engine, mech):
     the oracle): read_Spec and read_vis/conduct/diffusion_coefficients."""
     engine.set_species(mech.mi, n_tct=mech.n_tct, tct=mech.tct, tct_max=mech.tct_max, coef=mech.coef)
     engine.set_transport(mech.viscosity, mech.conductivity, mech.diffusion)
+
+
+def planar_jet(nx=256, ny=192, nz=128, seed=5, lx=0.024, ly=0.018, lz=0.012, width=0.002, delta=0.0002,
+               noise=0.05, kmax=8, ignite=True, T_ign=1800.0, x_ign=0.25, mech=None, cfl=0.5, iflag=1):
+    """Reacting planar H2 jet into a hot air coflow on a uniform box
+    [0, lx] x [-ly/2, ly/2] x [0, lz) (span periodic).
+
+    Jet: X_H2 = 0.10, X_N2 = 0.90 at 300 K, 300 m/s, |y| < width/2; coflow:
+    air at 1100 K, 20 m/s; 1 atm.  The streams blend with a tanh profile of
+    thickness delta.  The inlet velocity carries seeded broadband noise:
+    random-phase Fourier modes in (y, z) up to kmax per direction, flat
+    spectrum, scaled to an RMS of `noise` x 300 m/s inside the jet, on u, v
+    and w.  The initial field repeats the inflow plane downstream; with
+    `ignite`, the mixing layers (blend weight 0.1 .. 0.9) for x > x_ign lx
+    start at T_ign as an igniter.  WENO5, Steger-Warming, CD6, real gas,
+    Chemkin/Wilke transport with mixture-averaged diffusion, BC_JET with
+    BC_Para(1) = iflag, periodic span.
+
+    Returns cfg, mesh, span, state, bc (inlet2d), info (dt, mech)."""
+    from .mechanism import Mechanism
+    from . import BC_JET, SCHEME_WENO5
+    mech = mech or Mechanism()
+    ns = mech.n_spec
+    xf = np.zeros(ns)
+    xo = np.zeros(ns)
+    xf[mech.index("H2")], xf[mech.index("N2")] = 0.10, 0.90
+    xo[mech.index("O2")], xo[mech.index("N2")] = 0.21, 0.79
+    u1, T1, u2, T2, p0 = 300.0, 300.0, 20.0, 1100.0, 101300.0
+    x = np.linspace(0.0, lx, nx)
+    y = np.linspace(-0.5 * ly, 0.5 * ly, ny)
+    z = np.arange(nz) * (lz / nz)
+    s = 0.5 * (1.0 - np.tanh((np.abs(y) - 0.5 * width) / delta))          # 1 in the jet
+    Y2, Z2 = np.meshgrid(y, z, indexing="ij")
+    S2 = np.broadcast_to(s[:, None], Y2.shape)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    pert = []
+    for _ in range(3):
+        f = np.zeros(Y2.shape)
+        for ky in range(1, kmax + 1):
+            for kz in range(0, kmax + 1):
+                ph = rng.uniform(0.0, 2.0 * math.pi, 2)
+                f += np.sin(2 * math.pi * ky * (Y2 / ly) + ph[0]) * np.cos(2 * math.pi * kz * (Z2 / lz) + ph[1])
+        pert.append(f)
+    inj = S2 > 0.5
+    rms = math.sqrt(np.mean(np.concatenate([p[inj] for p in pert]) ** 2))
+    pert = [noise * u1 * p / rms * S2 for p in pert]
+    Ri = R0 / mech.mi
+    T_in = T2 + (T1 - T2) * S2
+    X_in = xo[:, None, None] + (xf - xo)[:, None, None] * S2[None]
+    di_in = p0 * X_in / (Ri[:, None, None] * T_in[None])
+    u_in = u2 + (u1 - u2) * S2 + pert[0]
+    v_in, w_in = pert[1], pert[2]
+    T = np.broadcast_to(T_in[None], (nx, ny, nz)).copy()
+    if ignite:
+        band = (S2 > 0.1) & (S2 < 0.9)
+        T[(x > x_ign * lx)[:, None, None] & band[None]] = T_ign
+    # constant pressure: partial densities from the local temperature
+    di = p0 * np.broadcast_to(X_in[:, None], (ns, nx, ny, nz)) / (Ri[:, None, None, None] * T[None])
+    u = np.broadcast_to(u_in[None], (nx, ny, nz)).copy()
+    v = np.broadcast_to(v_in[None], (nx, ny, nz)).copy()
+    w = np.broadcast_to(w_in[None], (nx, ny, nz)).copy()
+    d = di.sum(axis=0)
+    st = {}
+    for name, val, src in (("D", 1.0, d), ("U", 1.0, u), ("V", 1.0, v), ("W", 1.0, w), ("T", 300.0, T)):
+        a = halo_zeros(nx, ny, nz, val)
+        interior(a)[...] = src
+        st[name] = a
+    dih = np.ones((nx + 2 * LAP, ny + 2 * LAP, nz + 2 * LAP, ns), order="F")
+    for m in range(ns):
+        interior(dih[..., m])[...] = di[m]
+    st["DI"] = dih
+    E = mech.energy(di.reshape(ns, -1), T.reshape(-1)).reshape(d.shape)
+    f = np.zeros((nx, ny, nz, 5), order="F")
+    f[..., 0] = d
+    f[..., 1] = d * u
+    f[..., 2] = d * v
+    f[..., 3] = d * w
+    f[..., 4] = E + d * (u * u + v * v + w * w) * 0.5
+    st["F"] = f
+    st["P"] = halo_zeros(nx, ny, nz, 1.0)
+    st["ET"] = halo_zeros(nx, ny, nz, 1.0)
+    inlet = np.zeros((ny, nz, 5 + ns), order="F")
+    inlet[..., 0] = di_in.sum(axis=0)
+    inlet[..., 1], inlet[..., 2], inlet[..., 3], inlet[..., 4] = u_in, v_in, w_in, T_in
+    for m in range(ns):
+        inlet[..., 5 + m] = di_in[m]
+    mesh = mesh_1d(x, y, z)
+    span = np.array([0, 0, 0, 0, 0, 0, 0, 0, lz], dtype=np.float64)
+    cfg = make_cfg(nx, ny, nz, periodic=(0, 0, 1), scheme_invis=SCHEME_WENO5, scheme_vis=SCHEME_CD6,
+                   flux_splitting=SPLIT_SW, if_viscous=1, ibc=BC_JET, bc_para=[iflag], n_spec=ns, if_real_gas=1,
+                   flag_vis=1, flag_cond=1, flag_diff=1)
+    p = (di * (Ri[:, None, None, None] * T[None])).sum(axis=0)
+    c = np.sqrt((p / E + 1.0) * p / d)
+    lam = (np.abs(u) + c) / (lx / (nx - 1)) + (np.abs(v) + c) / (ly / (ny - 1)) + (np.abs(w) + c) / (lz / nz)
+    dt = cfl / float(np.max(lam))
+    info = dict(name="planar_jet", dt=dt, mech=mech)
+    return cfg, mesh, span, st, dict(inlet2d=inlet), info
