@@ -188,6 +188,13 @@ int32_t ocfd_flow_time_advance(double dt1);
 int32_t ocfd_ns_step(double dt, int32_t nsub, int32_t mode, int32_t reaction);
 /* Chemical_time_advance (OCFD_Time_Adv.f90:169-196) on every interior cell. */
 int32_t ocfd_chemical_time_advance(double dt2, int32_t mode);
+/* CFL time step (no reference counterpart: the reference reads dt from the
+ * namelist, OCFD_NS_Solver.f90:20): dt = cfl / max over interior cells of
+ * sum_dir (|A_dir . u| + c |A_dir|) / h_dir, with A_dir the metric rows
+ * (Akx.., Aix.., Asx..), c = sqrt(gamma p/d), gamma = p/Et + 1 and
+ * h_dir = 1/(n_dir - 1).  Deterministic (a maximum); synchronises. */
+int32_t ocfd_cfl_dt(double cfl, double* dt);
+
 /* Integrator of ocfd_chemical_time_advance / ocfd_ns_step (no reference
  * counterpart beyond OCFD_CHEM_FIXED): OCFD_CHEM_FIXED = chemical_timeAdv
  * with CHEM_TimeAdv = mode (default); OCFD_CHEM_ADAPTIVE = the adaptive

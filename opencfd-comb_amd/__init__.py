@@ -130,7 +130,7 @@ def load_library(path=None):
         "ocfd_set_transport": (i32, [vp, vp, vp]), "ocfd_set_bc_jet": (i32, [vp]),
         "ocfd_ns_step": (i32, [dbl, i32, i32, i32]), "ocfd_chemical_time_advance": (i32, [dbl, i32]),
         "ocfd_comput_T_P": (i32, []), "ocfd_du_species": (i32, []), "ocfd_species_time_adv": (i32, [i32, dbl]),
-        "ocfd_set_chem_method": (i32, [i32, dbl, dbl, dbl, i32, i32]),
+        "ocfd_set_chem_method": (i32, [i32, dbl, dbl, dbl, i32, i32]), "ocfd_cfl_dt": (i32, [dbl, P(dbl)]),
         "ocfd_chem_set_mechanism": (i32, [i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, i32, vp, dbl, vp, vp]),
         "ocfd_chem_timeadv_batch": (i32, [ctypes.c_int64, vp, vp, vp, i32, i32]),
         "ocfd_chem_adaptive_batch": (i32, [ctypes.c_int64, vp, vp, vp, dbl, dbl, dbl, i32, vp, i32]),
@@ -270,6 +270,12 @@ class FlowSolver:
     def NS_step(self, dt, nsub=1, mode=1, reaction=True):
         """One time step of NS_solver (OCFD_NS_Solver.f90:26-37)."""
         self._check(self.lib.ocfd_ns_step(float(dt), int(nsub), int(mode), int(bool(reaction))))
+
+    def cfl_dt(self, cfl):
+        """dt = cfl / max sum_dir (|u_n| + c |grad xi|) / h (device reduction)."""
+        out = ctypes.c_double(0.0)
+        self._check(self.lib.ocfd_cfl_dt(float(cfl), ctypes.byref(out)))
+        return out.value
 
     def set_chem_method(self, method, rtol_y=0.05, y_floor=1e-6, rtol_t=0.01, max_sub=4000, scheduler=1):
         """CHEM_FIXED (chemical_timeAdv, CHEM_TimeAdv = mode), CHEM_ADAPTIVE or

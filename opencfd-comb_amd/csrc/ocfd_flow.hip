@@ -823,6 +823,50 @@ __global__ __launch_bounds__(256) void k_sum_final(DevState S, const double* par
   }
 }
 
+// CFL time step (no reference counterpart; the reference takes dt from the
+// namelist): lambda = sum over xi, eta, zeta of (|A1 u + A2 v + A3 w| +
+// c |(A1, A2, A3)|) / h with c = sqrt(gamma p / d), gamma = p/Et + 1
+// (comput_cc, Comput_flow_variables.f90:50-63); dt = CFL / max lambda.  The
+// maximum is order independent, so the result does not depend on the grid.
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__global__ __launch_bounds__(256) void k_cfl_partial(DevState S, double* part) {
+  __shared__ double sm[4];
+  double acc = 0.0;
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < S.ni; c += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(c % S.nx) + 1;
+    const int j = (int)((c / S.nx) % S.ny) + 1;
+    const int k = (int)(c / ((long)S.nx * S.ny)) + 1;
+    const long q = hq(S, i, j, k);
+    const double uu = S.u[q], vv = S.v[q], ww = S.w[q];
+    const double cc = sqrt((S.p[q] / S.Et[q] + 1.0) * S.p[q] / S.d[q]);
+    const double a[3][3] = {{S.Akx[q], S.Aky[q], S.Akz[q]}, {S.Aix[q], S.Aiy[q], S.Aiz[q]}, {S.Asx[q], S.Asy[q], S.Asz[q]}};
+    double lam = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const double un = a[d][0] * uu + a[d][1] * vv + a[d][2] * ww;
+      const double g = sqrt(a[d][0] * a[d][0] + a[d][1] * a[d][1] + a[d][2] * a[d][2]);
+      lam = lam + (fabs(un) + cc * g) * S.h_1[d];
+    }
+    acc = fmax(acc, lam);
+  }
+  acc = wave_max(acc);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
+}
+__global__ __launch_bounds__(64) void k_cfl_final(const double* part, int nparts, double cfl, double* out) {
+  double acc = 0.0;
+  for (int a = threadIdx.x; a < nparts; a += 64) acc = fmax(acc, part[a]);
+  acc = wave_max(acc);
+  if (threadIdx.x == 0) {
+    out[0] = cfl / acc;
+    out[1] = acc;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_sensor_norm(DevState S) {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= S.ni) return;
@@ -1718,6 +1762,20 @@ int32_t ocfd_chemical_time_advance(double dt2, int32_t mode) {
   if (rc) return fail(std::string("ocfd_chemical_time_advance: ") + ocfd_chem_last_error());
   launch("k_chem_unpack", k_chem_unpack, grid1(S.ni), dim3(256), S, (const double*)G.chem_rho, (const double*)G.chem_E);
   HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int32_t ocfd_cfl_dt(double cfl, double* dt) {
+  if (check_ready()) return 1;
+  if (!(cfl > 0.0) || !dt) return fail("ocfd_cfl_dt: CFL must be positive and dt non-null");
+  const DevState& S = G.S;
+  launch("k_cfl_partial", k_cfl_partial, dim3(G.nparts), dim3(256), S, G.part);
+  launch("k_cfl_final", k_cfl_final, dim3(1), dim3(64), (const double*)G.part, G.nparts, cfl, S.red + 2);
+  double h[2];
+  HIPCHK(hipMemcpyAsync(h, S.red + 2, sizeof(h), hipMemcpyDeviceToHost, G.stream));
+  HIPCHK(hipStreamSynchronize(G.stream));
+  if (!(h[1] > 0.0) || !std::isfinite(h[1])) return fail("ocfd_cfl_dt: no finite positive wave speed (NaN in the state?)");
+  *dt = h[0];
   return 0;
 }
 

@@ -226,3 +226,25 @@ def test_tile_variants_bitwise_identical(pkg):
         g.du_inviscous()
         assert np.array_equal(g.download("DU").view(np.uint64), ref.view(np.uint64))
         g.close()
+
+
+def test_cfl_dt_closed_form(pkg):
+    """ocfd_cfl_dt against the closed form evaluated on the downloaded
+    state: dt = cfl / max sum_dir (|A_dir . u| + c |A_dir|) (n_dir - 1)."""
+    for case in (fc.small_isotropic(16), fc.small_flat_plate(48, 32, 16)):
+        g = pkg.FlowSolver(case[0])
+        fc.setup(g, case)
+        dt = g.cfl_dt(0.5)
+        I = fc.interior
+        u, v, w, p, et, d = (I(g.download(k)) for k in ("U", "V", "W", "P", "ET", "D"))
+        c = np.sqrt((p / et + 1.0) * p / d)
+        lam = 0.0
+        n = (case[0].nx, case[0].ny, case[0].nz)
+        for k, rows in enumerate((("AKX", "AKY", "AKZ"), ("AIX", "AIY", "AIZ"), ("ASX", "ASY", "ASZ"))):
+            a = [I(g.download(r)) for r in rows]
+            un = a[0] * u + a[1] * v + a[2] * w
+            lam = lam + (np.abs(un) + c * np.sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2])) * (n[k] - 1.0)
+        expect = 0.5 / float(np.max(lam))
+        print(case[5]["name"], dt, expect)
+        assert abs(dt - expect) <= 1e-13 * expect
+        g.close()
