@@ -134,6 +134,7 @@ def chem_bench(m, ncell=1 << 21, cpu=True, surrogate_cells=1 << 24, surrogate_ep
                         "dt log-U[1e-8,1e-6] s", dtype="f64",
                controller=dict(rtol_y=chemistry.RTOL_Y, y_floor=chemistry.Y_FLOOR, rtol_t=chemistry.RTOL_T),
                **res)
+    S = None
     if surrogate_cells:
         from ocfd_amd import surrogate
         t0 = time.perf_counter()
@@ -188,6 +189,93 @@ def chem_bench(m, ncell=1 << 21, cpu=True, surrogate_cells=1 << 24, surrogate_ep
         out["cpu_baseline"] = dict(value=nsample / el, unit="cells/s", cores=threads, kind="port",
                                    sample=f"first {nsample} cells of the same batch, adaptive point-implicit "
                                           f"(oracle, C, {threads} thread), {el:.1f} s")
+    return out, S
+
+
+def jet_bench(m, S, nx=256, ny=192, nz=128, steps=10, warmup=2, tau=2e-5, cfl=0.5):
+    """The reacting planar H2 jet (WENO5, 9 species, real gas, Chemkin/Wilke
+    transport, BC_JET) in NS_solver steps: flow dt/2, chemistry over dt,
+    comput_T_P, flow dt/2.  The field is first pre-reacted for tau so the
+    igniter has lit the mixing layers.  Runs the same steps with the plain
+    adaptive FP64 integrator and with the surrogate plus fallback, from the
+    same state, and compares the final fields."""
+    import numpy as np
+    import torch
+    from ocfd_amd import cases, chemistry
+    case = cases.planar_jet(nx, ny, nz)
+    cfg, mesh, span, st, bc, info = case
+    mech = info["mech"]
+    chemistry.ChemBatch(mech)
+    g = m.FlowSolver(cfg)
+    g.set_mesh(*mesh, span)
+    cases.jet_gas(g, mech)
+    g.set_bc_jet(bc["inlet2d"])
+    g.upload_state(st)
+    del st
+    g.comput_flow_variables()
+    g.OCFD_bc()
+    ctl = dict(rtol_y=chemistry.RTOL_Y, y_floor=chemistry.Y_FLOOR, rtol_t=chemistry.RTOL_T, max_sub=100000)
+    g.set_chem_method(m.CHEM_ADAPTIVE, scheduler=2, **ctl)
+    t = time.perf_counter()
+    g.Chemical_time_advance(tau)
+    g.comput_T_P()
+    g.synchronize()
+    t_pre = time.perf_counter() - t
+    dt = g.cfl_dt(cfl)
+    names = ("F", "DI", "D", "U", "V", "W", "T", "P", "ET")
+    saved = {k: g.download(k) for k in names}
+    T0max = float(np.max(saved["T"]))
+    g.set_option(m.OPT_USE_GRAPH, 1)
+    ncell = nx * ny * nz
+    ctl["max_sub"] = 4000
+
+    def run(method, reaction=True):
+        for k in names:
+            g.upload(k, saved[k])
+        g.set_chem_method(method, scheduler=2 if method == m.CHEM_ADAPTIVE else 1, **ctl)
+        for _ in range(warmup):
+            g.NS_step(dt, 1, 1, reaction)
+        g.synchronize()
+        for k in names:
+            g.upload(k, saved[k])
+        g.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            g.NS_step(dt, 1, 1, reaction)
+        g.synchronize()
+        el = time.perf_counter() - t
+        return el / steps * 1e3
+
+    out = dict(workload=f"reacting planar H2 jet {nx}x{ny}x{nz}, 9 species / 19 reactions, WENO5 + SW, CD6, "
+                        "real gas, Chemkin/Wilke transport, mixture-averaged diffusion, BC_JET",
+               dt=dt, cfl=cfl, pre_react_s=t_pre, pre_react_tau=tau, steps=steps,
+               step="NS_solver: flow RK3 dt/2, chemistry dt, comput_T_P, flow RK3 dt/2")
+    ms_flow = run(m.CHEM_FIXED, reaction=False)
+    out["flow_only_ms_per_step"] = ms_flow
+    out["flow_Mcell_steps_per_s"] = ncell * 2 / ms_flow / 1e3
+    ms_plain = run(m.CHEM_ADAPTIVE)
+    Tp, dip = g.download("T"), g.download("DI")
+    unfinished = None
+    out["plain"] = dict(ms_per_step=ms_plain, chem_ms_per_step=ms_plain - ms_flow,
+                        integrator="adaptive point-implicit FP64, work queue + cooperative kernel")
+    if S is not None:
+        ms_s = run(m.CHEM_SURROGATE)
+        nfb = S.fallbacks()
+        Ts, dis = g.download("T"), g.download("DI")
+        I = (slice(4, -4),) * 3
+        Tp_i, Ts_i = Tp[I], Ts[I]
+        Yp = dip[I] / dip[I].sum(axis=3, keepdims=True)
+        Ys = dis[I] / dis[I].sum(axis=3, keepdims=True)
+        major = [mech.index(k) for k in ("H2", "O2", "H2O", "N2")]
+        dY = np.abs(Ys[..., major] - Yp[..., major]).max()
+        out["surrogate"] = dict(ms_per_step=ms_s, chem_ms_per_step=ms_s - ms_flow, fallbacks_last_step=nfb,
+                                fallback_fraction=nfb / ncell, speedup_step_vs_plain=ms_plain / ms_s,
+                                speedup_chem_vs_plain=(ms_plain - ms_flow) / max(ms_s - ms_flow, 1e-9),
+                                error_vs_plain=dict(peak_T_rel=float(abs(Ts_i.max() - Tp_i.max()) / Tp_i.max()),
+                                                    dT_max_K=float(np.abs(Ts_i - Tp_i).max()),
+                                                    dY_major_max=float(dY)))
+    out["T_max_initial"] = T0max
+    g.close()
     return out
 
 
@@ -221,6 +309,11 @@ def main():
     ap.add_argument("--chem-cells", type=int, default=1 << 21)
     ap.add_argument("--surrogate-train-cells", type=int, default=1 << 24, help="0 skips the surrogate")
     ap.add_argument("--surrogate-epochs", type=int, default=25)
+    ap.add_argument("--no-jet", action="store_true", help="skip the reacting planar jet")
+    ap.add_argument("--jet-steps", type=int, default=10)
+    ap.add_argument("--jet-nx", type=int, default=256)
+    ap.add_argument("--jet-ny", type=int, default=192)
+    ap.add_argument("--jet-nz", type=int, default=128)
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a plain 512 MiB device copy (3 times): the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
@@ -333,9 +426,12 @@ def main():
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(m)
     g.close()
-    chem = None if args.no_chem else chem_bench(m, args.chem_cells, cpu=not args.no_cpu_baseline,
-                                                 surrogate_cells=args.surrogate_train_cells,
-                                                 surrogate_epochs=args.surrogate_epochs)
+    chem, S = (None, None) if args.no_chem else chem_bench(m, args.chem_cells, cpu=not args.no_cpu_baseline,
+                                                            surrogate_cells=args.surrogate_train_cells,
+                                                            surrogate_epochs=args.surrogate_epochs)
+    jet = None
+    if not args.no_jet and not args.no_chem:
+        jet = jet_bench(m, S, args.jet_nx, args.jet_ny, args.jet_nz, steps=args.jet_steps)
     out = {
         "metric": METRIC, "value": value, "unit": "Mcell-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
@@ -349,6 +445,7 @@ def main():
         "roofline": roofline, "step_roofline": step_roofline, "cpu_baseline": cpu,
         "kernels_ms_per_step": {k: v[0] / max(1, min(args.steps, 20)) for k, v in kstats.items()},
         "chem": chem,
+        "jet": jet,
     }
     print(json.dumps(out), flush=True)
 
