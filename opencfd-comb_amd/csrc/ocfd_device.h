@@ -72,6 +72,7 @@ struct DevState {
   double* dus;         // [ns][interior] species right-hand sides
   double* Evs;         // [3*ns][interior] species diffusive fluxes (xi, eta, zeta)
   double* Qp;          // [3][interior] enthalpy flux of species diffusion
+  double* cs;          // [ns][interior] species mass fractions di/d
   const double* jet_in;  // jet inflow flow_inlet2d (ny, nz, 5+ns)
   int jet_iflag;       // BC_Para(1) of the jet
   double* red;         // reduction scratch

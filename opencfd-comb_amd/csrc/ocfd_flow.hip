@@ -1189,7 +1189,10 @@ void enq_species(int krk, double dt1) {
   launch_species_sweep<0>();
   launch_species_sweep<1>();
   launch_species_sweep<2>();
-  if (S.viscous) launch("k_species_visc_prep", k_species_visc_prep, grid1(S.ni), dim3(256), S);
+  if (S.viscous) {
+    launch("k_species_conc", k_species_conc, grid1(S.ni), dim3(256), S);
+    launch("k_species_visc_prep", k_species_visc_prep, grid1(S.ni), dim3(256), S);
+  }
   launch("k_species_rk", k_species_rk, grid1(S.ni), dim3(256), S, krk, dt1, S.viscous);
 }
 
@@ -1387,7 +1390,7 @@ int32_t ocfd_init(const ocfd_cfg* cfg) {
   const long nh = S.nh, ni = S.ni, ns = S.ns, nms = S.ns > 1 ? S.ns : 0;
   struct { double** p; long n; double v; } arrs[] = {
       {&S.f, 5 * ni, 0.0}, {&S.fn, 5 * ni, 0.0}, {&S.du, 5 * ni, 0.0}, {&S.di, nh * ns, 1.0}, {&S.din, ni * ns, 1.0},
-      {&S.AmD, ni * nms, 0.0}, {&S.dus, ni * nms, 0.0}, {&S.Evs, 3 * ni * nms, 0.0}, {&S.Qp, 3 * ni, 0.0},
+      {&S.AmD, ni * nms, 0.0}, {&S.dus, ni * nms, 0.0}, {&S.Evs, 3 * ni * nms, 0.0}, {&S.Qp, 3 * ni, 0.0}, {&S.cs, ni * nms, 0.0},
       {(double**)&S.jet_in, (long)S.ny * S.nz * (5 + ns), 1.0},
       {&S.d, nh, 1.0}, {&S.u, nh, 1.0}, {&S.v, nh, 1.0}, {&S.w, nh, 1.0}, {&S.T, nh, 300.0}, {&S.p, nh, 1.0},
       {&S.Et, nh, 1.0}, {&S.Et_prev, ni, 1.0}, {&S.Amu, ni, 0.0}, {&S.Amk, ni, 0.0},
@@ -1481,6 +1484,11 @@ int32_t ocfd_set_species(int32_t n_tct, const double* tct, double tct_max, const
   auto P3 = [](double x) { double x2 = x * x; return x2 * x; };
   auto P4 = [](double x) { double x2 = x * x; return x2 * x2; };
   auto P5 = [](double x) { double x2 = x * x; return (x2 * x2) * x; };
+  for (int k = 0; k < ns; k++)
+    for (int j = 0; j < ns; j++) {
+      h.wsq[k][j] = std::sqrt(mi[j] / mi[k]);               // comput_vis_coeff, TRANSPORT_modules.f90:300-301
+      h.wden[k][j] = std::sqrt(8.0 * (1.0 + mi[k] / mi[j]));
+    }
   for (int k = 0; k < ns; k++) {
     h.Mi[k] = mi[k];
     h.Ri[k] = GAS_R0 / mi[k];

@@ -21,6 +21,9 @@ struct FlowGas {
   double Mi[OCFD_NS_MAX], Ri[OCFD_NS_MAX];
   double coef[OCFD_NS_MAX][GAS_NT_MAX][7], ect[OCFD_NS_MAX][GAS_NT_MAX];
   double mu_an[OCFD_NS_MAX][4], k_an[OCFD_NS_MAX][4], D_an[OCFD_NS_MAX][OCFD_NS_MAX][4];
+  // Wilke factors that depend only on the molar masses, evaluated on the host
+  // with the same IEEE operations: sqrt(M_j/M_k) and sqrt(8 (1 + M_k/M_j))
+  double wsq[OCFD_NS_MAX][OCFD_NS_MAX], wden[OCFD_NS_MAX][OCFD_NS_MAX];
 };
 __constant__ FlowGas fg;
 
@@ -183,14 +186,19 @@ __global__ __launch_bounds__(256) void k_transport_ms(DevState S) {
   cell_ijk(S, c, i, j, k);
   const long q = hq(S, i, j, k);
   const int ns = fg.ns;
+  // species loops run to the fixed bound with guards, fully unrolled, so the
+  // per-species arrays stay in registers
   double di0[OCFD_NS_MAX], Xi[OCFD_NS_MAX];
   double X0 = 0.0;
-  for (int m = 0; m < ns; m++) {
-    di0[m] = S.di[q + (long)m * S.nh];
-    Xi[m] = di0[m] / fg.Mi[m];
-    X0 = X0 + Xi[m];
+#pragma unroll
+  for (int m = 0; m < OCFD_NS_MAX; m++) {
+    di0[m] = m < ns ? S.di[q + (long)m * S.nh] : 0.0;
+    Xi[m] = m < ns ? di0[m] / fg.Mi[m] : 0.0;
+    if (m < ns) X0 = X0 + Xi[m];
   }
-  for (int m = 0; m < ns; m++) Xi[m] = Xi[m] / X0;
+#pragma unroll
+  for (int m = 0; m < OCFD_NS_MAX; m++)
+    if (m < ns) Xi[m] = Xi[m] / X0;
   const double T = S.T[q];
   double amu;
   if (fg.flag_vis == 0) {
@@ -198,17 +206,25 @@ __global__ __launch_bounds__(256) void k_transport_ms(DevState S) {
     const double tr = T / TA;
     amu = S.amu0 * sqrt(gp3(tr)) * (TA + TC) / (TC + T);
   } else {
+    const double lT = log(T);
     double mui[OCFD_NS_MAX];
-    for (int m = 0; m < ns; m++) mui[m] = chemkin_fit(fg.mu_an[m], log(T)) * 0.1;
+#pragma unroll
+    for (int m = 0; m < OCFD_NS_MAX; m++) mui[m] = m < ns ? chemkin_fit(fg.mu_an[m], lT) * 0.1 : 1.0;
     double mu = 0.0;
-    for (int kk = 0; kk < ns; kk++) {
-      double Pk0 = 0.0;
-      for (int jj = 0; jj < ns; jj++) {
-        const double a = 1.0 + sqrt(mui[kk] / mui[jj] * sqrt(fg.Mi[jj] / fg.Mi[kk]));
-        const double Pkj = (a * a) / sqrt(8.0 * (1.0 + fg.Mi[kk] / fg.Mi[jj]));
-        Pk0 = Pk0 + Xi[jj] * Pkj;
+#pragma unroll
+    for (int kk = 0; kk < OCFD_NS_MAX; kk++) {
+      if (kk < ns) {
+        double Pk0 = 0.0;
+#pragma unroll
+        for (int jj = 0; jj < OCFD_NS_MAX; jj++) {
+          if (jj < ns) {
+            const double a = 1.0 + sqrt(mui[kk] / mui[jj] * fg.wsq[kk][jj]);
+            const double Pkj = (a * a) / fg.wden[kk][jj];
+            Pk0 = Pk0 + Xi[jj] * Pkj;
+          }
+        }
+        mu = mu + Xi[kk] * mui[kk] / Pk0;
       }
-      mu = mu + Xi[kk] * mui[kk] / Pk0;
     }
     amu = mu;
   }
@@ -217,33 +233,51 @@ __global__ __launch_bounds__(256) void k_transport_ms(DevState S) {
   if (fg.flag_cond == 0) {
     amk = amu * Cp / S.pr;
   } else {
+    const double lT = log(T);
     double Amk1 = 0.0, Amk2 = 0.0;
-    for (int m = 0; m < ns; m++) {
-      const double ki = chemkin_fit(fg.k_an[m], log(T)) * 1.0e-5;
-      Amk1 = Amk1 + Xi[m] * ki;
-      Amk2 = Amk2 + Xi[m] / ki;
-    }
+#pragma unroll
+    for (int m = 0; m < OCFD_NS_MAX; m++)
+      if (m < ns) {
+        const double ki = chemkin_fit(fg.k_an[m], lT) * 1.0e-5;
+        Amk1 = Amk1 + Xi[m] * ki;
+        Amk2 = Amk2 + Xi[m] / ki;
+      }
     amk = 0.5 * (Amk1 + 1.0 / Amk2);
   }
   S.Amu[c] = amu;
   S.Amk[c] = amk;
   if (fg.flag_diff == 0) {
     double d0 = 0.0;
-    for (int m = 0; m < ns; m++) d0 = d0 + Xi[m] * fg.Mi[m];
-    for (int m = 0; m < ns; m++)
-      S.AmD[c + (long)m * S.ni] = amu * (1.0 - Xi[m] * fg.Mi[m] / d0) / ((1.0 - Xi[m]) * S.sc * S.d[q] + 1.0e-20);
+#pragma unroll
+    for (int m = 0; m < OCFD_NS_MAX; m++)
+      if (m < ns) d0 = d0 + Xi[m] * fg.Mi[m];
+#pragma unroll
+    for (int m = 0; m < OCFD_NS_MAX; m++)
+      if (m < ns)
+        S.AmD[c + (long)m * S.ni] = amu * (1.0 - Xi[m] * fg.Mi[m] / d0) / ((1.0 - Xi[m]) * S.sc * S.d[q] + 1.0e-20);
   } else {
-    // D2(k,j) from the fit of the pair k <= j (the reference fills the lower
-    // triangle from the upper one)
+    // binary coefficients D(k,j) = D(j,k) from the fit of the pair k < j (the
+    // reference fills the lower triangle from the upper one), each evaluated
+    // once.  Visiting the pairs (a, b), a < b, with a outer and b inner adds
+    // the terms of every ds_k in increasing j, the reference's order.
     const double lT = log(T);
-    for (int kk = 0; kk < ns; kk++) {
-      double ds = 0.0;
-      for (int jj = 0; jj < ns; jj++) {
-        if (jj == kk) continue;
-        const int a = kk < jj ? kk : jj, b = kk < jj ? jj : kk;
-        ds = ds + Xi[jj] / (chemkin_fit(fg.D_an[a][b], lT) * 1.0e-4);
-      }
-      S.AmD[c + (long)kk * S.ni] = (1.0 - Xi[kk]) / (ds + 1.0e-20) * GAS_ATM / S.p[q];
+    const double pq = S.p[q];
+    double ds[OCFD_NS_MAX];
+#pragma unroll
+    for (int m = 0; m < OCFD_NS_MAX; m++) ds[m] = 0.0;
+#pragma unroll
+    for (int a = 0; a < OCFD_NS_MAX; a++)
+#pragma unroll
+      for (int b = a + 1; b < OCFD_NS_MAX; b++)
+        if (b < ns) {
+          const double D = chemkin_fit(fg.D_an[a][b], lT) * 1.0e-4;
+          ds[a] = ds[a] + Xi[b] / D;
+          ds[b] = ds[b] + Xi[a] / D;
+        }
+#pragma unroll
+    for (int kk = 0; kk < OCFD_NS_MAX; kk++) {
+      if (kk < ns)
+        S.AmD[c + (long)kk * S.ni] = (1.0 - Xi[kk]) / (ds[kk] + 1.0e-20) * GAS_ATM / pq;
     }
   }
 }
@@ -264,8 +298,13 @@ template <int DIR>
 __global__ __launch_bounds__(256) void k_species_sweep(DevState S) {
   constexpr int NP = SpTile<DIR>::NP, TS = SpTile<DIR>::TS;
   constexpr int NPT = TS + 8, NIF = TS + 1;
-  __shared__ double sF[2 * OCFD_NS_MAX * NP * NPT];
-  __shared__ double sH[2 * OCFD_NS_MAX * NP * NIF];
+  // one species at a time through small LDS buffers: the split velocity,
+  // the point index and the reconstruction kind are formed once
+  __shared__ double sV[NP * NPT];
+  __shared__ long sQ[NP * NPT];
+  __shared__ double sF[2 * NP * NPT];
+  __shared__ double sH[2 * NP * NIF];
+  __shared__ uint8_t sK[2 * NP * NIF];
   const int ns = S.ns;
   const int n = S.n[DIR];
   const long nb = (long)gridDim.x * gridDim.y;
@@ -290,14 +329,9 @@ __global__ __launch_bounds__(256) void k_species_sweep(DevState S) {
     int a0, a1;
     pencil_ijk<DIR>(S, pen, a0, a1);
     const long q = sweep_hq<DIR>(S, a0, a1, c0 - 4 + sl);
-    const double vs = M1[q] * S.u[q] + M2[q] * S.v[q] + M3[q] * S.w[q];
-    for (int m = 0; m < ns; m++) {
-      const double fs = S.di[q + (long)m * S.nh];
-      sF[m * NP * NPT + lp(pl, sl)] = 0.5 * (vs + fabs(vs)) * fs;
-      sF[(ns + m) * NP * NPT + lp(pl, sl)] = 0.5 * (vs - fabs(vs)) * fs;
-    }
+    sQ[lp(pl, sl)] = q;
+    sV[lp(pl, sl)] = M1[q] * S.u[q] + M2[q] * S.v[q] + M3[q] * S.w[q];
   }
-  __syncthreads();
   const bool hyb = S.scheme == OCFD_SCHEME_HYBRID;
   for (int t = tid; t < 2 * NP * NIF; t += blockDim.x) {
     const int side = t / (NP * NIF);
@@ -308,37 +342,53 @@ __global__ __launch_bounds__(256) void k_species_sweep(DevState S) {
     if (pen >= npen || l > ncell) continue;
     const int s_if = c0 - 1 + l;
     const int mask = hyb ? (int)S.mask[DIR][mask_index<DIR>(S, pen, s_if)] : 0;
-    const int kind = recon_kind(S, DIR, s_if, side, mask);
-    for (int m = 0; m < ns; m++) {
-      const double* F = sF + (side * ns + m) * NP * NPT;
+    sK[t] = (uint8_t)recon_kind(S, DIR, s_if, side, mask);
+  }
+  const double h1 = S.h_1[DIR];
+  for (int m = 0; m < ns; m++) {
+    const double* Di = S.di + (long)m * S.nh;
+    for (int t = tid; t < NP * NPT; t += blockDim.x) {
+      const int pl = DIR == 0 ? t / NPT : t % NP;
+      const int sl = DIR == 0 ? t % NPT : t / NP;
+      if (pen0 + pl >= npen || sl > ncell + 7) continue;
+      const int a = lp(pl, sl);
+      const double vs = sV[a], fs = Di[sQ[a]];
+      sF[a] = 0.5 * (vs + fabs(vs)) * fs;
+      sF[NP * NPT + a] = 0.5 * (vs - fabs(vs)) * fs;
+    }
+    __syncthreads();  // sF complete; the previous species' outputs have read sH
+    for (int t = tid; t < 2 * NP * NIF; t += blockDim.x) {
+      const int side = t / (NP * NIF);
+      const int r = t % (NP * NIF);
+      const int pl = DIR == 0 ? r / NIF : r % NP;
+      const int l = DIR == 0 ? r % NIF : r / NP;
+      if (pen0 + pl >= npen || l > ncell) continue;
+      const double* F = sF + side * NP * NPT;
       double w[8];
 #pragma unroll
       for (int qq = 0; qq < 8; qq++) w[qq] = F[lp(pl, side ? l + 7 - qq : l + qq)];
-      sH[(side * ns + m) * NP * NIF + r] = recon(kind, w, S.ud7l);
+      sH[t] = recon(sK[t], w, S.ud7l);
     }
-  }
-  __syncthreads();
-  const double h1 = S.h_1[DIR];
-  for (int t = tid; t < ns * NP * TS; t += blockDim.x) {
-    const int m = t / (NP * TS);
-    const int r = t % (NP * TS);
-    const int pl = DIR == 0 ? r / TS : r % NP;
-    const int c = DIR == 0 ? r % TS : r / NP;
-    const long pen = pen0 + pl;
-    if (pen >= npen || c >= ncell) continue;
-    const double hp1 = sH[m * NP * NIF + lh(pl, c + 1)], hp0 = sH[m * NP * NIF + lh(pl, c)];
-    const double hm1 = sH[(ns + m) * NP * NIF + lh(pl, c + 1)], hm0 = sH[(ns + m) * NP * NIF + lh(pl, c)];
-    const double val = (hp1 - hp0 + hm1 - hm0) * h1;
-    int a0, a1;
-    pencil_ijk<DIR>(S, pen, a0, a1);
-    const long ci = sweep_iq<DIR>(S, a0, a1, c0 + c) + (long)m * S.ni;
-    if (DIR == 0) {
-      S.dus[ci] = val;
-    } else if (DIR == 1) {
-      S.dus[ci] = S.dus[ci] + val;
-    } else {
-      const long q = sweep_hq<DIR>(S, a0, a1, c0 + c);
-      S.dus[ci] = -(S.dus[ci] + val) * S.Ajac[q];
+    __syncthreads();  // sH complete; sF may be overwritten
+    for (int t = tid; t < NP * TS; t += blockDim.x) {
+      const int pl = DIR == 0 ? t / TS : t % NP;
+      const int c = DIR == 0 ? t % TS : t / NP;
+      const long pen = pen0 + pl;
+      if (pen >= npen || c >= ncell) continue;
+      const double hp1 = sH[lh(pl, c + 1)], hp0 = sH[lh(pl, c)];
+      const double hm1 = sH[NP * NIF + lh(pl, c + 1)], hm0 = sH[NP * NIF + lh(pl, c)];
+      const double val = (hp1 - hp0 + hm1 - hm0) * h1;
+      int a0, a1;
+      pencil_ijk<DIR>(S, pen, a0, a1);
+      const long ci = sweep_iq<DIR>(S, a0, a1, c0 + c) + (long)m * S.ni;
+      if (DIR == 0) {
+        S.dus[ci] = val;
+      } else if (DIR == 1) {
+        S.dus[ci] = S.dus[ci] + val;
+      } else {
+        const long q = sQ[lp(pl, c + 4)];
+        S.dus[ci] = -(S.dus[ci] + val) * S.Ajac[q];
+      }
     }
   }
 }
@@ -347,8 +397,19 @@ __global__ __launch_bounds__(256) void k_species_sweep(DevState S) {
 // du_scaler_viscous (OCFD_scaler_viscous.f90:4-79), part 1 for all species:
 // grad(c_m) with c_m = di_m/d, the flux rho D_m grad(c_m) in the three
 // computational directions, and Qp = sum_m rho D_m h_m grad(c_m) in species
-// order (zeroed at the first species).
+// order (zeroed at the first species).  c_m is formed once per cell by
+// k_species_conc (the reference's Ev1 = fs/d, :25-31).
 // ===========================================================================
+__global__ __launch_bounds__(256) void k_species_conc(DevState S) {
+  const long c = cell_index();
+  if (c >= S.ni) return;
+  int i, j, k;
+  cell_ijk(S, c, i, j, k);
+  const long q = hq(S, i, j, k);
+  const double d = S.d[q];
+  for (int m = 0; m < S.ns; m++) S.cs[c + (long)m * S.ni] = S.di[q + (long)m * S.nh] / d;
+}
+
 __global__ __launch_bounds__(256) void k_species_visc_prep(DevState S) {
   const long c = cell_index();
   if (c >= S.ni) return;
@@ -365,11 +426,11 @@ __global__ __launch_bounds__(256) void k_species_visc_prep(DevState S) {
   double Qpx = 0.0, Qpy = 0.0, Qpz = 0.0;
   const long ni = S.ni;
   for (int m = 0; m < S.ns; m++) {
-    const double* Di = S.di + (long)m * S.nh;
+    const double* C = S.cs + (long)m * ni;
     auto dc = [&](int dir, int s) {
       return deriv1(S, dir, s, [&](int o) {
-        const long off = (long)(wrap(s + o, S.n[dir], S.per[dir]) - s) * S.sh[dir];
-        return Di[q + off] / S.d[q + off];
+        const int t = wrap(s + o, S.n[dir], S.per[dir]);
+        return C[c + (long)(t - s) * S.si[dir]];
       });
     };
     const double uk = dc(0, i), ui = dc(1, j), us = dc(2, k);
