@@ -320,8 +320,10 @@ int32_t ocfd_chem_adaptive_batch(int64_t n, double* rho, double* E, const double
 /* Chemistry options: OCFD_CHEM_OPT_COOP_BUDGET = attempts a cell may take on
  * one lane before scheduler 2 moves it to the cooperative kernel (default
  * 16); OCFD_CHEM_OPT_FALLBACK_SCHEDULER = scheduler (1 or 2) of the
- * surrogate's fallback (default 1). */
-enum { OCFD_CHEM_OPT_COOP_BUDGET = 1, OCFD_CHEM_OPT_FALLBACK_SCHEDULER = 2 };
+ * surrogate's fallback (default 2); OCFD_CHEM_OPT_COOP_DIRECT_MAX = a
+ * fallback list of at most this many cells hands every cell that needs more
+ * than one attempt to the cooperative kernel (default 16384). */
+enum { OCFD_CHEM_OPT_COOP_BUDGET = 1, OCFD_CHEM_OPT_FALLBACK_SCHEDULER = 2, OCFD_CHEM_OPT_COOP_DIRECT_MAX = 3 };
 int32_t ocfd_chem_set_option(int32_t key, int32_t value);
 /* The chemistry work counters of the last batch call (synchronises):
  * [0] per-lane queue head, [1] unfinished cells, [2] surrogate fallbacks,

@@ -17,7 +17,7 @@ from . import load_library, OcfdError
 from .mechanism import Mechanism
 
 MODES = {1: "Euler", 2: "RK2", 3: "RK3", -1: "point-implicit Euler"}
-OPT_COOP_BUDGET, OPT_FALLBACK_SCHEDULER = 1, 2
+OPT_COOP_BUDGET, OPT_FALLBACK_SCHEDULER, OPT_COOP_DIRECT_MAX = 1, 2, 3
 
 # adaptive controller defaults: on jet-mixture cells along their ignition
 # trajectories, one step of dt in [1e-8, 1e-6] s lands within 0.25 K and 1e-3

@@ -74,7 +74,8 @@ struct ChemCtx {
   int* fb_list = nullptr;                  // cells handed to the fallback integrator
   long fb_cap = 0;
   HeavyBuf heavy{};                        // cells handed to the cooperative kernel
-  int fallback_scheduler = 1;              // scheduler of the surrogate's fallback
+  int fallback_scheduler = 2;              // scheduler of the surrogate's fallback
+  int coop_direct_max = 16384;             // fallback lists up to this size go to the cooperative kernel
   int coop_budget = 16;                    // attempts on one lane before the cooperative kernel
   unsigned char* heavy_mem = nullptr;
   bool surr_ready = false;
@@ -562,13 +563,16 @@ __global__ __launch_bounds__(128) void k_chem_adaptive(long n, double* rho, doub
 // With budget > 0, a cell still running after `budget` attempts is handed
 // to the cooperative kernel (k_chem_coop) with its full controller state, so
 // the few cells that need hundreds of substeps do not hold a lane for the
-// whole kernel.
+// whole kernel.  A list of at most direct_max cells (the surrogate's
+// rejects: few, and mostly stiff) hands every cell over after its first
+// attempt: each then has a wavefront to itself.
 __global__ __launch_bounds__(128) void k_chem_queue(long n, double* rho, double* Ecell, const double* dtcell,
                                                      ChemCtl ctl, int* nsub_out, unsigned long long* queue,
                                                      const int* list, const unsigned long long* list_count, int budget,
-                                                     HeavyBuf hb) {
+                                                     int direct_max, HeavyBuf hb) {
   const unsigned long long count =
       list ? (*list_count < (unsigned long long)n ? *list_count : (unsigned long long)n) : (unsigned long long)n;
+  if (list && budget > 0 && count <= (unsigned long long)direct_max) budget = 1;
   const int lane = threadIdx.x & 63;
   const unsigned long long lt = (1ull << lane) - 1ull;
   long cell = -1;  // -1: needs work, -2: queue exhausted
@@ -1073,7 +1077,7 @@ int launch_adaptive(long n, double* rho, double* E, const double* dt, const Chem
   const bool coop = scheduler == 2;
   if (coop && ensure_heavy(n)) return 1;
   hipLaunchKernelGGL(k_chem_queue, dim3((unsigned)blocks), dim3(128), 0, CX.stream, n, rho, E, dt, ctl, nsub_out,
-                     CX.counter, list, list_count, coop ? CX.coop_budget : 0, CX.heavy);
+                     CX.counter, list, list_count, coop ? CX.coop_budget : 0, CX.coop_direct_max, CX.heavy);
   if (coop) {
     int per2 = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per2, k_chem_coop, 64, 0) != hipSuccess)
@@ -1278,6 +1282,10 @@ int32_t ocfd_chem_set_option(int32_t key, int32_t value) {
     case 2:
       if (value < 1 || value > 2) return cfail("ocfd_chem_set_option: fallback scheduler 1 or 2");
       CX.fallback_scheduler = value;
+      return 0;
+    case 3:
+      if (value < 0) return cfail("ocfd_chem_set_option: the direct cooperative list size must be >= 0");
+      CX.coop_direct_max = value;
       return 0;
   }
   return cfail("ocfd_chem_set_option: unknown option");
