@@ -934,7 +934,7 @@ __global__ void k_fill_u64(unsigned long long* p, int count) {
   if ((int)threadIdx.x < count && blockIdx.x == 0) p[threadIdx.x] = 0;
 }
 
-__global__ __launch_bounds__(256) void k_surrogate(long n, double* rho, double* Ecell, const double* dtcell,
+__global__ __launch_bounds__(256, 2) void k_surrogate(long n, double* rho, double* Ecell, const double* dtcell,
                                                     int* flag_out, int* fb_list, unsigned long long* counters) {
   __shared__ __attribute__((aligned(16))) tbf16 H[MT_ROWS * MT_LDH];
   __shared__ int ok_in[MT_ROWS];

@@ -269,15 +269,18 @@ __device__ __forceinline__ int recon_kind(const DevState& S, int dir, int s, int
 // Central first derivative along dir at cell s (1..n) of a field f read at
 // logical positions s+o (o = -4..4) through `at(o)`:
 //   OCFD_dx0/dy0/dz0, OCFD_dxyz_viscous.f90:3-242 (CD6/CD8 + one-sided closures).
-template <class Fn>
+// CD = 0 / 1 fixes CD6 / CD8 at compile time (fewer live registers in the
+// kernels that take many derivatives); CD = -1 reads the scheme from S.
+template <int CD = -1, class Fn>
 __device__ __forceinline__ double deriv1(const DevState& S, int dir, int s, Fn at) {
   const int n = S.n[dir];
   const double* a = S.cd6c[dir];
-  const bool lo = !S.per[dir] && (s <= 3 || (s == 4 && S.cd8));
-  const bool hi = !S.per[dir] && (s >= n - 2 || (s == n - 3 && S.cd8));
+  const bool cd8 = CD < 0 ? (bool)S.cd8 : CD == 1;
+  const bool lo = !S.per[dir] && (s <= 3 || (s == 4 && cd8));
+  const bool hi = !S.per[dir] && (s >= n - 2 || (s == n - 3 && cd8));
   double r = 0.0;
   if (!lo && !hi) {
-    if (!S.cd8) {
+    if (!cd8) {
       r = a[0] * (at(3) - at(-3)) + a[1] * (at(2) - at(-2)) + a[2] * (at(1) - at(-1));
     } else {
       const double* c = S.cd8c[dir];

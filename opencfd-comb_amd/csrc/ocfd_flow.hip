@@ -428,6 +428,7 @@ __global__ __launch_bounds__(256) void k_transport(DevState S) {
 // ===========================================================================
 enum { RK_MASS = 1, RK_VISC = 2, RK_WRITE_DU = 4, RK_UPDATE_F = 8, RK_PRIMS = 16, RK_DI_FROM_D = 32, RK_MS = 64 };
 
+template <int CD>
 __global__ __launch_bounds__(256) void k_rk(DevState S, int krk, double dt, int flags) {
   const long c = cell_index();
   if (c >= S.ni) return;
@@ -451,7 +452,7 @@ __global__ __launch_bounds__(256) void k_rk(DevState S, int krk, double dt, int 
   if (flags & RK_VISC) {
     const double ajac = S.Ajac[q];
     auto dv = [&](const double* F, int dir, int s) {
-      return deriv1(S, dir, s, [&](int o) {
+      return deriv1<CD>(S, dir, s, [&](int o) {
         const int t = wrap(s + o, S.n[dir], S.per[dir]);
         return F[c + (long)(t - s) * S.si[dir]];
       });
@@ -1091,6 +1092,9 @@ void launch(const char* name, K kern, dim3 g, dim3 b, A... args) {
 
 inline dim3 grid1(long n) { return dim3((unsigned)((n + 255) / 256)); }
 
+// k_rk with the viscous scheme fixed at compile time
+inline void (*rk_kernel())(DevState, int, double, int) { return G.S.cd8 ? k_rk<1> : k_rk<0>; }
+
 template <int DIR, int V>
 void launch_sweep_v() {
   const DevState& S = G.S;
@@ -1167,7 +1171,7 @@ void enq_stage(int krk, double dt1) {
   else launch("k_transport", k_transport, grid1(S.ni), dim3(256), S);
   const int flags = (S.mass_force ? RK_MASS : 0) | (S.viscous ? RK_VISC : 0) | RK_UPDATE_F | RK_PRIMS | RK_DI_FROM_D;
   launch("k_negt_reset", k_negt_reset, dim3(1), dim3(64), S, 0);
-  launch("k_rk", k_rk, grid1(S.ni), dim3(256), S, krk, dt1, flags);
+  launch("k_rk", rk_kernel(), grid1(S.ni), dim3(256), S, krk, dt1, flags);
   launch("k_negt_repair", k_negt_repair, dim3(1), dim3(64), S);
   enq_bc();
 }
@@ -1191,9 +1195,9 @@ void enq_species(int krk, double dt1) {
   launch_species_sweep<2>();
   if (S.viscous) {
     launch("k_species_conc", k_species_conc, grid1(S.ni), dim3(256), S);
-    launch("k_species_visc_prep", k_species_visc_prep, grid1(S.ni), dim3(256), S);
+    launch("k_species_visc_prep", S.cd8 ? k_species_visc_prep<1> : k_species_visc_prep<0>, grid1(S.ni), dim3(256), S);
   }
-  launch("k_species_rk", k_species_rk, grid1(S.ni), dim3(256), S, krk, dt1, S.viscous);
+  launch("k_species_rk", S.cd8 ? k_species_rk<1> : k_species_rk<0>, grid1(S.ni), dim3(256), S, krk, dt1, S.viscous);
 }
 
 // one RK3 stage for N_SPEC > 1 (OCFD_Time_Adv.f90:33-91): transport, sensor,
@@ -1210,7 +1214,7 @@ void enq_stage_ms(int krk, double dt1) {
   if (S.viscous) launch("k_visc_prep", k_visc_prep<true>, grid1(S.ni), dim3(256), S);
   const int flags = (S.mass_force ? RK_MASS : 0) | (S.viscous ? RK_VISC : 0) | RK_UPDATE_F | RK_PRIMS | RK_MS;
   launch("k_negt_reset", k_negt_reset, dim3(1), dim3(64), S, 0);
-  launch("k_rk", k_rk, grid1(S.ni), dim3(256), S, krk, dt1, flags);
+  launch("k_rk", rk_kernel(), grid1(S.ni), dim3(256), S, krk, dt1, flags);
   launch("k_negt_repair", k_negt_repair, dim3(1), dim3(64), S);
   enq_bc();
 }
@@ -1682,7 +1686,7 @@ int32_t ocfd_du_inviscous(void) {
   launch_sweep<0>();
   launch_sweep<1>();
   launch_sweep<2>();
-  if (G.S.mass_force) launch("k_rk", k_rk, grid1(G.S.ni), dim3(256), G.S, 1, 0.0, (int)(RK_MASS | RK_WRITE_DU));
+  if (G.S.mass_force) launch("k_rk", rk_kernel(), grid1(G.S.ni), dim3(256), G.S, 1, 0.0, (int)(RK_MASS | RK_WRITE_DU));
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1690,21 +1694,21 @@ int32_t ocfd_du_viscous(void) {
   if (check_ready()) return 1;
   if (G.S.ns > 1) launch("k_visc_prep", k_visc_prep<true>, grid1(G.S.ni), dim3(256), G.S);
   else launch("k_visc_prep", k_visc_prep<false>, grid1(G.S.ni), dim3(256), G.S);
-  launch("k_rk", k_rk, grid1(G.S.ni), dim3(256), G.S, 1, 0.0, (int)(RK_VISC | RK_WRITE_DU));
+  launch("k_rk", rk_kernel(), grid1(G.S.ni), dim3(256), G.S, 1, 0.0, (int)(RK_VISC | RK_WRITE_DU));
   HIPCHK(hipGetLastError());
   return 0;
 }
 int32_t ocfd_time_adv_rk3(int32_t krk, double dt) {
   if (check_ready()) return 1;
   if (krk < 1 || krk > 3) return fail("ocfd_time_adv_rk3: KRK must be 1, 2 or 3");
-  launch("k_rk", k_rk, grid1(G.S.ni), dim3(256), G.S, (int)krk, dt, (int)RK_UPDATE_F);
+  launch("k_rk", rk_kernel(), grid1(G.S.ni), dim3(256), G.S, (int)krk, dt, (int)RK_UPDATE_F);
   HIPCHK(hipGetLastError());
   return 0;
 }
 int32_t ocfd_comput_flow_variables(void) {
   if (check_ready()) return 1;
   launch("k_negt_reset", k_negt_reset, dim3(1), dim3(64), G.S, 0);
-  launch("k_rk", k_rk, grid1(G.S.ni), dim3(256), G.S, 1, 0.0, (int)(G.S.ns > 1 ? RK_PRIMS | RK_MS : RK_PRIMS));
+  launch("k_rk", rk_kernel(), grid1(G.S.ni), dim3(256), G.S, 1, 0.0, (int)(G.S.ns > 1 ? RK_PRIMS | RK_MS : RK_PRIMS));
   launch("k_negt_repair", k_negt_repair, dim3(1), dim3(64), G.S);
   hipMemcpyAsync(G.h_negt, G.S.negt + 17, sizeof(int), hipMemcpyDeviceToHost, G.stream);
   HIPCHK(hipGetLastError());

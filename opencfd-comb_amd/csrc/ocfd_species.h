@@ -179,7 +179,7 @@ __device__ __forceinline__ void cell_ijk(const DevState& S, long c, int& i, int&
 // comput_vis_coeff (TRANSPORT_modules.f90:259-306), comput_Cp,
 // comput_thermal_conductivity (:312-372), comput_diffusion_coeff (:378-424).
 // ===========================================================================
-__global__ __launch_bounds__(256) void k_transport_ms(DevState S) {
+__global__ __launch_bounds__(256, 5) void k_transport_ms(DevState S) {
   const long c = cell_index();
   if (c >= S.ni) return;
   int i, j, k;
@@ -410,6 +410,7 @@ __global__ __launch_bounds__(256) void k_species_conc(DevState S) {
   for (int m = 0; m < S.ns; m++) S.cs[c + (long)m * S.ni] = S.di[q + (long)m * S.nh] / d;
 }
 
+template <int CD>
 __global__ __launch_bounds__(256) void k_species_visc_prep(DevState S) {
   const long c = cell_index();
   if (c >= S.ni) return;
@@ -428,7 +429,7 @@ __global__ __launch_bounds__(256) void k_species_visc_prep(DevState S) {
   for (int m = 0; m < S.ns; m++) {
     const double* C = S.cs + (long)m * ni;
     auto dc = [&](int dir, int s) {
-      return deriv1(S, dir, s, [&](int o) {
+      return deriv1<CD>(S, dir, s, [&](int o) {
         const int t = wrap(s + o, S.n[dir], S.per[dir]);
         return C[c + (long)(t - s) * S.si[dir]];
       });
@@ -455,6 +456,7 @@ __global__ __launch_bounds__(256) void k_species_visc_prep(DevState S) {
 // Species update: Dus += div(diffusive flux) Ajac (OCFD_scaler_viscous.f90:67-77)
 // and Scalar_time_adv_RK3 (OCFD_Time_Adv.f90:134-166) for every species;
 // krk = 0 stores the complete right-hand side in dus instead.
+template <int CD>
 __global__ __launch_bounds__(256) void k_species_rk(DevState S, int krk, double dt, int visc) {
   const long c = cell_index();
   if (c >= S.ni) return;
@@ -464,7 +466,7 @@ __global__ __launch_bounds__(256) void k_species_rk(DevState S, int krk, double 
   const long ni = S.ni;
   const double ajac = S.Ajac[q];
   auto dv = [&](const double* F, int dir, int s) {
-    return deriv1(S, dir, s, [&](int o) {
+    return deriv1<CD>(S, dir, s, [&](int o) {
       const int t = wrap(s + o, S.n[dir], S.per[dir]);
       return F[c + (long)(t - s) * S.si[dir]];
     });
