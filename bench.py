@@ -151,6 +151,9 @@ def chem_bench(m, ncell=1 << 21, cpu=True, surrogate_cells=1 << 24, surrogate_ep
         torch.cuda.synchronize()
         t2 = time.perf_counter()
         del x, y
+        rho.copy_(r0)
+        E.copy_(e0)
+        tile_rows, tile_waves, tile_ms = S.autotune(rho, E, dt, reps=3)
         flags = torch.zeros(ncell, dtype=torch.int32, device="cuda")
         ms_s, _ = timed(lambda: S.step(rho, E, dt, fallback=True, flags=flags))
         nfb = S.fallbacks()
@@ -167,7 +170,9 @@ def chem_bench(m, ncell=1 << 21, cpu=True, surrogate_cells=1 << 24, surrogate_ep
         dYm = np.abs(Ys[major] - Yp[major]).max(axis=0)
         f = flags.cpu().numpy()
         out["surrogate"] = dict(
-            network="12 -> 256 x5 -> 9 GELU, BF16 MFMA (FP32 accumulation), fused per-128-cell kernel",
+            network="12 -> 256 x5 -> 9 GELU, BF16 MFMA (FP32 accumulation), fused kernel",
+            tile=dict(cells_per_workgroup=tile_rows, waves=tile_waves,
+                      ms_per_shape={"128x4": tile_ms[0], "128x8": tile_ms[1], "64x4": tile_ms[2]}),
             train=dict(cells=surrogate_cells, epochs=len(hist), batch=8192, data_s=t1 - t0, train_s=t2 - t1,
                        final_loss=hist[-1]),
             ms=ms_s, cells_per_s=ncell / ms_s * 1e3, fallbacks=nfb,

@@ -149,6 +149,8 @@ def load_library(path=None):
         "ocfd_chem_surrogate_set": (i32, [vp, i32]),
         "ocfd_chem_surrogate_batch": (i32, [ctypes.c_int64, vp, vp, vp, i32, dbl, dbl, dbl, i32, vp]),
         "ocfd_chem_surrogate_fallbacks": (ctypes.c_int64, []),
+        "ocfd_chem_surrogate_autotune": (i32, [ctypes.c_int64, vp, vp, vp, i32, vp]),
+        "ocfd_chem_surrogate_tile": (i32, [i32, P(i32), P(i32)]),
         "ocfd_chem_cell_features": (i32, [ctypes.c_int64, vp, vp, vp, i32, vp]),
         "ocfd_chem_surrogate_targets": (i32, [ctypes.c_int64, vp, vp, vp]),
     }

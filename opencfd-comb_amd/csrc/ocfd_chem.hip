@@ -79,6 +79,8 @@ struct ChemCtx {
   int coop_budget = 16;                    // attempts on one lane before the cooperative kernel
   unsigned char* heavy_mem = nullptr;
   bool surr_ready = false;
+  int surr_variant = 0;                    // tile shape of k_surrogate (SurrTile)
+  float surr_tune_ms[8] = {};
   std::string err;
 };
 ChemCtx CX;
@@ -934,14 +936,26 @@ __global__ void k_fill_u64(unsigned long long* p, int count) {
   if ((int)threadIdx.x < count && blockIdx.x == 0) p[threadIdx.x] = 0;
 }
 
-__global__ __launch_bounds__(256, 2) void k_surrogate(long n, double* rho, double* Ecell, const double* dtcell,
-                                                    int* flag_out, int* fb_list, unsigned long long* counters) {
-  __shared__ __attribute__((aligned(16))) tbf16 H[MT_ROWS * MT_LDH];
-  __shared__ int ok_in[MT_ROWS];
+// Tile shapes of the fused surrogate kernel (cells per workgroup, waves,
+// workgroups per CU the register budget aims at).  All give bit-identical
+// results; ocfd_chem_surrogate_autotune picks the fastest.
+template <int V> struct SurrTile;
+template <> struct SurrTile<0> { static constexpr int ROWS = 128, NW = 4, OCC = 2; };
+template <> struct SurrTile<1> { static constexpr int ROWS = 128, NW = 8, OCC = 1; };
+template <> struct SurrTile<2> { static constexpr int ROWS = 64, NW = 4, OCC = 3; };
+#define SURR_VARIANTS 3
+
+template <int V>
+__global__ __launch_bounds__(SurrTile<V>::NW * 64, SurrTile<V>::OCC) void k_surrogate(
+    long n, double* rho, double* Ecell, const double* dtcell, int* flag_out, int* fb_list,
+    unsigned long long* counters) {
+  constexpr int ROWS = SurrTile<V>::ROWS, NW = SurrTile<V>::NW;
+  __shared__ __attribute__((aligned(16))) tbf16 H[ROWS * MT_LDH];
+  __shared__ int ok_in[ROWS];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const long c0 = (long)blockIdx.x * MT_ROWS;
+  const long c0 = (long)blockIdx.x * ROWS;
   Mech m = (Mech)&cm;
-  if (t < MT_ROWS) {
+  if (t < ROWS) {
     const long c = c0 + t;
     float x[12];
 #pragma unroll
@@ -976,11 +990,11 @@ __global__ __launch_bounds__(256, 2) void k_surrogate(long n, double* rho, doubl
   }
   __syncthreads();
   const int L = cnet.n_layers;
-  mt_layer256<true>(H, cnet.W[0], cnet.b[0], 64, cnet.dinp[0], lane, w);
-  for (int l = 1; l < L - 1; l++) mt_layer256<true>(H, cnet.W[l], cnet.b[l], 256, 256, lane, w);
+  mt_layer256<true, ROWS, NW>(H, cnet.W[0], cnet.b[0], 64, cnet.dinp[0], lane, w);
+  for (int l = 1; l < L - 1; l++) mt_layer256<true, ROWS, NW>(H, cnet.W[l], cnet.b[l], 256, 256, lane, w);
   float* O = (float*)H;
-  mt_out16(H, cnet.W[L - 1], cnet.b[L - 1], 256, 256, O, lane, w);
-  if (t < MT_ROWS) {
+  mt_out16<ROWS, NW>(H, cnet.W[L - 1], cnet.b[L - 1], 256, 256, O, lane, w);
+  if (t < ROWS) {
     const long c = c0 + t;
     if (c < n) {
       int why = ok_in[t] ? 0 : 1;
@@ -1032,6 +1046,44 @@ __global__ __launch_bounds__(256, 2) void k_surrogate(long n, double* rho, doubl
   }
 }
 
+
+// checks of the surrogate network, fallback list capacity, network view into
+// the constant bank, counters cleared (on CX.stream)
+int surr_prepare(long n, const char* who) {
+  if (!CX.ready || !CX.surr_ready) return cfail(std::string(who) + ": mechanism or surrogate not set");
+  const MlpDevice* net = ocfd_mlp_device_view();
+  if (!net) return cfail(std::string(who) + ": no network (ocfd_mlp_create)");
+  const int L = net->n_layers;
+  bool shape_ok = L >= 2 && net->din[0] == 12 && net->dinp[0] == 64 && net->dout[L - 1] == CH_NS;
+  for (int l = 0; l < L - 1; l++) shape_ok = shape_ok && net->dout[l] == 256;
+  if (!shape_ok) return cfail(std::string(who) + ": the network must be 12 -> 256 x k -> 9");
+  if (n > 0x7fffffffL) return cfail(std::string(who) + ": at most 2^31-1 cells");
+  if (n > CX.fb_cap) {
+    if (CX.fb_list) hipFree(CX.fb_list);
+    CX.fb_list = nullptr;
+    if (hipMalloc(&CX.fb_list, (size_t)n * sizeof(int)) != hipSuccess) return cfail(std::string(who) + ": hipMalloc");
+    CX.fb_cap = n;
+  }
+  if (hipMemcpyToSymbolAsync(HIP_SYMBOL(cnet), net, sizeof(MlpDevice), 0, hipMemcpyHostToDevice, CX.stream) !=
+      hipSuccess)
+    return cfail(std::string(who) + ": copy failed");
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, CX.stream, CX.counter, 8);
+  return 0;
+}
+
+template <int V>
+void launch_surrogate_v(long n, double* rho, double* E, const double* dt, int* flag_out) {
+  constexpr int R = SurrTile<V>::ROWS, NW = SurrTile<V>::NW;
+  hipLaunchKernelGGL(k_surrogate<V>, dim3((unsigned)((n + R - 1) / R)), dim3(NW * 64), 0, CX.stream, n, rho, E, dt,
+                     flag_out, CX.fb_list, CX.counter);
+}
+void launch_surrogate(int v, long n, double* rho, double* E, const double* dt, int* flag_out) {
+  switch (v) {
+    case 1: launch_surrogate_v<1>(n, rho, E, dt, flag_out); break;
+    case 2: launch_surrogate_v<2>(n, rho, E, dt, flag_out); break;
+    default: launch_surrogate_v<0>(n, rho, E, dt, flag_out); break;
+  }
+}
 
 int ensure_heavy(long n) {
   if (n <= CX.heavy.cap) return 0;
@@ -1199,35 +1251,75 @@ int32_t ocfd_chem_surrogate_set(const double* cfg, int32_t count) {
 
 int32_t ocfd_chem_surrogate_batch(int64_t n, double* rho, double* E, const double* dt, int32_t fallback,
                                   double rtol_y, double y_floor, double rtol_t, int32_t max_sub, int32_t* flag_out) {
-  if (!CX.ready || !CX.surr_ready) return cfail("ocfd_chem_surrogate_batch: mechanism or surrogate not set");
-  const MlpDevice* net = ocfd_mlp_device_view();
-  if (!net) return cfail("ocfd_chem_surrogate_batch: no network (ocfd_mlp_create)");
-  const int L = net->n_layers;
-  bool shape_ok = L >= 2 && net->din[0] == 12 && net->dinp[0] == 64 && net->dout[L - 1] == CH_NS;
-  for (int l = 0; l < L - 1; l++) shape_ok = shape_ok && net->dout[l] == 256;
-  if (!shape_ok) return cfail("ocfd_chem_surrogate_batch: the network must be 12 -> 256 x k -> 9");
   if (fallback && (max_sub < 1 || !(rtol_y > 0.0) || !(y_floor >= 0.0) || !(rtol_t > 0.0)))
     return cfail("ocfd_chem_surrogate_batch: bad controller parameters");
-  if (n > 0x7fffffffL) return cfail("ocfd_chem_surrogate_batch: at most 2^31-1 cells");
-  if (n > CX.fb_cap) {
-    if (CX.fb_list) hipFree(CX.fb_list);
-    CX.fb_list = nullptr;
-    if (hipMalloc(&CX.fb_list, (size_t)n * sizeof(int)) != hipSuccess) return cfail("ocfd_chem_surrogate_batch: hipMalloc");
-    CX.fb_cap = n;
-  }
-  if (hipMemcpyToSymbolAsync(HIP_SYMBOL(cnet), net, sizeof(MlpDevice), 0, hipMemcpyHostToDevice, CX.stream) !=
-      hipSuccess)
-    return cfail("ocfd_chem_surrogate_batch: copy failed");
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, CX.stream, CX.counter, 8);
+  if (surr_prepare((long)n, "ocfd_chem_surrogate_batch")) return 1;
   if (n <= 0) return hipGetLastError() == hipSuccess ? 0 : cfail("ocfd_chem_surrogate_batch: launch failed");
-  hipLaunchKernelGGL(k_surrogate, dim3((unsigned)((n + MT_ROWS - 1) / MT_ROWS)), dim3(256), 0, CX.stream, (long)n, rho,
-                     E, dt, (int*)flag_out, CX.fb_list, CX.counter);
+  launch_surrogate(CX.surr_variant, (long)n, rho, E, dt, (int*)flag_out);
   if (fallback) {
     const ChemCtl ctl{rtol_y, y_floor, rtol_t, (int)max_sub};
     if (launch_adaptive((long)n, rho, E, dt, ctl, nullptr, CX.fallback_scheduler, CX.fb_list, CX.counter + 2)) return 1;
   }
   if (hipGetLastError() != hipSuccess) return cfail("ocfd_chem_surrogate_batch: launch failed");
   return 0;
+}
+
+// Autotuner of the fused surrogate kernel: every tile shape runs `reps`
+// times on scratch copies of the batch (the inputs are not modified), timed
+// with HIP events on the chemistry stream; the fastest is kept.  Results are
+// bit-identical for every shape.  ms_out (may be NULL) receives the mean ms
+// per launch of each shape (SURR_VARIANTS floats).  Synchronises.
+int32_t ocfd_chem_surrogate_autotune(int64_t n, const double* rho, const double* E, const double* dt, int32_t reps,
+                                     float* ms_out) {
+  if (reps < 1 || n <= 0) return cfail("ocfd_chem_surrogate_autotune: need reps >= 1 and cells");
+  if (surr_prepare((long)n, "ocfd_chem_surrogate_autotune")) return 1;
+  double *r2 = nullptr, *e2 = nullptr;
+  if (hipMalloc(&r2, sizeof(double) * (size_t)n * CH_NS) != hipSuccess ||
+      hipMalloc(&e2, sizeof(double) * (size_t)n) != hipSuccess) {
+    if (r2) hipFree(r2);
+    return cfail("ocfd_chem_surrogate_autotune: out of device memory");
+  }
+  hipEvent_t a, b;
+  hipEventCreate(&a);
+  hipEventCreate(&b);
+  int best = 0;
+  for (int v = 0; v < SURR_VARIANTS; v++) {
+    float tot = 0.f;
+    for (int r = 0; r <= reps; r++) {  // the first run warms up and is not counted
+      hipMemcpyAsync(r2, rho, sizeof(double) * (size_t)n * CH_NS, hipMemcpyDeviceToDevice, CX.stream);
+      hipMemcpyAsync(e2, E, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, CX.stream);
+      hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, CX.stream, CX.counter, 8);
+      hipEventRecord(a, CX.stream);
+      launch_surrogate(v, (long)n, r2, e2, dt, nullptr);
+      hipEventRecord(b, CX.stream);
+      hipEventSynchronize(b);
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, a, b);
+      if (r > 0) tot += ms;
+    }
+    CX.surr_tune_ms[v] = tot / reps;
+    if (CX.surr_tune_ms[v] < CX.surr_tune_ms[best]) best = v;
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  hipFree(r2);
+  hipFree(e2);
+  CX.surr_variant = best;
+  if (ms_out)
+    for (int v = 0; v < SURR_VARIANTS; v++) ms_out[v] = CX.surr_tune_ms[v];
+  return hipGetLastError() == hipSuccess ? 0 : cfail("ocfd_chem_surrogate_autotune: launch failed");
+}
+
+// the tile shape in use (and force one with variant >= 0)
+int32_t ocfd_chem_surrogate_tile(int32_t variant, int32_t* rows, int32_t* waves) {
+  if (variant >= SURR_VARIANTS) return cfail("ocfd_chem_surrogate_tile: unknown variant");
+  if (variant >= 0) CX.surr_variant = variant;
+  const int v = CX.surr_variant;
+  const int R[SURR_VARIANTS] = {SurrTile<0>::ROWS, SurrTile<1>::ROWS, SurrTile<2>::ROWS};
+  const int W[SURR_VARIANTS] = {SurrTile<0>::NW, SurrTile<1>::NW, SurrTile<2>::NW};
+  if (rows) *rows = R[v];
+  if (waves) *waves = W[v];
+  return v;
 }
 
 // T [n] (mode 0), raw features [n][12] f64 (mode 1) or normalised surrogate

@@ -81,6 +81,30 @@ __device__ __forceinline__ long mask_index(const DevState& S, long pen, int s) {
   return pen + (long)S.nx * S.ny * (long)s;
 }
 
+// Per-pencil base indices of a sweep tile, formed once per workgroup: the
+// halo index, the interior index and the mask index of the pencil's point
+// s = 1 (s = 0 for the mask); a point s of the pencil then adds (s-1) (or s)
+// times the direction's stride, with no integer division in the hot loops.
+template <int DIR, int NP>
+struct PencilBase {
+  long hq1[NP], iq1[NP], mb[NP];
+};
+template <int DIR, int NP>
+__device__ __forceinline__ void pencil_bases(const DevState& S, long pen0, long npen, PencilBase<DIR, NP>& B) {
+  const int t = threadIdx.x;
+  if (t < NP && pen0 + t < npen) {
+    int a0, a1;
+    pencil_ijk<DIR>(S, pen0 + t, a0, a1);
+    B.hq1[t] = sweep_hq<DIR>(S, a0, a1, 1);
+    B.iq1[t] = sweep_iq<DIR>(S, a0, a1, 1);
+    B.mb[t] = mask_index<DIR>(S, pen0 + t, 0);
+  }
+}
+template <int DIR>
+__device__ __forceinline__ long mask_stride(const DevState& S) {
+  return DIR == 0 ? 1L : DIR == 1 ? (long)S.nx : (long)S.nx * S.ny;
+}
+
 // Flux splitting at one point (split_Stager_Warming, OCFD_flux_split.f90:6-72;
 // split_Local_LaxFriedrichs, :76-103).  The sound speed is comput_cc's
 // (Comput_flow_variables.f90:50-63), evaluated on the fly.
@@ -147,6 +171,7 @@ __global__ __launch_bounds__(256) void k_sweep(DevState S) {
   __shared__ unsigned char sKind[NE];
   __shared__ int sCnt[4][10];
   __shared__ int sTotal;
+  __shared__ PencilBase<DIR, NP> PB;
   const int n = S.n[DIR];
   // XCD-aware tile order over the flattened 2-D grid
   const long nb = (long)gridDim.x * gridDim.y;
@@ -163,6 +188,8 @@ __global__ __launch_bounds__(256) void k_sweep(DevState S) {
   const double* M2 = DIR == 0 ? S.Aky1 : DIR == 1 ? S.Aiy1 : S.Asy1;
   const double* M3 = DIR == 0 ? S.Akz1 : DIR == 1 ? S.Aiz1 : S.Asz1;
   const int tid = threadIdx.x;
+  pencil_bases<DIR, NP>(S, pen0, npen, PB);
+  __syncthreads();
 
   // phase 1: split fluxes at the tile's points s = c0-4 .. c0+ncell+3
   for (int t = tid; t < NP * NPT; t += blockDim.x) {
@@ -171,9 +198,7 @@ __global__ __launch_bounds__(256) void k_sweep(DevState S) {
     const long pen = pen0 + pl;
     const int s = c0 - 4 + sl;
     if (pen >= npen || sl > ncell + 7) continue;
-    int a0, a1;
-    pencil_ijk<DIR>(S, pen, a0, a1);
-    const long q = sweep_hq<DIR>(S, a0, a1, s);
+    const long q = PB.hq1[pl] + (long)(wrap(s, n, S.per[DIR]) - 1) * S.sh[DIR];
     double fp[5], fm[5];
     split_point(S.split, S.Et[q], S.p[q], S.d[q], S.u[q], S.v[q], S.w[q], M1[q], M2[q], M3[q], fp, fm);
 #pragma unroll
@@ -208,7 +233,7 @@ __global__ __launch_bounds__(256) void k_sweep(DevState S) {
         const long pen = pen0 + pl;
         if (pen < npen && l <= ncell) {
           const int s_if = c0 - 1 + l;
-          const int mask = hyb ? (int)S.mask[DIR][mask_index<DIR>(S, pen, s_if)] : 0;
+          const int mask = hyb ? (int)S.mask[DIR][PB.mb[pl] + (long)s_if * mask_stride<DIR>(S)] : 0;
           kind = recon_kind(S, DIR, s_if, side, mask);
         }
         sKind[e] = (unsigned char)kind;
@@ -293,15 +318,13 @@ __global__ __launch_bounds__(256) void k_sweep(DevState S) {
     const double hp1 = sH[m * NP * NIF + lh(pl, c + 1)], hp0 = sH[m * NP * NIF + lh(pl, c)];
     const double hm1 = sH[(5 + m) * NP * NIF + lh(pl, c + 1)], hm0 = sH[(5 + m) * NP * NIF + lh(pl, c)];
     const double val = (hp1 - hp0 + hm1 - hm0) * h1;
-    int a0, a1;
-    pencil_ijk<DIR>(S, pen, a0, a1);
-    const long ci = sweep_iq<DIR>(S, a0, a1, c0 + c) + (long)m * S.ni;
+    const long ci = PB.iq1[pl] + (long)(c0 + c - 1) * S.si[DIR] + (long)m * S.ni;
     if (DIR == 0) {
       S.du[ci] = val;
     } else if (DIR == 1) {
       S.du[ci] = S.du[ci] + val;
     } else {
-      const long q = sweep_hq<DIR>(S, a0, a1, c0 + c);
+      const long q = PB.hq1[pl] + (long)(c0 + c - 1) * S.sh[DIR];
       S.du[ci] = -(S.du[ci] + val) * S.Ajac[q];
     }
   }

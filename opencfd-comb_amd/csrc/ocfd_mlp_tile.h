@@ -1,7 +1,8 @@
 // ocfd_mlp_tile.h -- device code of the fused MLP forward used inside the
-// chemistry surrogate kernel: one 256-thread workgroup (4 waves) carries a
-// tile of 128 cells through every layer, the activations staying in LDS as
-// bfloat16 and each wave streaming its slice of the weights from L2.
+// chemistry surrogate kernel: one workgroup of NW waves (4 or 8) carries a
+// tile of cells (ROWS = 64 or 128) through every layer, the activations
+// staying in LDS as bfloat16 and each wave streaming its slice of the
+// weights from L2.
 //
 // Layer shapes this code takes: input padded to 64, hidden widths 256,
 // output padded; v_mfma_f32_16x16x32_bf16 with FP32 accumulation.  MFMA
@@ -14,7 +15,7 @@ typedef __bf16 tbf16;
 typedef __bf16 tbf16x8 __attribute__((ext_vector_type(8)));
 typedef float tfloatx4 __attribute__((ext_vector_type(4)));
 
-constexpr int MT_ROWS = 128;  // cells per workgroup
+constexpr int MT_ROWS = 128;  // largest number of cells per workgroup
 constexpr int MT_LDH = 264;   // LDS row pitch (bf16): 256 + 8 against bank conflicts
 
 // GELU (tanh form) as z * sigmoid(2u), the same evaluation as training
@@ -23,43 +24,47 @@ __device__ __forceinline__ float mt_gelu(float z) {
   return z / (1.0f + __expf(-2.0f * u));
 }
 
-// H[128][0:K] (LDS) -> act(H . W^T + b) written back into H[128][0:256].
-// W: [256][ldw] bf16 row-major (out x in).  Wave w computes output columns
-// [64w, 64w + 64) for all 128 rows: 8 x 4 accumulator tiles.
-template <bool GELU>
+// H[ROWS][0:K] (LDS) -> act(H . W^T + b) written back into H[ROWS][0:256].
+// W: [256][ldw] bf16 row-major (out x in).  NW waves; wave w computes output
+// columns [256/NW w, 256/NW (w+1)) for all ROWS rows.  Every output element
+// accumulates its K-chunks of 32 in increasing order whatever ROWS and NW
+// are, so all tile shapes give bit-identical results.
+template <bool GELU, int ROWS, int NW>
 __device__ __forceinline__ void mt_layer256(tbf16* H, const tbf16* __restrict__ W, const float* __restrict__ b, int K,
                                             int ldw, int lane, int w) {
-  tfloatx4 acc[8][4];
+  constexpr int IT = ROWS / 16, JT = 16 / NW;
+  tfloatx4 acc[IT][JT];
 #pragma unroll
-  for (int i = 0; i < 8; i++)
+  for (int i = 0; i < IT; i++)
 #pragma unroll
-    for (int j = 0; j < 4; j++) acc[i][j] = tfloatx4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < JT; j++) acc[i][j] = tfloatx4{0.f, 0.f, 0.f, 0.f};
   const int lr = lane & 15, lk = (lane >> 4) * 8;
-  tbf16x8 bn[4];
+  const int cb = 16 * JT * w;
+  tbf16x8 bn[JT];
 #pragma unroll
-  for (int j = 0; j < 4; j++) bn[j] = *(const tbf16x8*)&W[(long)(64 * w + 16 * j + lr) * ldw + lk];
+  for (int j = 0; j < JT; j++) bn[j] = *(const tbf16x8*)&W[(long)(cb + 16 * j + lr) * ldw + lk];
   for (int k0 = 0; k0 < K; k0 += 32) {
-    tbf16x8 bc[4];
+    tbf16x8 bc[JT];
 #pragma unroll
-    for (int j = 0; j < 4; j++) bc[j] = bn[j];
+    for (int j = 0; j < JT; j++) bc[j] = bn[j];
     if (k0 + 32 < K) {  // prefetch the next weight fragments
 #pragma unroll
-      for (int j = 0; j < 4; j++) bn[j] = *(const tbf16x8*)&W[(long)(64 * w + 16 * j + lr) * ldw + k0 + 32 + lk];
+      for (int j = 0; j < JT; j++) bn[j] = *(const tbf16x8*)&W[(long)(cb + 16 * j + lr) * ldw + k0 + 32 + lk];
     }
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
+    for (int i = 0; i < IT; i++) {
       const tbf16x8 a = *(const tbf16x8*)&H[(16 * i + lr) * MT_LDH + k0 + lk];
 #pragma unroll
-      for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bc[j], acc[i][j], 0, 0, 0);
+      for (int j = 0; j < JT; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bc[j], acc[i][j], 0, 0, 0);
     }
   }
   __syncthreads();  // every wave has read its input rows
 #pragma unroll
-  for (int j = 0; j < 4; j++) {
-    const int col = 64 * w + 16 * j + lr;
+  for (int j = 0; j < JT; j++) {
+    const int col = cb + 16 * j + lr;
     const float bias = b[col];
 #pragma unroll
-    for (int i = 0; i < 8; i++)
+    for (int i = 0; i < IT; i++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const float v = acc[i][j][r] + bias;
@@ -69,25 +74,30 @@ __device__ __forceinline__ void mt_layer256(tbf16* H, const tbf16* __restrict__ 
   __syncthreads();
 }
 
-// Output layer, first 16 output columns: O[128][16] (fp32, may alias H) =
-// H[128][0:K] . W[0:16]^T + b.  Wave w takes rows [32w, 32w + 32).
+// Output layer, first 16 output columns: O[ROWS][16] (fp32, may alias H) =
+// H[ROWS][0:K] . W[0:16]^T + b.  Wave w takes rows [ROWS/NW w, ROWS/NW (w+1)).
+template <int ROWS, int NW>
 __device__ __forceinline__ void mt_out16(const tbf16* H, const tbf16* __restrict__ W, const float* __restrict__ b,
                                          int K, int ldw, float* O, int lane, int w) {
-  tfloatx4 acc[2] = {tfloatx4{0.f, 0.f, 0.f, 0.f}, tfloatx4{0.f, 0.f, 0.f, 0.f}};
+  constexpr int RW = ROWS / NW, IT = RW / 16;
+  static_assert(IT >= 1, "at least 16 rows per wave");
+  tfloatx4 acc[IT];
+#pragma unroll
+  for (int i = 0; i < IT; i++) acc[i] = tfloatx4{0.f, 0.f, 0.f, 0.f};
   const int lr = lane & 15, lk = (lane >> 4) * 8;
   for (int k0 = 0; k0 < K; k0 += 32) {
     const tbf16x8 bf = *(const tbf16x8*)&W[(long)lr * ldw + k0 + lk];
 #pragma unroll
-    for (int i = 0; i < 2; i++) {
-      const tbf16x8 a = *(const tbf16x8*)&H[(32 * w + 16 * i + lr) * MT_LDH + k0 + lk];
+    for (int i = 0; i < IT; i++) {
+      const tbf16x8 a = *(const tbf16x8*)&H[(RW * w + 16 * i + lr) * MT_LDH + k0 + lk];
       acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bf, acc[i], 0, 0, 0);
     }
   }
   __syncthreads();  // O may alias H
   const float bias = b[lr];
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < IT; i++)
 #pragma unroll
-    for (int r = 0; r < 4; r++) O[(32 * w + 16 * i + (lane >> 4) * 4 + r) * 16 + lr] = acc[i][r] + bias;
+    for (int r = 0; r < 4; r++) O[(RW * w + 16 * i + (lane >> 4) * 4 + r) * 16 + lr] = acc[i][r] + bias;
   __syncthreads();
 }

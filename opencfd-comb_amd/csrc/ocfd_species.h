@@ -305,6 +305,7 @@ __global__ __launch_bounds__(256) void k_species_sweep(DevState S) {
   __shared__ double sF[2 * NP * NPT];
   __shared__ double sH[2 * NP * NIF];
   __shared__ uint8_t sK[2 * NP * NIF];
+  __shared__ PencilBase<DIR, NP> PB;
   const int ns = S.ns;
   const int n = S.n[DIR];
   const long nb = (long)gridDim.x * gridDim.y;
@@ -321,14 +322,14 @@ __global__ __launch_bounds__(256) void k_species_sweep(DevState S) {
   const double* M2 = DIR == 0 ? S.Aky1 : DIR == 1 ? S.Aiy1 : S.Asy1;
   const double* M3 = DIR == 0 ? S.Akz1 : DIR == 1 ? S.Aiz1 : S.Asz1;
   const int tid = threadIdx.x;
+  pencil_bases<DIR, NP>(S, pen0, npen, PB);
+  __syncthreads();
   for (int t = tid; t < NP * NPT; t += blockDim.x) {
     const int pl = DIR == 0 ? t / NPT : t % NP;
     const int sl = DIR == 0 ? t % NPT : t / NP;
     const long pen = pen0 + pl;
     if (pen >= npen || sl > ncell + 7) continue;
-    int a0, a1;
-    pencil_ijk<DIR>(S, pen, a0, a1);
-    const long q = sweep_hq<DIR>(S, a0, a1, c0 - 4 + sl);
+    const long q = PB.hq1[pl] + (long)(wrap(c0 - 4 + sl, n, S.per[DIR]) - 1) * S.sh[DIR];
     sQ[lp(pl, sl)] = q;
     sV[lp(pl, sl)] = M1[q] * S.u[q] + M2[q] * S.v[q] + M3[q] * S.w[q];
   }
@@ -341,7 +342,7 @@ __global__ __launch_bounds__(256) void k_species_sweep(DevState S) {
     const long pen = pen0 + pl;
     if (pen >= npen || l > ncell) continue;
     const int s_if = c0 - 1 + l;
-    const int mask = hyb ? (int)S.mask[DIR][mask_index<DIR>(S, pen, s_if)] : 0;
+    const int mask = hyb ? (int)S.mask[DIR][PB.mb[pl] + (long)s_if * mask_stride<DIR>(S)] : 0;
     sK[t] = (uint8_t)recon_kind(S, DIR, s_if, side, mask);
   }
   const double h1 = S.h_1[DIR];
@@ -378,9 +379,7 @@ __global__ __launch_bounds__(256) void k_species_sweep(DevState S) {
       const double hp1 = sH[lh(pl, c + 1)], hp0 = sH[lh(pl, c)];
       const double hm1 = sH[NP * NIF + lh(pl, c + 1)], hm0 = sH[NP * NIF + lh(pl, c)];
       const double val = (hp1 - hp0 + hm1 - hm0) * h1;
-      int a0, a1;
-      pencil_ijk<DIR>(S, pen, a0, a1);
-      const long ci = sweep_iq<DIR>(S, a0, a1, c0 + c) + (long)m * S.ni;
+      const long ci = PB.iq1[pl] + (long)(c0 + c - 1) * S.si[DIR] + (long)m * S.ni;
       if (DIR == 0) {
         S.dus[ci] = val;
       } else if (DIR == 1) {

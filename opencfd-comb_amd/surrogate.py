@@ -269,6 +269,25 @@ class Surrogate:
                                                        _vp(flags.data_ptr()) if flags is not None else None))
         return flags
 
+    def autotune(self, rho, E, dt, reps=3):
+        """Time every tile shape of the fused kernel on scratch copies of the
+        batch and keep the fastest (bit-identical results).  Returns
+        (rows, waves, ms per shape)."""
+        n = self.chem._args(rho, E, dt)
+        ms = (ctypes.c_float * 3)()
+        self._check(self.lib.ocfd_chem_surrogate_autotune(n, _vp(rho.data_ptr()), _vp(E.data_ptr()),
+                                                          _vp(dt.data_ptr()), int(reps), ms))
+        rows, waves = self.tile()
+        return rows, waves, list(ms)
+
+    def tile(self, variant=-1):
+        """(cells per workgroup, waves) of the shape in use; variant >= 0 forces one."""
+        r, w = ctypes.c_int32(0), ctypes.c_int32(0)
+        v = self.lib.ocfd_chem_surrogate_tile(int(variant), ctypes.byref(r), ctypes.byref(w))
+        if v < 0:
+            self._check(1)
+        return r.value, w.value
+
     def fallbacks(self):
         v = self.lib.ocfd_chem_surrogate_fallbacks()
         if v < 0:

@@ -132,3 +132,48 @@ def test_domain_check_sends_cells_to_fallback(sur):
     S.step(rho, E, dt, fallback=True, flags=flags)
     f = flags.cpu().numpy()
     assert np.all((f[:100] & 1) != 0)
+
+
+def test_tile_shapes_bitwise_identical_and_autotune(sur):
+    """Every tile shape of the fused kernel gives the same bits; the
+    autotuner leaves the inputs untouched and keeps one of the shapes."""
+    import torch
+    S, ch, M, test = sur
+    outs = []
+    for v in range(3):
+        S.tile(v)
+        rho, E, flags = run(S, test, fallback=False)
+        outs.append((rho.cpu().numpy(), E.cpu().numpy(), flags.cpu().numpy()))
+    for o in outs[1:]:
+        assert all(np.array_equal(a, b) for a, b in zip(outs[0], o))
+    r0, e0 = test["rho0"].clone(), test["E0"].clone()
+    rows, waves, ms = S.autotune(r0, e0, test["dt"], reps=2)
+    print("tile shapes (ms):", ms, "chosen", rows, "x", waves)
+    assert torch.equal(r0, test["rho0"]) and torch.equal(e0, test["E0"])
+    assert (rows, waves) in ((128, 4), (128, 8), (64, 4)) and all(m > 0 for m in ms)
+
+
+def test_training_gains_over_untrained_network(sur):
+    """What the training buys: the same network at its initial weights sends
+    most cells to the fallback or errs by far more; after training most cells
+    are accepted and the error against the plain integrator is small."""
+    import torch
+    S, ch, M, test = sur
+    trained = S.net.get_params()
+    Tp = S.temperature(test["rho1"], test["E1"]).cpu().numpy()
+    try:
+        S.net.set_params(S.net.init_params(5))       # untrained, same architecture
+        rho, E, flags = run(S, test, fallback=False)
+        acc_u = flags.cpu().numpy() == 0
+        err_u = np.abs(S.temperature(rho, E).cpu().numpy() - Tp)[acc_u]
+    finally:
+        S.net.set_params(trained)
+    rho, E, flags = run(S, test, fallback=False)
+    acc_t = flags.cpu().numpy() == 0
+    err_t = np.abs(S.temperature(rho, E).cpu().numpy() - Tp)[acc_t]
+    p99_u = np.percentile(err_u, 99) if err_u.size else np.inf
+    p99_t = np.percentile(err_t, 99)
+    print(f"accepted: untrained {acc_u.mean():.3f}, trained {acc_t.mean():.3f}; "
+          f"|dT| p99 untrained {p99_u:.1f} K, trained {p99_t:.2f} K")
+    assert acc_t.mean() > 0.5
+    assert acc_u.mean() < 0.5 * acc_t.mean() or p99_u > 10 * p99_t
