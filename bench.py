@@ -293,11 +293,14 @@ def measured_traffic(kernel):
     path = os.path.join(ROOT, "profiles", "flat_plate_640x120x64_traffic.md")
     if not os.path.exists(path):
         return None
+    col = None
     for line in open(path):
         cells = [c.strip() for c in line.split("|")]
-        if len(cells) > 5 and cells[1] == kernel:
+        if col is None and "traffic (MiB)" in cells:
+            col = cells.index("traffic (MiB)")
+        elif col is not None and len(cells) > col and cells[1] == kernel:
             try:
-                return float(cells[5]) * 2 ** 20
+                return float(cells[col]) * 2 ** 20
             except ValueError:
                 return None
     return None
