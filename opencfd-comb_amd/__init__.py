@@ -66,7 +66,7 @@ class OcfdCfg(ctypes.Structure):
 def make_cfg(nx, ny, nz, *, periodic=(1, 1, 1), scheme_invis=SCHEME_WENO7, scheme_vis=SCHEME_CD6,
              flux_splitting=SPLIT_SW, if_viscous=1, ibc=BC_NONE, ud7l_diss=1.0, hybrid_para=None,
              amu0=1.789e-5, pr=0.72, sc=1.0, mass_force=None, bc_para=None, scheme_boundary=None,
-             n_spec=1, if_real_gas=0, flag_vis=0, flag_cond=0, flag_diff=0):
+             n_spec=1, if_real_gas=0, flag_vis=0, flag_cond=0, flag_diff=0, if_scheme_character=0):
     """Build an ``ocfd_cfg`` with the namelist defaults of the reference
     (code/mpi-io/OCFD_readpara.f90:65-110) for fields not given."""
     c = OcfdCfg()
@@ -74,7 +74,7 @@ def make_cfg(nx, ny, nz, *, periodic=(1, 1, 1), scheme_invis=SCHEME_WENO7, schem
     for a in range(3):
         c.periodic[a] = int(periodic[a])
     c.scheme_invis, c.scheme_vis, c.flux_splitting = scheme_invis, scheme_vis, flux_splitting
-    c.if_viscous, c.if_scheme_character, c.ibc = if_viscous, 0, ibc
+    c.if_viscous, c.if_scheme_character, c.ibc = if_viscous, if_scheme_character, ibc
     for a in range(6):
         c.scheme_boundary[a] = 0 if scheme_boundary is None else int(scheme_boundary[a])
         c.ghost_cell[a] = 0

@@ -330,6 +330,179 @@ __global__ __launch_bounds__(256) void k_sweep(DevState S) {
   }
 }
 
+// ===========================================================================
+// Characteristic flux (IF_Scheme_Character = 1): du_inviscous_Character and
+// flux_charteric, OCFD_inviscous_character.f90:3-314.  At each interface the
+// split fluxes of the stencil are projected on the left eigenvectors S of the
+// arithmetic-mean state, reconstructed component by component (the same
+// stencils and closures as the non-characteristic sweep), summed and
+// projected back with S^-1.  Interfaces of the hybrid's linear scheme are not
+// projected; non-periodic faces with Scheme_boundary = 0 take the
+// non-reflecting one-sided formula on the physical fluxes (:288-312).
+// ===========================================================================
+__device__ __forceinline__ void char_matrices(double u1, double v1, double w1, double c1, double p1, double Et1,
+                                              double a1, double a2, double a3, double S[5][5], double S1[5][5]) {
+  double ss = sqrt(a1 * a1 + a2 * a2 + a3 * a3);
+  const double n1 = a1 / ss, n2 = a2 / ss, n3 = a3 / ss;
+  double l1, l2, l3;
+  if (fabs(n3) <= fabs(n2)) {
+    ss = sqrt(n1 * n1 + n2 * n2);
+    l1 = -n2 / ss;
+    l2 = n1 / ss;
+    l3 = 0.0;
+  } else {
+    ss = sqrt(n1 * n1 + n3 * n3);
+    l1 = -n3 / ss;
+    l2 = 0.0;
+    l3 = n1 / ss;
+  }
+  const double m1 = n2 * l3 - n3 * l2, m2 = n3 * l1 - n1 * l3, m3 = n1 * l2 - n2 * l1;
+  const double un = u1 * n1 + v1 * n2 + w1 * n3, ul = u1 * l1 + v1 * l2 + w1 * l3, um = u1 * m1 + v1 * m2 + w1 * m3;
+  const double V2 = (u1 * u1 + v1 * v1 + w1 * w1) * 0.5;
+  const double gamma = p1 / Et1 + 1.0;
+  const double KK = (gamma - 1.0) / (c1 * c1);
+  const double KK1 = KK * 0.5, X1 = 1.0 / (2.0 * c1), H = V2 + 1.0 / KK;
+  S[0][0] = 1.0 - KK * V2; S[0][1] = KK * u1; S[0][2] = KK * v1; S[0][3] = KK * w1; S[0][4] = -KK;
+  S[1][0] = -ul; S[1][1] = l1; S[1][2] = l2; S[1][3] = l3; S[1][4] = 0.0;
+  S[2][0] = -um; S[2][1] = m1; S[2][2] = m2; S[2][3] = m3; S[2][4] = 0.0;
+  S[3][0] = KK1 * V2 + X1 * un; S[3][1] = -X1 * n1 - KK1 * u1; S[3][2] = -X1 * n2 - KK1 * v1;
+  S[3][3] = -X1 * n3 - KK1 * w1; S[3][4] = KK1;
+  S[4][0] = KK1 * V2 - X1 * un; S[4][1] = X1 * n1 - KK1 * u1; S[4][2] = X1 * n2 - KK1 * v1;
+  S[4][3] = X1 * n3 - KK1 * w1; S[4][4] = KK1;
+  S1[0][0] = 1.0; S1[0][1] = 0.0; S1[0][2] = 0.0; S1[0][3] = 1.0; S1[0][4] = 1.0;
+  S1[1][0] = u1; S1[1][1] = l1; S1[1][2] = m1; S1[1][3] = u1 - c1 * n1; S1[1][4] = u1 + c1 * n1;
+  S1[2][0] = v1; S1[2][1] = l2; S1[2][2] = m2; S1[2][3] = v1 - c1 * n2; S1[2][4] = v1 + c1 * n2;
+  S1[3][0] = w1; S1[3][1] = l3; S1[3][2] = m3; S1[3][3] = w1 - c1 * n3; S1[3][4] = w1 + c1 * n3;
+  S1[4][0] = V2; S1[4][1] = ul; S1[4][2] = um; S1[4][3] = H - c1 * un; S1[4][4] = H + c1 * un;
+}
+
+template <int DIR>
+__global__ __launch_bounds__(256) void k_sweep_char(DevState S) {
+  constexpr int NP = SweepTile<DIR, 0>::NP, TS = SweepTile<DIR, 0>::TS;
+  constexpr int NPT = TS + 8, NIF = TS + 1;
+  __shared__ double sF[10 * NP * NPT];
+  __shared__ double sH[5 * NP * NIF];
+  __shared__ PencilBase<DIR, NP> PB;
+  const int n = S.n[DIR];
+  const long nb = (long)gridDim.x * gridDim.y;
+  const long lb = xcd_block((long)blockIdx.y * gridDim.x + blockIdx.x, nb);
+  const int bx = (int)(lb % gridDim.x);
+  const long by = lb / gridDim.x;
+  const int c0 = bx * TS + 1;
+  const int ncell = min(TS, n - c0 + 1);
+  const long npen = DIR == 0 ? (long)S.ny * S.nz : DIR == 1 ? (long)S.nx * S.nz : (long)S.nx * S.ny;
+  const long pen0 = by * NP;
+  auto lp = [](int pl, int s) { return DIR == 0 ? pl * NPT + s : s * NP + pl; };
+  auto lh = [](int pl, int l) { return DIR == 0 ? pl * NIF + l : l * NP + pl; };
+  const double* M1 = DIR == 0 ? S.Akx1 : DIR == 1 ? S.Aix1 : S.Asx1;
+  const double* M2 = DIR == 0 ? S.Aky1 : DIR == 1 ? S.Aiy1 : S.Asy1;
+  const double* M3 = DIR == 0 ? S.Akz1 : DIR == 1 ? S.Aiz1 : S.Asz1;
+  const int tid = threadIdx.x;
+  pencil_bases<DIR, NP>(S, pen0, npen, PB);
+  __syncthreads();
+  for (int t = tid; t < NP * NPT; t += blockDim.x) {
+    const int pl = DIR == 0 ? t / NPT : t % NP;
+    const int sl = DIR == 0 ? t % NPT : t / NP;
+    if (pen0 + pl >= npen || sl > ncell + 7) continue;
+    const long q = PB.hq1[pl] + (long)(wrap(c0 - 4 + sl, n, S.per[DIR]) - 1) * S.sh[DIR];
+    double fp[5], fm[5];
+    split_point(S.split, S.Et[q], S.p[q], S.d[q], S.u[q], S.v[q], S.w[q], M1[q], M2[q], M3[q], fp, fm);
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      sF[m * NP * NPT + lp(pl, sl)] = fp[m];
+      sF[(5 + m) * NP * NPT + lp(pl, sl)] = fm[m];
+    }
+  }
+  __syncthreads();
+  const bool hyb = S.scheme == OCFD_SCHEME_HYBRID;
+  const int blo = S.bidx[DIR][0], bhi = S.bidx[DIR][1];
+  const int sblo = S.sb[2 * DIR], sbhi = S.sb[2 * DIR + 1];
+  for (int t = tid; t < NP * NIF; t += blockDim.x) {
+    const int pl = DIR == 0 ? t / NIF : t % NP;
+    const int l = DIR == 0 ? t % NIF : t / NP;
+    if (pen0 + pl >= npen || l > ncell) continue;
+    const int s_if = c0 - 1 + l;
+    const int mask = hyb ? (int)S.mask[DIR][PB.mb[pl] + (long)s_if * mask_stride<DIR>(S)] : 0;
+    double hh[5];
+    const bool face_lo = blo == 1 && sblo == 0 && s_if == 0, face_hi = bhi == 1 && sbhi == 0 && s_if == n;
+    if (face_lo || face_hi) {
+      // tile point of position s: s - (c0 - 4)
+#pragma unroll
+      for (int m = 0; m < 5; m++) {
+        const double* P = sF + m * NP * NPT;
+        const double* M = sF + (5 + m) * NP * NPT;
+        auto at = [&](const double* F, int sp) { return F[lp(pl, sp - c0 + 4)]; };
+        hh[m] = face_lo ? (2.0 * at(P, 1) + 5.0 * at(P, 2) - at(P, 3) + 2.0 * at(M, 3) - 7.0 * at(M, 2) +
+                           11.0 * at(M, 1)) / 6.0
+                        : (2.0 * at(P, n - 2) - 7.0 * at(P, n - 1) + 11.0 * at(P, n) + 2.0 * at(M, n) +
+                           5.0 * at(M, n - 1) - at(M, n - 2)) / 6.0;
+      }
+    } else {
+      const int kP = recon_kind(S, DIR, s_if, 0, mask), kM = recon_kind(S, DIR, s_if, 1, mask);
+      if (hyb && mask == 1) {
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+          double wP[8], wM[8];
+#pragma unroll
+          for (int qq = 0; qq < 8; qq++) {
+            wP[qq] = sF[m * NP * NPT + lp(pl, l + qq)];
+            wM[qq] = sF[(5 + m) * NP * NPT + lp(pl, l + 7 - qq)];
+          }
+          hh[m] = recon(kP, wP, S.ud7l) + recon(kM, wM, S.ud7l);
+        }
+      } else {
+        const long qa = PB.hq1[pl] + (long)(wrap(s_if, n, S.per[DIR]) - 1) * S.sh[DIR];
+        const long qb = PB.hq1[pl] + (long)(wrap(s_if + 1, n, S.per[DIR]) - 1) * S.sh[DIR];
+        const double ca = sqrt((S.p[qa] / S.Et[qa] + 1.0) * S.p[qa] / S.d[qa]);
+        const double cb = sqrt((S.p[qb] / S.Et[qb] + 1.0) * S.p[qb] / S.d[qb]);
+        double Sm[5][5], S1[5][5];
+        char_matrices((S.u[qa] + S.u[qb]) * 0.5, (S.v[qa] + S.v[qb]) * 0.5, (S.w[qa] + S.w[qb]) * 0.5,
+                      (ca + cb) * 0.5, (S.p[qa] + S.p[qb]) * 0.5, (S.Et[qa] + S.Et[qb]) * 0.5,
+                      (M1[qa] + M1[qb]) * 0.5, (M2[qa] + M2[qb]) * 0.5, (M3[qa] + M3[qb]) * 0.5, Sm, S1);
+        double h0[5];
+#pragma unroll
+        for (int r = 0; r < 5; r++) {
+          double wP[8], wM[8];
+#pragma unroll
+          for (int qq = 0; qq < 8; qq++) {
+            const int a = lp(pl, l + qq), b = lp(pl, l + 7 - qq);
+            wP[qq] = Sm[r][0] * sF[a] + Sm[r][1] * sF[NP * NPT + a] + Sm[r][2] * sF[2 * NP * NPT + a] +
+                     Sm[r][3] * sF[3 * NP * NPT + a] + Sm[r][4] * sF[4 * NP * NPT + a];
+            wM[qq] = Sm[r][0] * sF[5 * NP * NPT + b] + Sm[r][1] * sF[6 * NP * NPT + b] +
+                     Sm[r][2] * sF[7 * NP * NPT + b] + Sm[r][3] * sF[8 * NP * NPT + b] +
+                     Sm[r][4] * sF[9 * NP * NPT + b];
+          }
+          h0[r] = recon(kP, wP, S.ud7l) + recon(kM, wM, S.ud7l);
+        }
+#pragma unroll
+        for (int r = 0; r < 5; r++)
+          hh[r] = S1[r][0] * h0[0] + S1[r][1] * h0[1] + S1[r][2] * h0[2] + S1[r][3] * h0[3] + S1[r][4] * h0[4];
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 5; m++) sH[m * NP * NIF + lh(pl, l)] = hh[m];
+  }
+  __syncthreads();
+  const double h1 = S.h_1[DIR];
+  for (int t = tid; t < 5 * NP * TS; t += blockDim.x) {
+    const int m = t / (NP * TS);
+    const int r = t % (NP * TS);
+    const int pl = DIR == 0 ? r / TS : r % NP;
+    const int c = DIR == 0 ? r % TS : r / NP;
+    if (pen0 + pl >= npen || c >= ncell) continue;
+    const double val = (sH[m * NP * NIF + lh(pl, c + 1)] - sH[m * NP * NIF + lh(pl, c)]) * h1;
+    const long ci = PB.iq1[pl] + (long)(c0 + c - 1) * S.si[DIR] + (long)m * S.ni;
+    if (DIR == 0) {
+      S.du[ci] = val;
+    } else if (DIR == 1) {
+      S.du[ci] = S.du[ci] + val;
+    } else {
+      const long q = PB.hq1[pl] + (long)(c0 + c - 1) * S.sh[DIR];
+      S.du[ci] = -(S.du[ci] + val) * S.Ajac[q];
+    }
+  }
+}
+
 #include "ocfd_species.h"
 
 // ===========================================================================
@@ -1139,7 +1312,18 @@ void launch_sweep_variant(int v) {
 }
 
 template <int DIR>
-void launch_sweep() { launch_sweep_variant<DIR>(G.sweep_variant[DIR]); }
+void launch_sweep() {
+  if (G.cfg.if_scheme_character) {
+    const DevState& S = G.S;
+    constexpr int NP = SweepTile<DIR, 0>::NP, TS = SweepTile<DIR, 0>::TS;
+    const long npen = DIR == 0 ? (long)S.ny * S.nz : DIR == 1 ? (long)S.nx * S.nz : (long)S.nx * S.ny;
+    dim3 g((unsigned)((S.n[DIR] + TS - 1) / TS), (unsigned)((npen + NP - 1) / NP));
+    static const char* names[3] = {"k_sweep_char_x", "k_sweep_char_y", "k_sweep_char_z"};
+    launch(names[DIR], k_sweep_char<DIR>, g, dim3(256), S);
+    return;
+  }
+  launch_sweep_variant<DIR>(G.sweep_variant[DIR]);
+}
 
 template <int DIR>
 void launch_masks() {
@@ -1334,7 +1518,7 @@ int32_t ocfd_init(const ocfd_cfg* cfg) {
   if (c.lap != OCFD_LAP) return fail("ocfd_init: LAP must be 4");
   if (c.nx < 8 || c.ny < 8 || c.nz < 8) return fail("ocfd_init: each dimension needs at least 8 points");
   if (c.n_spec < 1 || c.n_spec > OCFD_NS_MAX) return fail("ocfd_init: N_SPEC must be 1..9");
-  if (c.if_scheme_character != 0) return fail("ocfd_init: IF_Scheme_Character=1 is not supported");
+  if (c.if_scheme_character != 0 && c.if_scheme_character != 1) return fail("ocfd_init: IF_Scheme_Character must be 0 or 1");
   if (c.n_spec == 1 && c.if_real_gas != 0) return fail("ocfd_init: the N_SPEC == 1 flow path is the perfect gas (IF_RealGas=0)");
   if (c.n_spec == 1 && (c.flag_vis != 0 || c.flag_cond != 0))
     return fail("ocfd_init: N_SPEC == 1 uses Sutherland/Prandtl transport");
@@ -1850,6 +2034,7 @@ int32_t ocfd_ns_step(double dt, int32_t nsub, int32_t mode, int32_t reaction) {
 // of the choice.
 int32_t ocfd_autotune(int32_t reps) {
   if (check_ready()) return 1;
+  if (G.cfg.if_scheme_character) return 0;  // one tile shape for the characteristic sweep
   if (reps < 1) reps = 3;
   drop_graph();
   const int saved_timing = G.timing;

@@ -53,6 +53,7 @@ def load(ocfd_module):
         "orc_set_transport": (i32, [vp, vp, vp]), "orc_set_bc_jet": (i32, [vp]),
         "orc_ns_step": (i32, [dbl, i32, i32, i32]), "orc_chemical_time_advance": (i32, [dbl, i32]),
         "orc_comput_T_P": (i32, []), "orc_du_species": (i32, [vp]), "orc_download_qp": (i32, [vp]),
+        "orc_char_matrices": (i32, [vp, vp, vp]),
         "orc_comput_T": (dbl, [vp, dbl]), "orc_comput_E": (dbl, [vp, dbl]), "orc_comput_Cp": (dbl, [dbl, vp, dbl]),
     }
     for name, (res, args) in sig.items():

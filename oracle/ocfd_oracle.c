@@ -770,6 +770,253 @@ static void du_inviscous(void) {
 }
 
 /* ------------------------------------------------------------------ */
+/* Characteristic flux (IF_Scheme_Character = 1),
+ * OCFD_inviscous_character.f90:3-425. */
+
+/* S (left eigenvectors) and S1 = S^-1 (right eigenvectors) at the
+ * arithmetic mean state of an interface, flux_charteric :209-257 */
+static void char_matrices(double u1, double v1, double w1, double c1, double p1, double Et1, double a1, double a2,
+                          double a3, double S[5][5], double S1[5][5]) {
+  double ss = sqrt(a1 * a1 + a2 * a2 + a3 * a3);
+  double n1 = a1 / ss, n2 = a2 / ss, n3 = a3 / ss;
+  double l1, l2, l3;
+  if (fabs(n3) <= fabs(n2)) {
+    ss = sqrt(n1 * n1 + n2 * n2);
+    l1 = -n2 / ss; l2 = n1 / ss; l3 = 0.0;
+  } else {
+    ss = sqrt(n1 * n1 + n3 * n3);
+    l1 = -n3 / ss; l2 = 0.0; l3 = n1 / ss;
+  }
+  double m1 = n2 * l3 - n3 * l2, m2 = n3 * l1 - n1 * l3, m3 = n1 * l2 - n2 * l1;
+  double un = u1 * n1 + v1 * n2 + w1 * n3, ul = u1 * l1 + v1 * l2 + w1 * l3, um = u1 * m1 + v1 * m2 + w1 * m3;
+  double V2 = (u1 * u1 + v1 * v1 + w1 * w1) * 0.5;
+  double gamma = p1 / Et1 + 1.0;
+  double KK = (gamma - 1.0) / (c1 * c1);
+  double KK1 = KK * 0.5, X1 = 1.0 / (2.0 * c1), H = V2 + 1.0 / KK;
+  double s[5][5] = {{1.0 - KK * V2, KK * u1, KK * v1, KK * w1, -KK},
+                    {-ul, l1, l2, l3, 0.0},
+                    {-um, m1, m2, m3, 0.0},
+                    {KK1 * V2 + X1 * un, -X1 * n1 - KK1 * u1, -X1 * n2 - KK1 * v1, -X1 * n3 - KK1 * w1, KK1},
+                    {KK1 * V2 - X1 * un, X1 * n1 - KK1 * u1, X1 * n2 - KK1 * v1, X1 * n3 - KK1 * w1, KK1}};
+  double r[5][5] = {{1.0, 0.0, 0.0, 1.0, 1.0},
+                    {u1, l1, m1, u1 - c1 * n1, u1 + c1 * n1},
+                    {v1, l2, m2, v1 - c1 * n2, v1 + c1 * n2},
+                    {w1, l3, m3, w1 - c1 * n3, w1 + c1 * n3},
+                    {V2, ul, um, H - c1 * un, H + c1 * un}};
+  memcpy(S, s, sizeof s);
+  memcpy(S1, r, sizeof r);
+}
+int orc_char_matrices(const double* st, double* S, double* S1) {
+  char_matrices(st[0], st[1], st[2], st[3], st[4], st[5], st[6], st[7], st[8], (double(*)[5])S, (double(*)[5])S1);
+  return 0;
+}
+
+/* flux_x1 (flux+ at one interface on a window ff(mm), mm = Ka1..Kb1),
+ * OCFD_inviscous_character.f90:317-371 */
+static double char_flux_x1(const double* ff, int i, int n, const int* bidx, int ihyb, const int* sb) {
+  int Ka = G.Ka1, Kb = G.Kb1;
+  if ((bidx[0] == 0 || i > -Ka || sb[0] == -1) && (bidx[1] == 0 || i <= n - Kb || sb[1] == -1)) {
+    double s8[8];
+    switch (G.cfg.scheme_invis) {
+      case OCFD_SCHEME_OMP6:
+        for (int o = -3; o <= 4; o++) s8[o + 3] = ff[o];
+        return omp6_pt(s8);
+      case OCFD_SCHEME_WENO7: return weno7_pt(ff[-3], ff[-2], ff[-1], ff[0], ff[1], ff[2], ff[3]);
+      case OCFD_SCHEME_WENO5: return hh_weno5P(ff);
+      case OCFD_SCHEME_UD7L: return ud7l_pt(ff[-3], ff[-2], ff[-1], ff[0], ff[1], ff[2], ff[3], ff[4]);
+      default: /* hybrid, OCFD_Schemes_1.f90 hh_HybridP */
+        if (ihyb == 1) return ud7l_pt(ff[-3], ff[-2], ff[-1], ff[0], ff[1], ff[2], ff[3], ff[4]);
+        if (ihyb == 2) return weno7_pt(ff[-3], ff[-2], ff[-1], ff[0], ff[1], ff[2], ff[3]);
+        return hh_weno5P(ff);
+    }
+  } else if (i <= -Ka && bidx[0] == 1 && sb[0] != -1) {
+    if (sb[0] == 0) {
+      if (i == 1) return (2.0 * ff[0] + 5.0 * ff[1] - ff[2]) / 6.0;
+      if (i == 2) return hh_weno5P_boundary(ff, DEL_LIFT);
+      return hh_weno5P(ff);
+    }
+    return hh_weno5P(ff);
+  } else if (i > n - Kb && bidx[1] == 1 && sb[1] != -1) {
+    if (sb[1] == 0 && i == n - 1) return hh_weno5P_boundary(ff, DEL_RIGHT);
+    return hh_weno5P(ff);
+  }
+  return 0.0;
+}
+
+/* flux_x2 (flux-), OCFD_inviscous_character.f90:374-425 */
+static double char_flux_x2(const double* ff, int i, int n, const int* bidx, int ihyb, const int* sb) {
+  int Ka = G.Ka2, Kb = G.Kb2;
+  if ((bidx[0] == 0 || i > -Ka || sb[0] == -1) && (bidx[1] == 0 || i <= n - Kb || sb[1] == -1)) {
+    double s8[8];
+    switch (G.cfg.scheme_invis) {
+      case OCFD_SCHEME_OMP6:
+        for (int o = -3; o <= 4; o++) s8[o + 3] = ff[1 - o];
+        return omp6_pt(s8);
+      case OCFD_SCHEME_WENO7: return weno7_pt(ff[4], ff[3], ff[2], ff[1], ff[0], ff[-1], ff[-2]);
+      case OCFD_SCHEME_WENO5: return hh_weno5M(ff);
+      case OCFD_SCHEME_UD7L: return ud7l_pt(ff[4], ff[3], ff[2], ff[1], ff[0], ff[-1], ff[-2], ff[-3]);
+      default:
+        if (ihyb == 1) return ud7l_pt(ff[4], ff[3], ff[2], ff[1], ff[0], ff[-1], ff[-2], ff[-3]);
+        if (ihyb == 2) return weno7_pt(ff[4], ff[3], ff[2], ff[1], ff[0], ff[-1], ff[-2]);
+        return hh_weno5M(ff);
+    }
+  } else if (i <= -Ka && bidx[0] == 1 && sb[0] != -1) {
+    if (sb[0] == 0 && i == 1) return hh_weno5M_boundary(ff, DEL_LIFT);
+    return hh_weno5M(ff);
+  } else if (i > n - Kb && bidx[1] == 1 && sb[1] != -1) {
+    if (sb[1] == 0) {
+      if (i == n - 1) return (2.0 * ff[1] + 5.0 * ff[0] - ff[-1]) / 6.0;
+      if (i == n - 2) return hh_weno5M_boundary(ff, DEL_RIGHT);
+      return hh_weno5M(ff);
+    }
+    return hh_weno5M(ff);
+  }
+  return 0.0;
+}
+
+/* flux_charteric, OCFD_inviscous_character.f90:165-314: hh(0:n,5) of one
+ * pencil from fp/fm [5][n+2lap] and the pencil's primitives (offset o = s+lap-1) */
+static void flux_charteric(int n, int lap, double* hh, const double* fp, const double* fm, const double* A1,
+                           const double* A2, const double* A3, const double* EE, const double* pp, const double* cc1,
+                           const double* uu, const double* vv, const double* ww, const int* bidx,
+                           const unsigned char* scm, const int* sb) {
+  int np = n + 2 * lap;
+  int ia = (bidx[0] == 0 || sb[0] != 0) ? 0 : 1;
+  int ib = (bidx[1] == 0 || sb[1] != 0) ? n : n - 1;
+  static double fpc[5][9], fmc[5][9]; /* mm = -4..4 at index mm+4; keeps values between interfaces */
+  double S[5][5], S1[5][5], h0[5];
+#define O(s) ((s) + lap - 1)
+  for (int i = ia; i <= ib; i++) {
+    int sh = scm[i];
+    if (sh == 1) {
+      for (int mm = G.Ka1; mm <= G.Kb1; mm++)
+        for (int m = 0; m < 5; m++) fpc[m][mm + 4] = fp[(long)m * np + O(i + mm)];
+      for (int mm = G.Ka2; mm <= G.Kb2; mm++)
+        for (int m = 0; m < 5; m++) fmc[m][mm + 4] = fm[(long)m * np + O(i + mm)];
+    } else {
+      int ka1 = G.Ka1, kb1 = G.Kb1, ka2 = G.Ka2, kb2 = G.Kb2;
+      if (sh == 2) { ka1 = -3; kb1 = 3; ka2 = -2; kb2 = 4; }        /* OCFD_readpara.f90:404 */
+      else if (sh == 3) { ka1 = -2; kb1 = 2; ka2 = -1; kb2 = 3; }   /* :405 */
+      int a = O(i), b = O(i + 1);
+      char_matrices((uu[a] + uu[b]) * 0.5, (vv[a] + vv[b]) * 0.5, (ww[a] + ww[b]) * 0.5, (cc1[a] + cc1[b]) * 0.5,
+                    (pp[a] + pp[b]) * 0.5, (EE[a] + EE[b]) * 0.5, (A1[a] + A1[b]) * 0.5, (A2[a] + A2[b]) * 0.5,
+                    (A3[a] + A3[b]) * 0.5, S, S1);
+      for (int mm = ka1; mm <= kb1; mm++) {
+        const double* f = fp + O(i + mm);
+        for (int r = 0; r < 5; r++)
+          fpc[r][mm + 4] = S[r][0] * f[0] + S[r][1] * f[np] + S[r][2] * f[2L * np] + S[r][3] * f[3L * np] +
+                           S[r][4] * f[4L * np];
+      }
+      for (int mm = ka2; mm <= kb2; mm++) {
+        const double* f = fm + O(i + mm);
+        for (int r = 0; r < 5; r++)
+          fmc[r][mm + 4] = S[r][0] * f[0] + S[r][1] * f[np] + S[r][2] * f[2L * np] + S[r][3] * f[3L * np] +
+                           S[r][4] * f[4L * np];
+      }
+    }
+    for (int m = 0; m < 5; m++) {
+      double h1 = char_flux_x1(&fpc[m][4], i, n, bidx, sh, sb);
+      double h2 = char_flux_x2(&fmc[m][4], i, n, bidx, sh, sb);
+      h0[m] = h1 + h2;
+    }
+    for (int r = 0; r < 5; r++)
+      hh[(long)r * (n + 1) + i] =
+          sh == 1 ? h0[r] : S1[r][0] * h0[0] + S1[r][1] * h0[1] + S1[r][2] * h0[2] + S1[r][3] * h0[3] + S1[r][4] * h0[4];
+  }
+  /* non-reflecting faces, :288-312 */
+  if (bidx[0] == 1 && sb[0] == 0)
+    for (int m = 0; m < 5; m++) {
+      const double* P = fp + (long)m * np;
+      const double* M = fm + (long)m * np;
+      hh[(long)m * (n + 1)] = (2.0 * P[O(1)] + 5.0 * P[O(2)] - P[O(3)] + 2.0 * M[O(3)] - 7.0 * M[O(2)] + 11.0 * M[O(1)]) / 6.0;
+    }
+  if (bidx[1] == 1 && sb[1] == 0)
+    for (int m = 0; m < 5; m++) {
+      const double* P = fp + (long)m * np;
+      const double* M = fm + (long)m * np;
+      hh[(long)m * (n + 1) + n] =
+          (2.0 * P[O(n - 2)] - 7.0 * P[O(n - 1)] + 11.0 * P[O(n)] + 2.0 * M[O(n)] + 5.0 * M[O(n - 1)] - M[O(n - 2)]) / 6.0;
+    }
+#undef O
+}
+
+/* du_inviscous_Character, OCFD_inviscous_character.f90:3-162 */
+static void du_inviscous_character(void) {
+  int nmax = G.nx > G.ny ? G.nx : G.ny;
+  if (G.nz > nmax) nmax = G.nz;
+  int lap = G.lap, npm = nmax + 2 * lap;
+  double *Et1 = dalloc(npm, 0), *p1 = dalloc(npm, 0), *c1 = dalloc(npm, 0), *d1 = dalloc(npm, 0);
+  double *u1 = dalloc(npm, 0), *v1 = dalloc(npm, 0), *w1 = dalloc(npm, 0);
+  double *A1 = dalloc(npm, 0), *A2 = dalloc(npm, 0), *A3 = dalloc(npm, 0);
+  double *fp = dalloc(5L * npm, 0), *fm = dalloc(5L * npm, 0);
+  double* hh = dalloc(5L * (nmax + 1), 0);
+  unsigned char* scm = (unsigned char*)calloc((size_t)nmax + 1, 1);
+  int hyb = G.cfg.scheme_invis == OCFD_SCHEME_HYBRID;
+  double hx_1 = 1.0 / G.hx, hy_1 = 1.0 / G.hy, hz_1 = 1.0 / G.hz;
+  comput_cc();
+  for (int dir = 0; dir < 3; dir++) {
+    int n = dir == 0 ? G.nx : dir == 1 ? G.ny : G.nz;
+    long st = dir == 0 ? 1 : dir == 1 ? G.nxh : (long)G.nxh * G.nyh;
+    const double *M1 = dir == 0 ? G.Akx1 : dir == 1 ? G.Aix1 : G.Asx1;
+    const double *M2 = dir == 0 ? G.Aky1 : dir == 1 ? G.Aiy1 : G.Asy1;
+    const double *M3 = dir == 0 ? G.Akz1 : dir == 1 ? G.Aiz1 : G.Asz1;
+    const int* sb = &G.cfg.scheme_boundary[2 * dir];
+    int na = dir == 0 ? G.ny : G.nx, nb = dir == 2 ? G.ny : G.nz;
+    unsigned char* mask = dir == 0 ? G.maskx : dir == 1 ? G.masky : G.maskz;
+    for (int b = 1; b <= nb; b++)
+      for (int a = 1; a <= na; a++) {
+        int i0 = dir == 0 ? 0 : a, j0 = dir == 0 ? a : dir == 1 ? 0 : b, k0 = dir == 2 ? 0 : b;
+        long base = H(i0, j0, k0);
+        if (hyb)
+          for (int s = 0; s <= n; s++) scm[s] = set_hybrid_scheme(fmax(G.Rhybrid[base + s * st], G.Rhybrid[base + (s + 1) * st]));
+        for (int s = 1 - lap; s <= n + lap; s++) {
+          long q = base + s * st;
+          int o = s + lap - 1;
+          Et1[o] = G.Et[q]; p1[o] = G.p[q]; c1[o] = G.cc[q]; d1[o] = G.d[q];
+          u1[o] = G.u[q]; v1[o] = G.v[q]; w1[o] = G.w[q];
+          A1[o] = M1[q]; A2[o] = M2[q]; A3[o] = M3[q];
+        }
+        if (G.cfg.flux_splitting == OCFD_SPLIT_SW) split_SW(n, lap, Et1, p1, c1, d1, u1, v1, w1, A1, A2, A3, fp, fm);
+        else split_LLF(n, lap, Et1, p1, c1, d1, u1, v1, w1, A1, A2, A3, fp, fm);
+        flux_charteric(n, lap, hh, fp, fm, A1, A2, A3, Et1, p1, c1, u1, v1, w1, G.bound_index[dir], scm, sb);
+        for (int m = 0; m < 5; m++) {
+          const double* h = hh + (long)m * (n + 1);
+          for (int s = 1; s <= n; s++) {
+            int i = dir == 0 ? s : a, j = dir == 1 ? s : (dir == 0 ? a : b), k = dir == 2 ? s : b;
+            long c = I(i, j, k) + (long)m * G.ni;
+            if (dir == 0) G.du[c] = (h[s] - h[s - 1]) * hx_1;
+            else if (dir == 1) G.du[c] = G.du[c] + (h[s] - h[s - 1]) * hy_1;
+            else G.du[c] = -(G.du[c] + (h[s] - h[s - 1]) * hz_1) * G.Ajac[H(i, j, k)];
+          }
+        }
+        if (hyb) {
+          for (int s = 0; s <= n; s++) {
+            long mi;
+            if (dir == 0) mi = s + (long)(G.nx + 1) * ((a - 1) + (long)G.ny * (b - 1));
+            else if (dir == 1) mi = (a - 1) + (long)G.nx * (s + (long)(G.ny + 1) * (b - 1));
+            else mi = (a - 1) + (long)G.nx * ((b - 1) + (long)G.ny * s);
+            mask[mi] = scm[s];
+          }
+        }
+      }
+  }
+  if (G.cfg.if_mass_force) { /* :148-160 */
+    const double* mf = G.cfg.mass_force;
+    for (int k = 1; k <= G.nz; k++)
+      for (int j = 1; j <= G.ny; j++)
+        for (int i = 1; i <= G.nx; i++) {
+          long c = I(i, j, k), q = H(i, j, k);
+          G.du[c + G.ni] = G.du[c + G.ni] + G.d[q] * mf[0];
+          G.du[c + 2 * G.ni] = G.du[c + 2 * G.ni] + G.d[q] * mf[1];
+          G.du[c + 3 * G.ni] = G.du[c + 3 * G.ni] + G.d[q] * mf[2];
+          G.du[c + 4 * G.ni] = G.du[c + 4 * G.ni] + G.d[q] * (G.u[q] * mf[0] + G.v[q] * mf[1] + G.w[q] * mf[2]);
+        }
+  }
+  free(Et1); free(p1); free(c1); free(d1); free(u1); free(v1); free(w1);
+  free(A1); free(A2); free(A3); free(fp); free(fm); free(hh); free(scm);
+}
+
+/* ------------------------------------------------------------------ */
 /* Chemkin log fit exp(a1 + a2 lnT + a3 lnT^2 + a4 lnT^3) (TRANSPORT_modules.f90:276-277) */
 static double chemkin_fit(const double* a, double lT) {
   return exp(a[0] + a[1] * lT + a[2] * (lT * lT) + a[3] * pw3(lT));
@@ -1313,7 +1560,8 @@ int orc_flow_time_advance(double dt1) {
         if (G.cfg.if_viscous) du_scaler_viscous(m, G.di + (long)m * G.nh, G.dus);
         scalar_time_adv_rk3(krk, m, G.dus, dt1);
       }
-    du_inviscous();
+    if (G.cfg.if_scheme_character) du_inviscous_character();
+    else du_inviscous();
     if (G.cfg.if_viscous) du_viscous();
     time_adv_rk3(krk, dt1);
     if (comput_flow_variables()) return set_err("oracle: more than 10 negative-Et cells");
@@ -1380,7 +1628,11 @@ int orc_download_qp(double* out) {
 
 /* pieces, for unit-level comparisons */
 int orc_comput_amu_amk(void) { comput_Amu_Amk_AMD(); return 0; }
-int orc_du_inviscous(void) { du_inviscous(); return 0; }
+int orc_du_inviscous(void) {
+  if (G.cfg.if_scheme_character) du_inviscous_character();
+  else du_inviscous();
+  return 0;
+}
 int orc_du_viscous(void) { du_viscous(); return 0; }
 int orc_time_adv_rk3(int krk, double dt) { time_adv_rk3(krk, dt); return 0; }
 int orc_comput_flow_variables(void) { return comput_flow_variables(); }

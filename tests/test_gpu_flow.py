@@ -248,3 +248,58 @@ def test_cfl_dt_closed_form(pkg):
         print(case[5]["name"], dt, expect)
         assert abs(dt - expect) <= 1e-13 * expect
         g.close()
+
+
+# --------------------------------------------------------------- characteristic flux
+CHAR_SCHEMES = {"WENO5": 1, "WENO7": 2, "OMP6": 3, "UD7L": 4}
+
+
+@pytest.mark.parametrize("scheme", list(CHAR_SCHEMES))
+def test_characteristic_du_periodic(pkg, scheme):
+    case = list(fc.small_isotropic(16, scheme_invis=CHAR_SCHEMES[scheme]))
+    case[0].if_scheme_character = 1
+    g, o = pair(pkg, tuple(case))
+    g.stage_begin(1)
+    g.du_inviscous()
+    o.comput_Amu_Amk_AMD()
+    o.exchange_all()
+    o.du_inviscous()
+    d, same = fc.compare(g.download("DU"), o.download("DU"))
+    print("characteristic", scheme, d, same)
+    assert d <= 1e-12
+    g.close()
+
+
+@pytest.mark.parametrize("scheme", list(CHAR_SCHEMES) + ["HYBRID"])
+@pytest.mark.parametrize("sb", [0, 1, -1])
+def test_characteristic_du_boundaries(pkg, scheme, sb):
+    case = list(fc.small_flat_plate(48, 32, 16))
+    case[0].if_scheme_character = 1
+    if scheme != "HYBRID":
+        case[0].scheme_invis = CHAR_SCHEMES[scheme]
+    for a in range(6):
+        case[0].scheme_boundary[a] = sb
+    g, o = pair(pkg, tuple(case))
+    g.stage_begin(1)
+    g.du_inviscous()
+    o.comput_Amu_Amk_AMD()
+    o.exchange_all()
+    if scheme == "HYBRID":
+        o.comput_Rhybrid()
+    o.du_inviscous()
+    d, same = fc.compare(g.download("DU"), o.download("DU"))
+    print("characteristic", scheme, sb, d, same)
+    assert d <= 1e-12
+    g.close()
+
+
+def test_characteristic_rk3_step(pkg):
+    for case in (fc.small_isotropic(16), fc.small_flat_plate(48, 32, 16)):
+        case[0].if_scheme_character = 1
+        g, o = pair(pkg, case)
+        dt = case[5]["dt"]
+        g.flow_time_advance(dt)
+        o.flow_time_advance(dt)
+        rep = fc.compare_state(g, o, tol=1e-12)
+        print(case[5]["name"], {k: f"{v[0]:.1e}/{v[1]:.3f}" for k, v in rep.items()})
+        g.close()
