@@ -910,6 +910,145 @@ __global__ void k_bc_faces(DevState S) {
   }
 }
 
+// ===========================================================================
+// Ghost cells (Ghost_Cell(1:6) = 1 or 2, OCFD_GhostCell.f90).  Face f:
+// 0 i-, 1 i+, 2 j-, 3 j+, 4 k-, 5 k+; one thread per (ghost layer, tangential
+// point) over the interior tangential range.  Every ghost point reads only
+// interior points, so the points of a face are independent.
+// ===========================================================================
+struct GhostPt {
+  int dir, hi, n, sg, sb, sm;  // direction, high face, extent, ghost / boundary / mirror positions
+  int a, b;                    // tangential indices
+};
+__device__ __forceinline__ bool ghost_point(const DevState& S, int f, long t, GhostPt& g) {
+  g.dir = f >> 1;
+  g.hi = f & 1;
+  g.n = S.n[g.dir];
+  const int na = g.dir == 0 ? S.ny : S.nx, nb = g.dir == 2 ? S.ny : S.nz;
+  if (t >= (long)OCFD_LAP * na * nb) return false;
+  const int l = (int)(t % OCFD_LAP) + 1;
+  const long r = t / OCFD_LAP;
+  g.a = (int)(r % na) + 1;
+  g.b = (int)(r / na) + 1;
+  g.sg = g.hi ? g.n + l : 1 - l;
+  g.sb = g.hi ? g.n : 1;
+  g.sm = g.hi ? 2 * g.n - g.sg : 2 - g.sg;
+  return true;
+}
+__device__ __forceinline__ long ghost_hq(const DevState& S, const GhostPt& g, int s) {
+  return g.dir == 0 ? hq(S, s, g.a, g.b) : g.dir == 1 ? hq(S, g.a, s, g.b) : hq(S, g.a, g.b, s);
+}
+
+// Flow_Ghost_Extent_2nd (OCFD_GhostCell.f90:166-364): linear extrapolation of
+// u, v, w, d, T and the partial densities, d and T limited to 0.8..1.2 of the
+// boundary value, partial densities rescaled to d, then Et = E(di, T), p = p(di, T)
+__global__ void k_flow_ghost(DevState S, int f) {
+  GhostPt g;
+  if (!ghost_point(S, f, (long)blockIdx.x * blockDim.x + threadIdx.x, g)) return;
+  const long q = ghost_hq(S, g, g.sg), q0 = ghost_hq(S, g, g.sb), qm = ghost_hq(S, g, g.sm);
+  const double s1 = 0.8, s2 = 1.2;
+  S.u[q] = 2.0 * S.u[q0] - S.u[qm];
+  S.v[q] = 2.0 * S.v[q0] - S.v[qm];
+  S.w[q] = 2.0 * S.w[q0] - S.w[qm];
+  double d = 2.0 * S.d[q0] - S.d[qm];
+  double T = 2.0 * S.T[q0] - S.T[qm];
+  if (d < s1 * S.d[q0]) d = s1 * S.d[q0];
+  if (d > s2 * S.d[q0]) d = s2 * S.d[q0];
+  if (T < s1 * S.T[q0]) T = s1 * S.T[q0];
+  if (T > s2 * S.T[q0]) T = s2 * S.T[q0];
+  S.d[q] = d;
+  S.T[q] = T;
+  if (S.ns == 1) {
+    double di = 2.0 * S.di[q0] - S.di[qm];
+    const double d0 = 0.0 + di;
+    di = di * (d / d0);
+    S.di[q] = di;
+    S.Et[q] = 0.0 + 2.5 * S.Ri0 * di * T;  // comput_E_pgas
+    S.p[q] = 0.0 + S.Ri0 * di * T;          // comput_P
+    return;
+  }
+  double di1[OCFD_NS_MAX];
+  double d0 = 0.0;
+  for (int m = 0; m < S.ns; m++) {
+    di1[m] = 2.0 * S.di[q0 + (long)m * S.nh] - S.di[qm + (long)m * S.nh];
+    d0 = d0 + di1[m];
+  }
+  const double dtmp = d / d0;
+  for (int m = 0; m < S.ns; m++) {
+    di1[m] = di1[m] * dtmp;
+    S.di[q + (long)m * S.nh] = di1[m];
+  }
+  S.Et[q] = gas_E(di1, T);
+  S.p[q] = gas_P(di1, T);
+}
+
+// Jac_Ghost_Extent_1st (OCFD_GhostCell.f90:17-152): metrics copied from the
+// boundary point, coordinates extrapolated linearly
+__global__ void k_jac_ghost1(DevState S, int f) {
+  GhostPt g;
+  if (!ghost_point(S, f, (long)blockIdx.x * blockDim.x + threadIdx.x, g)) return;
+  const long q = ghost_hq(S, g, g.sg), q0 = ghost_hq(S, g, g.sb), q1 = ghost_hq(S, g, g.hi ? g.n - 1 : 2);
+  double* M[10] = {S.Akx, S.Aky, S.Akz, S.Aix, S.Aiy, S.Aiz, S.Asx, S.Asy, S.Asz, S.Ajac};
+  for (int m = 0; m < 10; m++) M[m][q] = M[m][q0];
+  double* X[3] = {S.Axx, S.Ayy, S.Azz};
+  for (int c = 0; c < 3; c++)
+    X[c][q] = g.hi ? X[c][q0] + (double)(g.sg - g.n) * (X[c][q0] - X[c][q1])
+                   : X[c][q0] + (1.0 - g.sg) * (X[c][q0] - X[c][q1]);
+}
+
+// Jac_Ghost_Extent_2nd (OCFD_GhostCell.f90:368-470): coordinates extrapolated
+// to second order ...
+__global__ void k_jac_ghost2_coords(DevState S, int f) {
+  GhostPt g;
+  if (!ghost_point(S, f, (long)blockIdx.x * blockDim.x + threadIdx.x, g)) return;
+  const long q = ghost_hq(S, g, g.sg), q0 = ghost_hq(S, g, g.sb), qm = ghost_hq(S, g, g.sm);
+  S.Axx[q] = 2.0 * S.Axx[q0] - S.Axx[qm];
+  S.Ayy[q] = 2.0 * S.Ayy[q0] - S.Ayy[qm];
+  S.Azz[q] = 2.0 * S.Azz[q0] - S.Azz[qm];
+}
+
+// ... and the metrics of the ghost block from them (comput_Jacobian3d_Ghost,
+// :473-580): per direction, one-sided 2nd order at the block's ends, central
+// 2nd order next to them, central 4th order inside; the block spans the
+// ghost layers plus the boundary plane along the face normal and the
+// interior along the tangential directions
+__global__ void k_jac_ghost2_metrics(DevState S, int f) {
+  GhostPt g;
+  if (!ghost_point(S, f, (long)blockIdx.x * blockDim.x + threadIdx.x, g)) return;
+  int pos[3];
+  pos[g.dir] = g.sg;
+  pos[g.dir == 0 ? 1 : 0] = g.a;
+  pos[g.dir == 2 ? 1 : 2] = g.b;
+  const long q = hq(S, pos[0], pos[1], pos[2]);
+  auto der = [&](const double* X, int e) {
+    const int r0 = e == g.dir ? (g.hi ? g.n : 1 - OCFD_LAP) : 1;
+    const int r1 = e == g.dir ? (g.hi ? g.n + OCFD_LAP : 1) : S.n[e];
+    const long st = S.sh[e];
+    const double h = S.dh[e];
+    const int p = pos[e];
+    auto x = [&](int o) { return X[q + (long)o * st]; };
+    if (p == r0) return (-3.0 * x(0) + 4.0 * x(1) - x(2)) / (2.0 * h);
+    if (p == r1) return (x(-2) - 4.0 * x(-1) + 3.0 * x(0)) / (2.0 * h);
+    if (p == r0 + 1 || p == r1 - 1) return (x(1) - x(-1)) / (2.0 * h);
+    return (8.0 * (x(1) - x(-1)) - (x(2) - x(-2))) / (12.0 * h);
+  };
+  const double xi1 = der(S.Axx, 0), yi1 = der(S.Ayy, 0), zi1 = der(S.Azz, 0);
+  const double xj1 = der(S.Axx, 1), yj1 = der(S.Ayy, 1), zj1 = der(S.Azz, 1);
+  const double xk1 = der(S.Axx, 2), yk1 = der(S.Ayy, 2), zk1 = der(S.Azz, 2);
+  const double Jac1 = 1.0 / (xi1 * yj1 * zk1 + yi1 * zj1 * xk1 + zi1 * xj1 * yk1 - zi1 * yj1 * xk1 - yi1 * xj1 * zk1 -
+                             xi1 * zj1 * yk1);
+  S.Ajac[q] = Jac1;
+  S.Akx[q] = Jac1 * (yj1 * zk1 - zj1 * yk1);
+  S.Aky[q] = Jac1 * (zj1 * xk1 - xj1 * zk1);
+  S.Akz[q] = Jac1 * (xj1 * yk1 - yj1 * xk1);
+  S.Aix[q] = Jac1 * (yk1 * zi1 - zk1 * yi1);
+  S.Aiy[q] = Jac1 * (zk1 * xi1 - xk1 * zi1);
+  S.Aiz[q] = Jac1 * (xk1 * yi1 - yk1 * xi1);
+  S.Asx[q] = Jac1 * (yi1 * zj1 - zi1 * yj1);
+  S.Asy[q] = Jac1 * (zi1 * xj1 - xi1 * zj1);
+  S.Asz[q] = Jac1 * (xi1 * yj1 - yi1 * xj1);
+}
+
 // fn = f, din = di  (OCFD_Time_Adv.f90:10-29)
 __global__ __launch_bounds__(256) void k_save0(DevState S) {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1288,6 +1427,13 @@ void launch(const char* name, K kern, dim3 g, dim3 b, A... args) {
 
 inline dim3 grid1(long n) { return dim3((unsigned)((n + 255) / 256)); }
 
+// ghost points of face f (LAP layers over the interior tangential range)
+inline long ghost_count(int f) {
+  const DevState& S = G.S;
+  const int dir = f >> 1;
+  return (long)OCFD_LAP * (dir == 0 ? S.ny : S.nx) * (dir == 2 ? S.ny : S.nz);
+}
+
 // k_rk with the viscous scheme fixed at compile time
 inline void (*rk_kernel())(DevState, int, double, int) { return G.S.cd8 ? k_rk<1> : k_rk<0>; }
 
@@ -1366,6 +1512,9 @@ void enq_bc() {
   const long nxi = S.nx > 2 ? S.nx - 2 : 0, nyi = S.ny > 2 ? S.ny - 2 : 0;
   const long nf = 2L * S.ny * S.nz + 2L * nxi * S.nz + 2L * nxi * nyi;
   launch("k_bc_faces", k_bc_faces, grid1(nf), dim3(256), S);
+  for (int f = 0; f < 6; f++)  // Flow_Ghost_boundary, OCFD_BC.f90:186
+    if (G.cfg.ghost_cell[f] == 1 || G.cfg.ghost_cell[f] == 2)
+      launch("k_flow_ghost", k_flow_ghost, grid1(ghost_count(f)), dim3(256), S, f);
 }
 
 void enq_stage(int krk, double dt1) {
@@ -1533,7 +1682,8 @@ int32_t ocfd_init(const ocfd_cfg* cfg) {
   }
   for (int a = 0; a < 6; a++) {
     if (c.scheme_boundary[a] < -1 || c.scheme_boundary[a] > 1) return fail("ocfd_init: Scheme_boundary must be -1, 0 or 1");
-    if (c.ghost_cell[a] != 0) return fail("ocfd_init: Ghost_Cell /= 0 is not supported");
+    if (c.ghost_cell[a] < 0 || c.ghost_cell[a] > 2) return fail("ocfd_init: Ghost_Cell must be 0, 1 or 2");
+    if (c.ghost_cell[a] != 0 && c.periodic[a / 2]) return fail("ocfd_init: Ghost_Cell on a periodic face");
   }
   if (c.ibc != OCFD_BC_NONE && c.ibc != OCFD_BC_BOUNDARYLAYER && c.ibc != OCFD_BC_JET)
     return fail("ocfd_init: unsupported boundary condition");
@@ -1773,6 +1923,15 @@ int32_t ocfd_set_mesh(const double* axx, const double* ayy, const double* azz, c
     if (S.per[1]) hipLaunchKernelGGL(k_fill_faces<1>, grid1(npy * 8), dim3(256), 0, G.stream, S, M[a], 0, 0.0);
     if (S.per[2]) hipLaunchKernelGGL(k_fill_faces<2>, grid1(npz * 8), dim3(256), 0, G.stream, S, M[a], 0, 0.0);
   }
+  // Jac_Ghost_boundary (OCFD_mesh.f90:36): first-order faces, then second-order
+  for (int f = 0; f < 6; f++)
+    if (G.cfg.ghost_cell[f] == 1)
+      hipLaunchKernelGGL(k_jac_ghost1, grid1(ghost_count(f)), dim3(256), 0, G.stream, S, f);
+  for (int f = 0; f < 6; f++)
+    if (G.cfg.ghost_cell[f] == 2) {
+      hipLaunchKernelGGL(k_jac_ghost2_coords, grid1(ghost_count(f)), dim3(256), 0, G.stream, S, f);
+      hipLaunchKernelGGL(k_jac_ghost2_metrics, grid1(ghost_count(f)), dim3(256), 0, G.stream, S, f);
+    }
   hipLaunchKernelGGL(k_metrics_over_jac, grid1(S.nh), dim3(256), 0, G.stream, S);
   HIPCHK(hipStreamSynchronize(G.stream));
   HIPCHK(hipGetLastError());

@@ -1476,6 +1476,8 @@ static void face_update(int i, int j, int k, double* di1) {
   G.f[c + 4 * G.ni] = G.Et[q] + dd * (uu * uu + vv * vv + ww * ww) * 0.5;
 }
 
+static void flow_ghost_extent_2nd(void);
+
 /* OCFD_bc, OCFD_BC.f90:2-188 (one rank: every face is a rank face) */
 static int bc_all(void) {
   if (G.cfg.ibc == OCFD_BC_BOUNDARYLAYER) bc_boundarylayer();
@@ -1491,6 +1493,7 @@ static int bc_all(void) {
   for (int j = 1; j <= G.ny; j++) for (int i = 1; i <= G.nx; i++) face_update(i, j, 1, di1);
   for (int j = 1; j <= G.ny; j++) for (int i = 1; i <= G.nx; i++) face_update(i, j, G.nz, di1);
   free(di1);
+  flow_ghost_extent_2nd(); /* Flow_Ghost_boundary, OCFD_BC.f90:186 */
   return 0;
 }
 
@@ -1650,6 +1653,137 @@ double orc_sensor_mean(void) { return G.sensor_dp_av; }
 /* ------------------------------------------------------------------ */
 /* Mesh and metrics: init_mesh / comput_Jacobian3d / updata_Axyz_periodic,
  * OCFD_mesh.f90:1-185.  Coordinates are halo arrays; interior is read. */
+/* ------------------------------------------------------------------ */
+/* Ghost cells (Ghost_Cell(1:6) = 1 or 2), OCFD_GhostCell.f90.  A face f
+ * (0 i-, 1 i+, 2 j-, 3 j+, 4 k-, 5 k+) covers the LAP ghost layers over the
+ * interior tangential range.  gpos(f, l, a, b) is the halo index of ghost
+ * layer l (1..LAP) at tangential (a, b); bpos the boundary point; mpos the
+ * mirrored interior point 2*boundary - ghost. */
+static long gidx(int dir, int s, int a, int b) {
+  return dir == 0 ? H(s, a, b) : dir == 1 ? H(a, s, b) : H(a, b, s);
+}
+static int face_n(int dir) { return dir == 0 ? G.nx : dir == 1 ? G.ny : G.nz; }
+static void face_tan(int dir, int* na, int* nb) {
+  *na = dir == 0 ? G.ny : G.nx;
+  *nb = dir == 2 ? G.ny : G.nz;
+}
+
+/* Jac_Ghost_Extent_1st, OCFD_GhostCell.f90:17-152 */
+static void jac_ghost_1st(int f) {
+  int dir = f / 2, hi = f & 1, n = face_n(dir), na, nb;
+  face_tan(dir, &na, &nb);
+  double* M[10] = {G.Akx, G.Aky, G.Akz, G.Aix, G.Aiy, G.Aiz, G.Asx, G.Asy, G.Asz, G.Ajac};
+  double* X[3] = {G.Axx, G.Ayy, G.Azz};
+  for (int b = 1; b <= nb; b++)
+    for (int a = 1; a <= na; a++)
+      for (int l = 1; l <= G.lap; l++) {
+        int sg = hi ? n + l : 1 - l, sb0 = hi ? n : 1, sb1 = hi ? n - 1 : 2;
+        long q = gidx(dir, sg, a, b), q0 = gidx(dir, sb0, a, b), q1 = gidx(dir, sb1, a, b);
+        for (int m = 0; m < 10; m++) M[m][q] = M[m][q0];
+        for (int c = 0; c < 3; c++)
+          X[c][q] = hi ? X[c][q0] + (double)(sg - n) * (X[c][q0] - X[c][q1]) : X[c][q0] + (1.0 - sg) * (X[c][q0] - X[c][q1]);
+      }
+}
+
+/* one-sided / central first derivative inside the ghost block range [r0, r1]
+ * (comput_Jacobian3d_Ghost, OCFD_GhostCell.f90:506-559) */
+static double ghost_deriv(const double* X, long q, long st, int p, int r0, int r1, double h) {
+#define XX(o) X[q + (long)(o) * st]
+  if (p == r0) return (-3.0 * XX(0) + 4.0 * XX(1) - XX(2)) / (2.0 * h);
+  if (p == r1) return (XX(-2) - 4.0 * XX(-1) + 3.0 * XX(0)) / (2.0 * h);
+  if (p == r0 + 1 || p == r1 - 1) return (XX(1) - XX(-1)) / (2.0 * h);
+  return (8.0 * (XX(1) - XX(-1)) - (XX(2) - XX(-2))) / (12.0 * h);
+#undef XX
+}
+
+/* Jac_Ghost_Extent_2nd + comput_Jacobian3d_Ghost, OCFD_GhostCell.f90:368-580 */
+static void jac_ghost_2nd(int f) {
+  int dir = f / 2, hi = f & 1, n = face_n(dir), na, nb;
+  face_tan(dir, &na, &nb);
+  double* X[3] = {G.Axx, G.Ayy, G.Azz};
+  for (int b = 1; b <= nb; b++)
+    for (int a = 1; a <= na; a++)
+      for (int l = 1; l <= G.lap; l++) {
+        int sg = hi ? n + l : 1 - l, sb0 = hi ? n : 1, sm = hi ? 2 * n - sg : 2 - sg;
+        long q = gidx(dir, sg, a, b), q0 = gidx(dir, sb0, a, b), qm = gidx(dir, sm, a, b);
+        for (int c = 0; c < 3; c++) X[c][q] = 2.0 * X[c][q0] - X[c][qm];
+      }
+  long st[3] = {1, G.nxh, (long)G.nxh * G.nyh};
+  double h[3] = {G.hx, G.hy, G.hz};
+  int lo[3] = {1, 1, 1}, up[3] = {G.nx, G.ny, G.nz};
+  lo[dir] = hi ? n : 1 - G.lap;
+  up[dir] = hi ? n + G.lap : 1;
+  for (int b = 1; b <= nb; b++)
+    for (int a = 1; a <= na; a++)
+      for (int l = 1; l <= G.lap; l++) {
+        int sg = hi ? n + l : 1 - l;
+        int pos[3];
+        pos[dir] = sg;
+        pos[dir == 0 ? 1 : 0] = a;
+        pos[dir == 2 ? 1 : 2] = b;
+        long q = H(pos[0], pos[1], pos[2]);
+        double dx[3][3]; /* dx[c][e]: d(coordinate c)/d(index e) */
+        for (int c = 0; c < 3; c++)
+          for (int e = 0; e < 3; e++) dx[c][e] = ghost_deriv(X[c], q, st[e], pos[e], lo[e], up[e], h[e]);
+        double xi1 = dx[0][0], yi1 = dx[1][0], zi1 = dx[2][0];
+        double xj1 = dx[0][1], yj1 = dx[1][1], zj1 = dx[2][1];
+        double xk1 = dx[0][2], yk1 = dx[1][2], zk1 = dx[2][2];
+        double Jac1 = 1.0 / (xi1 * yj1 * zk1 + yi1 * zj1 * xk1 + zi1 * xj1 * yk1 - zi1 * yj1 * xk1 - yi1 * xj1 * zk1 -
+                             xi1 * zj1 * yk1);
+        G.Ajac[q] = Jac1;
+        G.Akx[q] = Jac1 * (yj1 * zk1 - zj1 * yk1);
+        G.Aky[q] = Jac1 * (zj1 * xk1 - xj1 * zk1);
+        G.Akz[q] = Jac1 * (xj1 * yk1 - yj1 * xk1);
+        G.Aix[q] = Jac1 * (yk1 * zi1 - zk1 * yi1);
+        G.Aiy[q] = Jac1 * (zk1 * xi1 - xk1 * zi1);
+        G.Aiz[q] = Jac1 * (xk1 * yi1 - yk1 * xi1);
+        G.Asx[q] = Jac1 * (yi1 * zj1 - zi1 * yj1);
+        G.Asy[q] = Jac1 * (zi1 * xj1 - xi1 * zj1);
+        G.Asz[q] = Jac1 * (xi1 * yj1 - yi1 * xj1);
+      }
+}
+
+/* Flow_Ghost_Extent_2nd, OCFD_GhostCell.f90:166-364 */
+static void flow_ghost_extent_2nd(void) {
+  const double s1 = 0.8, s2 = 1.2;
+  int ns = G.nspec;
+  double* di1 = dalloc(ns, 0);
+  for (int f = 0; f < 6; f++) {
+    int gc = G.cfg.ghost_cell[f];
+    if (gc != 1 && gc != 2) continue;
+    int dir = f / 2, hi = f & 1, n = face_n(dir), na, nb;
+    face_tan(dir, &na, &nb);
+    for (int b = 1; b <= nb; b++)
+      for (int a = 1; a <= na; a++)
+        for (int l = 1; l <= G.lap; l++) {
+          int sg = hi ? n + l : 1 - l, sb0 = hi ? n : 1, sm = hi ? 2 * n - sg : 2 - sg;
+          long q = gidx(dir, sg, a, b), q0 = gidx(dir, sb0, a, b), qm = gidx(dir, sm, a, b);
+          G.u[q] = 2.0 * G.u[q0] - G.u[qm];
+          G.v[q] = 2.0 * G.v[q0] - G.v[qm];
+          G.w[q] = 2.0 * G.w[q0] - G.w[qm];
+          G.d[q] = 2.0 * G.d[q0] - G.d[qm];
+          G.T[q] = 2.0 * G.T[q0] - G.T[qm];
+          if (G.d[q] < s1 * G.d[q0]) G.d[q] = s1 * G.d[q0];
+          if (G.d[q] > s2 * G.d[q0]) G.d[q] = s2 * G.d[q0];
+          if (G.T[q] < s1 * G.T[q0]) G.T[q] = s1 * G.T[q0];
+          if (G.T[q] > s2 * G.T[q0]) G.T[q] = s2 * G.T[q0];
+          double d0 = 0.0;
+          for (int m = 0; m < ns; m++) {
+            di1[m] = 2.0 * G.di[q0 + (long)m * G.nh] - G.di[qm + (long)m * G.nh];
+            d0 = d0 + di1[m];
+          }
+          double dtmp = G.d[q] / d0;
+          for (int m = 0; m < ns; m++) {
+            di1[m] = di1[m] * dtmp;
+            G.di[q + (long)m * G.nh] = di1[m];
+          }
+          G.Et[q] = comput_E(di1, G.T[q]);
+          G.p[q] = comput_P(di1, G.T[q]);
+        }
+  }
+  free(di1);
+}
+
 int orc_set_mesh(const double* axx, const double* ayy, const double* azz, const double* span) {
   for (int k = 1; k <= G.nz; k++)
     for (int j = 1; j <= G.ny; j++)
@@ -1711,6 +1845,11 @@ int orc_set_mesh(const double* axx, const double* ayy, const double* azz, const 
   free(t);
   double* M[10] = {G.Akx, G.Aky, G.Akz, G.Aix, G.Aiy, G.Aiz, G.Asx, G.Asy, G.Asz, G.Ajac};
   for (int a = 0; a < 10; a++) exchange_xyz(M[a]);
+  /* Jac_Ghost_boundary, OCFD_mesh.f90:36 (first-order faces, then second-order) */
+  for (int f = 0; f < 6; f++)
+    if (G.cfg.ghost_cell[f] == 1) jac_ghost_1st(f);
+  for (int f = 0; f < 6; f++)
+    if (G.cfg.ghost_cell[f] == 2) jac_ghost_2nd(f);
   double* M1[9] = {G.Akx1, G.Aky1, G.Akz1, G.Aix1, G.Aiy1, G.Aiz1, G.Asx1, G.Asy1, G.Asz1};
   for (long q = 0; q < G.nh; q++)
     for (int a = 0; a < 9; a++) M1[a][q] = M[a][q] / G.Ajac[q];

@@ -303,3 +303,31 @@ def test_characteristic_rk3_step(pkg):
         rep = fc.compare_state(g, o, tol=1e-12)
         print(case[5]["name"], {k: f"{v[0]:.1e}/{v[1]:.3f}" for k, v in rep.items()})
         g.close()
+
+
+# --------------------------------------------------------------- ghost cells
+@pytest.mark.parametrize("gc,sb", [(2, 1), (1, -1), (2, 0)])
+def test_ghost_cells_flat_plate(pkg, gc, sb):
+    """Ghost_Cell on the x and y faces of the boundary layer: ghost metrics
+    (first- or second-order extension), flow ghosts after every OCFD_bc, and
+    the Scheme_boundary closures that read them; bit-identical metrics and
+    step, like the runs without ghosts."""
+    case = list(fc.small_flat_plate(48, 32, 16))
+    for a in range(4):
+        case[0].ghost_cell[a] = gc
+        case[0].scheme_boundary[a] = sb
+    g, o = pair(pkg, tuple(case))
+    g.set_option(pkg.OPT_HYBRID_SUM_MODE, 0)
+    for name in pkg.METRICS:
+        d, same = fc.compare(g.download(name), o.download(name), name)
+        assert same == 1.0, (name, d, same)
+    for name in ("D", "U", "T", "P", "ET"):  # ghosts included
+        assert np.array_equal(g.download(name), o.download(name)), name
+    dt1 = 0.5 * case[5]["dt"]
+    g.flow_time_advance(dt1)
+    o.flow_time_advance(dt1)
+    rep = fc.compare_state(g, o, tol=1e-12)
+    print(gc, sb, {k: f"{v[0]:.1e}/{v[1]:.3f}" for k, v in rep.items()})
+    for name in ("D", "U", "T", "P", "ET"):
+        assert np.array_equal(g.download(name), o.download(name)), name
+    g.close()

@@ -196,3 +196,18 @@ def test_ns_step_adaptive_chemistry(pkg):
     o.flow_time_advance(0.5 * dt)
     compare_all(g, o, tol=1e-10)
     g.close()
+
+
+def test_rk3_step_with_ghost_cells(pkg):
+    """Second-order ghost cells on the jet's x and y faces (species ghosts
+    rescaled to d, real-gas Et and p) read by WENO5-with-ghost closures."""
+    case = list(fc.small_jet())
+    for a in range(4):
+        case[0].ghost_cell[a] = 2
+        case[0].scheme_boundary[a] = 1
+    g, o = pair(pkg, tuple(case))
+    dt = case[5]["dt"]
+    g.flow_time_advance(dt)
+    o.flow_time_advance(dt)
+    compare_all(g, o)
+    g.close()

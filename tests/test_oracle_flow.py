@@ -556,3 +556,48 @@ def test_characteristic_flux_equals_componentwise_for_a_linear_scheme():
     o.du_inviscous()
     assert np.max(np.abs(o.download("DU"))) == 0.0
     o.close()
+
+
+# --------------------------------------------------------------- ghost cells
+def test_ghost_cells_closed_forms():
+    """Ghost_Cell = 2 on the x and y faces of a stretched, non-periodic mesh:
+    second-order coordinate extrapolation is exact for the linear x and
+    reproduces the mirror rule; ghost metrics of a uniform direction equal
+    the interior ones; flow ghosts extrapolate linear fields exactly and
+    clip d and T to 0.8..1.2 of the boundary value."""
+    m = load_package()
+    import oracle
+    from ocfd_amd import cases
+    nx, ny, nz = 16, 12, 8
+    x = np.linspace(0.0, 1.5, nx)
+    y = np.cumsum(np.r_[0.0, 0.01 * 1.15 ** np.arange(ny - 1)])
+    z = np.arange(nz) * 0.1
+    gc = [2, 2, 1, 1, 0, 0]
+    cfg = m.make_cfg(nx, ny, nz, periodic=(0, 0, 1), scheme_boundary=[1] * 6)
+    for a in range(6):
+        cfg.ghost_cell[a] = gc[a]
+    xs, ys, zs = np.meshgrid(x, y, z, indexing="ij")
+    d = 1.0 + 0.01 * xs
+    T = 300.0 + 100.0 * xs + 2500.0 * ys   # steep in y: the ghost T gets clipped at the j- face
+    u = 10.0 + 3.0 * xs
+    st = cases.state_from_primitives(d, u, 0 * u, 0 * u, T)
+    o = oracle.Oracle(m, cfg)
+    o.set_mesh(*cases.mesh_1d(x, y, z), np.array([0, 0, 0, 0, 0, 0, 0, 0, 0.8]))
+    o.upload_state(st)
+    o.comput_flow_variables()
+    o.OCFD_bc()
+    X = o.download("AKX")   # d xi / dx: uniform x -> (nx-1)/1.5 everywhere, also in the x ghosts
+    L = LAP
+    inner = X[L:L + nx, L:L + ny, L:L + nz]
+    assert np.allclose(X[:L, L:L + ny, L:L + nz], inner[0:1], rtol=1e-12)
+    assert np.allclose(X[L + nx:, L:L + ny, L:L + nz], inner[-1:], rtol=1e-12)
+    U, D, TT = o.download("U"), o.download("D"), o.download("T")
+    for g in range(1, L + 1):   # x- ghosts: linear fields continue exactly
+        assert np.allclose(U[L - g, L:L + ny, L:L + nz], 2 * U[L, L:L + ny, L:L + nz] - U[L + g, L:L + ny, L:L + nz], rtol=1e-14)
+        assert np.allclose(D[L - g, L:L + ny, L:L + nz], 2 * D[L, L:L + ny, L:L + nz] - D[L + g, L:L + ny, L:L + nz], rtol=1e-14)
+    # j- ghosts: T = 2 T(1) - T(2-j) falls below 0.8 T(1) deep in the ghost layers -> clipped
+    Tg, Tb = TT[L:L + nx, 0, L:L + nz], TT[L:L + nx, L, L:L + nz]
+    assert np.all(Tg >= 0.8 * Tb * (1 - 1e-15)) and np.all(Tg == 0.8 * Tb)
+    T1 = TT[L:L + nx, L - 1, L:L + nz]   # first ghost layer: not clipped, exact extrapolation
+    assert np.allclose(T1, 2 * Tb - TT[L:L + nx, L + 1, L:L + nz], rtol=1e-14)
+    o.close()
