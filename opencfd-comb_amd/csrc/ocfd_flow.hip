@@ -1124,9 +1124,15 @@ __global__ void k_sum_ordered(DevState S) {
   S.red[1] = dp_av;
 }
 
-// Fast mean: per-block tree sums, then one block sums the partials.
+// Fast (non-ordered) mean of the type-2 sensor: per-thread strided sums, a
+// wavefront tree by __shfl_xor, the four wave sums in fixed order; then one
+// wave over the partials.  Deterministic for a given grid.
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o, 64);
+  return v;
+}
 __global__ __launch_bounds__(256) void k_sum_partial(DevState S, double* part) {
-  __shared__ double sm[256];
+  __shared__ double sm[4];
   double acc = 0.0;
   for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < S.ni; c += (long)gridDim.x * blockDim.x) {
     const int i = (int)(c % S.nx) + 1;
@@ -1134,26 +1140,17 @@ __global__ __launch_bounds__(256) void k_sum_partial(DevState S, double* part) {
     const int k = (int)(c / ((long)S.nx * S.ny)) + 1;
     acc += S.Rh[hq(S, i, j, k)];
   }
-  sm[threadIdx.x] = acc;
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+  if (threadIdx.x == 0) part[blockIdx.x] = ((sm[0] + sm[1]) + sm[2]) + sm[3];
 }
-__global__ __launch_bounds__(256) void k_sum_final(DevState S, const double* part, int nparts) {
-  __shared__ double sm[256];
+__global__ __launch_bounds__(64) void k_sum_final(DevState S, const double* part, int nparts) {
   double acc = 0.0;
-  for (int a = threadIdx.x; a < nparts; a += blockDim.x) acc += part[a];
-  sm[threadIdx.x] = acc;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
-    __syncthreads();
-  }
+  for (int a = threadIdx.x; a < nparts; a += 64) acc += part[a];
+  acc = wave_sum(acc);
   if (threadIdx.x == 0) {
-    const double dp_av = sm[0] / (1.0 * S.nx * S.ny * S.nz);
+    const double dp_av = acc / (1.0 * S.nx * S.ny * S.nz);
     S.red[0] = 1.0 / (dp_av + 1.0e-30);
     S.red[1] = dp_av;
   }
@@ -1489,7 +1486,7 @@ void enq_rhybrid() {
       launch("k_sum_ordered", k_sum_ordered, dim3(1), dim3(64), S);
     } else {
       launch("k_sum_partial", k_sum_partial, dim3(G.nparts), dim3(256), S, G.part);
-      launch("k_sum_final", k_sum_final, dim3(1), dim3(256), S, (const double*)G.part, G.nparts);
+      launch("k_sum_final", k_sum_final, dim3(1), dim3(64), S, (const double*)G.part, G.nparts);
     }
     launch("k_sensor_norm", k_sensor_norm, grid1(S.ni), dim3(256), S);
   }
