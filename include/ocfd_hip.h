@@ -56,9 +56,9 @@ typedef struct ocfd_cfg {
   int32_t scheme_vis;          /* OCFD_SCHEME_CD6 or OCFD_SCHEME_CD8 */
   int32_t flux_splitting;      /* OCFD_SPLIT_SW or OCFD_SPLIT_LLF */
   int32_t if_viscous;          /* IF_Viscous */
-  int32_t if_scheme_character; /* IF_Scheme_Character (only 0 supported) */
+  int32_t if_scheme_character; /* IF_Scheme_Character: 0 or 1 (characteristic flux) */
   int32_t scheme_boundary[6];  /* Scheme_boundary(1:6): 0, 1 or -1 */
-  int32_t ghost_cell[6];       /* Ghost_Cell(1:6) (only 0 supported) */
+  int32_t ghost_cell[6];       /* Ghost_Cell(1:6): 0, 1 or 2 (non-periodic faces) */
   int32_t ibc;                 /* OCFD_BC_* */
   int32_t if_mass_force;       /* IF_Mass_Force */
   int32_t if_real_gas;         /* IF_RealGas (0 perfect gas, 1 NASA fits) */
@@ -111,7 +111,11 @@ enum {
   /* 1 = capture the RK3 step in a hipGraph on first use and replay it. */
   OCFD_OPT_USE_GRAPH = 2,
   /* 1 = bracket every kernel launch with HIP events (per-kernel timing). */
-  OCFD_OPT_KERNEL_TIMING = 3
+  OCFD_OPT_KERNEL_TIMING = 3,
+  /* k planes walked by one block of the viscous-flux and update kernels
+   * (their z-stencil windows stay in registers); 0 = one thread per cell.
+   * Default 4.  Results are bitwise independent of the value. */
+  OCFD_OPT_COLUMN_PLANES = 4
 };
 
 /* ---- lifetime ---------------------------------------------------------- */

@@ -271,8 +271,28 @@ __device__ __forceinline__ int recon_kind(const DevState& S, int dir, int s, int
 //   OCFD_dx0/dy0/dz0, OCFD_dxyz_viscous.f90:3-242 (CD6/CD8 + one-sided closures).
 // CD = 0 / 1 fixes CD6 / CD8 at compile time (fewer live registers in the
 // kernels that take many derivatives); CD = -1 reads the scheme from S.
-template <int CD = -1, class Fn>
+// INT = true: the caller guarantees R+1 <= s <= n-R (R = 3 for CD6, 4 for
+// CD8), where no closure applies and no periodic wrap is needed; the result
+// is then the central formula, in straight-line code the scheduler can
+// overlap with the loads of the next derivative.
+template <int CD, class Fn>
+__device__ __forceinline__ double deriv1_central(const DevState& S, int dir, Fn at) {
+  if (CD == 0) {
+    const double* a = S.cd6c[dir];
+    return a[0] * (at(3) - at(-3)) + a[1] * (at(2) - at(-2)) + a[2] * (at(1) - at(-1));
+  }
+  const double* c = S.cd8c[dir];
+  return c[0] * (at(1) - at(-1)) + c[1] * (at(2) - at(-2)) + c[2] * (at(3) - at(-3)) + c[3] * (at(4) - at(-4));
+}
+
+// cells whose CD6 (R = 3) or CD8 (R = 4) stencil along a direction of n
+// cells is the central one without a periodic wrap
+__device__ __forceinline__ bool central_cell(int s, int n, int R) { return s >= R + 1 && s <= n - R; }
+
+template <int CD = -1, bool INT = false, class Fn>
 __device__ __forceinline__ double deriv1(const DevState& S, int dir, int s, Fn at) {
+  static_assert(!INT || CD >= 0, "the interior form needs the scheme fixed at compile time");
+  if constexpr (INT) return deriv1_central<CD>(S, dir, at);
   const int n = S.n[dir];
   const double* a = S.cd6c[dir];
   const bool cd8 = CD < 0 ? (bool)S.cd8 : CD == 1;

@@ -513,16 +513,51 @@ __global__ __launch_bounds__(256) void k_sweep_char(DevState S) {
 // ===========================================================================
 __device__ __forceinline__ double pw3(double x) { double x2 = x * x; return x2 * x; }
 
+// 2.5-D marching launch for the per-cell kernels that differentiate in z.
+// A block covers a COL_TX x COL_TY tile of (i, j) and walks a chunk of
+// consecutive k planes.  The z-stencil window of every differentiated field
+// stays in registers and advances one plane per step, so a plane of those
+// fields is read once per chunk instead of once per stencil tap.  Block
+// order: tile i fastest, then tile j, then chunk, so the tiles whose rows
+// meet in the y stencil run together on one XCD.  Decoding a cell is two
+// increments per plane instead of the three 64-bit divisions of the flat
+// launch.
+constexpr int COL_TX = 64, COL_TY = 4;
+
+struct ColPos {
+  int i, j, k0, k1;
+};
+
+__device__ __forceinline__ bool col_pos(const DevState& S, int kc, ColPos& P) {
+  const int nti = (S.nx + COL_TX - 1) / COL_TX, ntj = (S.ny + COL_TY - 1) / COL_TY;
+  const int L = (int)xcd_block((long)blockIdx.x, (long)gridDim.x);
+  const int ti = L % nti, r = L / nti;
+  const int tj = r % ntj, ch = r / ntj;
+  P.i = ti * COL_TX + (int)(threadIdx.x % COL_TX) + 1;
+  P.j = tj * COL_TY + (int)(threadIdx.x / COL_TX) + 1;
+  P.k0 = ch * kc + 1;
+  P.k1 = min(P.k0 + kc, S.nz + 1);
+  return P.i <= S.nx && P.j <= S.ny;
+}
+
+inline dim3 col_grid(const DevState& S, int kc) {
+  const long nti = (S.nx + COL_TX - 1) / COL_TX, ntj = (S.ny + COL_TY - 1) / COL_TY;
+  return dim3((unsigned)(nti * ntj * ((S.nz + kc - 1) / kc)));
+}
+
+// Wave-uniform choice of the interior form: every active lane of the wave has
+// the central stencil in the named directions.  Interior waves then run
+// straight-line code (no closure branches), so the loads of all derivatives
+// can be in flight together; boundary waves take the general form.
+__device__ __forceinline__ bool wave_all(bool p) { return __all((int)p) != 0; }
+
 // MS (N_SPEC > 1): the transport coefficients come from k_transport_ms and
 // the species-diffusion enthalpy flux Qp from k_species_visc_prep.
-template <bool MS>
-__global__ __launch_bounds__(256) void k_visc_prep(DevState S) {
-  const long c = cell_index();
-  if (c >= S.ni) return;
-  const int i = (int)(c % S.nx) + 1;
-  const int j = (int)((c / S.nx) % S.ny) + 1;
-  const int k = (int)(c / ((long)S.nx * S.ny)) + 1;
-  const long q = hq(S, i, j, k);
+// (us, vs, ws, Ts) are the zeta derivatives of u, v, w, T at the cell.
+// INT: the cell's xi and eta stencils are central (no wrap, no closure).
+template <int CD, bool MS, bool INT>
+__device__ __forceinline__ void visc_prep_cell(const DevState& S, long c, long q, int i, int j, double us, double vs,
+                                               double ws, double Ts) {
   double amu, amk;
   if (MS) {
     amu = S.Amu[c];
@@ -541,14 +576,19 @@ __global__ __launch_bounds__(256) void k_visc_prep(DevState S) {
   }
 
   auto dh = [&](const double* F, int dir, int s) {
-    return deriv1(S, dir, s, [&](int o) {
-      const int t = wrap(s + o, S.n[dir], S.per[dir]);
-      return F[q + (long)(t - s) * S.sh[dir]];
-    });
+    const double* Fq = F + q;
+    const int st = dir == 0 ? 1 : S.nxh;
+    if constexpr (INT) {
+      return deriv1<CD, true>(S, dir, s, [&](int o) { return Fq[o * st]; });
+    } else {
+      return deriv1<CD>(S, dir, s, [&](int o) {
+        const int t = wrap(s + o, S.n[dir], S.per[dir]);
+        return Fq[(t - s) * st];
+      });
+    }
   };
   const double uk = dh(S.u, 0, i), vk = dh(S.v, 0, i), wk = dh(S.w, 0, i), Tk = dh(S.T, 0, i);
   const double ui = dh(S.u, 1, j), vi = dh(S.v, 1, j), wi = dh(S.w, 1, j), Ti = dh(S.T, 1, j);
-  const double us = dh(S.u, 2, k), vs = dh(S.v, 2, k), ws = dh(S.w, 2, k), Ts = dh(S.T, 2, k);
   const double Akx = S.Akx[q], Aky = S.Aky[q], Akz = S.Akz[q];
   const double Aix = S.Aix[q], Aiy = S.Aiy[q], Aiz = S.Aiz[q];
   const double Asx = S.Asx[q], Asy = S.Asy[q], Asz = S.Asz[q];
@@ -579,20 +619,85 @@ __global__ __launch_bounds__(256) void k_visc_prep(DevState S) {
   const double Akx1 = S.Akx1[q], Aky1 = S.Aky1[q], Akz1 = S.Akz1[q];
   const double Aix1 = S.Aix1[q], Aiy1 = S.Aiy1[q], Aiz1 = S.Aiz1[q];
   const double Asx1 = S.Asx1[q], Asy1 = S.Asy1[q], Asz1 = S.Asz1[q];
-  double* Ev = S.Ev;
+  double* Ev = S.Ev + c;
   const long ni = S.ni;
-  Ev[c + 0 * ni] = Akx1 * s11 + Aky1 * s12 + Akz1 * s13;
-  Ev[c + 1 * ni] = Akx1 * s12 + Aky1 * s22 + Akz1 * s23;
-  Ev[c + 2 * ni] = Akx1 * s13 + Aky1 * s23 + Akz1 * s33;
-  Ev[c + 3 * ni] = Akx1 * E1 + Aky1 * E2 + Akz1 * E3;
-  Ev[c + 4 * ni] = Aix1 * s11 + Aiy1 * s12 + Aiz1 * s13;
-  Ev[c + 5 * ni] = Aix1 * s12 + Aiy1 * s22 + Aiz1 * s23;
-  Ev[c + 6 * ni] = Aix1 * s13 + Aiy1 * s23 + Aiz1 * s33;
-  Ev[c + 7 * ni] = Aix1 * E1 + Aiy1 * E2 + Aiz1 * E3;
-  Ev[c + 8 * ni] = Asx1 * s11 + Asy1 * s12 + Asz1 * s13;
-  Ev[c + 9 * ni] = Asx1 * s12 + Asy1 * s22 + Asz1 * s23;
-  Ev[c + 10 * ni] = Asx1 * s13 + Asy1 * s23 + Asz1 * s33;
-  Ev[c + 11 * ni] = Asx1 * E1 + Asy1 * E2 + Asz1 * E3;
+  Ev[0 * ni] = Akx1 * s11 + Aky1 * s12 + Akz1 * s13;
+  Ev[1 * ni] = Akx1 * s12 + Aky1 * s22 + Akz1 * s23;
+  Ev[2 * ni] = Akx1 * s13 + Aky1 * s23 + Akz1 * s33;
+  Ev[3 * ni] = Akx1 * E1 + Aky1 * E2 + Akz1 * E3;
+  Ev[4 * ni] = Aix1 * s11 + Aiy1 * s12 + Aiz1 * s13;
+  Ev[5 * ni] = Aix1 * s12 + Aiy1 * s22 + Aiz1 * s23;
+  Ev[6 * ni] = Aix1 * s13 + Aiy1 * s23 + Aiz1 * s33;
+  Ev[7 * ni] = Aix1 * E1 + Aiy1 * E2 + Aiz1 * E3;
+  Ev[8 * ni] = Asx1 * s11 + Asy1 * s12 + Asz1 * s13;
+  Ev[9 * ni] = Asx1 * s12 + Asy1 * s22 + Asz1 * s23;
+  Ev[10 * ni] = Asx1 * s13 + Asy1 * s23 + Asz1 * s33;
+  Ev[11 * ni] = Asx1 * E1 + Asy1 * E2 + Asz1 * E3;
+}
+
+// one thread per cell
+template <int CD, bool MS>
+__global__ __launch_bounds__(256) void k_visc_prep(DevState S) {
+  constexpr int R = CD == 1 ? 4 : 3;
+  const long c = cell_index();
+  if (c >= S.ni) return;
+  const int i = (int)(c % S.nx) + 1;
+  const int j = (int)((c / S.nx) % S.ny) + 1;
+  const int k = (int)(c / ((long)S.nx * S.ny)) + 1;
+  const long q = hq(S, i, j, k);
+  const long plh = (long)S.nxh * S.nyh;
+  if (wave_all(central_cell(i, S.nx, R) && central_cell(j, S.ny, R) && central_cell(k, S.nz, R))) {
+    auto dz = [&](const double* F) { return deriv1<CD, true>(S, 2, k, [&](int o) { return F[q + o * plh]; }); };
+    visc_prep_cell<CD, MS, true>(S, c, q, i, j, dz(S.u), dz(S.v), dz(S.w), dz(S.T));
+  } else {
+    auto dz = [&](const double* F) {
+      return deriv1<CD>(S, 2, k, [&](int o) {
+        const int t = wrap(k + o, S.n[2], S.per[2]);
+        return F[q + (long)(t - k) * plh];
+      });
+    };
+    visc_prep_cell<CD, MS, false>(S, c, q, i, j, dz(S.u), dz(S.v), dz(S.w), dz(S.T));
+  }
+}
+
+// marching over k planes with the z windows of u, v, w, T in registers
+template <int CD, bool MS>
+__global__ __launch_bounds__(256) void k_visc_prep_col(DevState S, int kc) {
+  constexpr int R = CD == 1 ? 4 : 3, W = 2 * R + 1;
+  ColPos P;
+  if (!col_pos(S, kc, P)) return;
+  const long plh = (long)S.nxh * S.nyh, pli = (long)S.nx * S.ny;
+  const long qcol = hq(S, P.i, P.j, 0), ccol = iq(S, P.i, P.j, 1);
+  const bool xy_int = wave_all(central_cell(P.i, S.nx, R) && central_cell(P.j, S.ny, R));
+  double zu[W], zv[W], zw[W], zT[W];
+  // plane p (-3 .. nz+4 before the periodic wrap) lies inside the halo box
+  auto ld = [&](int slot, int p) {
+    const long qp = qcol + (long)wrap(p, S.nz, S.per[2]) * plh;
+    zu[slot] = S.u[qp];
+    zv[slot] = S.v[qp];
+    zw[slot] = S.w[qp];
+    zT[slot] = S.T[qp];
+  };
+#pragma unroll
+  for (int o = 0; o < W - 1; o++) ld(o, P.k0 - R + o);
+  for (int k = P.k0; k < P.k1; k++) {
+    ld(W - 1, k + R);
+    const long c = ccol + (long)(k - 1) * pli, q = qcol + (long)k * plh;
+    if (xy_int && central_cell(k, S.nz, R)) {
+      auto dz = [&](const double* z) { return deriv1<CD, true>(S, 2, k, [&](int o) { return z[o + R]; }); };
+      visc_prep_cell<CD, MS, true>(S, c, q, P.i, P.j, dz(zu), dz(zv), dz(zw), dz(zT));
+    } else {
+      auto dz = [&](const double* z) { return deriv1<CD>(S, 2, k, [&](int o) { return z[o + R]; }); };
+      visc_prep_cell<CD, MS, false>(S, c, q, P.i, P.j, dz(zu), dz(zv), dz(zw), dz(zT));
+    }
+#pragma unroll
+    for (int o = 0; o < W - 1; o++) {
+      zu[o] = zu[o + 1];
+      zv[o] = zv[o + 1];
+      zw[o] = zw[o + 1];
+      zT[o] = zT[o + 1];
+    }
+  }
 }
 
 // Transport only (when the viscous terms are off the reference still
@@ -624,20 +729,32 @@ __global__ __launch_bounds__(256) void k_transport(DevState S) {
 // ===========================================================================
 enum { RK_MASS = 1, RK_VISC = 2, RK_WRITE_DU = 4, RK_UPDATE_F = 8, RK_PRIMS = 16, RK_DI_FROM_D = 32, RK_MS = 64 };
 
-template <int CD>
-__global__ __launch_bounds__(256) void k_rk(DevState S, int krk, double dt, int flags) {
-  const long c = cell_index();
-  if (c >= S.ni) return;
-  const int i = (int)(c % S.nx) + 1;
-  const int j = (int)((c / S.nx) % S.ny) + 1;
-  const int k = (int)(c / ((long)S.nx * S.ny)) + 1;
-  const long q = hq(S, i, j, k);
+// The stepping path's flag sets are compile-time (F >= 0), so the kernel is
+// straight-line code and its loads can be in flight together; F = -1 reads
+// the flags at run time (per-stage entry points).
+// zs[m] is the zeta derivative of Ev3(m+1) at the cell (read only with RK_VISC).
+// INT: the cell's xi and eta stencils are central (no wrap, no closure).
+template <int CD, int F, bool INT>
+__device__ __forceinline__ void rk_cell(const DevState& S, long c, long q, int i, int j, int krk, double dt,
+                                        int flags_rt, const double* zs) {
+  const int flags = F >= 0 ? F : flags_rt;
   const long ni = S.ni;
   const double d_old = S.d[q];
   double du[5];
   if (flags & (RK_MASS | RK_VISC | RK_UPDATE_F)) {
 #pragma unroll
     for (int m = 0; m < 5; m++) du[m] = S.du[c + m * ni];
+  }
+  // the stage's old solution, read before the flux divergence so that its
+  // loads overlap the stencil loads
+  double fnv[5], fov[5];
+  if (flags & RK_UPDATE_F) {
+#pragma unroll
+    for (int m = 0; m < 5; m++) fnv[m] = S.fn[c + m * ni];
+    if (krk != 1) {
+#pragma unroll
+      for (int m = 0; m < 5; m++) fov[m] = S.f[c + m * ni];
+    }
   }
   if (flags & RK_MASS) {
     du[1] = du[1] + d_old * S.mf[0];
@@ -647,18 +764,23 @@ __global__ __launch_bounds__(256) void k_rk(DevState S, int krk, double dt, int 
   }
   if (flags & RK_VISC) {
     const double ajac = S.Ajac[q];
-    auto dv = [&](const double* F, int dir, int s) {
-      return deriv1<CD>(S, dir, s, [&](int o) {
-        const int t = wrap(s + o, S.n[dir], S.per[dir]);
-        return F[c + (long)(t - s) * S.si[dir]];
-      });
+    auto dv = [&](const double* Fv, int dir, int s) {
+      const double* Fc = Fv + c;
+      const int st = dir == 0 ? 1 : S.nx;
+      if constexpr (INT) {
+        return deriv1<CD, true>(S, dir, s, [&](int o) { return Fc[o * st]; });
+      } else {
+        return deriv1<CD>(S, dir, s, [&](int o) {
+          const int t = wrap(s + o, S.n[dir], S.per[dir]);
+          return Fc[(t - s) * st];
+        });
+      }
     };
 #pragma unroll
     for (int m = 0; m < 4; m++) {
       const double uk = dv(S.Ev + (long)m * ni, 0, i);
       const double ui = dv(S.Ev + (long)(4 + m) * ni, 1, j);
-      const double us = dv(S.Ev + (long)(8 + m) * ni, 2, k);
-      du[m + 1] = du[m + 1] + (uk + ui + us) * ajac;
+      du[m + 1] = du[m + 1] + (uk + ui + zs[m]) * ajac;
     }
   }
   if (flags & RK_WRITE_DU) {
@@ -669,12 +791,12 @@ __global__ __launch_bounds__(256) void k_rk(DevState S, int krk, double dt, int 
   if (flags & RK_UPDATE_F) {
     if (krk == 1) {
 #pragma unroll
-      for (int m = 0; m < 5; m++) f[m] = S.fn[c + m * ni] + dt * du[m];
+      for (int m = 0; m < 5; m++) f[m] = fnv[m] + dt * du[m];
     } else {
       const double ra = krk == 2 ? 3.0 / 4.0 : 1.0 / 3.0;
       const double rb = krk == 2 ? 1.0 / 4.0 : 2.0 / 3.0;
 #pragma unroll
-      for (int m = 0; m < 5; m++) f[m] = ra * S.fn[c + m * ni] + rb * (S.f[c + m * ni] + dt * du[m]);
+      for (int m = 0; m < 5; m++) f[m] = ra * fnv[m] + rb * (fov[m] + dt * du[m]);
     }
 #pragma unroll
     for (int m = 0; m < 5; m++) S.f[c + m * ni] = f[m];
@@ -720,6 +842,90 @@ __global__ __launch_bounds__(256) void k_rk(DevState S, int krk, double dt, int 
   S.di[q] = di;
   S.T[q] = T;
   S.p[q] = p;
+}
+
+// one thread per cell
+template <int CD, int F>
+__global__ __launch_bounds__(256) void k_rk(DevState S, int krk, double dt, int flags_rt) {
+  constexpr int R = CD == 1 ? 4 : 3;
+  const int flags = F >= 0 ? F : flags_rt;
+  const long c = cell_index();
+  if (c >= S.ni) return;
+  const int i = (int)(c % S.nx) + 1;
+  const int j = (int)((c / S.nx) % S.ny) + 1;
+  const int k = (int)(c / ((long)S.nx * S.ny)) + 1;
+  const long q = hq(S, i, j, k);
+  const long pli = (long)S.nx * S.ny;
+  double zs[4] = {0.0, 0.0, 0.0, 0.0};
+  if (wave_all(central_cell(i, S.nx, R) && central_cell(j, S.ny, R) && central_cell(k, S.nz, R))) {
+    if (flags & RK_VISC) {
+#pragma unroll
+      for (int m = 0; m < 4; m++) {
+        const double* Fv = S.Ev + (long)(8 + m) * S.ni + c;
+        zs[m] = deriv1<CD, true>(S, 2, k, [&](int o) { return Fv[o * pli]; });
+      }
+    }
+    rk_cell<CD, F, true>(S, c, q, i, j, krk, dt, flags_rt, zs);
+  } else {
+    if (flags & RK_VISC) {
+#pragma unroll
+      for (int m = 0; m < 4; m++) {
+        const double* Fv = S.Ev + (long)(8 + m) * S.ni;
+        zs[m] = deriv1<CD>(S, 2, k, [&](int o) {
+          const int t = wrap(k + o, S.n[2], S.per[2]);
+          return Fv[c + (long)(t - k) * pli];
+        });
+      }
+    }
+    rk_cell<CD, F, false>(S, c, q, i, j, krk, dt, flags_rt, zs);
+  }
+}
+
+// marching over k planes with the z windows of Ev3(1:4) in registers
+template <int CD, int F>
+__global__ __launch_bounds__(256) void k_rk_col(DevState S, int kc, int krk, double dt, int flags_rt) {
+  constexpr int R = CD == 1 ? 4 : 3, W = 2 * R + 1;
+  const int flags = F >= 0 ? F : flags_rt;
+  ColPos P;
+  if (!col_pos(S, kc, P)) return;
+  const long plh = (long)S.nxh * S.nyh, pli = (long)S.nx * S.ny;
+  const long qcol = hq(S, P.i, P.j, 0), ccol = iq(S, P.i, P.j, 1);
+  const bool visc = flags & RK_VISC;
+  const bool xy_int = wave_all(central_cell(P.i, S.nx, R) && central_cell(P.j, S.ny, R));
+  double z[4][W];
+  // Ev is stored over the interior only: planes outside 1..nz of a
+  // non-periodic z are never read by the one-sided closures and stay zero
+  auto ld = [&](int slot, int p) {
+    p = wrap(p, S.nz, S.per[2]);
+    if (visc && p >= 1 && p <= S.nz) {
+      const double* Fv = S.Ev + 8 * S.ni + ccol + (long)(p - 1) * pli;
+#pragma unroll
+      for (int m = 0; m < 4; m++) z[m][slot] = Fv[m * S.ni];
+    } else {
+#pragma unroll
+      for (int m = 0; m < 4; m++) z[m][slot] = 0.0;
+    }
+  };
+#pragma unroll
+  for (int o = 0; o < W - 1; o++) ld(o, P.k0 - R + o);
+  for (int k = P.k0; k < P.k1; k++) {
+    ld(W - 1, k + R);
+    const long c = ccol + (long)(k - 1) * pli, q = qcol + (long)k * plh;
+    double zs[4];
+    if (xy_int && central_cell(k, S.nz, R)) {
+#pragma unroll
+      for (int m = 0; m < 4; m++) zs[m] = visc ? deriv1<CD, true>(S, 2, k, [&](int o) { return z[m][o + R]; }) : 0.0;
+      rk_cell<CD, F, true>(S, c, q, P.i, P.j, krk, dt, flags_rt, zs);
+    } else {
+#pragma unroll
+      for (int m = 0; m < 4; m++) zs[m] = visc ? deriv1<CD>(S, 2, k, [&](int o) { return z[m][o + R]; }) : 0.0;
+      rk_cell<CD, F, false>(S, c, q, P.i, P.j, krk, dt, flags_rt, zs);
+    }
+#pragma unroll
+    for (int o = 0; o < W - 1; o++)
+#pragma unroll
+      for (int m = 0; m < 4; m++) z[m][o] = z[m][o + 1];
+  }
 }
 
 // di = d over the whole halo box for N_SPEC == 1 (OCFD_Time_Adv.f90:45-52)
@@ -1335,6 +1541,7 @@ struct Ctx {
   int hybrid_sum_mode = 0;
   int sweep_variant[3] = {0, 0, 0};
   float tune_ms[3][8] = {};
+  int col_planes = 4;     // k planes per block of the marching kernels (0: one thread per cell)
   int use_graph = 0;
   int timing = 0;
   hipGraphExec_t gexec = nullptr;
@@ -1431,8 +1638,47 @@ inline long ghost_count(int f) {
   return (long)OCFD_LAP * (dir == 0 ? S.ny : S.nx) * (dir == 2 ? S.ny : S.nz);
 }
 
-// k_rk with the viscous scheme fixed at compile time
-inline void (*rk_kernel())(DevState, int, double, int) { return G.S.cd8 ? k_rk<1> : k_rk<0>; }
+// k_rk with the viscous scheme and (for the stepping path's flag sets) the
+// flags fixed at compile time, flat or marching
+template <int CD, int F>
+void launch_rk_f(int krk, double dt, int flags) {
+  const DevState& S = G.S;
+  const int kc = G.col_planes;
+  if (kc > 0) launch("k_rk", k_rk_col<CD, F>, col_grid(S, kc), dim3(256), S, kc, krk, dt, flags);
+  else launch("k_rk", k_rk<CD, F>, grid1(S.ni), dim3(256), S, krk, dt, flags);
+}
+
+template <int CD>
+void launch_rk_cd(int krk, double dt, int flags) {
+  constexpr int STEP1 = RK_VISC | RK_UPDATE_F | RK_PRIMS | RK_DI_FROM_D;
+  constexpr int STEPMS = RK_VISC | RK_UPDATE_F | RK_PRIMS | RK_MS;
+  switch (flags) {
+    case STEP1: launch_rk_f<CD, STEP1>(krk, dt, flags); break;
+    case STEP1 & ~RK_VISC: launch_rk_f<CD, (STEP1 & ~RK_VISC)>(krk, dt, flags); break;
+    case STEPMS: launch_rk_f<CD, STEPMS>(krk, dt, flags); break;
+    case STEPMS & ~RK_VISC: launch_rk_f<CD, (STEPMS & ~RK_VISC)>(krk, dt, flags); break;
+    default: launch_rk_f<CD, -1>(krk, dt, flags); break;
+  }
+}
+
+void launch_rk(int krk, double dt, int flags) {
+  if (G.S.cd8) launch_rk_cd<1>(krk, dt, flags);
+  else launch_rk_cd<0>(krk, dt, flags);
+}
+
+void launch_visc_prep(bool ms) {
+  const DevState& S = G.S;
+  const int kc = G.col_planes;
+  if (kc > 0) {
+    auto kern = ms ? (S.cd8 ? k_visc_prep_col<1, true> : k_visc_prep_col<0, true>)
+                   : (S.cd8 ? k_visc_prep_col<1, false> : k_visc_prep_col<0, false>);
+    launch("k_visc_prep", kern, col_grid(S, kc), dim3(256), S, kc);
+  } else {
+    auto kern = ms ? (S.cd8 ? k_visc_prep<1, true> : k_visc_prep<0, true>)
+                   : (S.cd8 ? k_visc_prep<1, false> : k_visc_prep<0, false>);
+    launch("k_visc_prep", kern, grid1(S.ni), dim3(256), S);
+  }
+}
 
 template <int DIR, int V>
 void launch_sweep_v() {
@@ -1520,11 +1766,11 @@ void enq_stage(int krk, double dt1) {
   launch_sweep<0>();
   launch_sweep<1>();
   launch_sweep<2>();
-  if (S.viscous) launch("k_visc_prep", k_visc_prep<false>, grid1(S.ni), dim3(256), S);
+  if (S.viscous) launch_visc_prep(false);
   else launch("k_transport", k_transport, grid1(S.ni), dim3(256), S);
   const int flags = (S.mass_force ? RK_MASS : 0) | (S.viscous ? RK_VISC : 0) | RK_UPDATE_F | RK_PRIMS | RK_DI_FROM_D;
   launch("k_negt_reset", k_negt_reset, dim3(1), dim3(64), S, 0);
-  launch("k_rk", rk_kernel(), grid1(S.ni), dim3(256), S, krk, dt1, flags);
+  launch_rk(krk, dt1, flags);
   launch("k_negt_repair", k_negt_repair, dim3(1), dim3(64), S);
   enq_bc();
 }
@@ -1564,10 +1810,10 @@ void enq_stage_ms(int krk, double dt1) {
   launch_sweep<0>();
   launch_sweep<1>();
   launch_sweep<2>();
-  if (S.viscous) launch("k_visc_prep", k_visc_prep<true>, grid1(S.ni), dim3(256), S);
+  if (S.viscous) launch_visc_prep(true);
   const int flags = (S.mass_force ? RK_MASS : 0) | (S.viscous ? RK_VISC : 0) | RK_UPDATE_F | RK_PRIMS | RK_MS;
   launch("k_negt_reset", k_negt_reset, dim3(1), dim3(64), S, 0);
-  launch("k_rk", rk_kernel(), grid1(S.ni), dim3(256), S, krk, dt1, flags);
+  launch_rk(krk, dt1, flags);
   launch("k_negt_repair", k_negt_repair, dim3(1), dim3(64), S);
   enq_bc();
 }
@@ -1809,6 +2055,11 @@ int32_t ocfd_set_option(int32_t key, int32_t value) {
     case OCFD_OPT_HYBRID_SUM_MODE: G.hybrid_sum_mode = value; drop_graph(); return 0;
     case OCFD_OPT_USE_GRAPH: G.use_graph = value; drop_graph(); return 0;
     case OCFD_OPT_KERNEL_TIMING: collect_timing(); G.timing = value; drop_graph(); return 0;
+    case OCFD_OPT_COLUMN_PLANES:
+      if (value < 0) return fail("ocfd_set_option: column planes must be >= 0");
+      G.col_planes = value;
+      drop_graph();
+      return 0;
   }
   return fail("ocfd_set_option: unknown key");
 }
@@ -2049,29 +2300,28 @@ int32_t ocfd_du_inviscous(void) {
   launch_sweep<0>();
   launch_sweep<1>();
   launch_sweep<2>();
-  if (G.S.mass_force) launch("k_rk", rk_kernel(), grid1(G.S.ni), dim3(256), G.S, 1, 0.0, (int)(RK_MASS | RK_WRITE_DU));
+  if (G.S.mass_force) launch_rk(1, 0.0, (int)(RK_MASS | RK_WRITE_DU));
   HIPCHK(hipGetLastError());
   return 0;
 }
 int32_t ocfd_du_viscous(void) {
   if (check_ready()) return 1;
-  if (G.S.ns > 1) launch("k_visc_prep", k_visc_prep<true>, grid1(G.S.ni), dim3(256), G.S);
-  else launch("k_visc_prep", k_visc_prep<false>, grid1(G.S.ni), dim3(256), G.S);
-  launch("k_rk", rk_kernel(), grid1(G.S.ni), dim3(256), G.S, 1, 0.0, (int)(RK_VISC | RK_WRITE_DU));
+  launch_visc_prep(G.S.ns > 1);
+  launch_rk(1, 0.0, (int)(RK_VISC | RK_WRITE_DU));
   HIPCHK(hipGetLastError());
   return 0;
 }
 int32_t ocfd_time_adv_rk3(int32_t krk, double dt) {
   if (check_ready()) return 1;
   if (krk < 1 || krk > 3) return fail("ocfd_time_adv_rk3: KRK must be 1, 2 or 3");
-  launch("k_rk", rk_kernel(), grid1(G.S.ni), dim3(256), G.S, (int)krk, dt, (int)RK_UPDATE_F);
+  launch_rk((int)krk, dt, (int)RK_UPDATE_F);
   HIPCHK(hipGetLastError());
   return 0;
 }
 int32_t ocfd_comput_flow_variables(void) {
   if (check_ready()) return 1;
   launch("k_negt_reset", k_negt_reset, dim3(1), dim3(64), G.S, 0);
-  launch("k_rk", rk_kernel(), grid1(G.S.ni), dim3(256), G.S, 1, 0.0, (int)(G.S.ns > 1 ? RK_PRIMS | RK_MS : RK_PRIMS));
+  launch_rk(1, 0.0, (int)(G.S.ns > 1 ? RK_PRIMS | RK_MS : RK_PRIMS));
   launch("k_negt_repair", k_negt_repair, dim3(1), dim3(64), G.S);
   hipMemcpyAsync(G.h_negt, G.S.negt + 17, sizeof(int), hipMemcpyDeviceToHost, G.stream);
   HIPCHK(hipGetLastError());

@@ -93,6 +93,27 @@ def test_rk3_step(pkg, iflag):
     g.close()
 
 
+def test_column_planes_bitwise(pkg):
+    """The k-marching viscous-flux and update kernels (non-periodic z, where
+    the windows meet the one-sided closures) give the one-thread-per-cell
+    launch bit for bit for chunk lengths that do and do not divide nz."""
+    case = fc.small_jet(nz=22)
+    dt = case[5]["dt"]
+    ref = None
+    for kc in (0, 1, 5, 16, 22):
+        g = pkg.FlowSolver(case[0])
+        g.set_option(pkg.OPT_COLUMN_PLANES, kc)
+        fc.setup_jet(g, case)
+        g.flow_time_advance(dt)
+        snap = {n: g.download(n) for n in ("F", "DI", "T", "P")}
+        g.close()
+        if ref is None:
+            ref = snap
+            continue
+        for n in snap:
+            assert np.array_equal(snap[n].view(np.uint64), ref[n].view(np.uint64)), (kc, n)
+
+
 def test_stage_pieces_equal_fused_step(pkg):
     case = fc.small_jet()
     dt = case[5]["dt"]
