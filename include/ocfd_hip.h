@@ -113,8 +113,9 @@ enum {
   /* 1 = bracket every kernel launch with HIP events (per-kernel timing). */
   OCFD_OPT_KERNEL_TIMING = 3,
   /* k planes walked by one block of the viscous-flux and update kernels
-   * (their z-stencil windows stay in registers); 0 = one thread per cell.
-   * Default 4.  Results are bitwise independent of the value. */
+   * (their z-stencil windows stay in registers); 0 = one thread per cell;
+   * -1 (default) = 4 for N_SPEC = 1, one thread per cell for N_SPEC > 1.
+   * Results are bitwise independent of the value. */
   OCFD_OPT_COLUMN_PLANES = 4
 };
 

@@ -289,6 +289,12 @@ __device__ __forceinline__ double deriv1_central(const DevState& S, int dir, Fn 
 // cells is the central one without a periodic wrap
 __device__ __forceinline__ bool central_cell(int s, int n, int R) { return s >= R + 1 && s <= n - R; }
 
+// Wave-uniform choice of the interior form: every active lane of the wave has
+// the central stencil in the named directions.  Interior waves then run
+// straight-line code (no closure branches), so the loads of all derivatives
+// can be in flight together; boundary waves take the general form.
+__device__ __forceinline__ bool wave_all(bool p) { return __all((int)p) != 0; }
+
 template <int CD = -1, bool INT = false, class Fn>
 __device__ __forceinline__ double deriv1(const DevState& S, int dir, int s, Fn at) {
   static_assert(!INT || CD >= 0, "the interior form needs the scheme fixed at compile time");

@@ -409,13 +409,10 @@ __global__ __launch_bounds__(256) void k_species_conc(DevState S) {
   for (int m = 0; m < S.ns; m++) S.cs[c + (long)m * S.ni] = S.di[q + (long)m * S.nh] / d;
 }
 
-template <int CD>
-__global__ __launch_bounds__(256) void k_species_visc_prep(DevState S) {
-  const long c = cell_index();
-  if (c >= S.ni) return;
-  int i, j, k;
-  cell_ijk(S, c, i, j, k);
-  const long q = hq(S, i, j, k);
+// INT: every stencil of the cell is central (no wrap, no closure), so each
+// species' derivatives are straight-line code whose loads issue together.
+template <int CD, bool INT>
+__device__ __forceinline__ void species_visc_prep_cell(const DevState& S, long c, long q, int i, int j, int k) {
   const double T0 = S.T[q], d0 = S.d[q];
   const double Akx = S.Akx[q], Aky = S.Aky[q], Akz = S.Akz[q];
   const double Aix = S.Aix[q], Aiy = S.Aiy[q], Aiz = S.Aiz[q];
@@ -426,19 +423,24 @@ __global__ __launch_bounds__(256) void k_species_visc_prep(DevState S) {
   double Qpx = 0.0, Qpy = 0.0, Qpz = 0.0;
   const long ni = S.ni;
   for (int m = 0; m < S.ns; m++) {
-    const double* C = S.cs + (long)m * ni;
+    const double* C = S.cs + (long)m * ni + c;
     auto dc = [&](int dir, int s) {
-      return deriv1<CD>(S, dir, s, [&](int o) {
-        const int t = wrap(s + o, S.n[dir], S.per[dir]);
-        return C[c + (long)(t - s) * S.si[dir]];
-      });
+      const long st = S.si[dir];
+      if constexpr (INT) {
+        return deriv1<CD, true>(S, dir, s, [&](int o) { return C[o * st]; });
+      } else {
+        return deriv1<CD>(S, dir, s, [&](int o) {
+          const int t = wrap(s + o, S.n[dir], S.per[dir]);
+          return C[(t - s) * st];
+        });
+      }
     };
     const double uk = dc(0, i), ui = dc(1, j), us = dc(2, k);
+    const double dD = d0 * S.AmD[c + (long)m * ni];
     const double hi = gas_hi(m, T0);
     const double cix = uk * Akx + ui * Aix + us * Asx;
     const double ciy = uk * Aky + ui * Aiy + us * Asy;
     const double ciz = uk * Akz + ui * Aiz + us * Asz;
-    const double dD = d0 * S.AmD[c + (long)m * ni];
     const double E1 = dD * cix, E2 = dD * ciy, E3 = dD * ciz;
     Qpx = Qpx + E1 * hi;
     Qpy = Qpy + E2 * hi;
@@ -452,42 +454,75 @@ __global__ __launch_bounds__(256) void k_species_visc_prep(DevState S) {
   S.Qp[c + 2 * ni] = Qpz;
 }
 
-// Species update: Dus += div(diffusive flux) Ajac (OCFD_scaler_viscous.f90:67-77)
-// and Scalar_time_adv_RK3 (OCFD_Time_Adv.f90:134-166) for every species;
-// krk = 0 stores the complete right-hand side in dus instead.
 template <int CD>
-__global__ __launch_bounds__(256) void k_species_rk(DevState S, int krk, double dt, int visc) {
+__global__ __launch_bounds__(256) void k_species_visc_prep(DevState S) {
+  constexpr int R = CD == 1 ? 4 : 3;
   const long c = cell_index();
   if (c >= S.ni) return;
   int i, j, k;
   cell_ijk(S, c, i, j, k);
   const long q = hq(S, i, j, k);
+  if (wave_all(central_cell(i, S.nx, R) && central_cell(j, S.ny, R) && central_cell(k, S.nz, R)))
+    species_visc_prep_cell<CD, true>(S, c, q, i, j, k);
+  else
+    species_visc_prep_cell<CD, false>(S, c, q, i, j, k);
+}
+
+// Species update: Dus += div(diffusive flux) Ajac (OCFD_scaler_viscous.f90:67-77)
+// and Scalar_time_adv_RK3 (OCFD_Time_Adv.f90:134-166) for every species;
+// krk = 0 stores the complete right-hand side in dus instead.
+template <int CD, bool INT>
+__device__ __forceinline__ void species_rk_cell(const DevState& S, long c, long q, int i, int j, int k, int krk,
+                                                double dt, int visc) {
   const long ni = S.ni;
   const double ajac = S.Ajac[q];
   auto dv = [&](const double* F, int dir, int s) {
-    return deriv1<CD>(S, dir, s, [&](int o) {
-      const int t = wrap(s + o, S.n[dir], S.per[dir]);
-      return F[c + (long)(t - s) * S.si[dir]];
-    });
+    const double* Fc = F + c;
+    const long st = S.si[dir];
+    if constexpr (INT) {
+      return deriv1<CD, true>(S, dir, s, [&](int o) { return Fc[o * st]; });
+    } else {
+      return deriv1<CD>(S, dir, s, [&](int o) {
+        const int t = wrap(s + o, S.n[dir], S.per[dir]);
+        return Fc[(t - s) * st];
+      });
+    }
   };
   const double ra = krk == 2 ? 3.0 / 4.0 : 1.0 / 3.0;
   const double rb = krk == 2 ? 1.0 / 4.0 : 2.0 / 3.0;
   for (int m = 0; m < S.ns; m++) {
-    double val = S.dus[c + (long)m * ni];
+    const long qm = q + (long)m * S.nh, cm = c + (long)m * ni;
+    // the stage's old values, read with the stencil loads
+    const double dn = krk >= 1 ? S.din[cm] : 0.0;
+    const double dold = krk >= 2 ? S.di[qm] : 0.0;
+    double val = S.dus[cm];
     if (visc) {
       const double uk = dv(S.Evs + (long)(3 * m) * ni, 0, i);
       const double ui = dv(S.Evs + (long)(3 * m + 1) * ni, 1, j);
       const double us = dv(S.Evs + (long)(3 * m + 2) * ni, 2, k);
       val = val + (uk + ui + us) * ajac;
     }
-    const long qm = q + (long)m * S.nh, cm = c + (long)m * ni;
     if (krk == 0) {  // right-hand side only
       S.dus[cm] = val;
       continue;
     }
-    if (krk == 1) S.di[qm] = S.din[cm] + dt * val;
-    else S.di[qm] = ra * S.din[cm] + rb * (S.di[qm] + dt * val);
+    if (krk == 1) S.di[qm] = dn + dt * val;
+    else S.di[qm] = ra * dn + rb * (dold + dt * val);
   }
+}
+
+template <int CD>
+__global__ __launch_bounds__(256) void k_species_rk(DevState S, int krk, double dt, int visc) {
+  constexpr int R = CD == 1 ? 4 : 3;
+  const long c = cell_index();
+  if (c >= S.ni) return;
+  int i, j, k;
+  cell_ijk(S, c, i, j, k);
+  const long q = hq(S, i, j, k);
+  if (wave_all(central_cell(i, S.nx, R) && central_cell(j, S.ny, R) && central_cell(k, S.nz, R)))
+    species_rk_cell<CD, true>(S, c, q, i, j, k, krk, dt, visc);
+  else
+    species_rk_cell<CD, false>(S, c, q, i, j, k, krk, dt, visc);
 }
 
 // Rescaling_di (Comput_flow_variables.f90:101-121) and comput_T_P (:32-46)
