@@ -246,16 +246,18 @@ __global__ __launch_bounds__(256) void k_sweep(DevState S) {
       for (int k = 0; k < 10; k++) sCnt[wave][k] = cnt[k];
     }
     __syncthreads();
-    int off[10];
+    int off[10], ktot[10];
     int acc = 0;
 #pragma unroll
     for (int k = 1; k < 10; k++) {
       int before = 0;
+      ktot[k] = 0;
       for (int w = 0; w < 4; w++) {
         if (w < wave) before += sCnt[w][k];
-        acc += sCnt[w][k];
+        ktot[k] += sCnt[w][k];
       }
-      off[k] = acc - (sCnt[0][k] + sCnt[1][k] + sCnt[2][k] + sCnt[3][k]) + before;
+      off[k] = acc + before;
+      acc += ktot[k];
     }
     if (tid == 0) sTotal = acc;
     const unsigned long long lt = (1ull << lane) - 1ull;
@@ -264,6 +266,7 @@ __global__ __launch_bounds__(256) void k_sweep(DevState S) {
       const int kind = e < e1 ? (int)sKind[e] : 0;
 #pragma unroll
       for (int k = 1; k < 10; k++) {
+        if (ktot[k] == 0) continue;  // kind absent from the tile (uniform)
         const unsigned long long bal = __ballot(kind == k);
         if (kind == k) sList[off[k] + __popcll(bal & lt)] = (unsigned short)e;
         off[k] += __popcll(bal);
@@ -277,15 +280,27 @@ __global__ __launch_bounds__(256) void k_sweep(DevState S) {
   // Results are held in registers until every lane has read its windows,
   // then written over the split buffer.
   const int inner = S.scheme == 1 ? K_WENO5 : S.scheme == 2 ? K_WENO7 : S.scheme == 3 ? K_OMP6 : K_UD7L;
+  // item t = tid + it*256 of 5*total is (comp, idx) = divmod(t, total),
+  // advanced by 256 per iteration without a division
+  constexpr int LPS = DIR == 0 ? 1 : NP;  // LDS stride between neighbouring points of a pencil
   double res[MAXT];
   int dst[MAXT];
+  int comp = 0, idx = tid;
+  while (idx >= total && comp < 5) {
+    idx -= total;
+    comp++;
+  }
 #pragma unroll
   for (int it = 0; it < MAXT; it++) {
     dst[it] = -1;
-    const int t = tid + it * 256;
-    if (t >= 5 * total) continue;
-    const int comp = t / total;
-    const int idx = t - comp * total;
+    if (it > 0) {
+      idx += 256;
+      while (idx >= total && comp < 5) {
+        idx -= total;
+        comp++;
+      }
+    }
+    if (comp >= 5) continue;
     const int e = uniform ? idx : (int)sList[idx];
     const int side = e / (NP * NIF);
     const int r = e - side * (NP * NIF);
@@ -293,10 +308,12 @@ __global__ __launch_bounds__(256) void k_sweep(DevState S) {
     const int l = DIR == 0 ? r % NIF : r / NP;
     if (uniform && (pen0 + pl >= npen || l > ncell)) continue;
     const int kind = uniform ? inner : (int)sKind[e];
-    const double* F = sF + (side * 5 + comp) * NP * NPT;
+    // window w[qq] = v(l-3+qq) for flux+ and v(l+4-qq) for flux-
+    const double* F = sF + (side * 5 + comp) * NP * NPT + lp(pl, side ? l + 7 : l);
+    const int step = side ? -LPS : LPS;
     double w[8];
 #pragma unroll
-    for (int qq = 0; qq < 8; qq++) w[qq] = F[lp(pl, side ? l + 7 - qq : l + qq)];
+    for (int qq = 0; qq < 8; qq++) w[qq] = F[qq * step];
     res[it] = recon(kind, w, S.ud7l);
     dst[it] = (side * 5 + comp) * NP * NIF + r;
   }
