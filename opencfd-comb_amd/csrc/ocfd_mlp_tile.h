@@ -18,10 +18,11 @@ typedef float tfloatx4 __attribute__((ext_vector_type(4)));
 constexpr int MT_ROWS = 128;  // largest number of cells per workgroup
 constexpr int MT_LDH = 264;   // LDS row pitch (bf16): 256 + 8 against bank conflicts
 
-// GELU (tanh form) as z * sigmoid(2u), the same evaluation as training
+// GELU (tanh form) as z * sigmoid(2u) with the hardware exp and reciprocal,
+// the same evaluation as training
 __device__ __forceinline__ float mt_gelu(float z) {
   const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
-  return z / (1.0f + __expf(-2.0f * u));
+  return z * __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
 }
 
 // H[ROWS][0:K] (LDS) -> act(H . W^T + b) written back into H[ROWS][0:256].
