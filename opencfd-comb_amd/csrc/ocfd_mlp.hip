@@ -38,17 +38,18 @@ constexpr int TM = 64, TN = 64, TK = 32, LDT = TK + 8;  // LDS row: 40 bf16 = 80
 enum { EPI_STORE = 0, EPI_BIAS_GELU = 1, EPI_DGELU = 2 };
 
 // GELU in its tanh form, 0.5 z (1 + tanh u) with u = c (z + 0.044715 z^3),
-// evaluated as z * sigmoid(2u) = z * rcp(1 + exp(-2u)) with the hardware exp
-// and reciprocal (v_exp_f32, v_rcp_f32: 1 ulp; a correctly rounded division
-// costs a quarter of the fused inference kernel's time and is far below the
-// bfloat16 rounding of the activations).  The inference tile uses the same.
-__device__ __forceinline__ float gelu_f(float z) {
-  const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
-  return z * __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
+// evaluated as z * sigmoid(2u) = z * rcp(1 + exp2(-2 c log2(e) (z + 0.044715
+// z^3))) with the hardware exp2 and reciprocal (v_exp_f32, v_rcp_f32: 1 ulp,
+// far below the bfloat16 rounding of the activations; a correctly rounded
+// division cost a sixth of the fused inference kernel's time).  The inference
+// tile evaluates the same expression.
+__device__ __forceinline__ float gelu_sig(float z) {
+  const float w = fmaf(0.044715f * z * z, z, z);
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f((-2.0f * 0.7978845608028654f * 1.4426950408889634f) * w));
 }
+__device__ __forceinline__ float gelu_f(float z) { return z * gelu_sig(z); }
 __device__ __forceinline__ float dgelu_f(float z) {
-  const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
-  const float s = __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
+  const float s = gelu_sig(z);
   return s + z * s * (1.0f - s) * 2.0f * 0.7978845608028654f * (1.0f + 3.0f * 0.044715f * z * z);
 }
 

@@ -21,8 +21,8 @@ constexpr int MT_LDH = 264;   // LDS row pitch (bf16): 256 + 8 against bank conf
 // GELU (tanh form) as z * sigmoid(2u) with the hardware exp and reciprocal,
 // the same evaluation as training
 __device__ __forceinline__ float mt_gelu(float z) {
-  const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
-  return z * __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
+  const float w = fmaf(0.044715f * z * z, z, z);
+  return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f((-2.0f * 0.7978845608028654f * 1.4426950408889634f) * w));
 }
 
 // H[ROWS][0:K] (LDS) -> act(H . W^T + b) written back into H[ROWS][0:256].
