@@ -25,7 +25,7 @@ __device__ __forceinline__ float mt_gelu(float z) {
   return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f((-2.0f * 0.7978845608028654f * 1.4426950408889634f) * w));
 }
 
-// H[ROWS][0:K] (LDS) -> act(H . W^T + b) written back into H[ROWS][0:256].
+// H[ROWS][0:K] (LDS) -> act(b + H . W^T) written back into H[ROWS][0:256].
 // W: [256][ldw] bf16 row-major (out x in).  NW waves; wave w computes output
 // columns [256/NW w, 256/NW (w+1)) for all ROWS rows.  Every output element
 // accumulates its K-chunks of 32 in increasing order whatever ROWS and NW
@@ -34,13 +34,16 @@ template <bool GELU, int ROWS, int NW>
 __device__ __forceinline__ void mt_layer256(tbf16* H, const tbf16* __restrict__ W, const float* __restrict__ b, int K,
                                             int ldw, int lane, int w) {
   constexpr int IT = ROWS / 16, JT = 16 / NW;
-  tfloatx4 acc[IT][JT];
-#pragma unroll
-  for (int i = 0; i < IT; i++)
-#pragma unroll
-    for (int j = 0; j < JT; j++) acc[i][j] = tfloatx4{0.f, 0.f, 0.f, 0.f};
   const int lr = lane & 15, lk = (lane >> 4) * 8;
   const int cb = 16 * JT * w;
+  // the accumulators start from the bias of their column
+  tfloatx4 acc[IT][JT];
+#pragma unroll
+  for (int j = 0; j < JT; j++) {
+    const float bias = b[cb + 16 * j + lr];
+#pragma unroll
+    for (int i = 0; i < IT; i++) acc[i][j] = tfloatx4{bias, bias, bias, bias};
+  }
   tbf16x8 bn[JT];
 #pragma unroll
   for (int j = 0; j < JT; j++) bn[j] = *(const tbf16x8*)&W[(long)(cb + 16 * j + lr) * ldw + lk];
@@ -63,12 +66,11 @@ __device__ __forceinline__ void mt_layer256(tbf16* H, const tbf16* __restrict__ 
 #pragma unroll
   for (int j = 0; j < JT; j++) {
     const int col = cb + 16 * j + lr;
-    const float bias = b[col];
 #pragma unroll
     for (int i = 0; i < IT; i++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const float v = acc[i][j][r] + bias;
+        const float v = acc[i][j][r];
         H[(16 * i + (lane >> 4) * 4 + r) * MT_LDH + col] = (tbf16)(GELU ? mt_gelu(v) : v);
       }
   }
@@ -76,16 +78,17 @@ __device__ __forceinline__ void mt_layer256(tbf16* H, const tbf16* __restrict__ 
 }
 
 // Output layer, first 16 output columns: O[ROWS][16] (fp32, may alias H) =
-// H[ROWS][0:K] . W[0:16]^T + b.  Wave w takes rows [ROWS/NW w, ROWS/NW (w+1)).
+// b + H[ROWS][0:K] . W[0:16]^T.  Wave w takes rows [ROWS/NW w, ROWS/NW (w+1)).
 template <int ROWS, int NW>
 __device__ __forceinline__ void mt_out16(const tbf16* H, const tbf16* __restrict__ W, const float* __restrict__ b,
                                          int K, int ldw, float* O, int lane, int w) {
   constexpr int RW = ROWS / NW, IT = RW / 16;
   static_assert(IT >= 1, "at least 16 rows per wave");
+  const int lr = lane & 15, lk = (lane >> 4) * 8;
+  const float bias = b[lr];
   tfloatx4 acc[IT];
 #pragma unroll
-  for (int i = 0; i < IT; i++) acc[i] = tfloatx4{0.f, 0.f, 0.f, 0.f};
-  const int lr = lane & 15, lk = (lane >> 4) * 8;
+  for (int i = 0; i < IT; i++) acc[i] = tfloatx4{bias, bias, bias, bias};
   for (int k0 = 0; k0 < K; k0 += 32) {
     const tbf16x8 bf = *(const tbf16x8*)&W[(long)lr * ldw + k0 + lk];
 #pragma unroll
@@ -95,10 +98,9 @@ __device__ __forceinline__ void mt_out16(const tbf16* H, const tbf16* __restrict
     }
   }
   __syncthreads();  // O may alias H
-  const float bias = b[lr];
 #pragma unroll
   for (int i = 0; i < IT; i++)
 #pragma unroll
-    for (int r = 0; r < 4; r++) O[(RW * w + 16 * i + (lane >> 4) * 4 + r) * 16 + lr] = acc[i][r] + bias;
+    for (int r = 0; r < 4; r++) O[(RW * w + 16 * i + (lane >> 4) * 4 + r) * 16 + lr] = acc[i][r];
   __syncthreads();
 }
