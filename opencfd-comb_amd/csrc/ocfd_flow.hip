@@ -158,12 +158,20 @@ __device__ __forceinline__ void split_point(int kind, double Et1, double p1, dou
   }
 }
 
+// Kind lists: the sorted list of a tile depends only on the hybrid masks and
+// the position, which change once per step, so the first sweep after the
+// masks are built stores it (KC_WRITE: entries at kc + tile*NE, count at
+// kct[tile]) and the later stages read it back (KC_READ) instead of
+// classifying and sorting again.  A list entry is (kind << 12) | entry.
+enum { KC_NONE = 0, KC_WRITE = 1, KC_READ = 2 };
+
 template <int DIR, int V>
-__global__ __launch_bounds__(256) void k_sweep(DevState S) {
+__global__ __launch_bounds__(256) void k_sweep(DevState S, int kmode, unsigned short* kc, int* kct) {
   constexpr int NP = SweepTile<DIR, V>::NP, TS = SweepTile<DIR, V>::TS;
   constexpr int NPT = TS + 8, NIF = TS + 1, NE = 2 * NP * NIF;
   constexpr int MAXT = (5 * NE + 255) / 256;
   static_assert(10 * NP * NIF <= 10 * NP * NPT, "interface values must fit in the split buffer");
+  static_assert(NE < 0x1000, "entry index must fit in 12 bits");
   // one LDS buffer: split fluxes (phase 1-2), then interface values (phase 3)
   __shared__ double sF[10 * NP * NPT];
   double* sH = sF;
@@ -218,7 +226,12 @@ __global__ __launch_bounds__(256) void k_sweep(DevState S) {
   constexpr int PERW = (NE + 3) / 4;
   const int e0 = wave * PERW, e1 = min(NE, e0 + PERW);
   int total;
-  if (!uniform) {
+  if (!uniform && kmode == KC_READ) {
+    const long base = lb * NE;
+    const int cnt = kct[lb];
+    for (int i = tid; i < cnt; i += blockDim.x) sList[i] = kc[base + i];
+    if (tid == 0) sTotal = cnt;
+  } else if (!uniform) {
     int cnt[10];
 #pragma unroll
     for (int k = 0; k < 10; k++) cnt[k] = 0;
@@ -268,9 +281,15 @@ __global__ __launch_bounds__(256) void k_sweep(DevState S) {
       for (int k = 1; k < 10; k++) {
         if (ktot[k] == 0) continue;  // kind absent from the tile (uniform)
         const unsigned long long bal = __ballot(kind == k);
-        if (kind == k) sList[off[k] + __popcll(bal & lt)] = (unsigned short)e;
+        if (kind == k) sList[off[k] + __popcll(bal & lt)] = (unsigned short)((k << 12) | e);
         off[k] += __popcll(bal);
       }
+    }
+    if (kmode == KC_WRITE) {
+      __syncthreads();
+      const long base = lb * NE;
+      for (int i = tid; i < acc; i += blockDim.x) kc[base + i] = sList[i];
+      if (tid == 0) kct[lb] = acc;
     }
   }
   __syncthreads();
@@ -301,13 +320,14 @@ __global__ __launch_bounds__(256) void k_sweep(DevState S) {
       }
     }
     if (comp >= 5) continue;
-    const int e = uniform ? idx : (int)sList[idx];
+    const int u = uniform ? idx : (int)sList[idx];
+    const int e = u & 0xFFF;
     const int side = e / (NP * NIF);
     const int r = e - side * (NP * NIF);
     const int pl = DIR == 0 ? r / NIF : r % NP;
     const int l = DIR == 0 ? r % NIF : r / NP;
     if (uniform && (pen0 + pl >= npen || l > ncell)) continue;
-    const int kind = uniform ? inner : (int)sKind[e];
+    const int kind = uniform ? inner : u >> 12;
     // window w[qq] = v(l-3+qq) for flux+ and v(l+4-qq) for flux-
     const double* F = sF + (side * 5 + comp) * NP * NPT + lp(pl, side ? l + 7 : l);
     const int step = side ? -LPS : LPS;
@@ -1553,6 +1573,9 @@ struct Ctx {
   int sweep_variant[3] = {0, 0, 0};
   float tune_ms[3][8] = {};
   int col_planes = -1;    // k planes per block of the marching kernels (0: one thread per cell, -1: auto)
+  unsigned short* kc[3] = {nullptr, nullptr, nullptr};  // sweep kind lists per direction
+  int* kct[3] = {nullptr, nullptr, nullptr};            // their per-tile counts
+  int kc_var[3] = {-1, -1, -1};  // tile variant the stored lists belong to (-1: none / masks rebuilt)
   int use_graph = 0;
   int timing = 0;
   hipGraphExec_t gexec = nullptr;
@@ -1704,7 +1727,30 @@ void launch_sweep_v() {
   const long npen = DIR == 0 ? (long)S.ny * S.nz : DIR == 1 ? (long)S.nx * S.nz : (long)S.nx * S.ny;
   dim3 g((unsigned)((S.n[DIR] + TS - 1) / TS), (unsigned)((npen + NP - 1) / NP));
   static const char* names[3] = {"k_sweep_x", "k_sweep_y", "k_sweep_z"};
-  launch(names[DIR], k_sweep<DIR, V>, g, dim3(256), S);
+  // the first sweep after the masks are built (or after a tile change)
+  // stores the kind lists, the following ones read them
+  int mode = KC_WRITE;
+  if (G.kc_var[DIR] == V) mode = KC_READ;
+  G.kc_var[DIR] = V;
+  launch(names[DIR], k_sweep<DIR, V>, g, dim3(256), S, mode, G.kc[DIR], G.kct[DIR]);
+}
+
+// entries and tiles of the largest kind-list layout over the tile variants
+template <int DIR, int V>
+void kc_size(long& entries, long& tiles) {
+  const DevState& S = G.S;
+  constexpr int NP = SweepTile<DIR, V>::NP, TS = SweepTile<DIR, V>::TS;
+  const long npen = DIR == 0 ? (long)S.ny * S.nz : DIR == 1 ? (long)S.nx * S.nz : (long)S.nx * S.ny;
+  const long nt = (long)((S.n[DIR] + TS - 1) / TS) * ((npen + NP - 1) / NP);
+  entries = std::max(entries, nt * 2L * NP * (TS + 1));
+  tiles = std::max(tiles, nt);
+}
+template <int DIR>
+void kc_sizes(long& entries, long& tiles) {
+  kc_size<DIR, 0>(entries, tiles);
+  kc_size<DIR, 1>(entries, tiles);
+  kc_size<DIR, 2>(entries, tiles);
+  kc_size<DIR, 3>(entries, tiles);
 }
 
 template <int DIR>
@@ -1740,6 +1786,7 @@ void launch_masks() {
 
 void enq_rhybrid() {
   const DevState& S = G.S;
+  for (int d = 0; d < 3; d++) G.kc_var[d] = -1;  // new masks: the stored kind lists are stale
   const int sensor = (int)std::lround(G.cfg.hybrid_para[0]);
   if (sensor == 1) {
     launch("k_sensor1", k_sensor1, grid1(S.ni), dim3(256), S);
@@ -1909,6 +1956,11 @@ int32_t ocfd_finalize(void) {
   G.allocs.clear();
   G.chem_rho = G.chem_E = G.chem_dt = nullptr;
   G.part = nullptr;
+  for (int d = 0; d < 3; d++) {
+    G.kc[d] = nullptr;
+    G.kct[d] = nullptr;
+    G.kc_var[d] = -1;
+  }
   G.hg = FlowGas{};
   G.chem_method = OCFD_CHEM_FIXED;
   if (G.h_negt) hipHostFree(G.h_negt);
@@ -2041,6 +2093,21 @@ int32_t ocfd_init(const ocfd_cfg* cfg) {
     G.allocs.push_back(m);
     hipMemsetAsync(m, 0, sizeof(int) * 32, G.stream);
     S.negt = (int*)m;
+  }
+  for (int d = 0; d < 3; d++) {
+    long entries = 0, tiles = 0;
+    if (d == 0) kc_sizes<0>(entries, tiles);
+    else if (d == 1) kc_sizes<1>(entries, tiles);
+    else kc_sizes<2>(entries, tiles);
+    void* a = nullptr;
+    void* b = nullptr;
+    if (hipMalloc(&a, sizeof(unsigned short) * (size_t)entries) != hipSuccess) return fail("ocfd_init: hipMalloc failed");
+    G.allocs.push_back(a);
+    if (hipMalloc(&b, sizeof(int) * (size_t)tiles) != hipSuccess) return fail("ocfd_init: hipMalloc failed");
+    G.allocs.push_back(b);
+    G.kc[d] = (unsigned short*)a;
+    G.kct[d] = (int*)b;
+    G.kc_var[d] = -1;
   }
   G.nparts = 1024;
   G.part = dalloc(G.nparts);
