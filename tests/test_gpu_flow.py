@@ -126,11 +126,14 @@ def test_column_planes_bitwise(pkg, vis):
     """The k-marching viscous-flux and update kernels give the one-thread-per-
     cell launch bit for bit for every chunk length (one plane, lengths that do
     not divide nz, the whole column), and the oracle after an RK3 step, with
-    the CD6 and the CD8 viscous scheme on a periodic box and a flat plate."""
+    the CD6 and the CD8 viscous scheme on a periodic box and on a flat plate
+    with periodic and with non-periodic span."""
     import oracle
-    for case in (fc.small_flat_plate(48, 32, 19), fc.small_isotropic(20)):
+    for case, zper in ((fc.small_flat_plate(48, 32, 19), 1), (fc.small_flat_plate(48, 32, 19), 0),
+                       (fc.small_isotropic(20), 1)):
         case = list(case)
         case[0].scheme_vis = getattr(pkg, "SCHEME_" + vis)
+        case[0].periodic[2] = zper  # non-periodic z: windows meet the one-sided closures
         case = tuple(case)
         dt1 = 0.5 * case[5]["dt"]
         ref = None
