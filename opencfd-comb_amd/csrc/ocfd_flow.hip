@@ -2232,8 +2232,8 @@ int32_t ocfd_set_mesh(const double* axx, const double* ayy, const double* azz, c
   if (check_ready()) return 1;
   DevState& S = G.S;
   const size_t bytes = sizeof(double) * (size_t)S.nh;
-  double* tmp = dalloc(S.nh);
-  if (!tmp) return fail("ocfd_set_mesh: hipMalloc failed");
+  double* tmp = nullptr;
+  if (hipMalloc(&tmp, bytes) != hipSuccess) return fail("ocfd_set_mesh: hipMalloc failed");
   const double* src[3] = {axx, ayy, azz};
   double* dst[3] = {S.Axx, S.Ayy, S.Azz};
   for (int a = 0; a < 3; a++) {
@@ -2266,6 +2266,7 @@ int32_t ocfd_set_mesh(const double* axx, const double* ayy, const double* azz, c
     }
   hipLaunchKernelGGL(k_metrics_over_jac, grid1(S.nh), dim3(256), 0, G.stream, S);
   HIPCHK(hipStreamSynchronize(G.stream));
+  hipFree(tmp);
   HIPCHK(hipGetLastError());
   drop_graph();
   return 0;
