@@ -2179,8 +2179,8 @@ int32_t ocfd_set_species(int32_t n_tct, const double* tct, double tct_max, const
   auto P5 = [](double x) { double x2 = x * x; return (x2 * x2) * x; };
   for (int k = 0; k < ns; k++)
     for (int j = 0; j < ns; j++) {
-      h.wsq[k][j] = std::sqrt(mi[j] / mi[k]);               // comput_vis_coeff, TRANSPORT_modules.f90:300-301
-      h.wden[k][j] = std::sqrt(8.0 * (1.0 + mi[k] / mi[j]));
+      h.wq[k][j] = std::sqrt(std::sqrt(mi[j] / mi[k]));       // comput_vis_coeff, TRANSPORT_modules.f90:300-301
+      h.wrec[k][j] = 1.0 / std::sqrt(8.0 * (1.0 + mi[k] / mi[j]));
     }
   for (int k = 0; k < ns; k++) {
     h.Mi[k] = mi[k];

@@ -21,9 +21,9 @@ struct FlowGas {
   double Mi[OCFD_NS_MAX], Ri[OCFD_NS_MAX];
   double coef[OCFD_NS_MAX][GAS_NT_MAX][7], ect[OCFD_NS_MAX][GAS_NT_MAX];
   double mu_an[OCFD_NS_MAX][4], k_an[OCFD_NS_MAX][4], D_an[OCFD_NS_MAX][OCFD_NS_MAX][4];
-  // Wilke factors that depend only on the molar masses, evaluated on the host
-  // with the same IEEE operations: sqrt(M_j/M_k) and sqrt(8 (1 + M_k/M_j))
-  double wsq[OCFD_NS_MAX][OCFD_NS_MAX], wden[OCFD_NS_MAX][OCFD_NS_MAX];
+  // Wilke factors that depend only on the molar masses, evaluated on the
+  // host: wq = (M_j/M_k)^(1/4) and wrec = 1/sqrt(8 (1 + M_k/M_j))
+  double wq[OCFD_NS_MAX][OCFD_NS_MAX], wrec[OCFD_NS_MAX][OCFD_NS_MAX];
 };
 __constant__ FlowGas fg;
 
@@ -210,6 +210,17 @@ __global__ __launch_bounds__(256, 5) void k_transport_ms(DevState S) {
     double mui[OCFD_NS_MAX];
 #pragma unroll
     for (int m = 0; m < OCFD_NS_MAX; m++) mui[m] = m < ns ? chemkin_fit(fg.mu_an[m], lT) * 0.1 : 1.0;
+    // Wilke's Phi_kj = (1 + sqrt(mu_k/mu_j) (M_j/M_k)^(1/4))^2 / sqrt(8 (1 +
+    // M_k/M_j)) (comput_vis_coeff, TRANSPORT_modules.f90:296-305) with
+    // sqrt(mu_k/mu_j) = sqrt(mu_k) / sqrt(mu_j): 9 square roots and 9
+    // divisions per cell instead of 81 and 162 (a few ulps from the
+    // reference's evaluation)
+    double smu[OCFD_NS_MAX], rmu[OCFD_NS_MAX];
+#pragma unroll
+    for (int m = 0; m < OCFD_NS_MAX; m++) {
+      smu[m] = sqrt(mui[m]);
+      rmu[m] = 1.0 / smu[m];
+    }
     double mu = 0.0;
 #pragma unroll
     for (int kk = 0; kk < OCFD_NS_MAX; kk++) {
@@ -218,8 +229,8 @@ __global__ __launch_bounds__(256, 5) void k_transport_ms(DevState S) {
 #pragma unroll
         for (int jj = 0; jj < OCFD_NS_MAX; jj++) {
           if (jj < ns) {
-            const double a = 1.0 + sqrt(mui[kk] / mui[jj] * fg.wsq[kk][jj]);
-            const double Pkj = (a * a) / fg.wden[kk][jj];
+            const double a = 1.0 + smu[kk] * rmu[jj] * fg.wq[kk][jj];
+            const double Pkj = (a * a) * fg.wrec[kk][jj];
             Pk0 = Pk0 + Xi[jj] * Pkj;
           }
         }
@@ -270,9 +281,9 @@ __global__ __launch_bounds__(256, 5) void k_transport_ms(DevState S) {
 #pragma unroll
       for (int b = a + 1; b < OCFD_NS_MAX; b++)
         if (b < ns) {
-          const double D = chemkin_fit(fg.D_an[a][b], lT) * 1.0e-4;
-          ds[a] = ds[a] + Xi[b] / D;
-          ds[b] = ds[b] + Xi[a] / D;
+          const double rD = 1.0 / (chemkin_fit(fg.D_an[a][b], lT) * 1.0e-4);
+          ds[a] = ds[a] + Xi[b] * rD;
+          ds[b] = ds[b] + Xi[a] * rD;
         }
 #pragma unroll
     for (int kk = 0; kk < OCFD_NS_MAX; kk++) {
