@@ -394,10 +394,10 @@ int32_t ocfd_chem_surrogate_batch(int64_t n, double* rho, double* E, const doubl
                                   double rtol_y, double y_floor, double rtol_t, int32_t max_sub, int32_t* flag_out);
 /* Tile-shape autotuner of the fused surrogate kernel (no reference
  * counterpart): each shape (cells per workgroup x waves: 128x4, 128x8,
- * 64x4) runs reps times on scratch copies of the batch (rho, E are not
+ * 64x4, 96x4) runs reps times on scratch copies of the batch (rho, E are not
  * modified), timed with HIP events; the fastest is kept.  Results are
  * bit-identical for every shape.  ms_out (may be NULL) gets the mean ms per
- * launch of each shape (3 floats).  Synchronises. */
+ * launch of each shape (4 floats).  Synchronises. */
 int32_t ocfd_chem_surrogate_autotune(int64_t n, const double* rho, const double* E, const double* dt, int32_t reps,
                                      float* ms_out);
 /* Returns the shape index in use and its cells per workgroup and waves;

@@ -943,7 +943,8 @@ template <int V> struct SurrTile;
 template <> struct SurrTile<0> { static constexpr int ROWS = 128, NW = 4, OCC = 2; };
 template <> struct SurrTile<1> { static constexpr int ROWS = 128, NW = 8, OCC = 1; };
 template <> struct SurrTile<2> { static constexpr int ROWS = 64, NW = 4, OCC = 3; };
-#define SURR_VARIANTS 3
+template <> struct SurrTile<3> { static constexpr int ROWS = 96, NW = 4, OCC = 3; };
+#define SURR_VARIANTS 4
 
 template <int V>
 __global__ __launch_bounds__(SurrTile<V>::NW * 64, SurrTile<V>::OCC) void k_surrogate(
@@ -1081,6 +1082,7 @@ void launch_surrogate(int v, long n, double* rho, double* E, const double* dt, i
   switch (v) {
     case 1: launch_surrogate_v<1>(n, rho, E, dt, flag_out); break;
     case 2: launch_surrogate_v<2>(n, rho, E, dt, flag_out); break;
+    case 3: launch_surrogate_v<3>(n, rho, E, dt, flag_out); break;
     default: launch_surrogate_v<0>(n, rho, E, dt, flag_out); break;
   }
 }
@@ -1315,8 +1317,8 @@ int32_t ocfd_chem_surrogate_tile(int32_t variant, int32_t* rows, int32_t* waves)
   if (variant >= SURR_VARIANTS) return cfail("ocfd_chem_surrogate_tile: unknown variant");
   if (variant >= 0) CX.surr_variant = variant;
   const int v = CX.surr_variant;
-  const int R[SURR_VARIANTS] = {SurrTile<0>::ROWS, SurrTile<1>::ROWS, SurrTile<2>::ROWS};
-  const int W[SURR_VARIANTS] = {SurrTile<0>::NW, SurrTile<1>::NW, SurrTile<2>::NW};
+  const int R[SURR_VARIANTS] = {SurrTile<0>::ROWS, SurrTile<1>::ROWS, SurrTile<2>::ROWS, SurrTile<3>::ROWS};
+  const int W[SURR_VARIANTS] = {SurrTile<0>::NW, SurrTile<1>::NW, SurrTile<2>::NW, SurrTile<3>::NW};
   if (rows) *rows = R[v];
   if (waves) *waves = W[v];
   return v;

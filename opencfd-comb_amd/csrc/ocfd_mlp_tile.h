@@ -1,6 +1,6 @@
 // ocfd_mlp_tile.h -- device code of the fused MLP forward used inside the
 // chemistry surrogate kernel: one workgroup of NW waves (4 or 8) carries a
-// tile of cells (ROWS = 64 or 128) through every layer, the activations
+// tile of cells (ROWS = 64, 96 or 128) through every layer, the activations
 // staying in LDS as bfloat16 and each wave streaming its slice of the
 // weights from L2.
 //
@@ -78,29 +78,38 @@ __device__ __forceinline__ void mt_layer256(tbf16* H, const tbf16* __restrict__ 
 }
 
 // Output layer, first 16 output columns: O[ROWS][16] (fp32, may alias H) =
-// b + H[ROWS][0:K] . W[0:16]^T.  Wave w takes rows [ROWS/NW w, ROWS/NW (w+1)).
+// b + H[ROWS][0:K] . W[0:16]^T.  The 16-row tiles are dealt to the waves in
+// turn (wave w takes tiles w, w + NW, ...), so ROWS need only be a multiple
+// of 16; each element accumulates its K-chunks in increasing order.
 template <int ROWS, int NW>
 __device__ __forceinline__ void mt_out16(const tbf16* H, const tbf16* __restrict__ W, const float* __restrict__ b,
                                          int K, int ldw, float* O, int lane, int w) {
-  constexpr int RW = ROWS / NW, IT = RW / 16;
-  static_assert(IT >= 1, "at least 16 rows per wave");
+  static_assert(ROWS % 16 == 0, "whole 16-row tiles");
+  constexpr int NT = ROWS / 16, TPW = (NT + NW - 1) / NW;
   const int lr = lane & 15, lk = (lane >> 4) * 8;
   const float bias = b[lr];
-  tfloatx4 acc[IT];
+  tfloatx4 acc[TPW];
 #pragma unroll
-  for (int i = 0; i < IT; i++) acc[i] = tfloatx4{bias, bias, bias, bias};
+  for (int it = 0; it < TPW; it++) acc[it] = tfloatx4{bias, bias, bias, bias};
   for (int k0 = 0; k0 < K; k0 += 32) {
     const tbf16x8 bf = *(const tbf16x8*)&W[(long)lr * ldw + k0 + lk];
 #pragma unroll
-    for (int i = 0; i < IT; i++) {
-      const tbf16x8 a = *(const tbf16x8*)&H[(RW * w + 16 * i + lr) * MT_LDH + k0 + lk];
-      acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bf, acc[i], 0, 0, 0);
+    for (int it = 0; it < TPW; it++) {
+      const int tt = w + it * NW;
+      if (tt < NT) {
+        const tbf16x8 a = *(const tbf16x8*)&H[(16 * tt + lr) * MT_LDH + k0 + lk];
+        acc[it] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bf, acc[it], 0, 0, 0);
+      }
     }
   }
   __syncthreads();  // O may alias H
 #pragma unroll
-  for (int i = 0; i < IT; i++)
+  for (int it = 0; it < TPW; it++) {
+    const int tt = w + it * NW;
+    if (tt < NT) {
 #pragma unroll
-    for (int r = 0; r < 4; r++) O[(RW * w + 16 * i + (lane >> 4) * 4 + r) * 16 + lr] = acc[i][r];
+      for (int r = 0; r < 4; r++) O[(16 * tt + (lane >> 4) * 4 + r) * 16 + lr] = acc[it][r];
+    }
+  }
   __syncthreads();
 }

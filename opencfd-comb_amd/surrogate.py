@@ -274,7 +274,7 @@ class Surrogate:
         batch and keep the fastest (bit-identical results).  Returns
         (rows, waves, ms per shape)."""
         n = self.chem._args(rho, E, dt)
-        ms = (ctypes.c_float * 3)()
+        ms = (ctypes.c_float * 4)()
         self._check(self.lib.ocfd_chem_surrogate_autotune(n, _vp(rho.data_ptr()), _vp(E.data_ptr()),
                                                           _vp(dt.data_ptr()), int(reps), ms))
         rows, waves = self.tile()
