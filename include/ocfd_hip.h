@@ -116,7 +116,11 @@ enum {
    * (their z-stencil windows stay in registers); 0 = one thread per cell;
    * -1 (default) = 4 for N_SPEC = 1, one thread per cell for N_SPEC > 1.
    * Results are bitwise independent of the value. */
-  OCFD_OPT_COLUMN_PLANES = 4
+  OCFD_OPT_COLUMN_PLANES = 4,
+  /* 1 (default) = the viscous-flux preparation of each stage runs on a
+   * second stream beside the inviscid sweeps; 0 = one stream.  Results are
+   * bitwise independent of the value. */
+  OCFD_OPT_CONCURRENT = 5
 };
 
 /* ---- lifetime ---------------------------------------------------------- */

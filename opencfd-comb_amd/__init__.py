@@ -39,7 +39,7 @@ FIELDS = ["F", "FN", "DU", "DI", "D", "U", "V", "W", "T", "P", "ET", "AMU", "AMK
 FIELD = {name: i for i, name in enumerate(FIELDS)}
 METRICS = FIELDS[FIELD["AKX"]:FIELD["ASZ1"] + 1]
 
-OPT_HYBRID_SUM_MODE, OPT_USE_GRAPH, OPT_KERNEL_TIMING, OPT_COLUMN_PLANES = 1, 2, 3, 4
+OPT_HYBRID_SUM_MODE, OPT_USE_GRAPH, OPT_KERNEL_TIMING, OPT_COLUMN_PLANES, OPT_CONCURRENT = 1, 2, 3, 4, 5
 CHEM_FIXED, CHEM_ADAPTIVE, CHEM_SURROGATE = 0, 1, 2
 
 

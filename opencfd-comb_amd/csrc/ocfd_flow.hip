@@ -1576,6 +1576,9 @@ struct Ctx {
   unsigned short* kc[3] = {nullptr, nullptr, nullptr};  // sweep kind lists per direction
   int* kct[3] = {nullptr, nullptr, nullptr};            // their per-tile counts
   int kc_var[3] = {-1, -1, -1};  // tile variant the stored lists belong to (-1: none / masks rebuilt)
+  hipStream_t side = nullptr;    // the viscous-flux preparation runs here beside the sweeps
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  int concurrent = 1;
   int use_graph = 0;
   int timing = 0;
   hipGraphExec_t gexec = nullptr;
@@ -1628,6 +1631,7 @@ hipEvent_t get_event() {
 void collect_timing() {
   if (G.pending.empty()) return;
   hipStreamSynchronize(G.stream);
+  if (G.side) hipStreamSynchronize(G.side);
   for (auto& p : G.pending) {
     float ms = 0.f;
     hipEventElapsedTime(&ms, p.a, p.b);
@@ -1650,17 +1654,21 @@ int kid(const char* name) {
 }
 
 template <typename K, typename... A>
-void launch(const char* name, K kern, dim3 g, dim3 b, A... args) {
+void launch_s(hipStream_t st, const char* name, K kern, dim3 g, dim3 b, A... args) {
   if (G.timing) {
     hipEvent_t a = get_event(), e = get_event();
-    hipEventRecord(a, G.stream);
-    hipLaunchKernelGGL(kern, g, b, 0, G.stream, args...);
-    hipEventRecord(e, G.stream);
+    hipEventRecord(a, st);
+    hipLaunchKernelGGL(kern, g, b, 0, st, args...);
+    hipEventRecord(e, st);
     G.pending.push_back({kid(name), a, e});
     if (G.pending.size() > 4096) collect_timing();
   } else {
-    hipLaunchKernelGGL(kern, g, b, 0, G.stream, args...);
+    hipLaunchKernelGGL(kern, g, b, 0, st, args...);
   }
+}
+template <typename K, typename... A>
+void launch(const char* name, K kern, dim3 g, dim3 b, A... args) {
+  launch_s(G.stream, name, kern, g, b, args...);
 }
 
 inline dim3 grid1(long n) { return dim3((unsigned)((n + 255) / 256)); }
@@ -1706,18 +1714,37 @@ void launch_rk(int krk, double dt, int flags) {
   else launch_rk_cd<0>(krk, dt, flags);
 }
 
-void launch_visc_prep(bool ms) {
+void launch_visc_prep(bool ms, hipStream_t st = nullptr) {
   const DevState& S = G.S;
+  if (!st) st = G.stream;
   const int kc = col_planes();
   if (kc > 0) {
     auto kern = ms ? (S.cd8 ? k_visc_prep_col<1, true> : k_visc_prep_col<0, true>)
                    : (S.cd8 ? k_visc_prep_col<1, false> : k_visc_prep_col<0, false>);
-    launch("k_visc_prep", kern, col_grid(S, kc), dim3(256), S, kc);
+    launch_s(st, "k_visc_prep", kern, col_grid(S, kc), dim3(256), S, kc);
   } else {
     auto kern = ms ? (S.cd8 ? k_visc_prep<1, true> : k_visc_prep<0, true>)
                    : (S.cd8 ? k_visc_prep<1, false> : k_visc_prep<0, false>);
-    launch("k_visc_prep", kern, grid1(S.ni), dim3(256), S);
+    launch_s(st, "k_visc_prep", kern, grid1(S.ni), dim3(256), S);
   }
+}
+
+// The viscous-flux preparation reads only primitives, metrics and transport
+// data and writes only Ev, Amu, Amk (and reads Qp), none of which the
+// inviscid sweeps touch; k_rk needs both.  It therefore runs on a second
+// stream beside the sweeps (VALU-bound sweeps, memory-bound preparation):
+// fork after the work it depends on, join before k_rk.  Under graph capture
+// the fork and join become parallel branches.
+hipStream_t fork_side() {
+  if (!G.concurrent) return G.stream;
+  hipEventRecord(G.ev_fork, G.stream);
+  hipStreamWaitEvent(G.side, G.ev_fork, 0);
+  return G.side;
+}
+void join_side(hipStream_t st) {
+  if (st == G.stream) return;
+  hipEventRecord(G.ev_join, st);
+  hipStreamWaitEvent(G.stream, G.ev_join, 0);
 }
 
 template <int DIR, int V>
@@ -1827,11 +1854,13 @@ void enq_bc() {
 void enq_stage(int krk, double dt1) {
   const DevState& S = G.S;
   if (krk == 1 && G.cfg.scheme_invis == OCFD_SCHEME_HYBRID) enq_rhybrid();
+  const hipStream_t vs = fork_side();
+  if (S.viscous) launch_visc_prep(false, vs);
+  else launch_s(vs, "k_transport", k_transport, grid1(S.ni), dim3(256), S);
   launch_sweep<0>();
   launch_sweep<1>();
   launch_sweep<2>();
-  if (S.viscous) launch_visc_prep(false);
-  else launch("k_transport", k_transport, grid1(S.ni), dim3(256), S);
+  join_side(vs);
   const int flags = (S.mass_force ? RK_MASS : 0) | (S.viscous ? RK_VISC : 0) | RK_UPDATE_F | RK_PRIMS | RK_DI_FROM_D;
   launch("k_negt_reset", k_negt_reset, dim3(1), dim3(64), S, 0);
   launch_rk(krk, dt1, flags);
@@ -1871,10 +1900,12 @@ void enq_stage_ms(int krk, double dt1) {
   launch("k_transport_ms", k_transport_ms, grid1(S.ni), dim3(256), S);
   if (krk == 1 && G.cfg.scheme_invis == OCFD_SCHEME_HYBRID) enq_rhybrid();
   enq_species(krk, dt1);
+  const hipStream_t vs = S.viscous ? fork_side() : G.stream;
+  if (S.viscous) launch_visc_prep(true, vs);
   launch_sweep<0>();
   launch_sweep<1>();
   launch_sweep<2>();
-  if (S.viscous) launch_visc_prep(true);
+  join_side(vs);
   const int flags = (S.mass_force ? RK_MASS : 0) | (S.viscous ? RK_VISC : 0) | RK_UPDATE_F | RK_PRIMS | RK_MS;
   launch("k_negt_reset", k_negt_reset, dim3(1), dim3(64), S, 0);
   launch_rk(krk, dt1, flags);
@@ -1965,6 +1996,11 @@ int32_t ocfd_finalize(void) {
   G.chem_method = OCFD_CHEM_FIXED;
   if (G.h_negt) hipHostFree(G.h_negt);
   G.h_negt = nullptr;
+  if (G.side) hipStreamDestroy(G.side);
+  if (G.ev_fork) hipEventDestroy(G.ev_fork);
+  if (G.ev_join) hipEventDestroy(G.ev_join);
+  G.side = nullptr;
+  G.ev_fork = G.ev_join = nullptr;
   hipStreamDestroy(G.stream);
   G.stream = nullptr;
   G.inited = false;
@@ -2060,6 +2096,10 @@ int32_t ocfd_init(const ocfd_cfg* cfg) {
   }
 
   if (hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking) != hipSuccess) return fail("ocfd_init: no HIP device/stream");
+  if (hipStreamCreateWithFlags(&G.side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&G.ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&G.ev_join, hipEventDisableTiming) != hipSuccess)
+    return fail("ocfd_init: stream or event creation failed");
   const long nh = S.nh, ni = S.ni, ns = S.ns, nms = S.ns > 1 ? S.ns : 0;
   struct { double** p; long n; double v; } arrs[] = {
       {&S.f, 5 * ni, 0.0}, {&S.fn, 5 * ni, 0.0}, {&S.du, 5 * ni, 0.0}, {&S.di, nh * ns, 1.0}, {&S.din, ni * ns, 1.0},
@@ -2139,6 +2179,7 @@ int32_t ocfd_set_option(int32_t key, int32_t value) {
     case OCFD_OPT_HYBRID_SUM_MODE: G.hybrid_sum_mode = value; drop_graph(); return 0;
     case OCFD_OPT_USE_GRAPH: G.use_graph = value; drop_graph(); return 0;
     case OCFD_OPT_KERNEL_TIMING: collect_timing(); G.timing = value; drop_graph(); return 0;
+    case OCFD_OPT_CONCURRENT: G.concurrent = value ? 1 : 0; drop_graph(); return 0;
     case OCFD_OPT_COLUMN_PLANES:
       if (value < -1) return fail("ocfd_set_option: column planes must be >= -1");
       G.col_planes = value;
@@ -2543,13 +2584,22 @@ int32_t ocfd_autotune(int32_t reps) {
         else launch_sweep_variant<2>(v);
       };
       run();  // warm
+      // a hybrid step rebuilds the masks once per RK3 step, so its sweeps
+      // classify and store their kind lists in one stage of three and read
+      // them back in the other two: time that mix; otherwise the lists are
+      // stored once and every launch reads them
+      const bool hyb = G.cfg.scheme_invis == OCFD_SCHEME_HYBRID;
+      const int per = hyb ? 3 : 1;
       HIPCHK(hipEventRecord(a, G.stream));
-      for (int r = 0; r < reps; r++) run();
+      for (int r = 0; r < reps; r++) {
+        if (hyb) G.kc_var[dir] = -1;
+        for (int l = 0; l < per; l++) run();
+      }
       HIPCHK(hipEventRecord(b, G.stream));
       HIPCHK(hipEventSynchronize(b));
       float ms = 0.f;
       HIPCHK(hipEventElapsedTime(&ms, a, b));
-      G.tune_ms[dir][v] = ms / reps;
+      G.tune_ms[dir][v] = ms / (reps * per);
       if (ms < best) { best = ms; bestv = v; }
     }
     G.sweep_variant[dir] = bestv;
