@@ -1577,7 +1577,7 @@ struct Ctx {
   int* kct[3] = {nullptr, nullptr, nullptr};            // their per-tile counts
   int kc_var[3] = {-1, -1, -1};  // tile variant the stored lists belong to (-1: none / masks rebuilt)
   hipStream_t side = nullptr;    // the viscous-flux preparation runs here beside the sweeps
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_qp = nullptr;
   int concurrent = 1;
   int use_graph = 0;
   int timing = 0;
@@ -1853,10 +1853,11 @@ void enq_bc() {
 
 void enq_stage(int krk, double dt1) {
   const DevState& S = G.S;
-  if (krk == 1 && G.cfg.scheme_invis == OCFD_SCHEME_HYBRID) enq_rhybrid();
+  // the viscous preparation does not read the hybrid masks: fork before the sensor
   const hipStream_t vs = fork_side();
   if (S.viscous) launch_visc_prep(false, vs);
   else launch_s(vs, "k_transport", k_transport, grid1(S.ni), dim3(256), S);
+  if (krk == 1 && G.cfg.scheme_invis == OCFD_SCHEME_HYBRID) enq_rhybrid();
   launch_sweep<0>();
   launch_sweep<1>();
   launch_sweep<2>();
@@ -1869,27 +1870,33 @@ void enq_stage(int krk, double dt1) {
 }
 
 template <int DIR>
-void launch_species_sweep() {
+void launch_species_sweep(hipStream_t st) {
   const DevState& S = G.S;
   constexpr int NP = SpTile<DIR>::NP, TS = SpTile<DIR>::TS;
   const long npen = DIR == 0 ? (long)S.ny * S.nz : DIR == 1 ? (long)S.nx * S.nz : (long)S.nx * S.ny;
   dim3 g((unsigned)((S.n[DIR] + TS - 1) / TS), (unsigned)((npen + NP - 1) / NP));
   static const char* names[3] = {"k_species_sweep_x", "k_species_sweep_y", "k_species_sweep_z"};
-  launch(names[DIR], k_species_sweep<DIR>, g, dim3(256), S);
+  launch_s(st, names[DIR], k_species_sweep<DIR>, g, dim3(256), S);
 }
 
 // species right-hand sides (krk = 0: leave them in dus) or the species RK3
 // stage (OCFD_Time_Adv.f90:64-71)
-void enq_species(int krk, double dt1) {
+// on stream st (default: the library stream); after_prep, if given, is
+// recorded once Qp (species viscous preparation) is written
+void enq_species(int krk, double dt1, hipStream_t st = nullptr, hipEvent_t after_prep = nullptr) {
   const DevState& S = G.S;
-  launch_species_sweep<0>();
-  launch_species_sweep<1>();
-  launch_species_sweep<2>();
+  if (!st) st = G.stream;
+  launch_species_sweep<0>(st);
+  launch_species_sweep<1>(st);
+  launch_species_sweep<2>(st);
   if (S.viscous) {
-    launch("k_species_conc", k_species_conc, grid1(S.ni), dim3(256), S);
-    launch("k_species_visc_prep", S.cd8 ? k_species_visc_prep<1> : k_species_visc_prep<0>, grid1(S.ni), dim3(256), S);
+    launch_s(st, "k_species_conc", k_species_conc, grid1(S.ni), dim3(256), S);
+    launch_s(st, "k_species_visc_prep", S.cd8 ? k_species_visc_prep<1> : k_species_visc_prep<0>, grid1(S.ni),
+             dim3(256), S);
   }
-  launch("k_species_rk", S.cd8 ? k_species_rk<1> : k_species_rk<0>, grid1(S.ni), dim3(256), S, krk, dt1, S.viscous);
+  if (after_prep) hipEventRecord(after_prep, st);
+  launch_s(st, "k_species_rk", S.cd8 ? k_species_rk<1> : k_species_rk<0>, grid1(S.ni), dim3(256), S, krk, dt1,
+           S.viscous);
 }
 
 // one RK3 stage for N_SPEC > 1 (OCFD_Time_Adv.f90:33-91): transport, sensor,
@@ -1899,13 +1906,19 @@ void enq_stage_ms(int krk, double dt1) {
   const DevState& S = G.S;
   launch("k_transport_ms", k_transport_ms, grid1(S.ni), dim3(256), S);
   if (krk == 1 && G.cfg.scheme_invis == OCFD_SCHEME_HYBRID) enq_rhybrid();
-  enq_species(krk, dt1);
-  const hipStream_t vs = S.viscous ? fork_side() : G.stream;
-  if (S.viscous) launch_visc_prep(true, vs);
+  // the species chain (sweeps, concentrations, diffusion, update of di) on
+  // the second stream beside the flow sweeps, which read no di; the flow's
+  // viscous preparation waits for Qp, k_rk (Rescaling_di) for the whole chain
+  const hipStream_t sp = fork_side();
+  enq_species(krk, dt1, sp, sp != G.stream ? G.ev_qp : nullptr);
   launch_sweep<0>();
   launch_sweep<1>();
   launch_sweep<2>();
-  join_side(vs);
+  if (S.viscous) {
+    if (sp != G.stream) hipStreamWaitEvent(G.stream, G.ev_qp, 0);
+    launch_visc_prep(true);
+  }
+  join_side(sp);
   const int flags = (S.mass_force ? RK_MASS : 0) | (S.viscous ? RK_VISC : 0) | RK_UPDATE_F | RK_PRIMS | RK_MS;
   launch("k_negt_reset", k_negt_reset, dim3(1), dim3(64), S, 0);
   launch_rk(krk, dt1, flags);
@@ -1999,8 +2012,9 @@ int32_t ocfd_finalize(void) {
   if (G.side) hipStreamDestroy(G.side);
   if (G.ev_fork) hipEventDestroy(G.ev_fork);
   if (G.ev_join) hipEventDestroy(G.ev_join);
+  if (G.ev_qp) hipEventDestroy(G.ev_qp);
   G.side = nullptr;
-  G.ev_fork = G.ev_join = nullptr;
+  G.ev_fork = G.ev_join = G.ev_qp = nullptr;
   hipStreamDestroy(G.stream);
   G.stream = nullptr;
   G.inited = false;
@@ -2098,7 +2112,8 @@ int32_t ocfd_init(const ocfd_cfg* cfg) {
   if (hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking) != hipSuccess) return fail("ocfd_init: no HIP device/stream");
   if (hipStreamCreateWithFlags(&G.side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&G.ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&G.ev_join, hipEventDisableTiming) != hipSuccess)
+      hipEventCreateWithFlags(&G.ev_join, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&G.ev_qp, hipEventDisableTiming) != hipSuccess)
     return fail("ocfd_init: stream or event creation failed");
   const long nh = S.nh, ni = S.ni, ns = S.ns, nms = S.ns > 1 ? S.ns : 0;
   struct { double** p; long n; double v; } arrs[] = {
