@@ -317,6 +317,9 @@ def main():
     ap.add_argument("--nz", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--serial", action="store_true",
+                    help="one stream throughout (no viscous preparation beside the sweeps), e.g. for a kernel "
+                         "trace whose per-kernel durations are solo ones")
     ap.add_argument("--no-chem", action="store_true", help="skip the chemistry batch")
     ap.add_argument("--chem-cells", type=int, default=1 << 21)
     ap.add_argument("--surrogate-train-cells", type=int, default=1 << 24, help="0 skips the surrogate")
@@ -360,6 +363,7 @@ def main():
     g = m.FlowSolver(cfg)
     setup(g, case)
     g.set_option(m.OPT_HYBRID_SUM_MODE, 1)
+    g.set_option(m.OPT_CONCURRENT, 0 if args.serial else 1)
     tiles = g.autotune(3)
     g.set_option(m.OPT_USE_GRAPH, 1)
     step = run_steps(g, info, args.warmup, 0)
@@ -389,7 +393,9 @@ def main():
     kstats = {}
     if not args.no_kernel_timing:
         # per-launch HIP events on the library stream, over a second timed
-        # region of the same steps (graph replay off while events are on)
+        # region of the same steps (graph replay off while events are on);
+        # kernels run one at a time here so each duration is the kernel's own
+        g.set_option(m.OPT_CONCURRENT, 0)
         g.set_option(m.OPT_KERNEL_TIMING, 1)
         g.kernel_stats_reset()
         nprof = min(args.steps, 20)
@@ -397,6 +403,7 @@ def main():
         g.synchronize()
         kstats = g.kernel_stats()
         g.set_option(m.OPT_KERNEL_TIMING, 0)
+        g.set_option(m.OPT_CONCURRENT, 0 if args.serial else 1)
         name, (ms, n) = max(kstats.items(), key=lambda kv: kv[1][0])
         avg_s = ms / n / 1e3
         per_cell = {
@@ -430,7 +437,7 @@ def main():
                                 traffic=traffic, flops_per_cell=per_cell, hbm_achieved_GBps=hbm)
             roofline.update(cells_per_launch=ncell, avg_launch_ms=avg_s * 1e3,
                             timing="HIP events around every launch on the library stream, second pass of "
-                                   f"{nprof} steps (graph replay off)")
+                                   f"{nprof} steps (graph replay off, one kernel at a time)")
 
     # whole-step roofline: measured rate / min(HBM / B_alg, FP64 / F_alg)
     mean_frac = {v: sum(frac[d][v] for d in range(3)) / 3 for v in (1, 2, 3)}
@@ -457,6 +464,7 @@ def main():
                    "scheme": "HYBRID UD7L/WENO7/WENO5, pressure-gradient sensor (fast mean)",
                    "splitting": "Steger-Warming", "viscous": "CD6", "gas": "perfect gas, Sutherland",
                    "graph": "hipGraph replay", "parallelism": "single GPU",
+                   "streams": 1 if args.serial else 2,
                    "sweep_tiles": {"xyz"[d]: {"NP": tiles[d][1], "TS": tiles[d][2]} for d in range(3)},
                    "mask_fractions": {"x": frac[0], "y": frac[1], "z": frac[2]}},
         "roofline": roofline, "step_roofline": step_roofline, "cpu_baseline": cpu,
