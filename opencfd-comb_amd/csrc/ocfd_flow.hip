@@ -31,19 +31,21 @@
 // adjacent in i, so loads coalesce across pencils.
 // ===========================================================================
 // Tile shapes (pencils per workgroup NP x cells per pencil TS) offered to
-// the autotuner; variant 0 is the default.  Every variant computes every
+// the autotuner; variant 0 is the default, the shape that measured fastest
+// inside steps of the flat plate, and the autotuner leaves it only for a
+// shape more than 3% faster.  Every variant computes every
 // interface with the same arithmetic, so results do not depend on the choice.
 template <int DIR, int V> struct SweepTile;
 template <> struct SweepTile<0, 0> { static constexpr int NP = 4, TS = 64; };
 template <> struct SweepTile<0, 1> { static constexpr int NP = 8, TS = 32; };
 template <> struct SweepTile<0, 2> { static constexpr int NP = 2, TS = 128; };
 template <> struct SweepTile<0, 3> { static constexpr int NP = 8, TS = 64; };
-template <> struct SweepTile<1, 0> { static constexpr int NP = 16, TS = 16; };
-template <> struct SweepTile<1, 1> { static constexpr int NP = 8, TS = 32; };
+template <> struct SweepTile<1, 0> { static constexpr int NP = 8, TS = 32; };
+template <> struct SweepTile<1, 1> { static constexpr int NP = 16, TS = 16; };
 template <> struct SweepTile<1, 2> { static constexpr int NP = 16, TS = 32; };
 template <> struct SweepTile<1, 3> { static constexpr int NP = 32, TS = 16; };
-template <> struct SweepTile<2, 0> { static constexpr int NP = 16, TS = 16; };
-template <> struct SweepTile<2, 1> { static constexpr int NP = 8, TS = 32; };
+template <> struct SweepTile<2, 0> { static constexpr int NP = 8, TS = 32; };
+template <> struct SweepTile<2, 1> { static constexpr int NP = 16, TS = 16; };
 template <> struct SweepTile<2, 2> { static constexpr int NP = 16, TS = 32; };
 template <> struct SweepTile<2, 3> { static constexpr int NP = 32, TS = 16; };
 #define OCFD_SWEEP_VARIANTS 4
@@ -2615,7 +2617,9 @@ int32_t ocfd_autotune(int32_t reps) {
       float ms = 0.f;
       HIPCHK(hipEventElapsedTime(&ms, a, b));
       G.tune_ms[dir][v] = ms / (reps * per);
-      if (ms < best) { best = ms; bestv = v; }
+      // isolated launches on the set-up state rank near-equal shapes by
+      // noise: keep the default unless another shape is clearly faster
+      if (v == 0 || ms < 0.97f * best) { best = ms; bestv = v; }
     }
     G.sweep_variant[dir] = bestv;
   }
