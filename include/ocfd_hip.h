@@ -245,6 +245,22 @@ int32_t ocfd_time_adv_rk3(int32_t krk, double dt); /* OCFD_Time_Adv.f90:98-130 *
 int32_t ocfd_comput_flow_variables(void);   /* Comput_flow_variables.f90:3-46 */
 int32_t ocfd_bc(void);                      /* OCFD_BC.f90:2-188 */
 
+/* OCFD2d_flux1 (side 0, flux+) and OCFD2d_flux2 (side 1, flux-) over npen
+ * pencils at once (code/flow-solver/OCFD_inviscous.f90:209-281, 286-362).
+ * Device pointers, run on the library stream:
+ *   v   npen x (n + 2*LAP) doubles, pencil p holding f1d(1-LAP:n+LAP);
+ *   hh  npen x (n + 1) doubles, receives hh(0:n);
+ *   scm npen x (n + 1) bytes, Scm_Hy(0:n) (1 UD7L, 2 WENO7, 3 WENO5); read
+ *       only by the hybrid scheme, may be null otherwise.  An interior
+ *       interface whose hybrid mask is outside 1..3 gets NaN.
+ * Host pointers: bound_index[2] (Bound_index, 0 or 1) and
+ * scheme_boundary[2] (Scheme_boundary, -1, 0 or 1).  scheme is one of the
+ * inviscid OCFD_SCHEME_* values; the UD7L coefficients are the context's
+ * (OCFD_Config.ud7l_diss).  Results are bit-identical to the sweep's
+ * reconstructions and to the reference subroutines. */
+int32_t ocfd_flux1d_batch(int32_t side, int32_t scheme, int64_t npen, int32_t n, const double* v, double* hh,
+                          const uint8_t* scm, const int32_t* bound_index, const int32_t* scheme_boundary);
+
 /* Largest number of cells repaired by handle_NegativeEt in one
  * comput_flow_variables since the last ocfd_save_stage0 / step start
  * (code/others/OCFD_handle_NegativeT.f90:1-20).  More than 10 makes the next

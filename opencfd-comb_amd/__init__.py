@@ -122,7 +122,8 @@ def load_library(path=None):
         "ocfd_stage_begin": (i32, [i32]), "ocfd_comput_amu_amk": (i32, []),
         "ocfd_comput_rhybrid": (i32, []), "ocfd_du_inviscous": (i32, []), "ocfd_du_viscous": (i32, []),
         "ocfd_time_adv_rk3": (i32, [i32, dbl]), "ocfd_comput_flow_variables": (i32, []),
-        "ocfd_bc": (i32, []), "ocfd_negative_et_count": (i32, []), "ocfd_synchronize": (i32, []),
+        "ocfd_bc": (i32, []), "ocfd_negative_et_count": (i32, []),
+        "ocfd_flux1d_batch": (i32, [i32, i32, ctypes.c_int64, i32, vp, vp, vp, P(i32), P(i32)]), "ocfd_synchronize": (i32, []),
         "ocfd_stream": (vp, []), "ocfd_kernel_count": (i32, []), "ocfd_kernel_name": (ctypes.c_char_p, [i32]),
         "ocfd_kernel_stats": (i32, [i32, P(dbl), P(ctypes.c_int64)]), "ocfd_kernel_stats_reset": (i32, []),
         "ocfd_autotune": (i32, [i32]), "ocfd_set_tile": (i32, [i32, i32]),
@@ -325,6 +326,36 @@ class FlowSolver:
 
     def OCFD_bc(self):
         self._check(self.lib.ocfd_bc())
+
+    def OCFD2d_flux(self, side, scheme, v, bound_index=(0, 0), scheme_boundary=(0, 0), scm=None):
+        """OCFD2d_flux1 (side 0) / OCFD2d_flux2 (side 1) over a batch of
+        pencils (OCFD_inviscous.f90:209-362).  v is a float64 CUDA tensor of
+        shape (npen, n + 2*LAP) holding f1d(1-LAP:n+LAP); scm, for the hybrid
+        scheme, a uint8 CUDA tensor of shape (npen, n + 1).  Returns hh(0:n)
+        per pencil as a (npen, n + 1) float64 CUDA tensor."""
+        import torch
+        if not (isinstance(v, torch.Tensor) and v.is_cuda and v.dtype == torch.float64 and v.dim() == 2):
+            raise TypeError("v must be a 2-D float64 CUDA tensor")
+        v = v.contiguous()
+        npen, n = v.shape[0], v.shape[1] - 2 * LAP
+        if n < 1:
+            raise ValueError("v needs more than 2*LAP points per pencil")
+        hh = torch.empty((npen, n + 1), dtype=torch.float64, device=v.device)
+        sp = 0
+        if scm is not None:
+            if not (isinstance(scm, torch.Tensor) and scm.is_cuda and scm.dtype == torch.uint8):
+                raise TypeError("scm must be a uint8 CUDA tensor")
+            if tuple(scm.shape) != (npen, n + 1):
+                raise ValueError(f"scm must have shape {(npen, n + 1)}")
+            scm = scm.contiguous()
+            sp = scm.data_ptr()
+        b = (ctypes.c_int32 * 2)(*bound_index)
+        sb = (ctypes.c_int32 * 2)(*scheme_boundary)
+        # the library runs on its own stream: order it after torch's work
+        torch.cuda.current_stream().synchronize()
+        self._check(self.lib.ocfd_flux1d_batch(int(side), int(scheme), npen, n, v.data_ptr(), hh.data_ptr(), sp, b, sb))
+        self._check(self.lib.ocfd_synchronize())
+        return hh
 
     def negative_et_count(self):
         return int(self.lib.ocfd_negative_et_count())

@@ -229,20 +229,16 @@ __device__ __forceinline__ double recon(int kind, const double* w, const double*
 // Encodes OCFD2d_flux1/OCFD2d_flux2 (OCFD_inviscous.f90:209-362): inner range
 // [ib,ie] from the scheme's stencil [Ka,Kb] (OCFD_readpara.f90:378-405) and
 // the WENO5-type boundary closures otherwise.
-__device__ __forceinline__ int recon_kind(const DevState& S, int dir, int s, int side, int mask) {
-  const int n = S.n[dir];
+__device__ __forceinline__ int recon_kind_p(int scheme, int n, int blo, int bhi, int sblo, int sbhi, int s, int side,
+                                            int mask) {
   int Ka, Kb, inner;
-  switch (S.scheme) {
+  switch (scheme) {
     case 1: Ka = side ? -1 : -2; Kb = side ? 3 : 2; inner = K_WENO5; break;
     case 2: Ka = side ? -2 : -3; Kb = side ? 4 : 3; inner = K_WENO7; break;
     case 3: Ka = -3; Kb = 4; inner = K_OMP6; break;
     case 4: Ka = -3; Kb = 4; inner = K_UD7L; break;
     default: Ka = -3; Kb = 4; inner = mask; break;  // hybrid: 1 UD7L, 2 WENO7, 3 WENO5
   }
-  const int blo = S.bidx[dir][0], bhi = S.bidx[dir][1];
-  const int sblo = S.sb[2 * dir], sbhi = S.sb[2 * dir + 1];
-  const int ib = (blo == 0 || sblo == -1) ? 0 : -Ka + 1;
-  const int ie = (bhi == 0 || sbhi == -1) ? n : n - Kb;
   int kind = inner;
   if (blo == 1 && s <= -Ka) {
     if (sblo == 0) {
@@ -260,9 +256,11 @@ __device__ __forceinline__ int recon_kind(const DevState& S, int dir, int s, int
       kind = K_WENO5;
     }
   }
-  (void)ib;
-  (void)ie;
   return kind;
+}
+__device__ __forceinline__ int recon_kind(const DevState& S, int dir, int s, int side, int mask) {
+  return recon_kind_p(S.scheme, S.n[dir], S.bidx[dir][0], S.bidx[dir][1], S.sb[2 * dir], S.sb[2 * dir + 1], s, side,
+                      mask);
 }
 
 // ---------------------------------------------------------------------------

@@ -369,6 +369,34 @@ __global__ __launch_bounds__(256) void k_sweep(DevState S, int kmode, unsigned s
   }
 }
 
+// OCFD2d_flux1 / OCFD2d_flux2 over a batch of pencils (the 1-D scheme entry
+// points of OCFD_inviscous.f90:209-362): one thread per (pencil, interface),
+// the sweep's window, kind selection and reconstructions.  v holds
+// v(1-LAP:n+LAP) of each pencil, hh gets hh(0:n), scm the hybrid mask of
+// each interface.
+__global__ __launch_bounds__(256) void k_flux1d(DevState S, int side, int scheme, long npen, int n,
+                                                const double* __restrict__ v, double* __restrict__ hh,
+                                                const uint8_t* __restrict__ scm, int blo, int bhi, int sblo, int sbhi) {
+  const long npts = (long)n + 1;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npen * npts) return;
+  const long p = t / npts;
+  const int l = (int)(t - p * npts);
+  // a hybrid mask outside 1..3 names no scheme: such interfaces get NaN
+  int mask = scm ? (int)scm[t] : 0;
+  if (mask < K_UD7L || mask > K_WENO5) mask = -1;
+  const int kind = recon_kind_p(scheme, n, blo, bhi, sblo, sbhi, l, side, mask);
+  if (kind < 0) {
+    hh[t] = __longlong_as_double(0x7ff8000000000000LL);
+    return;
+  }
+  const double* vp = v + p * (n + 2 * OCFD_LAP);
+  double w[8];
+#pragma unroll
+  for (int qq = 0; qq < 8; qq++) w[qq] = vp[side ? l + 7 - qq : l + qq];
+  hh[t] = recon(kind, w, S.ud7l);
+}
+
 // ===========================================================================
 // Characteristic flux (IF_Scheme_Character = 1): du_inviscous_Character and
 // flux_charteric, OCFD_inviscous_character.f90:3-314.  At each interface the
@@ -2447,6 +2475,29 @@ int32_t ocfd_du_inviscous(void) {
   HIPCHK(hipGetLastError());
   return 0;
 }
+int32_t ocfd_flux1d_batch(int32_t side, int32_t scheme, int64_t npen, int32_t n, const double* v, double* hh,
+                          const uint8_t* scm, const int32_t* bound_index, const int32_t* scheme_boundary) {
+  if (check_ready()) return 1;
+  if (side != 0 && side != 1) return fail("ocfd_flux1d_batch: side must be 0 (flux1) or 1 (flux2)");
+  if (scheme != OCFD_SCHEME_WENO5 && scheme != OCFD_SCHEME_WENO7 && scheme != OCFD_SCHEME_OMP6 &&
+      scheme != OCFD_SCHEME_UD7L && scheme != OCFD_SCHEME_HYBRID)
+    return fail("ocfd_flux1d_batch: unknown inviscid scheme");
+  if (npen < 0 || n < 1) return fail("ocfd_flux1d_batch: npen >= 0 and n >= 1 required");
+  if (npen > 0 && (!v || !hh)) return fail("ocfd_flux1d_batch: null v or hh");
+  if (scheme == OCFD_SCHEME_HYBRID && npen > 0 && !scm) return fail("ocfd_flux1d_batch: the hybrid scheme needs scm");
+  if (!bound_index || !scheme_boundary) return fail("ocfd_flux1d_batch: null bound_index or scheme_boundary");
+  for (int a = 0; a < 2; a++) {
+    if (bound_index[a] != 0 && bound_index[a] != 1) return fail("ocfd_flux1d_batch: Bound_index must be 0 or 1");
+    if (scheme_boundary[a] < -1 || scheme_boundary[a] > 1) return fail("ocfd_flux1d_batch: Scheme_boundary must be -1, 0 or 1");
+  }
+  const long total = npen * ((long)n + 1);
+  if (total > 0)
+    launch("k_flux1d", k_flux1d, grid1(total), dim3(256), G.S, (int)side, (int)scheme, (long)npen, (int)n, v, hh,
+           scm, (int)bound_index[0], (int)bound_index[1], (int)scheme_boundary[0], (int)scheme_boundary[1]);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 int32_t ocfd_du_viscous(void) {
   if (check_ready()) return 1;
   launch_visc_prep(G.S.ns > 1);
