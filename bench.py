@@ -75,6 +75,10 @@ def cpu_baseline(m, nsteps=20):
                 sample=f"oracle (plain C, 1 thread) on the flat plate at 160x120x32, {nsteps} RK3 steps, {el:.1f} s")
 
 
+SURR_TOL = 5e-4           # mass-fraction error the increment bound is calibrated to
+SURR_CAL_CELLS = 1 << 20  # held-out labelled cells for that calibration
+
+
 def chem_bench(m, ncell=1 << 21, cpu=True, surrogate_cells=1 << 24, surrogate_epochs=25):
     """Finite-rate chemistry on the jet-flame batch: ncell independent cells
     (jet fuel/oxidiser mixtures, T 900-2500 K, p 0.5-5 atm, taken along their
@@ -151,24 +155,53 @@ def chem_bench(m, ncell=1 << 21, cpu=True, surrogate_cells=1 << 24, surrogate_ep
         torch.cuda.synchronize()
         t2 = time.perf_counter()
         del x, y
+        # increment bound from held-out labelled cells (a different seed)
+        cal = S.calibrate(S.dataset(SURR_CAL_CELLS, seed=2), tol=SURR_TOL)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
         rho.copy_(r0)
         E.copy_(e0)
         tile_rows, tile_waves, tile_ms = S.autotune(rho, E, dt, reps=3)
         flags = torch.zeros(ncell, dtype=torch.int32, device="cuda")
-        ms_s, _ = timed(lambda: S.step(rho, E, dt, fallback=True, flags=flags))
-        nfb = S.fallbacks()
-        ms_nofb, _ = timed(lambda: S.step(rho, E, dt, fallback=False))
-        rho.copy_(r0)
-        E.copy_(e0)
-        S.step(rho, E, dt, fallback=True, flags=flags)
         Tp = S.temperature(rp, Ep).cpu().numpy()
-        Ts = S.temperature(rho, E).cpu().numpy()
         Yp = (rp / rp.sum(0)).cpu().numpy()
-        Ys = (rho / rho.sum(0)).cpu().numpy()
-        dT = np.abs(Ts - Tp)
         major = [M.index(k) for k in ("H2", "O2", "H2O", "N2")]
-        dYm = np.abs(Ys[major] - Yp[major]).max(axis=0)
-        f = flags.cpu().numpy()
+
+        def surrogate_run():
+            ms_s, _ = timed(lambda: S.step(rho, E, dt, fallback=True, flags=flags))
+            nfb = S.fallbacks()
+            ms_nofb, _ = timed(lambda: S.step(rho, E, dt, fallback=False))
+            rho.copy_(r0)
+            E.copy_(e0)
+            S.step(rho, E, dt, fallback=True, flags=flags)
+            Ts = S.temperature(rho, E).cpu().numpy()
+            Ys = (rho / rho.sum(0)).cpu().numpy()
+            dT = np.abs(Ts - Tp)
+            dYm = np.abs(Ys[major] - Yp[major]).max(axis=0)
+            dYa = np.abs(Ys - Yp).max(axis=0)
+            f = flags.cpu().numpy()
+            return dict(
+                ms=ms_s, cells_per_s=ncell / ms_s * 1e3, fallbacks=nfb,
+                fallback_reasons={k: int(np.sum((f & b) != 0)) for k, b in
+                                  (("domain", 1), ("element_balance", 2), ("negative_Y", 4), ("T_bounds", 8),
+                                   ("increment_bound", 16))},
+                ms_without_fallback=ms_nofb,
+                speedup_vs_plain=best_plain / ms_s, plain_ms=best_plain,
+                error_vs_plain=dict(dT_max_K=float(dT.max()), dT_p99_K=float(np.percentile(dT, 99)),
+                                    dT_rel_max=float(np.max(dT / Tp)),
+                                    peak_T_rel=float(abs(Ts.max() - Tp.max()) / Tp.max()),
+                                    dY_major_max=float(dYm.max()), dY_major_p99=float(np.percentile(dYm, 99)),
+                                    dY_all_species_max=float(dYa.max())))
+
+        res_b = surrogate_run()
+        # the same network without the increment bound, for comparison
+        cfg_b = S.cfg.copy()
+        cfg_u = cfg_b.copy()
+        cfg_u[S.DY_MAX_INDEX] = np.inf
+        S.set_config(cfg_u)
+        res_u = surrogate_run()
+        S.set_config(cfg_b)
+        keep = ("ms", "fallbacks", "speedup_vs_plain", "error_vs_plain")
         out["surrogate"] = dict(
             network="12 -> 256 x5 -> 9 GELU, BF16 MFMA (FP32 accumulation), fused kernel",
             tile=dict(cells_per_workgroup=tile_rows, waves=tile_waves,
@@ -176,14 +209,9 @@ def chem_bench(m, ncell=1 << 21, cpu=True, surrogate_cells=1 << 24, surrogate_ep
                                     "96x4": tile_ms[3]}),
             train=dict(cells=surrogate_cells, epochs=len(hist), batch=8192, data_s=t1 - t0, train_s=t2 - t1,
                        final_loss=hist[-1]),
-            ms=ms_s, cells_per_s=ncell / ms_s * 1e3, fallbacks=nfb,
-            fallback_reasons={k: int(np.sum((f & b) != 0)) for k, b in
-                              (("domain", 1), ("element_balance", 2), ("negative_Y", 4), ("T_bounds", 8))},
-            ms_without_fallback=ms_nofb,
-            speedup_vs_plain=best_plain / ms_s, plain_ms=best_plain,
-            error_vs_plain=dict(dT_max_K=float(dT.max()), dT_p99_K=float(np.percentile(dT, 99)),
-                                dT_rel_max=float(np.max(dT / Tp)), peak_T_rel=float(abs(Ts.max() - Tp.max()) / Tp.max()),
-                                dY_major_max=float(dYm.max()), dY_major_p99=float(np.percentile(dYm, 99))))
+            calibration=dict(cal, seconds=t3 - t2),
+            **res_b,
+            without_increment_bound={k: res_u[k] for k in keep})
     if cpu:
         import oracle
         orc = oracle.ChemOracle(m, M)

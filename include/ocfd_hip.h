@@ -393,12 +393,13 @@ int32_t ocfd_mlp_train_step(int64_t B, const float* x, const float* y, float lr,
 int32_t ocfd_mlp_gemm_bf16(int32_t M, int32_t N, int32_t K, int32_t ta, int32_t tb, const void* A, int64_t lda,
                            const void* B, int64_t ldb, float* C, int64_t ldc, int32_t splits);
 
-/* Surrogate configuration: 151 doubles, in this order:
+/* Surrogate configuration: 152 doubles, in this order:
  *   in_shift[12], in_scale[12]  inputs x = (f - in_shift) * in_scale of the raw
  *                               features f = [T, ln p, cbrt(Y_0..Y_8), ln dt];
  *   out_scale[9]                dY_k = z_k * out_scale_k (z = network output);
  *   T_lo, T_hi, p_lo, p_hi, dt_lo, dt_hi   domain; cells outside fall back;
- *   y_neg_tol, elem_tol, T_out_lo, T_out_hi  acceptance bounds;
+ *   y_neg_tol, elem_tol, T_out_lo, T_out_hi, dy_max  acceptance bounds
+ *                               (dy_max bounds every |dY_k|; +inf: no bound);
  *   P[9][9]                     dY <- P dY (projection onto element balance);
  *   Cel[3][9]                   element moles per kg of each species (H, O, N). */
 int32_t ocfd_chem_surrogate_set(const double* cfg, int32_t count);
@@ -408,7 +409,8 @@ int32_t ocfd_chem_surrogate_set(const double* cfg, int32_t count);
  * (work-queue scheduler, controller arguments as ocfd_chem_adaptive_batch).
  * flag_out [n] (device, may be NULL): 0 surrogate, else the failed checks
  * (1 outside the training domain, 2 element balance, 4 negative mass
- * fraction, 8 temperature bounds) and the cell went to the fallback.  The
+ * fraction, 8 temperature bounds, 16 an increment above dy_max) and the cell
+ * went to the fallback.  The
  * network must be 12 -> 256 x k -> 9. */
 int32_t ocfd_chem_surrogate_batch(int64_t n, double* rho, double* E, const double* dt, int32_t fallback,
                                   double rtol_y, double y_floor, double rtol_t, int32_t max_sub, int32_t* flag_out);

@@ -863,16 +863,18 @@ __global__ void k_chem_reset(unsigned long long* queue) {
 // the MLP (BF16 MFMA, FP32 accumulation) gives z, and in FP64
 //   dY = P (z * out_scale)          (P projects onto element conservation)
 // is accepted when the cell lies in the training domain, the H/O/N balance of
-// dY is within elem_tol of the element content, Y + dY >= -y_neg_tol and the
-// temperature after the update (Newton on the new state, update_Et energy) is
-// finite and within [T_out_lo, T_out_hi].  Accepted cells are written
+// dY is within elem_tol of the element content, Y + dY >= -y_neg_tol, every
+// |dY_k| <= dy_max (the increment size up to which the network's error was
+// measured to stay within tolerance) and the temperature after the update
+// (Newton on the new state, update_Et energy) is finite and within
+// [T_out_lo, T_out_hi].  Accepted cells are written
 // (rho_k += rho dY_k, E by update_Et); the others go to a list for the plain
 // FP64 integrator.
 struct SurrCfg {
   double in_shift[12], in_scale[12];
   double out_scale[CH_NS];
   double T_lo, T_hi, p_lo, p_hi, dt_lo, dt_hi;
-  double y_neg_tol, elem_tol, T_out_lo, T_out_hi;
+  double y_neg_tol, elem_tol, T_out_lo, T_out_hi, dy_max;
   double P[CH_NS][CH_NS];
   double Cel[3][CH_NS];
 };
@@ -1027,6 +1029,7 @@ __global__ __launch_bounds__(SurrTile<V>::NW * 64, SurrTile<V>::OCC) void k_surr
 #pragma unroll
       for (int i = 0; i < CH_NS; i++) {
         if (!(di[i] / r + dYp[i] >= -scfg.y_neg_tol)) why |= 4;
+        if (!(fabs(dYp[i]) <= scfg.dy_max)) why |= 16;  // also NaN
         dinew[i] = di[i] + r * dYp[i];
       }
       double E = Ecell[c];

@@ -112,13 +112,35 @@ class MLP:
 # mass-fraction increments of one chemistry step.
 # ---------------------------------------------------------------------------
 DIMS = (12, 256, 256, 256, 256, 256, 9)
-CFG_LEN = 151
+CFG_LEN = 152
 N_SPEC = 9
 
 
 def element_matrix(mech):
     """Cel[e][k]: moles of element e (H, O, N) per kg of species k."""
     return (mech.elements / mech.mi[:, None]).T.copy()
+
+
+def trust_bound(m, err, tol):
+    """Largest increment bound t such that every calibration cell with
+    m <= t has err <= tol (m: the cell's largest |dY_k| as the surrogate
+    predicts it, err: its error against the plain integrator).  The bound
+    sits midway between the last cell kept and the first cell whose error
+    exceeds tol, so rounding of m cannot let that cell through; with no such
+    cell it is just above the largest m seen (cells beyond it were never
+    measured).  0 when the smallest increment already errs by more than tol."""
+    m = np.asarray(m, dtype=np.float64)
+    err = np.asarray(err, dtype=np.float64)
+    if m.size == 0:
+        return 0.0
+    o = np.argsort(m, kind="stable")
+    ms, es = m[o], err[o]
+    bad = np.nonzero(~(es <= tol))[0]
+    if bad.size == 0:
+        return float(ms[-1] * (1.0 + 1e-6))
+    mv = ms[bad[0]]
+    kept = ms[ms < mv]
+    return 0.5 * (float(kept.max()) + float(mv)) if kept.size else 0.0
 
 
 def projection(cel, sigma):
@@ -189,8 +211,9 @@ class Surrogate:
         return y
 
     # -- configuration ---------------------------------------------------------
-    def configure(self, data, sample=1 << 18, y_neg_tol=1e-5, elem_tol=1e-9, T_out=(250.0, None)):
-        """Normalisation and acceptance domain from the training data."""
+    def configure(self, data, sample=1 << 18, y_neg_tol=1e-5, elem_tol=1e-9, T_out=(250.0, None), dy_max=np.inf):
+        """Normalisation and acceptance domain from the training data
+        (dy_max: no increment bound until calibrate sets one)."""
         n = data["E0"].shape[0]
         k = min(n, sample)
         f = self.features(data["rho0"], data["E0"], data["dt"], mode=1)[:k].cpu().numpy()
@@ -206,11 +229,42 @@ class Surrogate:
         cfg = np.concatenate([
             shift, scale, sigma,
             [T.min(), T.max(), np.exp(lp.min()), np.exp(lp.max()), np.exp(ldt.min()), np.exp(ldt.max())],
-            [y_neg_tol, elem_tol, T_out[0], T_out[1] if T_out[1] is not None else 1.1 * T.max()],
+            [y_neg_tol, elem_tol, T_out[0], T_out[1] if T_out[1] is not None else 1.1 * T.max(), dy_max],
             P.ravel(), cel.ravel()])
         assert cfg.size == CFG_LEN
         self.set_config(cfg)
         return cfg
+
+    DY_MAX_INDEX = 12 + 12 + N_SPEC + 6 + 4
+
+    def calibrate(self, data, tol=5e-4):
+        """Sets the increment bound dy_max from labelled cells the network
+        was not trained on: the fused kernel runs on them with no bound and
+        no fallback, and among the cells it accepts, dy_max is the largest
+        increment size below which every cell's mass fractions (all species)
+        are within tol of the plain integrator's (trust_bound).  Returns the
+        bound and how many of those cells it sends to the fallback."""
+        import torch
+        if self.cfg is None:
+            raise RuntimeError("calibrate: configure the surrogate first")
+        cfg = self.cfg.copy()
+        cfg[self.DY_MAX_INDEX] = np.inf
+        self.set_config(cfg)
+        rho, E = data["rho0"].clone(), data["E0"].clone()
+        flags = torch.zeros(E.shape[0], dtype=torch.int32, device=E.device)
+        self.step(rho, E, data["dt"], fallback=False, flags=flags)
+        r0 = data["rho0"].sum(0)
+        m = ((rho - data["rho0"]) / r0).abs().amax(0)
+        Ys = rho / rho.sum(0)
+        Yt = data["rho1"] / data["rho1"].sum(0)
+        err = (Ys - Yt).abs().amax(0)
+        acc = flags == 0
+        m, err = m[acc].cpu().numpy(), err[acc].cpu().numpy()
+        bound = trust_bound(m, err, tol)
+        cfg[self.DY_MAX_INDEX] = bound
+        self.set_config(cfg)
+        return dict(dy_max=bound, tol=tol, cells=int(E.shape[0]), accepted_unbounded=int(m.size),
+                    sent_to_fallback=int(np.sum(~(m <= bound))), err_max_unbounded=float(err.max()) if err.size else 0.0)
 
     def set_config(self, cfg):
         cfg = np.ascontiguousarray(cfg, dtype=np.float64)
@@ -259,7 +313,8 @@ class Surrogate:
     def step(self, rho, E, dt, fallback=True, controller=None, max_sub=4000, flags=None):
         """One chemistry step of every cell in place; fills the per-cell flags
         if given (0 surrogate, else the failed checks: 1 domain, 2 element
-        balance, 4 negative mass fraction, 8 temperature bounds)."""
+        balance, 4 negative mass fraction, 8 temperature bounds, 16 an
+        increment above the calibrated bound)."""
         from .chemistry import RTOL_Y, Y_FLOOR, RTOL_T
         c = controller or (RTOL_Y, Y_FLOOR, RTOL_T)
         n = self.chem._args(rho, E, dt)

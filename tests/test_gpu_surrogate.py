@@ -60,7 +60,7 @@ def test_fused_kernel_matches_generic_forward(sur):
     z = S.net.forward(x).double().cpu().numpy()
     cfg = S.cfg
     sigma = cfg[24:33]
-    P = cfg[43:124].reshape(9, 9)
+    P = cfg[S.DY_MAX_INDEX + 1:S.DY_MAX_INDEX + 82].reshape(9, 9)
     dY_gen = (z * sigma) @ P.T
     r0 = test["rho0"].cpu().numpy()
     r = r0.sum(axis=0)
@@ -177,3 +177,47 @@ def test_training_gains_over_untrained_network(sur):
           f"|dT| p99 untrained {p99_u:.1f} K, trained {p99_t:.2f} K")
     assert acc_t.mean() > 0.5
     assert acc_u.mean() < 0.5 * acc_t.mean() or p99_u > 10 * p99_t
+
+
+def test_calibrated_increment_bound(sur):
+    """calibrate() on labelled cells sets dy_max so that, on those same cells,
+    every accepted cell is within tol of the plain integrator in every mass
+    fraction (the fused kernel is deterministic, so this is exact); exactly
+    the cells it reports go to the fallback with flag 16 alone, and there
+    they match the plain integrator bit for bit.  On held-out cells the
+    bound keeps the error within a few tol."""
+    import torch
+    S, ch, M, test = sur
+    n = test["E0"].shape[0]
+    h = n // 2
+    part = lambda a, b: {k: v[..., a:b].contiguous() for k, v in test.items()}
+    calset, held = part(0, h), part(h, n)
+    tol = 2e-4
+    saved = S.cfg.copy()
+    try:
+        cal = S.calibrate(calset, tol=tol)
+        print("calibration:", cal)
+        assert 0 < cal["dy_max"] < np.inf and cal["sent_to_fallback"] < cal["accepted_unbounded"]
+
+        def err_of(data, fallback):
+            rho, E, flags = run(S, data, fallback=fallback)
+            Ys = (rho / rho.sum(0)).cpu().numpy()
+            Yt = (data["rho1"] / data["rho1"].sum(0)).cpu().numpy()
+            return np.abs(Ys - Yt).max(axis=0), flags.cpu().numpy(), rho, E
+
+        err, f, _, _ = err_of(calset, False)
+        acc = f == 0
+        assert err[acc].max() <= tol
+        assert int(np.sum(f == 16)) == cal["sent_to_fallback"]
+        # flag-16 cells through the fallback equal the plain integrator
+        _, f2, rho, E = err_of(calset, True)
+        only16 = torch.from_numpy(f2 == 16).cuda()
+        rp, Ep = calset["rho0"].clone(), calset["E0"].clone()
+        ch.adaptive(rp, Ep, calset["dt"])
+        assert torch.equal(rho[:, only16], rp[:, only16]) and torch.equal(E[only16], Ep[only16])
+        err_h, fh, _, _ = err_of(held, False)
+        acc_h = fh == 0
+        print(f"held-out: accepted {acc_h.mean():.3f}, max error {err_h[acc_h].max():.2e} (tol {tol:.0e})")
+        assert acc_h.mean() > 0.05 and err_h[acc_h].max() <= 10 * tol
+    finally:
+        S.set_config(saved)
