@@ -121,7 +121,7 @@ def chem_bench(m, ncell=1 << 21, cpu=True, surrogate_cells=1 << 24, surrogate_ep
                                  substeps=dict(mean=float(k.mean()), p50=float(np.median(k)),
                                                p99=float(np.percentile(k, 99)), max=int(k.max())))
     ms, _ = timed(lambda: ch.adaptive(rho, E, dt, scheduler=2))
-    res["plain_adaptive_queue_plus_cooperative"] = dict(ms=ms, cells_per_s=ncell / ms * 1e3,
+    res["plain_adaptive_queue_plus_cooperative"] = dict(ms=ms, cells_per_s=ncell / ms * 1e3, lane_budget=128,
                                                         cooperative_cells=ch.counters()["cooperative"])
     ms, _ = timed(lambda: ch.adaptive(rho, E, dt, scheduler=0))
     res["plain_adaptive_thread"] = dict(ms=ms, cells_per_s=ncell / ms * 1e3)
@@ -193,15 +193,16 @@ def chem_bench(m, ncell=1 << 21, cpu=True, surrogate_cells=1 << 24, surrogate_ep
                                     dY_major_max=float(dYm.max()), dY_major_p99=float(np.percentile(dYm, 99)),
                                     dY_all_species_max=float(dYa.max())))
 
-        res_b = surrogate_run()
-        # the same network without the increment bound, for comparison
+        # headline: the network with the domain, balance, sign and T checks;
+        # then the same network with the calibrated increment bound as well
         cfg_b = S.cfg.copy()
         cfg_u = cfg_b.copy()
         cfg_u[S.DY_MAX_INDEX] = np.inf
         S.set_config(cfg_u)
         res_u = surrogate_run()
         S.set_config(cfg_b)
-        keep = ("ms", "fallbacks", "speedup_vs_plain", "error_vs_plain")
+        res_b = surrogate_run()
+        keep = ("ms", "cells_per_s", "fallbacks", "fallback_reasons", "speedup_vs_plain", "error_vs_plain")
         out["surrogate"] = dict(
             network="12 -> 256 x5 -> 9 GELU, BF16 MFMA (FP32 accumulation), fused kernel",
             tile=dict(cells_per_workgroup=tile_rows, waves=tile_waves,
@@ -209,9 +210,8 @@ def chem_bench(m, ncell=1 << 21, cpu=True, surrogate_cells=1 << 24, surrogate_ep
                                     "96x4": tile_ms[3]}),
             train=dict(cells=surrogate_cells, epochs=len(hist), batch=8192, data_s=t1 - t0, train_s=t2 - t1,
                        final_loss=hist[-1]),
-            calibration=dict(cal, seconds=t3 - t2),
-            **res_b,
-            without_increment_bound={k: res_u[k] for k in keep})
+            **res_u,
+            with_increment_bound=dict(calibration=dict(cal, seconds=t3 - t2), **{k: res_b[k] for k in keep}))
     if cpu:
         import oracle
         orc = oracle.ChemOracle(m, M)
@@ -296,21 +296,32 @@ def jet_bench(m, S, nx=256, ny=192, nz=128, steps=10, warmup=2, tau=2e-5, cfl=0.
     out["plain"] = dict(ms_per_step=ms_plain, chem_ms_per_step=ms_plain - ms_flow,
                         integrator="adaptive point-implicit FP64, persistent work queue")
     if S is not None:
-        ms_s = run(m.CHEM_SURROGATE)
-        nfb = S.fallbacks()
-        Ts, dis = g.download("T"), g.download("DI")
         I = (slice(4, -4),) * 3
-        Tp_i, Ts_i = Tp[I], Ts[I]
+        Tp_i = Tp[I]
         Yp = dip[I] / dip[I].sum(axis=3, keepdims=True)
-        Ys = dis[I] / dis[I].sum(axis=3, keepdims=True)
         major = [mech.index(k) for k in ("H2", "O2", "H2O", "N2")]
-        dY = np.abs(Ys[..., major] - Yp[..., major]).max()
-        out["surrogate"] = dict(ms_per_step=ms_s, chem_ms_per_step=ms_s - ms_flow, fallbacks_last_step=nfb,
-                                fallback_fraction=nfb / ncell, speedup_step_vs_plain=ms_plain / ms_s,
-                                speedup_chem_vs_plain=(ms_plain - ms_flow) / max(ms_s - ms_flow, 1e-9),
-                                error_vs_plain=dict(peak_T_rel=float(abs(Ts_i.max() - Tp_i.max()) / Tp_i.max()),
-                                                    dT_max_K=float(np.abs(Ts_i - Tp_i).max()),
-                                                    dY_major_max=float(dY)))
+
+        def surrogate_run(cfg):
+            S.set_config(cfg)
+            ms_s = run(m.CHEM_SURROGATE)
+            nfb = S.fallbacks()
+            Ts, dis = g.download("T"), g.download("DI")
+            Ts_i = Ts[I]
+            Ys = dis[I] / dis[I].sum(axis=3, keepdims=True)
+            dY = np.abs(Ys[..., major] - Yp[..., major]).max()
+            return dict(ms_per_step=ms_s, chem_ms_per_step=ms_s - ms_flow, fallbacks_last_step=nfb,
+                        fallback_fraction=nfb / ncell, speedup_step_vs_plain=ms_plain / ms_s,
+                        speedup_chem_vs_plain=(ms_plain - ms_flow) / max(ms_s - ms_flow, 1e-9),
+                        error_vs_plain=dict(peak_T_rel=float(abs(Ts_i.max() - Tp_i.max()) / Tp_i.max()),
+                                            dT_max_K=float(np.abs(Ts_i - Tp_i).max()), dY_major_max=float(dY)))
+
+        cfg_b = S.cfg.copy()
+        cfg_u = cfg_b.copy()
+        cfg_u[S.DY_MAX_INDEX] = np.inf
+        out["surrogate"] = surrogate_run(cfg_u)
+        if np.isfinite(cfg_b[S.DY_MAX_INDEX]):
+            out["surrogate"]["with_increment_bound"] = surrogate_run(cfg_b)
+        S.set_config(cfg_b)
     out["T_max_initial"] = T0max
     g.close()
     return out

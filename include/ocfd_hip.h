@@ -344,7 +344,8 @@ int32_t ocfd_chem_adaptive_batch(int64_t n, double* rho, double* E, const double
 
 /* Chemistry options: OCFD_CHEM_OPT_COOP_BUDGET = attempts a cell may take on
  * one lane before scheduler 2 moves it to the cooperative kernel (default
- * 16); OCFD_CHEM_OPT_FALLBACK_SCHEDULER = scheduler (1 or 2) of the
+ * 128, the fastest of 16..1024 on a 2^21-cell jet-flame batch);
+ * OCFD_CHEM_OPT_FALLBACK_SCHEDULER = scheduler (1 or 2) of the
  * surrogate's fallback (default 2); OCFD_CHEM_OPT_COOP_DIRECT_MAX = a
  * fallback list of at most this many cells hands every cell that needs more
  * than one attempt to the cooperative kernel (default 16384). */

@@ -76,7 +76,7 @@ struct ChemCtx {
   HeavyBuf heavy{};                        // cells handed to the cooperative kernel
   int fallback_scheduler = 2;              // scheduler of the surrogate's fallback
   int coop_direct_max = 16384;             // fallback lists up to this size go to the cooperative kernel
-  int coop_budget = 16;                    // attempts on one lane before the cooperative kernel
+  int coop_budget = 128;                   // attempts on one lane before the cooperative kernel
   unsigned char* heavy_mem = nullptr;
   bool surr_ready = false;
   int surr_variant = 0;                    // tile shape of k_surrogate (SurrTile)

@@ -81,12 +81,17 @@ def test_adaptive_matches_oracle(env, scheduler):
 def test_schedulers_bitwise_identical(env):
     """One cell per thread, the work queue and the work queue with the
     cooperative kernel run the same per-cell arithmetic: identical results.
-    The long steps make some cells outlast the per-lane budget."""
+    The long steps and a per-lane budget of 16 attempts make some cells move
+    to the cooperative kernel."""
     pkg, chem, b, M, orc = env
     out = []
     for sched in (0, 1, 2):
         rho, E, d = to_dev(b["rho"], b["E"], b["dt"] * 50.0)
-        ns = chem.adaptive(rho, E, d, max_sub=3000, scheduler=sched)
+        chem.set_option(1, 16)
+        try:
+            ns = chem.adaptive(rho, E, d, max_sub=3000, scheduler=sched)
+        finally:
+            chem.set_option(1, 128)
         cnt = chem.counters()
         out.append((rho.cpu().numpy(), E.cpu().numpy(), ns.cpu().numpy()))
         if sched == 2:
