@@ -348,8 +348,15 @@ int32_t ocfd_chem_adaptive_batch(int64_t n, double* rho, double* E, const double
  * OCFD_CHEM_OPT_FALLBACK_SCHEDULER = scheduler (1 or 2) of the
  * surrogate's fallback (default 2); OCFD_CHEM_OPT_COOP_DIRECT_MAX = a
  * fallback list of at most this many cells hands every cell that needs more
- * than one attempt to the cooperative kernel (default 16384). */
-enum { OCFD_CHEM_OPT_COOP_BUDGET = 1, OCFD_CHEM_OPT_FALLBACK_SCHEDULER = 2, OCFD_CHEM_OPT_COOP_DIRECT_MAX = 3 };
+ * than one attempt to the cooperative kernel (default 16384);
+ * OCFD_CHEM_OPT_ORDER = 1 (default): schedulers 1 and 2 of
+ * ocfd_chem_adaptive_batch, and the surrogate's fallback, take the cells in
+ * order of decreasing dt (stable device radix sort on the top 24 bits of dt,
+ * i.e. to within 0.03%), so the longest integrations start first; 0: index
+ * order.  Either order gives
+ * bit-identical results. */
+enum { OCFD_CHEM_OPT_COOP_BUDGET = 1, OCFD_CHEM_OPT_FALLBACK_SCHEDULER = 2, OCFD_CHEM_OPT_COOP_DIRECT_MAX = 3,
+       OCFD_CHEM_OPT_ORDER = 4 };
 int32_t ocfd_chem_set_option(int32_t key, int32_t value);
 /* The chemistry work counters of the last batch call (synchronises):
  * [0] per-lane queue head, [1] unfinished cells, [2] surrogate fallbacks,

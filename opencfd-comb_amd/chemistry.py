@@ -17,7 +17,7 @@ from . import load_library, OcfdError
 from .mechanism import Mechanism
 
 MODES = {1: "Euler", 2: "RK2", 3: "RK3", -1: "point-implicit Euler"}
-OPT_COOP_BUDGET, OPT_FALLBACK_SCHEDULER, OPT_COOP_DIRECT_MAX = 1, 2, 3
+OPT_COOP_BUDGET, OPT_FALLBACK_SCHEDULER, OPT_COOP_DIRECT_MAX, OPT_ORDER = 1, 2, 3, 4
 
 # adaptive controller defaults: on jet-mixture cells along their ignition
 # trajectories, one step of dt in [1e-8, 1e-6] s lands within 0.25 K and 1e-3
@@ -92,7 +92,8 @@ class ChemBatch:
         return nsub_out
 
     def set_option(self, key, value):
-        """OPT_COOP_BUDGET (1) or OPT_FALLBACK_SCHEDULER (2)."""
+        """OPT_COOP_BUDGET (1), OPT_FALLBACK_SCHEDULER (2), OPT_COOP_DIRECT_MAX (3)
+        or OPT_ORDER (4: 1 longest dt first, 0 index order)."""
         self._check(self.lib.ocfd_chem_set_option(int(key), int(value)))
 
     def counters(self):

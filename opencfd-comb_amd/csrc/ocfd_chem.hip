@@ -17,6 +17,7 @@
 // operation order (-ffp-contract=off); exp/log/log10/pow are the device
 // library's, so results agree with the CPU restatement to a few ulps per call.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <cmath>
 #include <cstdio>
@@ -78,6 +79,15 @@ struct ChemCtx {
   int coop_direct_max = 16384;             // fallback lists up to this size go to the cooperative kernel
   int coop_budget = 128;                   // attempts on one lane before the cooperative kernel
   unsigned char* heavy_mem = nullptr;
+  // longest-first order of a batch: cell indices sorted by decreasing dt
+  int order = 1;                           // 0: index order, 1: longest dt first
+  int *ord_in = nullptr, *ord_out = nullptr;
+  unsigned int *ord_keys = nullptr, *ord_keys_in = nullptr;
+  unsigned long long* ord_count = nullptr;
+  void* ord_temp = nullptr;
+  size_t ord_temp_bytes = 0;
+  long ord_cap = 0;
+  unsigned char* ord_mem = nullptr;
   bool surr_ready = false;
   int surr_variant = 0;                    // tile shape of k_surrogate (SurrTile)
   float surr_tune_ms[8] = {};
@@ -595,6 +605,11 @@ __global__ __launch_bounds__(128) void k_chem_queue(long n, double* rho, double*
         const unsigned long long my = base + (unsigned long long)__popcll(mask & lt);
         if (my < count) {
           cell = list ? (long)list[my] : (long)my;
+          if (cell < 0 || cell >= n) cell = -1;  // a list entry outside the batch is skipped
+        } else {
+          cell = -2;
+        }
+        if (cell >= 0) {
 #pragma unroll
           for (int i = 0; i < CH_NS; i++) di[i] = rho[(long)i * n + cell];
           E = Ecell[cell];
@@ -603,12 +618,10 @@ __global__ __launch_bounds__(128) void k_chem_queue(long n, double* rho, double*
           h_cap = dt;
           nsub = 0;
           nrej = 0;
-        } else {
-          cell = -2;
         }
       }
     }
-    if (__ballot(cell >= 0) == 0) break;
+    if (__ballot(cell != -2) == 0) break;  // every lane has found the queue empty
     if (cell >= 0) {
       bool done = !(t < dt && nsub < ctl.max_sub && nrej < ctl.max_sub);
       double h = 0.0;
@@ -1111,12 +1124,100 @@ int ensure_heavy(long n) {
   return 0;
 }
 
+// Sort key of a cell: the top 32 bits of dt (sign, exponent and 20 mantissa
+// bits, which order like the values for a positive finite dt: dt to within
+// 1e-6), else 0 (such cells go last)
+__device__ __forceinline__ unsigned int dt_key(double d) {
+  return (d > 0.0 && d <= 1.7976931348623157e308) ? (unsigned int)((unsigned long long)__double_as_longlong(d) >> 32)
+                                                  : 0u;
+}
+
+// keys[c] = dt_key(dt[c]), idx[c] = c for c < n, and *count = n
+__global__ void k_order_keys(long n, const double* __restrict__ dt, unsigned int* __restrict__ keys, int* idx,
+                             unsigned long long* count) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) {
+    keys[c] = dt_key(dt[c]);
+    idx[c] = (int)c;
+  }
+  if (c == 0) *count = (unsigned long long)n;
+}
+
+int ensure_order(long n) {
+  if (n <= CX.ord_cap) return 0;
+  if (CX.ord_mem) hipFree(CX.ord_mem);
+  CX.ord_mem = nullptr;
+  CX.ord_cap = 0;
+  size_t temp = 0;
+  if (hipcub::DeviceRadixSort::SortPairsDescending(nullptr, temp, (const unsigned int*)nullptr, (unsigned int*)nullptr,
+                                                   (const int*)nullptr, (int*)nullptr, (int)n) != hipSuccess)
+    return cfail("chemistry: radix sort query failed");
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t b_keys = up(sizeof(unsigned int) * (size_t)n), b_idx = up(sizeof(int) * (size_t)n), b_cnt = 256;
+  if (hipMalloc(&CX.ord_mem, 2 * b_keys + 2 * b_idx + b_cnt + up(temp)) != hipSuccess)
+    return cfail("chemistry: out of device memory (cell order)");
+  unsigned char* p = CX.ord_mem;
+  CX.ord_keys = (unsigned int*)p;
+  CX.ord_keys_in = (unsigned int*)(p + b_keys);
+  p += 2 * b_keys;
+  CX.ord_in = (int*)p;
+  CX.ord_out = (int*)(p + b_idx);
+  CX.ord_count = (unsigned long long*)(p + 2 * b_idx);
+  CX.ord_temp = p + 2 * b_idx + b_cnt;
+  CX.ord_temp_bytes = temp;
+  CX.ord_cap = n;
+  return 0;
+}
+
+// Cell indices in order of decreasing dt (to within 1e-6), ties in index
+// order (a stable radix sort): the cells with the longest integrations start first, so the batch
+// does not end waiting on a stiff cell that a lane took up late.  The result
+// is CX.ord_out[0 .. n) with *CX.ord_count = n.
+int order_longest_first(long n, const double* dt) {
+  if (n > 0x7fffffffL) return cfail("chemistry: the cell order needs n < 2^31");
+  if (ensure_order(n)) return 1;
+  hipLaunchKernelGGL(k_order_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, CX.stream, n, dt, CX.ord_keys_in,
+                     CX.ord_in, CX.ord_count);
+  size_t temp = CX.ord_temp_bytes;
+  const hipError_t e = hipcub::DeviceRadixSort::SortPairsDescending(CX.ord_temp, temp, CX.ord_keys_in, CX.ord_keys,
+                                                                    CX.ord_in, CX.ord_out, (int)n, 0, 32, CX.stream);
+  if (e != hipSuccess) return cfail(std::string("chemistry: radix sort failed: ") + hipGetErrorString(e));
+  return 0;
+}
+
+// Sort keys of a cell list whose length is on the device: dt_key of each
+// listed cell, 0 past the end.  A stable descending sort keeps every listed
+// cell ahead of the slots past the end (their keys are the smallest, and
+// among equal keys the listed cells come first in index order).
+__global__ void k_list_keys(long cap, const int* __restrict__ list, const unsigned long long* __restrict__ count,
+                            const double* __restrict__ dt, unsigned int* __restrict__ keys) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cap) return;
+  keys[i] = (unsigned long long)i < *count ? dt_key(dt[list[i]]) : 0u;
+}
+
+// The cells of list[0 .. *count) (count <= cap <= n) in order of decreasing
+// dt, into CX.ord_out[0 .. *count).
+int order_list_longest_first(long n, long cap, const int* list, const unsigned long long* count, const double* dt) {
+  if (n > 0x7fffffffL || cap > n) return cfail("chemistry: the cell order needs cap <= n < 2^31");
+  if (ensure_order(n)) return 1;
+  hipLaunchKernelGGL(k_list_keys, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, CX.stream, cap, list, count, dt,
+                     CX.ord_keys_in);
+  size_t temp = CX.ord_temp_bytes;
+  const hipError_t e = hipcub::DeviceRadixSort::SortPairsDescending(CX.ord_temp, temp, CX.ord_keys_in, CX.ord_keys,
+                                                                    list, CX.ord_out, (int)cap, 0, 32, CX.stream);
+  if (e != hipSuccess) return cfail(std::string("chemistry: radix sort failed: ") + hipGetErrorString(e));
+  return 0;
+}
+
 // the adaptive integrator over cells 0..n-1, or over list[0 .. *list_count)
 // when list is given; counters [0,1,3,4] must be zero.  scheduler 0: one
 // cell per thread; 1: per-lane work queue; 2: work queue with the cooperative
-// kernel taking the cells that outlast CX.coop_budget attempts.
+// kernel taking the cells that outlast CX.coop_budget attempts.  direct: a
+// list of at most CX.coop_direct_max cells goes to the cooperative kernel
+// after one attempt per cell (the surrogate's rejects).
 int launch_adaptive(long n, double* rho, double* E, const double* dt, const ChemCtl& ctl, int* nsub_out, int scheduler,
-                    const int* list, const unsigned long long* list_count) {
+                    const int* list, const unsigned long long* list_count, bool direct = true) {
   if (scheduler == 0 && !list) {
     hipLaunchKernelGGL(k_chem_adaptive, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, CX.stream, n, rho, E, dt, ctl,
                        nsub_out, CX.counter);
@@ -1134,7 +1235,8 @@ int launch_adaptive(long n, double* rho, double* E, const double* dt, const Chem
   const bool coop = scheduler == 2;
   if (coop && ensure_heavy(n)) return 1;
   hipLaunchKernelGGL(k_chem_queue, dim3((unsigned)blocks), dim3(128), 0, CX.stream, n, rho, E, dt, ctl, nsub_out,
-                     CX.counter, list, list_count, coop ? CX.coop_budget : 0, CX.coop_direct_max, CX.heavy);
+                     CX.counter, list, list_count, coop ? CX.coop_budget : 0, direct ? CX.coop_direct_max : 0,
+                     CX.heavy);
   if (coop) {
     int per2 = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per2, k_chem_coop, 64, 0) != hipSuccess)
@@ -1148,6 +1250,46 @@ int launch_adaptive(long n, double* rho, double* E, const double* dt, const Chem
 }
 
 // ===========================================================================
+// The batch entry behind ocfd_chem_adaptive_batch; the flow solver calls it
+// with order = false (its cells share one dt, so the order is the index order).
+int chem_adaptive_batch(long n, double* rho, double* E, const double* dt, double rtol_y, double y_floor, double rtol_t,
+                        int max_sub, int* nsub_out, int scheduler, bool order) {
+  if (!CX.ready) return cfail("ocfd_chem_adaptive_batch: call ocfd_chem_set_mechanism first");
+  if (max_sub < 1 || !(rtol_y > 0.0) || !(y_floor >= 0.0) || !(rtol_t > 0.0))
+    return cfail("ocfd_chem_adaptive_batch: bad controller parameters");
+  if (scheduler < 0 || scheduler > 2) return cfail("ocfd_chem_adaptive_batch: scheduler 0, 1 or 2");
+  const ChemCtl ctl{rtol_y, y_floor, rtol_t, max_sub};
+  hipLaunchKernelGGL(k_chem_reset, dim3(1), dim3(64), 0, CX.stream, CX.counter);
+  if (n <= 0) return hipGetLastError() == hipSuccess ? 0 : cfail("ocfd_chem_adaptive_batch: launch failed");
+  if (order && scheduler != 0 && n > 1) {
+    if (order_longest_first(n, dt)) return 1;
+    return launch_adaptive(n, rho, E, dt, ctl, nsub_out, scheduler, CX.ord_out, CX.ord_count, false);
+  }
+  return launch_adaptive(n, rho, E, dt, ctl, nsub_out, scheduler, nullptr, nullptr);
+}
+
+// The batch entry behind ocfd_chem_surrogate_batch; with order, the fallback
+// list is taken by decreasing dt (the flow solver passes false: one dt).
+int chem_surrogate_batch(long n, double* rho, double* E, const double* dt, bool fallback, double rtol_y, double y_floor,
+                         double rtol_t, int max_sub, int* flag_out, bool order) {
+  if (fallback && (max_sub < 1 || !(rtol_y > 0.0) || !(y_floor >= 0.0) || !(rtol_t > 0.0)))
+    return cfail("ocfd_chem_surrogate_batch: bad controller parameters");
+  if (surr_prepare(n, "ocfd_chem_surrogate_batch")) return 1;
+  if (n <= 0) return hipGetLastError() == hipSuccess ? 0 : cfail("ocfd_chem_surrogate_batch: launch failed");
+  launch_surrogate(CX.surr_variant, n, rho, E, dt, flag_out);
+  if (fallback) {
+    const ChemCtl ctl{rtol_y, y_floor, rtol_t, max_sub};
+    const int* list = CX.fb_list;
+    if (order && CX.fallback_scheduler != 0 && n > 1) {
+      if (order_list_longest_first(n, n, CX.fb_list, CX.counter + 2, dt)) return 1;
+      list = CX.ord_out;
+    }
+    if (launch_adaptive(n, rho, E, dt, ctl, nullptr, CX.fallback_scheduler, list, CX.counter + 2)) return 1;
+  }
+  if (hipGetLastError() != hipSuccess) return cfail("ocfd_chem_surrogate_batch: launch failed");
+  return 0;
+}
+
 extern "C" {
 
 const char* ocfd_chem_last_error(void) { return CX.err.c_str(); }
@@ -1231,14 +1373,8 @@ int32_t ocfd_chem_timeadv_batch(int64_t n, double* rho, double* E, const double*
 
 int32_t ocfd_chem_adaptive_batch(int64_t n, double* rho, double* E, const double* dt, double rtol_y, double y_floor,
                                  double rtol_t, int32_t max_sub, int32_t* nsub_out, int32_t scheduler) {
-  if (!CX.ready) return cfail("ocfd_chem_adaptive_batch: call ocfd_chem_set_mechanism first");
-  if (max_sub < 1 || !(rtol_y > 0.0) || !(y_floor >= 0.0) || !(rtol_t > 0.0))
-    return cfail("ocfd_chem_adaptive_batch: bad controller parameters");
-  if (scheduler < 0 || scheduler > 2) return cfail("ocfd_chem_adaptive_batch: scheduler 0, 1 or 2");
-  const ChemCtl ctl{rtol_y, y_floor, rtol_t, (int)max_sub};
-  hipLaunchKernelGGL(k_chem_reset, dim3(1), dim3(64), 0, CX.stream, CX.counter);
-  if (n <= 0) return hipGetLastError() == hipSuccess ? 0 : cfail("ocfd_chem_adaptive_batch: launch failed");
-  return launch_adaptive((long)n, rho, E, dt, ctl, (int*)nsub_out, (int)scheduler, nullptr, nullptr);
+  return chem_adaptive_batch((long)n, rho, E, dt, rtol_y, y_floor, rtol_t, (int)max_sub, (int*)nsub_out,
+                             (int)scheduler, CX.order != 0);
 }
 
 int32_t ocfd_chem_surrogate_set(const double* cfg, int32_t count) {
@@ -1256,17 +1392,8 @@ int32_t ocfd_chem_surrogate_set(const double* cfg, int32_t count) {
 
 int32_t ocfd_chem_surrogate_batch(int64_t n, double* rho, double* E, const double* dt, int32_t fallback,
                                   double rtol_y, double y_floor, double rtol_t, int32_t max_sub, int32_t* flag_out) {
-  if (fallback && (max_sub < 1 || !(rtol_y > 0.0) || !(y_floor >= 0.0) || !(rtol_t > 0.0)))
-    return cfail("ocfd_chem_surrogate_batch: bad controller parameters");
-  if (surr_prepare((long)n, "ocfd_chem_surrogate_batch")) return 1;
-  if (n <= 0) return hipGetLastError() == hipSuccess ? 0 : cfail("ocfd_chem_surrogate_batch: launch failed");
-  launch_surrogate(CX.surr_variant, (long)n, rho, E, dt, (int*)flag_out);
-  if (fallback) {
-    const ChemCtl ctl{rtol_y, y_floor, rtol_t, (int)max_sub};
-    if (launch_adaptive((long)n, rho, E, dt, ctl, nullptr, CX.fallback_scheduler, CX.fb_list, CX.counter + 2)) return 1;
-  }
-  if (hipGetLastError() != hipSuccess) return cfail("ocfd_chem_surrogate_batch: launch failed");
-  return 0;
+  return chem_surrogate_batch((long)n, rho, E, dt, fallback != 0, rtol_y, y_floor, rtol_t, (int)max_sub,
+                              (int*)flag_out, CX.order != 0);
 }
 
 // Autotuner of the fused surrogate kernel: every tile shape runs `reps`
@@ -1383,6 +1510,10 @@ int32_t ocfd_chem_set_option(int32_t key, int32_t value) {
     case 3:
       if (value < 0) return cfail("ocfd_chem_set_option: the direct cooperative list size must be >= 0");
       CX.coop_direct_max = value;
+      return 0;
+    case 4:
+      if (value < 0 || value > 1) return cfail("ocfd_chem_set_option: cell order 0 or 1");
+      CX.order = value;
       return 0;
   }
   return cfail("ocfd_chem_set_option: unknown option");

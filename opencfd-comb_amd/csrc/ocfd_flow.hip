@@ -20,6 +20,13 @@
 #include "../../include/ocfd_hip.h"
 #include "ocfd_device.h"
 
+// ocfd_chem_adaptive_batch / ocfd_chem_surrogate_batch with the longest-first
+// cell order switched by the caller (ocfd_chem.hip)
+int chem_adaptive_batch(long n, double* rho, double* E, const double* dt, double rtol_y, double y_floor, double rtol_t,
+                        int max_sub, int* nsub_out, int scheduler, bool order);
+int chem_surrogate_batch(long n, double* rho, double* E, const double* dt, bool fallback, double rtol_y, double y_floor,
+                         double rtol_t, int max_sub, int* flag_out, bool order);
+
 // ===========================================================================
 // Inviscid sweep (du_inviscous, OCFD_inviscous.f90:47-202).
 //
@@ -2570,11 +2577,11 @@ int32_t ocfd_chemical_time_advance(double dt2, int32_t mode) {
   if (ocfd_chem_set_stream((void*)G.stream)) return fail(std::string("ocfd_chemical_time_advance: ") + ocfd_chem_last_error());
   int rc;
   if (G.chem_method == OCFD_CHEM_ADAPTIVE)
-    rc = ocfd_chem_adaptive_batch(S.ni, G.chem_rho, G.chem_E, G.chem_dt, G.chem_rtol_y, G.chem_y_floor, G.chem_rtol_t,
-                                  G.chem_max_sub, nullptr, G.chem_scheduler);
+    rc = chem_adaptive_batch(S.ni, G.chem_rho, G.chem_E, G.chem_dt, G.chem_rtol_y, G.chem_y_floor, G.chem_rtol_t,
+                             G.chem_max_sub, nullptr, G.chem_scheduler, false);
   else if (G.chem_method == OCFD_CHEM_SURROGATE)
-    rc = ocfd_chem_surrogate_batch(S.ni, G.chem_rho, G.chem_E, G.chem_dt, 1, G.chem_rtol_y, G.chem_y_floor, G.chem_rtol_t,
-                                   G.chem_max_sub, nullptr);
+    rc = chem_surrogate_batch(S.ni, G.chem_rho, G.chem_E, G.chem_dt, true, G.chem_rtol_y, G.chem_y_floor, G.chem_rtol_t,
+                              G.chem_max_sub, nullptr, false);
   else
     rc = ocfd_chem_timeadv_batch(S.ni, G.chem_rho, G.chem_E, G.chem_dt, mode, 1);
   ocfd_chem_set_stream(prev);

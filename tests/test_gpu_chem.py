@@ -102,6 +102,33 @@ def test_schedulers_bitwise_identical(env):
             assert np.array_equal(a, c)
 
 
+def test_longest_first_order_is_bitwise_identical(env):
+    """Schedulers 1 and 2 take the cells by decreasing dt (OPT_ORDER = 1,
+    the default) or in index order (0): the same cells, the same per-cell
+    arithmetic, so the same bits and substep counts, on a batch with dt over
+    two decades (ties included) and on one with a single dt."""
+    pkg, chem, b, M, orc = env
+    from ocfd_amd import chemistry
+    rng = np.random.default_rng(4)
+    dt_var = b["dt"] * 10.0 ** rng.uniform(0.0, 2.0, b["dt"].shape[0])
+    dt_var[::7] = dt_var[0]
+    for dts in (dt_var, np.full_like(b["dt"], b["dt"].max())):
+        for sched in (1, 2):
+            out = []
+            for order in (0, 1):
+                chem.set_option(chemistry.OPT_ORDER, order)
+                try:
+                    rho, E, d = to_dev(b["rho"], b["E"], dts)
+                    ns = chem.adaptive(rho, E, d, max_sub=3000, scheduler=sched)
+                finally:
+                    chem.set_option(chemistry.OPT_ORDER, 1)
+                out.append((rho.cpu().numpy(), E.cpu().numpy(), ns.cpu().numpy(), chem.counters()["unfinished"]))
+            for a, c in zip(out[0], out[1]):
+                assert np.array_equal(a, c)
+    with pytest.raises(pkg.OcfdError):
+        chem.set_option(chemistry.OPT_ORDER, 2)
+
+
 def test_unfinished_cells_are_counted(env):
     """A budget too small to cover dt leaves cells unfinished; both
     schedulers count them as the oracle does."""
@@ -175,6 +202,15 @@ def test_conservation_at_scale(env):
     d = torch.tensor(s["dt"], dtype=torch.float64, device="cuda")
     nsub = chem.adaptive(rho, E, d, max_sub=1000)
     assert chem.unfinished() == 0
+    # the longest-first order (default) against index order, at full size
+    from ocfd_amd import chemistry
+    rho_i, E_i = r0.clone(), e0.clone()
+    chem.set_option(chemistry.OPT_ORDER, 0)
+    try:
+        nsub_i = chem.adaptive(rho_i, E_i, d, max_sub=1000)
+    finally:
+        chem.set_option(chemistry.OPT_ORDER, 1)
+    assert torch.equal(rho, rho_i) and torch.equal(E, E_i) and torch.equal(nsub, nsub_i)
     torch.cuda.synchronize()
     mi = torch.tensor(M.mi, dtype=torch.float64, device="cuda")[:, None]
     el = torch.tensor(M.elements, dtype=torch.float64, device="cuda")

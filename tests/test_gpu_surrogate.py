@@ -221,3 +221,21 @@ def test_calibrated_increment_bound(sur):
         assert acc_h.mean() > 0.05 and err_h[acc_h].max() <= 10 * tol
     finally:
         S.set_config(saved)
+
+
+def test_fallback_order_is_bitwise_identical(sur):
+    """The fallback list taken by decreasing dt (the default) or in the order
+    the surrogate kernel appended it gives the same bits."""
+    S, ch, M, test = sur
+    from ocfd_amd import chemistry
+    out = []
+    for order in (0, 1):
+        ch.set_option(chemistry.OPT_ORDER, order)
+        try:
+            rho, E, flags = run(S, test, fallback=True)
+        finally:
+            ch.set_option(chemistry.OPT_ORDER, 1)
+        out.append((rho.cpu().numpy(), E.cpu().numpy(), flags.cpu().numpy(), S.fallbacks()))
+    assert out[0][3] > 0
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
