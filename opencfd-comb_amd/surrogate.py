@@ -247,12 +247,17 @@ class Surrogate:
         import torch
         if self.cfg is None:
             raise RuntimeError("calibrate: configure the surrogate first")
-        cfg = self.cfg.copy()
+        saved = self.cfg.copy()
+        cfg = saved.copy()
         cfg[self.DY_MAX_INDEX] = np.inf
         self.set_config(cfg)
         rho, E = data["rho0"].clone(), data["E0"].clone()
         flags = torch.zeros(E.shape[0], dtype=torch.int32, device=E.device)
-        self.step(rho, E, data["dt"], fallback=False, flags=flags)
+        try:
+            self.step(rho, E, data["dt"], fallback=False, flags=flags)
+        except Exception:
+            self.set_config(saved)
+            raise
         r0 = data["rho0"].sum(0)
         m = ((rho - data["rho0"]) / r0).abs().amax(0)
         Ys = rho / rho.sum(0)
