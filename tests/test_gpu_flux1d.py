@@ -81,6 +81,21 @@ def test_flux1d_batch_bitwise(ctx, name, side):
         assert same, (name, side, n, bidx, sb, float(np.max(np.abs(got - ref))))
 
 
+@pytest.mark.parametrize("name", list(SCHEMES))
+@pytest.mark.parametrize("side", [0, 1])
+def test_flux1d_batch_short_pencils(ctx, name, side):
+    # n = 1, 2 and 3 with no boundary closure (Bound_index 0): every interface
+    # reads the ghost points, as on a periodic or exchanged pencil
+    import torch
+    pkg, g, o = ctx
+    for n, seed in ((1, 21), (2, 22), (3, 23)):
+        v, scm = data(6, n, seed)
+        ref = oracle_hh(o, SCHEMES[name], side, v, scm, (0, 0), (0, 0))
+        got = g.OCFD2d_flux(side, SCHEMES[name], torch.from_numpy(v).cuda(), (0, 0), (0, 0),
+                            scm=torch.from_numpy(scm).cuda()).cpu().numpy()
+        assert np.array_equal(got.view(np.uint64), ref.view(np.uint64)), (name, side, n)
+
+
 def test_flux1d_batch_large_batch_matches_pencils(ctx):
     # many pencils in one launch (more workgroups than CUs): every pencil equals
     # the same pencil computed alone, and the hybrid mask is honoured per pencil
