@@ -132,7 +132,7 @@ def chem_bench(m, ncell=1 << 21, cpu=True, surrogate_cells=1 << 24, surrogate_ep
     res["fixed_euler_M5"] = dict(ms=ms, cells_per_s=ncell / ms * 1e3)
     ms, _ = timed(lambda: ch.chemical_timeAdv(rho, E, dt, mode=-1, nsub=5))
     res["fixed_point_implicit_M5"] = dict(ms=ms, cells_per_s=ncell / ms * 1e3)
-    # plain reference result of the batch (judge of the surrogate's error)
+    # plain reference result of the batch (the measure of the surrogate's error)
     rho.copy_(r0)
     E.copy_(e0)
     ch.adaptive(rho, E, dt, scheduler=1)

@@ -154,7 +154,7 @@ def projection(cel, sigma):
 class Surrogate:
     """Learned chemistry step with element-balance / bounds checks and the
     plain FP64 integrator as fallback (and as the source of its training
-    labels and the judge of its error)."""
+    labels and the measure of its error)."""
 
     def __init__(self, chem, dims=DIMS, seed=0):
         self.chem = chem
