@@ -352,8 +352,9 @@ int32_t ocfd_chem_adaptive_batch(int64_t n, double* rho, double* E, const double
  * OCFD_CHEM_OPT_ORDER = 1 (default): schedulers 1 and 2 of
  * ocfd_chem_adaptive_batch, and the surrogate's fallback, take the cells in
  * order of decreasing dt T^2 (an estimate of each cell's work; stable device
- * radix sort), so the longest integrations start first; 0: index order.
- * Either order gives bit-identical results. */
+ * radix sort), so the longest integrations start first (batches of 2^31
+ * cells or more run in index order); 0: index order.  Either order gives
+ * bit-identical results. */
 enum { OCFD_CHEM_OPT_COOP_BUDGET = 1, OCFD_CHEM_OPT_FALLBACK_SCHEDULER = 2, OCFD_CHEM_OPT_COOP_DIRECT_MAX = 3,
        OCFD_CHEM_OPT_ORDER = 4 };
 int32_t ocfd_chem_set_option(int32_t key, int32_t value);

@@ -1279,7 +1279,8 @@ int chem_adaptive_batch(long n, double* rho, double* E, const double* dt, double
   const ChemCtl ctl{rtol_y, y_floor, rtol_t, max_sub};
   hipLaunchKernelGGL(k_chem_reset, dim3(1), dim3(64), 0, CX.stream, CX.counter);
   if (n <= 0) return hipGetLastError() == hipSuccess ? 0 : cfail("ocfd_chem_adaptive_batch: launch failed");
-  if (order && scheduler != 0 && n > 1) {
+  // the order lists cells as int: a batch of 2^31 cells or more runs in index order
+  if (order && scheduler != 0 && n > 1 && n <= 0x7fffffffL) {
     if (order_longest_first(n, rho, E, dt)) return 1;
     return launch_adaptive(n, rho, E, dt, ctl, nsub_out, scheduler, CX.ord_out, CX.ord_count, false);
   }
