@@ -121,6 +121,30 @@ def test_rk3_step_flat_plate_hybrid(pkg):
     g.close()
 
 
+def test_concurrent_streams_bitwise(pkg):
+    """The viscous preparation on a second stream beside the sweeps
+    (OPT_CONCURRENT = 1, the default) gives the one-stream run bit for bit,
+    eagerly and under graph replay, over three RK3 steps of the hybrid flat
+    plate (the first step writes the sweeps' kind lists, the others read
+    them)."""
+    case = fc.small_flat_plate(48, 32, 16)
+    dt1 = 0.5 * case[5]["dt"]
+    out = []
+    for conc, graph in ((0, 0), (1, 0), (1, 1), (0, 1)):
+        g = pkg.FlowSolver(case[0])
+        g.set_option(pkg.OPT_CONCURRENT, conc)
+        g.set_option(pkg.OPT_USE_GRAPH, graph)
+        fc.setup(g, case)
+        for _ in range(3):
+            g.flow_time_advance(dt1)
+        g.synchronize()
+        out.append({n: g.download(n) for n in ("F",) + fc.PRIMS})
+        g.close()
+    for k, o in enumerate(out[1:], 1):
+        for n in o:
+            assert np.array_equal(o[n].view(np.uint64), out[0][n].view(np.uint64)), (k, n)
+
+
 @pytest.mark.parametrize("vis", ["CD6", "CD8"])
 def test_column_planes_bitwise(pkg, vis):
     """The k-marching viscous-flux and update kernels give the one-thread-per-

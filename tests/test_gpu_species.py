@@ -153,6 +153,28 @@ def test_perfect_gas_sutherland_mixture(pkg):
     g.close()
 
 
+def test_concurrent_streams_bitwise(pkg):
+    """The species chain on a second stream beside the flow sweeps
+    (OPT_CONCURRENT = 1, the default) gives the one-stream run bit for bit,
+    eagerly and under graph replay, over two RK3 steps of the jet."""
+    case = fc.small_jet()
+    dt = case[5]["dt"]
+    out = []
+    for conc, graph in ((0, 0), (1, 0), (1, 1)):
+        g = pkg.FlowSolver(case[0])
+        g.set_option(pkg.OPT_CONCURRENT, conc)
+        g.set_option(pkg.OPT_USE_GRAPH, graph)
+        fc.setup_jet(g, case)
+        for _ in range(2):
+            g.flow_time_advance(dt)
+        g.synchronize()
+        out.append({n: g.download(n) for n in ("F", "DI", "T", "P")})
+        g.close()
+    for k, o in enumerate(out[1:], 1):
+        for n in o:
+            assert np.array_equal(o[n].view(np.uint64), out[0][n].view(np.uint64)), (k, n)
+
+
 def test_three_steps_graph(pkg):
     case = fc.small_jet()
     g, o = pair(pkg, case)
