@@ -105,8 +105,12 @@ enum {
    * (MPI_ALLREDUCE at code/schemes/OCFD_SetHybrid.f90:64):
    *   0 = ordered: one sequential sum in (k,j,i) order, bit-identical to a
    *       single-rank reference run (slow: one wavefront);
-   *   1 = fast: parallel tree sum (mask may differ where R is within
-   *       rounding of a threshold). */
+   *   1 (default) = fast and certified: a parallel tree sum; every
+   *       interface whose normalised sensor lies within the rounding bound
+   *       of the two sums (relative eta, see ocfd_hybrid_certificate) of a
+   *       threshold is counted, and when the count is not zero the
+   *       sequential sum and the masks are recomputed on the device.  The
+   *       masks and Rhybrid are therefore the ordered mode's, bit for bit. */
   OCFD_OPT_HYBRID_SUM_MODE = 1,
   /* 1 = capture the RK3 step in a hipGraph on first use and replay it. */
   OCFD_OPT_USE_GRAPH = 2,
@@ -120,7 +124,11 @@ enum {
   /* 1 (default) = the viscous-flux preparation of each stage runs on a
    * second stream beside the inviscid sweeps; 0 = one stream.  Results are
    * bitwise independent of the value. */
-  OCFD_OPT_CONCURRENT = 5
+  OCFD_OPT_CONCURRENT = 5,
+  /* Widens the certificate band of OCFD_OPT_HYBRID_SUM_MODE = 1 to
+   * eta * 2^value (0 <= value <= 60; default 0).  Tests use it to force the
+   * ordered recomputation; the masks stay the ordered ones for any value. */
+  OCFD_OPT_HYBRID_CERT_WIDEN = 6
 };
 
 /* ---- lifetime ---------------------------------------------------------- */
@@ -269,6 +277,16 @@ int32_t ocfd_flux1d_batch(int32_t side, int32_t scheme, int64_t npen, int32_t n,
  * ocfd_synchronize, a download or the next step. */
 int32_t ocfd_negative_et_count(void);
 
+/* Certificate of the last fast sensor mean (OCFD_OPT_HYBRID_SUM_MODE = 1,
+ * sensor type 2; no reference counterpart).  Synchronises.  uncertain: the
+ * interfaces whose normalised max(R_i, R_i+1) fell within the relative band
+ * eta of Hybrid_para(2) or (3), where the fast and the sequential mean could
+ * disagree (0: the fast masks are certified equal to the ordered ones; > 0:
+ * the library recomputed the ordered mean and masks).  margin: the smallest
+ * |R / theta - 1| over all interfaces and both thresholds.  eta: the band in
+ * use.  Any pointer may be NULL.  Fails when no certified mean has run. */
+int32_t ocfd_hybrid_certificate(int64_t* uncertain, double* margin, double* eta);
+
 /* ---- tile autotuner (no reference counterpart) ---------------------------
  * The inviscid sweeps come in several workgroup tile shapes (pencils per
  * workgroup x cells per pencil).  ocfd_autotune times each shape per
@@ -353,8 +371,8 @@ int32_t ocfd_chem_adaptive_batch(int64_t n, double* rho, double* E, const double
  * ocfd_chem_adaptive_batch, and the surrogate's fallback, take the cells in
  * order of decreasing dt T^2 (an estimate of each cell's work; stable device
  * radix sort), so the longest integrations start first (batches of 2^31
- * cells or more run in index order); 0: index order.  Either order gives
- * bit-identical results. */
+ * cells or more run in index order; scheduler 2 refuses them); 0: index
+ * order.  Either order gives bit-identical results. */
 enum { OCFD_CHEM_OPT_COOP_BUDGET = 1, OCFD_CHEM_OPT_FALLBACK_SCHEDULER = 2, OCFD_CHEM_OPT_COOP_DIRECT_MAX = 3,
        OCFD_CHEM_OPT_ORDER = 4 };
 int32_t ocfd_chem_set_option(int32_t key, int32_t value);

@@ -40,6 +40,7 @@ FIELD = {name: i for i, name in enumerate(FIELDS)}
 METRICS = FIELDS[FIELD["AKX"]:FIELD["ASZ1"] + 1]
 
 OPT_HYBRID_SUM_MODE, OPT_USE_GRAPH, OPT_KERNEL_TIMING, OPT_COLUMN_PLANES, OPT_CONCURRENT = 1, 2, 3, 4, 5
+OPT_HYBRID_CERT_WIDEN = 6
 CHEM_FIXED, CHEM_ADAPTIVE, CHEM_SURROGATE = 0, 1, 2
 
 
@@ -123,6 +124,7 @@ def load_library(path=None):
         "ocfd_comput_rhybrid": (i32, []), "ocfd_du_inviscous": (i32, []), "ocfd_du_viscous": (i32, []),
         "ocfd_time_adv_rk3": (i32, [i32, dbl]), "ocfd_comput_flow_variables": (i32, []),
         "ocfd_bc": (i32, []), "ocfd_negative_et_count": (i32, []),
+        "ocfd_hybrid_certificate": (i32, [P(ctypes.c_int64), P(dbl), P(dbl)]),
         "ocfd_flux1d_batch": (i32, [i32, i32, ctypes.c_int64, i32, vp, vp, vp, P(i32), P(i32)]), "ocfd_synchronize": (i32, []),
         "ocfd_stream": (vp, []), "ocfd_kernel_count": (i32, []), "ocfd_kernel_name": (ctypes.c_char_p, [i32]),
         "ocfd_kernel_stats": (i32, [i32, P(dbl), P(ctypes.c_int64)]), "ocfd_kernel_stats_reset": (i32, []),
@@ -356,6 +358,15 @@ class FlowSolver:
         self._check(self.lib.ocfd_flux1d_batch(int(side), int(scheme), npen, n, v.data_ptr(), hh.data_ptr(), sp, b, sb))
         self._check(self.lib.ocfd_synchronize())
         return hh
+
+    def hybrid_certificate(self):
+        """(uncertain interfaces, smallest |R/theta - 1|, band eta) of the
+        last certified fast sensor mean; uncertain == 0 certifies that the
+        fast masks are the ordered ones (otherwise the library recomputed
+        them from the ordered mean)."""
+        n, margin, eta = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.ocfd_hybrid_certificate(ctypes.byref(n), ctypes.byref(margin), ctypes.byref(eta)))
+        return int(n.value), float(margin.value), float(eta.value)
 
     def negative_et_count(self):
         return int(self.lib.ocfd_negative_et_count())

@@ -1,6 +1,7 @@
 """Build recipe for the HIP library (gfx950).
 
-Compiles csrc/*.hip into libocfd_hip.so next to this file.  FP64 kernels are
+Compiles csrc/*.hip (one object each, in parallel, under _build/) and links
+libocfd_hip.so next to this file.  FP64 kernels are
 built with -ffp-contract=off so every operation rounds as the reference
 Fortran does, and with packed-FP32 instructions disabled.
 """
@@ -31,11 +32,26 @@ HIPCC_FLAGS = [
 
 
 def build(verbose: bool = True) -> str:
+    """Compile each source to an object in parallel, then link the library."""
+    from concurrent.futures import ThreadPoolExecutor
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    cmd = [hipcc] + HIPCC_FLAGS + SOURCES + ["-o", LIB + ".tmp"]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    jobs = []
+    for src in SOURCES:
+        obj = os.path.join(objdir, os.path.basename(src) + ".o")
+        jobs.append(([hipcc] + compile_flags + ["-c", src, "-o", obj], obj))
+
+    def run(cmd):
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+
+    with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
+        for f in [ex.submit(run, c) for c, _ in jobs]:
+            f.result()
+    run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC"] + [o for _, o in jobs] + ["-o", LIB + ".tmp"])
     os.replace(LIB + ".tmp", LIB)
     return LIB
 

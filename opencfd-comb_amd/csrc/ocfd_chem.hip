@@ -1276,6 +1276,8 @@ int chem_adaptive_batch(long n, double* rho, double* E, const double* dt, double
   if (max_sub < 1 || !(rtol_y > 0.0) || !(y_floor >= 0.0) || !(rtol_t > 0.0))
     return cfail("ocfd_chem_adaptive_batch: bad controller parameters");
   if (scheduler < 0 || scheduler > 2) return cfail("ocfd_chem_adaptive_batch: scheduler 0, 1 or 2");
+  // the cooperative hand-off records a cell index as int
+  if (scheduler == 2 && n > 0x7fffffffL) return cfail("ocfd_chem_adaptive_batch: scheduler 2 takes at most 2^31-1 cells");
   const ChemCtl ctl{rtol_y, y_floor, rtol_t, max_sub};
   hipLaunchKernelGGL(k_chem_reset, dim3(1), dim3(64), 0, CX.stream, CX.counter);
   if (n <= 0) return hipGetLastError() == hipSuccess ? 0 : cfail("ocfd_chem_adaptive_batch: launch failed");

@@ -46,6 +46,8 @@ struct DevState {
   double cd8c[3][4];   // c1..c4
   double ud7l[8];
   double hyb_t1, hyb_t2;
+  double hyb_eta;      // relative bound on |1/mean_fast - 1/mean_ordered| of the type-2 sensor
+  double hyb_inv_t1, hyb_inv_t2;
   double Ri0, Mi0;     // N_SPEC == 1 gas constant and molar mass
   double amu0, pr, sc;
   double mf[3];

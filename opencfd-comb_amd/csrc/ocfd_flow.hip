@@ -1377,7 +1377,7 @@ __global__ __launch_bounds__(256) void k_sensor2(DevState S) {
 
 // Ordered mean: one lane sums in the reference's (k,j,i) order, so the
 // result is bit-identical to the single-rank MPI_ALLREDUCE of :64.
-__global__ void k_sum_ordered(DevState S) {
+__device__ void k_sum_ordered_body(DevState S) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double dp0 = 0.0;
   for (int k = 1; k <= S.nz; k++)
@@ -1397,6 +1397,7 @@ __global__ void k_sum_ordered(DevState S) {
   S.red[0] = 1.0 / (dp_av + 1.0e-30);
   S.red[1] = dp_av;
 }
+__global__ void k_sum_ordered(DevState S) { k_sum_ordered_body(S); }
 
 // Fast (non-ordered) mean of the type-2 sensor: per-thread strided sums, a
 // wavefront tree by __shfl_xor, the four wave sums in fixed order; then one
@@ -1427,7 +1428,19 @@ __global__ __launch_bounds__(64) void k_sum_final(DevState S, const double* part
     const double dp_av = acc / (1.0 * S.nx * S.ny * S.nz);
     S.red[0] = 1.0 / (dp_av + 1.0e-30);
     S.red[1] = dp_av;
+    // certificate counters of k_masks<.., 2>: uncertain interfaces, and
+    // the smallest relative distance of a normalised R to a threshold
+    reinterpret_cast<unsigned long long*>(S.red)[4] = 0ull;
+    reinterpret_cast<unsigned long long*>(S.red)[5] = 0x7ff0000000000000ull;  // +inf
   }
+}
+
+// The sequential mean again, only when the certificate of the fast one
+// failed (k_masks counted an interface inside the rounding band); otherwise
+// the lane exits at once.  Launched on one wave so it costs nothing when idle.
+__global__ void k_sum_ordered_if(DevState S) {
+  if (reinterpret_cast<const unsigned long long*>(S.red)[4] == 0ull) return;
+  k_sum_ordered_body(S);
 }
 
 // CFL time step (no reference counterpart; the reference takes dt from the
@@ -1484,12 +1497,20 @@ __global__ __launch_bounds__(256) void k_sensor_norm(DevState S) {
   S.Rh[q] = S.Rh[q] * S.red[0];
 }
 
-template <int DIR>
-__global__ __launch_bounds__(256) void k_masks(DevState S) {
+// Interface masks from the sensor (set_hybrid_scheme, OCFD_SetHybrid.f90:2-14):
+// kind 1/2/3 as max(R_i, R_i+1) <= t1, <= t2, above.  With a normalised
+// sensor (type 2) Rh still holds the raw |grad p| here and the mask uses
+// fl(max(r0, r1) * inv), which equals max(fl(r0 * inv), fl(r1 * inv)) (rounding
+// is monotone), i.e. the mask of the normalised field; k_sensor_norm
+// normalises Rh afterwards.  SCALE: 0 type 1 (R as stored), 1 normalised,
+// 2 normalised and certified: an interface whose normalised R lies within the
+// relative band hyb_eta of a threshold could flip between the fast and the
+// ordered mean; such interfaces are counted (red[4]) and the smallest
+// relative distance to a threshold is kept (red[5]) for the report.
+template <int DIR, int SCALE>
+__device__ __forceinline__ void mask_one(const DevState& S, long t, bool& unsure, double& dist) {
   const int n = S.n[DIR];
   const long npen = DIR == 0 ? (long)S.ny * S.nz : DIR == 1 ? (long)S.nx * S.nz : (long)S.nx * S.ny;
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= npen * (n + 1)) return;
   long pen;
   int s;
   if (DIR == 0) { s = (int)(t % (n + 1)); pen = t / (n + 1); }
@@ -1499,9 +1520,50 @@ __global__ __launch_bounds__(256) void k_masks(DevState S) {
   // exchanged Rhybrid: periodic faces wrap; non-periodic ghosts keep 0
   const double r0 = S.Rh[sweep_hq<DIR>(S, a0, a1, s)];
   const double r1 = S.Rh[sweep_hq<DIR>(S, a0, a1, s + 1)];
-  const double R0 = fmax(r0, r1);
+  const double R0 = SCALE == 0 ? fmax(r0, r1) : fmax(r0, r1) * S.red[0];
   const uint8_t m = R0 <= S.hyb_t1 ? 1 : (R0 <= S.hyb_t2 ? 2 : 3);
   S.mask[DIR][mask_index<DIR>(S, pen, s)] = m;
+  if (SCALE == 2) {
+    const double e = S.hyb_eta;
+    unsure = (R0 >= S.hyb_t1 * (1.0 - e) && R0 <= S.hyb_t1 * (1.0 + e)) ||
+             (R0 >= S.hyb_t2 * (1.0 - e) && R0 <= S.hyb_t2 * (1.0 + e));
+    dist = fmin(fabs(R0 * S.hyb_inv_t1 - 1.0), fabs(R0 * S.hyb_inv_t2 - 1.0));
+  }
+}
+
+template <int DIR, int SCALE>
+__global__ __launch_bounds__(256) void k_masks(DevState S) {
+  const int n = S.n[DIR];
+  const long npen = DIR == 0 ? (long)S.ny * S.nz : DIR == 1 ? (long)S.nx * S.nz : (long)S.nx * S.ny;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = t < npen * (n + 1);
+  bool unsure = false;
+  double dist = __builtin_inf();
+  if (live) mask_one<DIR, SCALE>(S, t, unsure, dist);
+  if (SCALE == 2) {
+    const unsigned long long b = __ballot(unsure);
+    for (int o = 32; o > 0; o >>= 1) dist = fmin(dist, __shfl_xor(dist, o, 64));
+    if ((threadIdx.x & 63) == 0) {
+      unsigned long long* cnt = reinterpret_cast<unsigned long long*>(S.red);
+      if (b) atomicAdd(cnt + 4, (unsigned long long)__popcll(b));
+      // non-negative doubles order as their bit patterns
+      if (dist < __builtin_inf()) atomicMin(cnt + 5, (unsigned long long)__double_as_longlong(dist));
+    }
+  }
+}
+
+// Masks again from the ordered mean, only when k_masks<.., 2> counted an
+// uncertain interface (a fixed grid striding over the interfaces, so the
+// idle case is a few hundred waves that read one counter and exit).
+template <int DIR>
+__global__ __launch_bounds__(256) void k_masks_redo(DevState S) {
+  if (reinterpret_cast<const unsigned long long*>(S.red)[4] == 0ull) return;
+  const int n = S.n[DIR];
+  const long npen = DIR == 0 ? (long)S.ny * S.nz : DIR == 1 ? (long)S.nx * S.nz : (long)S.nx * S.ny;
+  bool unsure;
+  double dist;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < npen * (n + 1); t += (long)gridDim.x * blockDim.x)
+    mask_one<DIR, 1>(S, t, unsure, dist);
 }
 
 // ===========================================================================
@@ -1606,7 +1668,9 @@ struct Ctx {
   std::vector<void*> allocs;
   double* part = nullptr;  // fast-sum partials
   int nparts = 0;
-  int hybrid_sum_mode = 0;
+  int hybrid_sum_mode = 1;
+  double hyb_eta0 = 0.0;      // certificate band before OCFD_OPT_HYBRID_CERT_WIDEN
+  bool hyb_cert_ran = false;
   int sweep_variant[3] = {0, 0, 0};
   float tune_ms[3][8] = {};
   int col_planes = -1;    // k planes per block of the marching kernels (0: one thread per cell, -1: auto)
@@ -1841,11 +1905,11 @@ void launch_sweep() {
   launch_sweep_variant<DIR>(G.sweep_variant[DIR]);
 }
 
-template <int DIR>
+template <int DIR, int SCALE>
 void launch_masks() {
   const DevState& S = G.S;
   const long npen = DIR == 0 ? (long)S.ny * S.nz : DIR == 1 ? (long)S.nx * S.nz : (long)S.nx * S.ny;
-  launch("k_masks", k_masks<DIR>, grid1(npen * (S.n[DIR] + 1)), dim3(256), S);
+  launch("k_masks", k_masks<DIR, SCALE>, grid1(npen * (S.n[DIR] + 1)), dim3(256), S);
 }
 
 void enq_rhybrid() {
@@ -1854,19 +1918,31 @@ void enq_rhybrid() {
   const int sensor = (int)std::lround(G.cfg.hybrid_para[0]);
   if (sensor == 1) {
     launch("k_sensor1", k_sensor1, grid1(S.ni), dim3(256), S);
-  } else {
-    launch("k_sensor2", k_sensor2, grid1(S.ni), dim3(256), S);
-    if (G.hybrid_sum_mode == 0) {
-      launch("k_sum_ordered", k_sum_ordered, dim3(1), dim3(64), S);
-    } else {
-      launch("k_sum_partial", k_sum_partial, dim3(G.nparts), dim3(256), S, G.part);
-      launch("k_sum_final", k_sum_final, dim3(1), dim3(64), S, (const double*)G.part, G.nparts);
-    }
-    launch("k_sensor_norm", k_sensor_norm, grid1(S.ni), dim3(256), S);
+    launch_masks<0, 0>();
+    launch_masks<1, 0>();
+    launch_masks<2, 0>();
+    return;
   }
-  launch_masks<0>();
-  launch_masks<1>();
-  launch_masks<2>();
+  launch("k_sensor2", k_sensor2, grid1(S.ni), dim3(256), S);
+  if (G.hybrid_sum_mode == 0) {
+    launch("k_sum_ordered", k_sum_ordered, dim3(1), dim3(64), S);
+    launch_masks<0, 1>();
+    launch_masks<1, 1>();
+    launch_masks<2, 1>();
+  } else {
+    // fast mean, certified: the masks are the ordered mean's in every case
+    G.hyb_cert_ran = true;
+    launch("k_sum_partial", k_sum_partial, dim3(G.nparts), dim3(256), S, G.part);
+    launch("k_sum_final", k_sum_final, dim3(1), dim3(64), S, (const double*)G.part, G.nparts);
+    launch_masks<0, 2>();
+    launch_masks<1, 2>();
+    launch_masks<2, 2>();
+    launch("k_sum_ordered_if", k_sum_ordered_if, dim3(1), dim3(64), S);
+    launch("k_masks_redo", k_masks_redo<0>, dim3(512), dim3(256), S);
+    launch("k_masks_redo", k_masks_redo<1>, dim3(512), dim3(256), S);
+    launch("k_masks_redo", k_masks_redo<2>, dim3(512), dim3(256), S);
+  }
+  launch("k_sensor_norm", k_sensor_norm, grid1(S.ni), dim3(256), S);
 }
 
 void enq_bc() {
@@ -2061,6 +2137,13 @@ int32_t ocfd_finalize(void) {
 int32_t ocfd_init(const ocfd_cfg* cfg) {
   ocfd_finalize();
   G.err.clear();
+  // every context starts from the documented option defaults
+  G.hybrid_sum_mode = 1;
+  G.hyb_cert_ran = false;
+  G.col_planes = -1;
+  G.concurrent = 1;
+  G.use_graph = 0;
+  G.timing = 0;
   if (!cfg) return fail("ocfd_init: null config");
   const ocfd_cfg& c = *cfg;
   if (c.lap != OCFD_LAP) return fail("ocfd_init: LAP must be 4");
@@ -2203,6 +2286,22 @@ int32_t ocfd_init(const ocfd_cfg* cfg) {
   }
   G.nparts = 1024;
   G.part = dalloc(G.nparts);
+  {
+    // Certificate of the fast sensor mean (k_sum_partial/k_sum_final against
+    // the reference's sequential sum, k_sum_ordered).  Both sums of the same
+    // ni non-negative values lie within gamma_k * (exact sum) of it, with k =
+    // ni - 1 for the sequential sum and k = the depth of the tree (each
+    // thread's strided run, 6 wave levels, 3 wave sums, each final lane's run
+    // over the partials, 6 wave levels).  Dividing by N, adding 1e-30, the
+    // reciprocal, the product R * inv and the band ends t*(1 +- eta) add one
+    // rounding each per side.  eta covers all of it with a factor 1.1 of room.
+    const double u = std::ldexp(1.0, -53);
+    const double run = std::ceil((double)S.ni / ((double)G.nparts * 256.0));
+    const double depth = run + 6.0 + 3.0 + std::ceil(G.nparts / 64.0) + 6.0;
+    S.hyb_eta = G.hyb_eta0 = 1.1 * ((double)S.ni - 1.0 + depth + 16.0) * u;
+    S.hyb_inv_t1 = S.hyb_t1 > 0.0 ? 1.0 / S.hyb_t1 : 0.0;
+    S.hyb_inv_t2 = S.hyb_t2 > 0.0 ? 1.0 / S.hyb_t2 : 0.0;
+  }
   double* bl = dalloc((long)S.ny * 6 + 6 + (long)S.nx * S.nz);
   void* nr = nullptr;
   if (!G.part || !bl || hipMalloc(&nr, sizeof(int) * (size_t)S.nx * S.nz) != hipSuccess) return fail("ocfd_init: hipMalloc failed");
@@ -2232,6 +2331,11 @@ int32_t ocfd_set_option(int32_t key, int32_t value) {
     case OCFD_OPT_USE_GRAPH: G.use_graph = value; drop_graph(); return 0;
     case OCFD_OPT_KERNEL_TIMING: collect_timing(); G.timing = value; drop_graph(); return 0;
     case OCFD_OPT_CONCURRENT: G.concurrent = value ? 1 : 0; drop_graph(); return 0;
+    case OCFD_OPT_HYBRID_CERT_WIDEN:
+      if (value < 0 || value > 60) return fail("ocfd_set_option: certificate widening must be 0..60");
+      G.S.hyb_eta = G.hyb_eta0 * std::ldexp(1.0, value);
+      drop_graph();
+      return 0;
     case OCFD_OPT_COLUMN_PLANES:
       if (value < -1) return fail("ocfd_set_option: column planes must be >= -1");
       G.col_planes = value;
@@ -2325,8 +2429,13 @@ int32_t ocfd_set_mesh(const double* axx, const double* ayy, const double* azz, c
   if (check_ready()) return 1;
   DevState& S = G.S;
   const size_t bytes = sizeof(double) * (size_t)S.nh;
-  double* tmp = nullptr;
-  if (hipMalloc(&tmp, bytes) != hipSuccess) return fail("ocfd_set_mesh: hipMalloc failed");
+  // staging buffer, freed on every return path (hipFree waits for the device)
+  struct Staging {
+    double* p = nullptr;
+    ~Staging() { if (p) hipFree(p); }
+  } stage;
+  if (hipMalloc(&stage.p, bytes) != hipSuccess) return fail("ocfd_set_mesh: hipMalloc failed");
+  double* tmp = stage.p;
   const double* src[3] = {axx, ayy, azz};
   double* dst[3] = {S.Axx, S.Ayy, S.Azz};
   for (int a = 0; a < 3; a++) {
@@ -2359,7 +2468,6 @@ int32_t ocfd_set_mesh(const double* axx, const double* ayy, const double* azz, c
     }
   hipLaunchKernelGGL(k_metrics_over_jac, grid1(S.nh), dim3(256), 0, G.stream, S);
   HIPCHK(hipStreamSynchronize(G.stream));
-  hipFree(tmp);
   HIPCHK(hipGetLastError());
   drop_graph();
   return 0;
@@ -2716,6 +2824,18 @@ int32_t ocfd_negative_et_count(void) {
   if (!G.inited) return 0;
   hipStreamSynchronize(G.stream);
   return *G.h_negt;
+}
+
+int32_t ocfd_hybrid_certificate(int64_t* uncertain, double* margin, double* eta) {
+  if (check_ready()) return 1;
+  if (!G.hyb_cert_ran) return fail("ocfd_hybrid_certificate: no certified sensor mean has run");
+  unsigned long long v[2];
+  HIPCHK(hipMemcpyAsync(v, G.S.red + 4, sizeof(v), hipMemcpyDeviceToHost, G.stream));
+  HIPCHK(hipStreamSynchronize(G.stream));
+  if (uncertain) *uncertain = (int64_t)v[0];
+  if (margin) std::memcpy(margin, &v[1], sizeof(double));
+  if (eta) *eta = G.S.hyb_eta;
+  return 0;
 }
 
 int32_t ocfd_synchronize(void) {

@@ -178,8 +178,10 @@ def convert(case_dir, out=JET_JSON):
 class Mechanism:
     """Arrays of a mechanism in the layout the C-ABI takes."""
 
-    def __init__(self, path=JET_JSON):
-        d = json.load(open(path))
+    def __init__(self, path=JET_JSON, data=None):
+        """From the JSON form at ``path``, or from the same structure given
+        as ``data`` (a dict)."""
+        d = data if data is not None else json.load(open(path))
         self.names = d["species"]
         m, t = d["mechanism"], d["thermo"]
         self.n_spec, self.n_reac, self.rereac = m["n_spec"], m["n_reac"], m["rereac"]
@@ -193,6 +195,13 @@ class Mechanism:
             for a, abc in enumerate(r["arrhenius"]):
                 arr[j, a] = abc
         self.arrhenius = arr
+        # reverse-rate data per reaction: ReReac = 1 (A_r, b_r, E_r, 0, 0),
+        # ReReac = 2 the Park fit A1..A5 (read_Reac, CHEM_reactions.f90:46-47)
+        rev = np.zeros((self.n_reac, 5))
+        for j, r in enumerate(R):
+            if r.get("reverse") is not None:
+                rev[j, :len(r["reverse"])] = r["reverse"]
+        self.reverse = rev
         self.vf = np.array([r["vf"] for r in R], dtype=np.int32)  # [reac][spec]
         self.vr = np.array([r["vr"] for r in R], dtype=np.int32)
         self.eff = np.array([r["eff"] if r["eff"] is not None else [0.0] * self.n_spec for r in R])
@@ -203,7 +212,7 @@ class Mechanism:
         self.viscosity = np.array(d.get("viscosity", []))
         self.conductivity = np.array(d.get("conductivity", []))
         self.diffusion = np.array(d.get("diffusion", []))
-        self.elements = np.array([ELEMENTS[n] for n in self.names], dtype=np.float64)  # [spec][H,O,N]
+        self.elements = np.array([ELEMENTS.get(n) or (0, 0, 0) for n in self.names], dtype=np.float64)  # [spec][H,O,N]
 
     def index(self, name):
         return self.names.index(name)

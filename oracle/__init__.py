@@ -221,7 +221,7 @@ class ChemOracle:
             "orc_chem_batch": (i32, [i32, ctypes.c_long, vp, vp, vp, i32]),
             "orc_chem_adaptive": (i32, [vp, vp, dbl, dbl, dbl, dbl, i32, vp, vp]),
             "orc_chem_adaptive_batch": (i32, [ctypes.c_long, vp, vp, vp, dbl, dbl, dbl, i32, vp, vp]),
-            "orc_set_threads": (i32, [i32]),
+            "orc_set_threads": (i32, [i32]), "orc_chem_set_reverse": (i32, [vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -238,6 +238,9 @@ class ChemOracle:
                                       float(mech.tct_max), _ptr(k[8]), _ptr(k[9]))
         if rc:
             raise RuntimeError("orc_chem_set_mechanism failed")
+        if mech.rereac != 0:
+            self._rev = np.ascontiguousarray(mech.reverse, dtype=np.float64)
+            L.orc_chem_set_reverse(_ptr(self._rev))
 
     def set_threads(self, n):
         """OpenMP threads of the batch functions; returns the count in use."""

@@ -7,7 +7,8 @@
  * Arrhenius rate constants, reverse rates from Gibbs equilibrium constants,
  * the analytic rate Jacobian, Gauss elimination with partial pivoting, and
  * the explicit Euler / RK2 / RK3 and point-implicit Euler integrators of
- * chemical_timeAdv.  Built with -ffp-contract=off; transcendentals are glibc's.
+ * chemical_timeAdv.  Reverse rates: Gibbs equilibrium constants (ReReac = 0),
+ * Arrhenius fits (1) or the Park curve fit of Kc (2).  Built with -ffp-contract=off; transcendentals are glibc's.
  *
  * Pinned by tests/test_oracle_chem.py: element and mass conservation per
  * step, Gauss == numpy.linalg.solve, Arrhenius / Troe / K_c against closed
@@ -37,6 +38,8 @@ static struct {
   int ns, nr, rereac;
   int af_type[NR_MAX], tb[NR_MAX], sgm[NR_MAX];
   double fc[NR_MAX], arr[NR_MAX][2][3], eff[NR_MAX][NS_MAX];
+  double rev[NR_MAX][5];  /* ReReac = 1: Ar, br, Er; ReReac = 2: A1r..A5r */
+  int rev_set;
   int vf[NR_MAX][NS_MAX], vr[NR_MAX][NS_MAX];
   int n_tct;
   double tct[NT_MAX], tct_max;
@@ -67,8 +70,9 @@ static double ipow(double x, int n) {
 int orc_chem_set_mechanism(int ns, int nr, int rereac, const int* af_type, const int* tb, const double* fc,
                            const double* arr, const int* vf, const int* vr, const double* eff, int n_tct,
                            const double* tct, double tct_max, const double* mi, const double* coef) {
-  if (ns > NS_MAX || nr > NR_MAX || n_tct > NT_MAX) return 1;
+  if (ns > NS_MAX || nr > NR_MAX || n_tct > NT_MAX || rereac < 0 || rereac > 2) return 1;
   C.ns = ns; C.nr = nr; C.rereac = rereac;
+  C.rev_set = 0;
   for (int j = 0; j < nr; j++) {
     C.af_type[j] = af_type[j]; C.tb[j] = tb[j]; C.fc[j] = fc[j];
     for (int a = 0; a < 2; a++) for (int b = 0; b < 3; b++) C.arr[j][a][b] = arr[(j * 2 + a) * 3 + b];
@@ -93,6 +97,15 @@ int orc_chem_set_mechanism(int ns, int nr, int rereac, const int* af_type, const
                           (C.coef[k][j + 1][4] - C.coef[k][j][4]) * p5(t) / 5.0) + C.ect[k][j];
     }
   }
+  return 0;
+}
+
+/* Reverse-rate data of ReReac = 1 or 2 (the extra line per reaction that
+ * read_Reac reads, CHEM_reactions.f90:46-47): rev[j*5 .. j*5+4]. */
+int orc_chem_set_reverse(const double* rev) {
+  for (int j = 0; j < C.nr; j++)
+    for (int a = 0; a < 5; a++) C.rev[j][a] = rev[j * 5 + a];
+  C.rev_set = 1;
   return 0;
 }
 
@@ -188,11 +201,22 @@ static void comput_kjx(double T, double* kjx) {
 /* Arrhenius, CHEM_reactions.f90:106-111 */
 static double arrhenius(double A, double b, double E, double T) { return A * pow(T, b) * exp(-E / T); }
 
-/* Reaction_rate_coef, CHEM_reactions.f90:130-197 (ReReac = 0 path) */
+/* ParkCurveFit, CHEM_reactions.f90:113-118: Kc = exp(A1 + A2 z + A3 z^2 +
+ * A4 z^3 + A5 z^4), z = 10000/T */
+static double park_curve_fit(const double* a, double T) {
+  double z = 10000.0 / T;
+  return exp(a[0] + a[1] * z + a[2] * p2(z) + a[3] * p3(z) + a[4] * p4(z));
+}
+
+/* Reaction_rate_coef, CHEM_reactions.f90:130-197: reverse rates from the
+ * Gibbs equilibrium constant (ReReac = 0), an Arrhenius fit (1) or
+ * k_f / Kc with the Park fit of Kc floored at 1e-30 (2). */
 int orc_chem_rate_coef(const double* ci, double T, double* wf, double* wr, double* kjx) {
   double kj[NR_MAX];
-  if (C.rereac != 0) return 1;
-  comput_kjx(T, kj);
+  if (C.rereac != 0 && !C.rev_set) return 1;
+  if (C.rereac == 0) comput_kjx(T, kj);
+  else
+    for (int j = 0; j < C.nr; j++) kj[j] = 0.0;
   for (int j = 0; j < C.nr; j++) {
     double Ctrb = 0.0;
     if (C.tb[j] == 1)
@@ -213,7 +237,15 @@ int orc_chem_rate_coef(const double* ci, double T, double* wf, double* wr, doubl
       double F = pow(10.0, log10(Fc) / (1.0 + p2((log10(Prs) + ac) / (an - 0.14 * (log10(Prs) + ac)))));
       wf[j] = wf2 * Prs / (1.0 + Prs) * F;
     }
-    wr[j] = wf[j] / kj[j];
+    if (C.rereac == 0) {
+      wr[j] = wf[j] / kj[j];
+    } else if (C.rereac == 1) {
+      wr[j] = arrhenius(C.rev[j][0], C.rev[j][1], C.rev[j][2], T);
+    } else {
+      kj[j] = park_curve_fit(C.rev[j], T);
+      if (kj[j] < 1.0e-30) kj[j] = 1.0e-30;
+      wr[j] = wf[j] / kj[j];
+    }
     if (C.tb[j] == 1 && C.af_type[j] != 1) {
       wf[j] = wf[j] * Ctrb;
       wr[j] = wr[j] * Ctrb;

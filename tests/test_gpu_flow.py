@@ -103,10 +103,13 @@ def test_rk3_step_isotropic(pkg):
     g.close()
 
 
-def test_rk3_step_flat_plate_hybrid(pkg):
+@pytest.mark.parametrize("sum_mode", [0, 1])
+def test_rk3_step_flat_plate_hybrid(pkg, sum_mode):
+    """Ordered sensor mean (0) and the certified fast mean (1, the default):
+    both give the reference's masks and the whole step bit for bit."""
     case = fc.small_flat_plate(64, 40, 16)
     g, o = pair(pkg, case)
-    g.set_option(pkg.OPT_HYBRID_SUM_MODE, 0)  # ordered mean: bit-identical masks
+    g.set_option(pkg.OPT_HYBRID_SUM_MODE, sum_mode)
     dt1 = 0.5 * case[5]["dt"]
     g.flow_time_advance(dt1)
     g.synchronize()
@@ -239,22 +242,88 @@ def test_graph_replay_matches_eager(pkg, name):
         assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
 
 
-def test_fast_sensor_mean_masks(pkg):
-    """Fast (tree) mean: masks may only differ where R is within rounding of a
-    threshold; report how many interfaces differ."""
+@pytest.mark.parametrize("widen", [0, 40])
+def test_fast_sensor_mean_masks(pkg, widen):
+    """Certified fast (tree) mean: the masks equal the reference's (ordered
+    sum) bit for bit.  widen = 0: the certificate holds and nothing is
+    recomputed; widen = 40 (band x 2^40) forces interfaces into the band, so
+    the ordered mean and masks are recomputed on the device -- the result is
+    the same in both cases, and Rhybrid is the ordered one too."""
     case = fc.small_flat_plate(64, 40, 16)
     g, o = pair(pkg, case)
     g.set_option(pkg.OPT_HYBRID_SUM_MODE, 1)
+    g.set_option(pkg.OPT_HYBRID_CERT_WIDEN, widen)
     g.comput_Rhybrid()
     o.exchange_all()
     o.comput_Rhybrid()
+    unsure, margin, eta = g.hybrid_certificate()
+    print(f"widen {widen}: uncertain {unsure}, margin {margin:.3e}, eta {eta:.3e}")
+    assert eta > 0.0 and margin >= 0.0
+    if widen == 0:
+        assert unsure == 0 and margin > eta
+    else:
+        assert unsure > 0 and margin <= eta
     R_g, R_o = fc.interior(g.download("RHYBRID")), fc.interior(o.download("RHYBRID"))
-    d, _ = fc.compare(R_g, R_o)
-    assert d <= 1e-13
+    if unsure:
+        assert np.array_equal(R_g.view(np.uint64), R_o.view(np.uint64))
+    else:
+        assert fc.compare(R_g, R_o)[0] <= 1e-13
     o.du_inviscous()
-    ndiff = sum(int(np.sum(g.download_mask(k) != o.download_mask(k))) for k in range(3))
-    print("fast-mean mask differences:", ndiff)
-    assert ndiff <= 3
+    for k in range(3):
+        gm, om = g.download_mask(k), o.download_mask(k)
+        assert np.array_equal(gm, om), f"mask {k}: {np.sum(gm != om)} differ"
+    g.close()
+
+
+def _sensor1_case(pkg):
+    """Flat plate with Hybrid_para(1) = 1 (the second-difference pressure
+    sensor, OCFD_SetHybrid.f90:34-43) and 1% noise on the energy, so that R
+    takes a spread of values; the thresholds are set at R's 60% and 93%
+    quantiles (from the oracle), giving all three interface kinds."""
+    import oracle
+    case = list(fc.small_flat_plate(64, 40, 16))
+    case[0].hybrid_para[0] = 1.0
+    st = dict(case[3])
+    f = st["F"].copy(order="F")
+    rng = np.random.default_rng(7)
+    f[..., 4] *= 1.0 + 0.01 * rng.standard_normal(f.shape[:3])
+    st["F"] = f
+    case[3] = st
+    o = oracle.Oracle(pkg, case[0])
+    fc.setup(o, tuple(case))
+    o.exchange_all()
+    o.comput_Rhybrid()
+    R = fc.interior(o.download("RHYBRID"))
+    case[0].hybrid_para[1], case[0].hybrid_para[2] = np.percentile(R, [60.0, 93.0])
+    return tuple(case)
+
+
+def test_sensor_type1_masks_and_step(pkg):
+    case = _sensor1_case(pkg)
+    g, o = pair(pkg, case)
+    g.stage_begin(1)
+    g.du_inviscous()
+    o.comput_Amu_Amk_AMD()
+    o.exchange_all()
+    o.comput_Rhybrid()
+    o.du_inviscous()
+    Rg, Ro = fc.interior(g.download("RHYBRID")), fc.interior(o.download("RHYBRID"))
+    assert np.array_equal(Rg.view(np.uint64), Ro.view(np.uint64))
+    for k in range(3):
+        gm, om = g.download_mask(k), o.download_mask(k)
+        frac = [float(np.mean(om == v)) for v in (1, 2, 3)]
+        print("mask", k, frac)
+        assert min(frac) > 0.0, frac
+        assert np.array_equal(gm, om), f"mask {k}: {np.sum(gm != om)} differ"
+    d, same = fc.compare(g.download("DU"), o.download("DU"))
+    assert same == 1.0, d
+    g.close()
+    g, o = pair(pkg, case)
+    dt1 = 0.5 * case[5]["dt"]
+    g.flow_time_advance(dt1)
+    o.flow_time_advance(dt1)
+    rep = fc.compare_state(g, o, tol=1e-12)
+    assert all(v[1] == 1.0 for v in rep.values()), rep
     g.close()
 
 

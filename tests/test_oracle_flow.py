@@ -601,3 +601,199 @@ def test_ghost_cells_closed_forms():
     T1 = TT[L:L + nx, L - 1, L:L + nz]   # first ghost layer: not clipped, exact extrapolation
     assert np.allclose(T1, 2 * Tb - TT[L:L + nx, L + 1, L:L + nz], rtol=1e-14)
     o.close()
+
+
+# ------------------------------------------------- curvilinear (GRID3D) metrics
+def curvilinear_box(n, a=0.05, L=2.0):
+    """A periodic box of side L under the non-separable smooth map
+    X = L (s_i + a sin 2pi s_j sin 2pi s_k), Y = L (s_j + a sin 2pi s_k sin 2pi s_i),
+    Z = L (s_k + a sin 2pi s_i sin 2pi s_j), s = (index - 1)/n, given as 3-D
+    coordinate arrays (the GRID3D input of init_mesh).  Returns the mesh, the
+    span and the exact d(X, Y, Z)/d(xi, eta, zeta) with xi = (i - 1) h,
+    h = 1/(n - 1) (OCFD_readpara.f90:351-353)."""
+    from ocfd_amd import cases
+    s = np.arange(n) / n
+    S1, S2, S3 = np.meshgrid(s, s, s, indexing="ij")
+    w = 2 * np.pi
+    sn = lambda t: np.sin(w * t)
+    cs = lambda t: np.cos(w * t)
+    X = L * (S1 + a * sn(S2) * sn(S3))
+    Y = L * (S2 + a * sn(S3) * sn(S1))
+    Z = L * (S3 + a * sn(S1) * sn(S2))
+    ds = (n - 1) / n            # d s / d xi
+    Jm = np.empty((3, 3) + X.shape)   # Jm[c][e] = d coord c / d index-coordinate e
+    Jm[0, 0] = L * ds * np.ones_like(X)
+    Jm[0, 1] = L * a * w * cs(S2) * sn(S3) * ds
+    Jm[0, 2] = L * a * w * sn(S2) * cs(S3) * ds
+    Jm[1, 0] = L * a * w * sn(S3) * cs(S1) * ds
+    Jm[1, 1] = L * ds * np.ones_like(X)
+    Jm[1, 2] = L * a * w * cs(S3) * sn(S1) * ds
+    Jm[2, 0] = L * a * w * cs(S1) * sn(S2) * ds
+    Jm[2, 1] = L * a * w * sn(S1) * cs(S2) * ds
+    Jm[2, 2] = L * ds * np.ones_like(X)
+    mesh = []
+    for A in (X, Y, Z):
+        h = cases.halo_zeros(n, n, n, 1.0)
+        cases.interior(h)[...] = A
+        mesh.append(h)
+    span = np.array([L, 0, 0, 0, L, 0, 0, 0, L], dtype=np.float64)
+    return mesh, span, Jm
+
+
+def _exact_metrics(Jm):
+    M = np.moveaxis(Jm, (0, 1), (-2, -1))          # [..., c, e]
+    inv = np.linalg.inv(M)                         # [..., e, c] = d index-coord e / d coord c
+    det = np.linalg.det(M)
+    return inv, det
+
+
+def _metric_errors(n):
+    m = load_package()
+    import oracle
+    mesh, span, Jm = curvilinear_box(n)
+    o = oracle.Oracle(m, m.make_cfg(n, n, n))
+    o.set_mesh(*mesh, span)
+    inv, det = _exact_metrics(Jm)
+    names = (("AKX", 0, 0), ("AKY", 0, 1), ("AKZ", 0, 2), ("AIX", 1, 0), ("AIY", 1, 1), ("AIZ", 1, 2),
+             ("ASX", 2, 0), ("ASY", 2, 1), ("ASZ", 2, 2))
+    err = {}
+    for name, e, c in names:
+        a = fc.interior(o.download(name))
+        err[name] = float(np.max(np.abs(a - inv[..., e, c])) / np.max(np.abs(inv[..., e, e])))
+    err["AJAC"] = float(np.max(np.abs(fc.interior(o.download("AJAC")) - 1.0 / det)) / np.max(np.abs(1.0 / det)))
+    # metric identity (geometric conservation law): sum over directions of
+    # d/d xi_d of the cofactor A_d1 = A_d / Ajac vanishes in exact arithmetic;
+    # with CD6 differences of the discrete cofactors it converges to 0
+    h = 1.0 / (n - 1)
+    cd6 = lambda f, ax: (np.roll(f, -1, ax) - np.roll(f, 1, ax)) * 0.75 / h - (
+        np.roll(f, -2, ax) - np.roll(f, 2, ax)) * 0.15 / h + (np.roll(f, -3, ax) - np.roll(f, 3, ax)) / 60.0 / h
+    gcl = 0.0
+    for c, comp in enumerate(("X", "Y", "Z")):
+        terms = [fc.interior(o.download(nm + "1")) for nm in (f"AK{comp}", f"AI{comp}", f"AS{comp}")]
+        div = cd6(terms[0], 0) + cd6(terms[1], 1) + cd6(terms[2], 2)
+        gcl = max(gcl, float(np.max(np.abs(div)) / (np.max(np.abs(terms[0])) / h)))
+    o.close()
+    return err, gcl
+
+
+def test_grid3d_metrics_converge_at_sixth_order():
+    """init_mesh / comput_Jacobian3d (OCFD_mesh.f90:60-107) on a non-separable
+    map: all nine metrics and the Jacobian approach the exact inverse of
+    d(X,Y,Z)/d(xi,eta,zeta) at the CD6 rate (error ratio ~64 from n = 16 to
+    32), and the discrete cofactors satisfy the metric identity to the same
+    order.  A transposed cofactor or a swapped derivative direction leaves an
+    O(1) error that does not converge."""
+    e16, g16 = _metric_errors(16)
+    e32, g32 = _metric_errors(32)
+    print({k: (f"{e16[k]:.2e}", f"{e32[k]:.2e}", f"{e16[k] / e32[k]:.1f}") for k in e16}, g16, g32)
+    for k in e16:
+        assert e32[k] < 1e-4, (k, e32[k])
+        assert e16[k] / e32[k] > 40.0, (k, e16[k], e32[k])
+    assert g32 < 1e-5 and g16 / g32 > 40.0, (g16, g32)
+
+
+# ------------------------------------------------------ boundary-layer BC
+def _bl_case(outlet, Tw, nx=24, ny=16, nz=8, shear=(0.15, 0.1)):
+    """A small flat-plate deck with a sheared mesh (the wall surface is the
+    plane y = s x + t z, so its unit normal is (-s, 1, -t)/sqrt(1+s^2+t^2)),
+    random perturbations of the state, a random blowing pattern, upper
+    points split between 0-D data and non-reflecting, and the given outlet
+    type and wall temperature (BC_Para(4), BC_Para(5))."""
+    from ocfd_amd import cases
+    case = list(fc.small_flat_plate(nx, ny, nz))
+    cfg, mesh, span, st, bc, info = case
+    s, t = shear
+    axx, ayy, azz = (a.copy(order="F") for a in mesh)
+    fc.interior(ayy)[...] += s * fc.interior(axx) + t * fc.interior(azz)
+    span = np.array(span, dtype=np.float64)
+    span[7] = t * span[8]                        # Ayy shifts by t * lz across the periodic z face
+    cfg.bc_para[3] = outlet
+    cfg.bc_para[4] = Tw
+    x = fc.interior(axx)[:, 0, 0]
+    cfg.bc_para[5] = 0.5 * (x[0] + x[-1])        # Wall_Xinit: copy upstream of mid-plate
+    cfg.bc_para[7] = 37.0                        # disA
+    rng = np.random.default_rng(11)
+    st = dict(st)
+    f = st["F"].copy(order="F")
+    f *= 1.0 + 0.02 * rng.standard_normal(f.shape)
+    st["F"] = f
+    bc = dict(bc)
+    bc["wall_pertb"] = np.asfortranarray(rng.uniform(-1.0, 1.0, (nx, nz)))
+    nonref = np.ones((nx, nz), dtype=np.int32, order="F")
+    nonref[::3, :] = 0
+    bc["nonreflect_upper"] = nonref
+    return (cfg, (axx, ayy, azz), span, st, bc, info), (s, t)
+
+
+@pytest.mark.parametrize("outlet,Tw", [(1, 321.9), (2, 321.9), (0, -1.0)])
+def test_bc_boundarylayer_known_answers(outlet, Tw):
+    """ocfd_bc_BoundaryLayer (OCFD_BC_Boundarylayers.f90:20-243) followed by
+    the face re-synthesis of OCFD_bc, against values the BC must produce:
+      inlet (i=1): the 1-D profile (d, u, v, w, T);
+      upper (j=ny): the 0-D data where the point is not non-reflecting, the
+        state before the call elsewhere;
+      wall (j=1), x < Wall_Xinit: a copy of j=2;
+      wall, x >= Wall_Xinit: velocity A0 n with A0 = disA ht pertb and n the
+        geometric unit normal of the sheared wall, T = T_w (T_w > 0) or
+        (4 T_2 - T_3)/3, p = (4 p_2 - p_3)/3 and d = p / (R T);
+      outlet (i=nx): type 1 copies i=nx-1, type 2 extrapolates 2 f(nx-1) - f(nx-2),
+        type 0 leaves the state before the call."""
+    m = load_package()
+    import oracle
+    from ocfd_amd import cases
+    case, (s, t) = _bl_case(outlet, Tw)
+    cfg, mesh, span, st, bc, info = case
+    ht = 0.37
+    o = oracle.Oracle(m, cfg)
+    o.set_mesh(*mesh, span)
+    o.set_bc_boundarylayer(bc["inlet1d"], bc["upper0d"], bc["nonreflect_upper"], bc["wall_pertb"])
+    o.set_bc_time(ht)
+    o.upload_state(st)
+    o.comput_flow_variables()
+    pre = {k: fc.interior(o.download(k)).copy() for k in ("D", "U", "V", "W", "T", "P")}
+    o.OCFD_bc()
+    post = {k: fc.interior(o.download(k)) for k in ("D", "U", "V", "W", "T", "P")}
+    nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
+    R = cases.R_AIR
+    close = lambda a, b, tol=1e-13: np.max(np.abs(a - b)) <= tol * max(np.max(np.abs(b)), 1e-300)
+    # inlet
+    inl = bc["inlet1d"]
+    for c, k in enumerate(("D", "U", "V", "W", "T")):
+        assert close(post[k][0, 1:-1, :], np.broadcast_to(inl[1:-1, c][:, None], (ny - 2, nz))), k
+    # upper
+    up = bc["upper0d"]
+    dat = bc["nonreflect_upper"] == 0
+    for c, k in enumerate(("D", "U", "V", "W", "T")):
+        top, before = post[k][1:-1, -1, :], pre[k][1:-1, -1, :]
+        d1 = dat[1:-1]
+        assert close(top[d1], np.full(int(d1.sum()), up[c])), k
+        assert close(top[~d1], before[~d1]), k
+    # wall
+    x = fc.interior(mesh[0])[:, 0, 0]
+    xin = cfg.bc_para[5]
+    cp = x < xin
+    wall = ~cp
+    wall[0] = wall[-1] = False          # inlet / outlet columns belong to those faces
+    for k in ("D", "U", "V", "W", "T"):
+        assert close(post[k][cp, 0, :][1:], pre[k][cp, 1, :][1:]), k
+    nrm = np.array([-s, 1.0, -t]) / math.sqrt(1.0 + s * s + t * t)
+    A0 = cfg.bc_para[7] * ht * bc["wall_pertb"][wall]
+    for c, k in enumerate(("U", "V", "W")):
+        assert close(post[k][wall, 0, :], A0 * nrm[c], 1e-12), k
+    T2, T3 = pre["T"][wall, 1, :], pre["T"][wall, 2, :]
+    Tw_expect = np.full(T2.shape, Tw) if Tw > 0 else (4.0 * T2 - T3) / 3.0
+    assert close(post["T"][wall, 0, :], Tw_expect)
+    p1 = (4.0 * pre["P"][wall, 1, :] - pre["P"][wall, 2, :]) / 3.0
+    assert close(post["P"][wall, 0, :], p1, 1e-14)
+    assert close(post["D"][wall, 0, :], p1 / (R * Tw_expect), 1e-14)
+    # outlet
+    for k in ("D", "U", "V", "W", "T"):
+        got = post[k][-1, 1:-1, :]
+        if outlet == 1:
+            expect = post[k][-2, 1:-1, :]
+        elif outlet == 2:
+            expect = 2.0 * post[k][-2, 1:-1, :] - post[k][-3, 1:-1, :]
+        else:
+            expect = pre[k][-1, 1:-1, :]
+        assert close(got, expect), k
+    o.close()
