@@ -231,6 +231,22 @@ int32_t ocfd_comput_T_P(void);
  * Scalar_time_adv_RK3 of stage krk. */
 int32_t ocfd_du_species(void);
 int32_t ocfd_species_time_adv(int32_t krk, double dt);
+/* One species at a time, for a host that keeps the reference's species loop
+ * (OCFD_Time_Adv.f90:64-71); m is the 1-based species index.  Called as
+ *   for m = 1..N_SPEC: ocfd_du_scaler_invis(m); [ocfd_du_scaler_viscous(m);]
+ *                      ocfd_scalar_time_adv_rk3(krk, m, dt);
+ * they give ocfd_species_time_adv(krk, dt) bit for bit.
+ *   ocfd_du_scaler_invis(m)        du_scaler_invis(fs,Dus), OCFD_scaler_inviscous.f90:1-84:
+ *                                  Dus(m) = inviscid right-hand side of species m;
+ *   ocfd_du_scaler_viscous(m)      du_scaler_viscous(K_SPEC,fs,Dus), OCFD_scaler_viscous.f90:4-79:
+ *                                  Dus(m) += diffusive right-hand side; Qp zeroed at m = 1
+ *                                  and accumulated for later m (needs IF_Viscous = 1);
+ *   ocfd_scalar_time_adv_rk3(krk, m, dt)  Scalar_time_adv_RK3(KRK,m,dus,dt),
+ *                                  OCFD_Time_Adv.f90:134-166: di(m) from din(m) and Dus(m).
+ * Dus(m) is the slice m of OCFD_FIELD_DUS (the reference reuses du(:,:,:,1)). */
+int32_t ocfd_du_scaler_invis(int32_t m);
+int32_t ocfd_du_scaler_viscous(int32_t m);
+int32_t ocfd_scalar_time_adv_rk3(int32_t krk, int32_t m, double dt);
 
 /* Per-stage pieces, for a host that keeps the reference's own stage loop
  * (OCFD_Time_Adv.f90:33-91).  Called in this order they reproduce
@@ -324,12 +340,20 @@ int32_t ocfd_kernel_stats_reset(void);
  * fc_troe [nr]; arrhenius [nr][2][3] (A, b, E for the low / second set);
  * vf, vr, eff [nr][ns]; tct [n_tct-1], tct_max; mi [ns];
  * coef [ns][n_tct][7] (NASA-7 A..G per temperature range).  This build
- * takes ns = 9, nr <= 32, n_tct <= 4 and ReReac = 0 (reverse rates from
- * Gibbs equilibrium constants, comput_KjX CHEM_reactions.f90:76-103). */
+ * takes ns = 9, nr <= 32, n_tct <= 4.  ReReac (Reaction_rate_coef,
+ * CHEM_reactions.f90:130-197): 0 = reverse rates from Gibbs equilibrium
+ * constants (comput_KjX, :76-103); 1 = Arrhenius reverse rates; 2 = k_f / Kc
+ * with the Park curve fit of Kc floored at 1e-30 (ParkCurveFit, :113-118).
+ * For 1 and 2 the batch calls fail until ocfd_chem_set_reverse_rates has
+ * given the reverse data. */
 int32_t ocfd_chem_set_mechanism(int32_t ns, int32_t nr, int32_t rereac, const int32_t* af_type,
                                 const int32_t* third_body, const double* fc_troe, const double* arrhenius,
                                 const int32_t* vf, const int32_t* vr, const double* eff, int32_t n_tct,
                                 const double* tct, double tct_max, const double* mi, const double* coef);
+/* Reverse-rate data of ReReac = 1 or 2: rev [nr][5], per reaction the line
+ * read_Reac reads after the Arrhenius lines (CHEM_reactions.f90:46-47):
+ * (A_r, b_r, E_r, -, -) for ReReac = 1, (A1r .. A5r) for ReReac = 2. */
+int32_t ocfd_chem_set_reverse_rates(const double* rev);
 
 /* Chemical_time_advance over a batch (OCFD_Time_Adv.f90:169-196 calling
  * chemical_timeAdv, CHEM_reactions.f90:374-390, per cell): nsub substeps of

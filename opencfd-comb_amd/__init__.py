@@ -133,8 +133,11 @@ def load_library(path=None):
         "ocfd_set_transport": (i32, [vp, vp, vp]), "ocfd_set_bc_jet": (i32, [vp]),
         "ocfd_ns_step": (i32, [dbl, i32, i32, i32]), "ocfd_chemical_time_advance": (i32, [dbl, i32]),
         "ocfd_comput_T_P": (i32, []), "ocfd_du_species": (i32, []), "ocfd_species_time_adv": (i32, [i32, dbl]),
+        "ocfd_du_scaler_invis": (i32, [i32]), "ocfd_du_scaler_viscous": (i32, [i32]),
+        "ocfd_scalar_time_adv_rk3": (i32, [i32, i32, dbl]),
         "ocfd_set_chem_method": (i32, [i32, dbl, dbl, dbl, i32, i32]), "ocfd_cfl_dt": (i32, [dbl, P(dbl)]),
         "ocfd_chem_set_mechanism": (i32, [i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, i32, vp, dbl, vp, vp]),
+        "ocfd_chem_set_reverse_rates": (i32, [vp]),
         "ocfd_chem_timeadv_batch": (i32, [ctypes.c_int64, vp, vp, vp, i32, i32]),
         "ocfd_chem_adaptive_batch": (i32, [ctypes.c_int64, vp, vp, vp, dbl, dbl, dbl, i32, vp, i32]),
         "ocfd_chem_synchronize": (i32, []), "ocfd_chem_stream": (vp, []), "ocfd_chem_set_stream": (i32, [vp]),
@@ -301,6 +304,20 @@ class FlowSolver:
     def Scalar_time_adv_RK3(self, krk, dt):
         """Species right-hand sides and update of stage krk (OCFD_Time_Adv.f90:64-71)."""
         self._check(self.lib.ocfd_species_time_adv(int(krk), float(dt)))
+
+    def du_scaler_invis(self, m):
+        """Inviscid right-hand side of species m (1-based) into DUS[..., m-1]
+        (du_scaler_invis, OCFD_scaler_inviscous.f90:1-84)."""
+        self._check(self.lib.ocfd_du_scaler_invis(int(m)))
+
+    def du_scaler_viscous(self, m):
+        """DUS[..., m-1] += diffusive right-hand side of species m; Qp zeroed at
+        m = 1 (du_scaler_viscous, OCFD_scaler_viscous.f90:4-79)."""
+        self._check(self.lib.ocfd_du_scaler_viscous(int(m)))
+
+    def scalar_time_adv_rk3(self, krk, m, dt):
+        """Scalar_time_adv_RK3(KRK, m, dus, dt) of species m (OCFD_Time_Adv.f90:134-166)."""
+        self._check(self.lib.ocfd_scalar_time_adv_rk3(int(krk), int(m), float(dt)))
 
     def save_stage0(self):
         self._check(self.lib.ocfd_save_stage0())

@@ -46,6 +46,9 @@ class ChemBatch:
             m.n_spec, m.n_reac, m.rereac, _vp(arrs["af"]), _vp(arrs["tb"]), _vp(arrs["fc"]), _vp(arrs["arr"]),
             _vp(arrs["vf"]), _vp(arrs["vr"]), _vp(arrs["eff"]), m.n_tct, _vp(arrs["tct"]), float(m.tct_max),
             _vp(arrs["mi"]), _vp(arrs["coef"])))
+        if m.rereac != 0:
+            arrs["rev"] = f64(m.reverse)
+            self._check(self.lib.ocfd_chem_set_reverse_rates(_vp(arrs["rev"])))
         self.n_spec = m.n_spec
 
     def _check(self, rc):

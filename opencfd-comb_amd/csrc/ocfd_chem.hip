@@ -36,6 +36,8 @@
 
 struct ChemMech {
   int ns, nr, n_tct;
+  int rereac;                      // 0 Gibbs Kc, 1 Arrhenius reverse rates, 2 Park fit of Kc
+  double rev[CH_NR_MAX][5];        // ReReac 1: A_r, b_r, E_r; ReReac 2: A1r..A5r
   int af_type[CH_NR_MAX], tb[CH_NR_MAX], sgm[CH_NR_MAX];
   double fc[CH_NR_MAX], arr[CH_NR_MAX][2][3], eff[CH_NR_MAX][CH_NS];
   int vf[CH_NR_MAX][CH_NS], vr[CH_NR_MAX][CH_NS];
@@ -69,6 +71,7 @@ struct HeavyBuf {
 namespace {
 struct ChemCtx {
   bool ready = false;
+  bool rev_ready = false;  // reverse-rate data present (always for ReReac = 0)
   ChemMech h{};
   hipStream_t stream = nullptr, own = nullptr;
   unsigned long long* counter = nullptr;  // [0] queue head, [1] unfinished, [2] surrogate fallbacks
@@ -186,8 +189,9 @@ __device__ __forceinline__ double chem_gibbs(Mech m, int k, double T, double log
 // (:230-272).  ri in mol/cm3/s; A[i][j] = d ri / d cj if JAC.
 // Constants of one reaction (read_Reac, CHEM_reactions.f90:15-71).
 struct RxnConst {
-  int af_type, tb, sgm;
+  int af_type, tb, sgm, rereac;
   double fc, a0, b0, e0, a1, b1, e1;
+  double r[5];                     // reverse-rate data (ReReac 1 or 2)
   double eff[CH_NS];
   int vf[CH_NS], vr[CH_NS];
 };
@@ -204,6 +208,9 @@ __device__ __forceinline__ RxnConst load_rxn(Mech m, int k) {
   R.a1 = m->arr[k][1][0];
   R.b1 = m->arr[k][1][1];
   R.e1 = m->arr[k][1][2];
+  R.rereac = m->rereac;
+#pragma unroll
+  for (int a = 0; a < 5; a++) R.r[a] = m->rev[k][a];
 #pragma unroll
   for (int i = 0; i < CH_NS; i++) {
     R.eff[i] = m->eff[k][i];
@@ -219,11 +226,14 @@ __device__ __forceinline__ RxnConst load_rxn(Mech m, int k) {
 __device__ __forceinline__ void chem_reaction_c(const RxnConst& R, double T, const double* gi, const double* ci,
                                                 double& wf1, double& wr1) {
   const double Rs = CH_R0 / CH_ATM * 1.0e6;
-  // comput_KjX (CHEM_reactions.f90:76-103)
-  double dgi = 0.0;
+  // comput_KjX (CHEM_reactions.f90:76-103), used by ReReac = 0 only
+  double kjx = 0.0;
+  if (R.rereac == 0) {
+    double dgi = 0.0;
 #pragma unroll
-  for (int i = 0; i < CH_NS; i++) dgi = dgi + gi[i] * (R.vr[i] - R.vf[i]);
-  const double kjx = exp(-dgi / T) * ipow(Rs * T, -R.sgm);
+    for (int i = 0; i < CH_NS; i++) dgi = dgi + gi[i] * (R.vr[i] - R.vf[i]);
+    kjx = exp(-dgi / T) * ipow(Rs * T, -R.sgm);
+  }
   double Ctrb = 0.0;
   if (R.tb == 1) {
 #pragma unroll
@@ -247,7 +257,17 @@ __device__ __forceinline__ void chem_reaction_c(const RxnConst& R, double T, con
     const double F = pow(10.0, lfc / (1.0 + q2((lp + ac) / (an - 0.14 * (lp + ac)))));
     wf = wf2 * Prs / (1.0 + Prs) * F;
   }
-  wr = wf / kjx;
+  if (R.rereac == 0) {
+    wr = wf / kjx;
+  } else if (R.rereac == 1) {
+    wr = R.r[0] * pow(T, R.r[1]) * exp(-R.r[2] / T);      // Arrhenius reverse rate (:180-181)
+  } else {
+    // ParkCurveFit (:113-118) with the 1e-30 floor (:182-186)
+    const double z = 10000.0 / T;
+    double kp = exp(R.r[0] + R.r[1] * z + R.r[2] * q2(z) + R.r[3] * q3(z) + R.r[4] * q4(z));
+    if (kp < 1.0e-30) kp = 1.0e-30;
+    wr = wf / kp;
+  }
   if (R.tb == 1 && R.af_type != 1) {
     wf = wf * Ctrb;
     wr = wr * Ctrb;
@@ -279,10 +299,12 @@ __device__ __forceinline__ double chem_dA(Mech m, int k, int j, double wf1, doub
 // the fully unrolled i/j loops keep ri and A in registers.
 template <bool JAC>
 __device__ void chem_rates(Mech m, const double* ci, double T, double* ri, double (*A)[CH_NS]) {
-  const double logT = log(T);
   double gi[CH_NS];
+  if (m->rereac == 0) {
+    const double logT = log(T);
 #pragma unroll
-  for (int i = 0; i < CH_NS; i++) gi[i] = chem_gibbs(m, i, T, logT);
+    for (int i = 0; i < CH_NS; i++) gi[i] = chem_gibbs(m, i, T, logT);
+  }
 #pragma unroll
   for (int i = 0; i < CH_NS; i++) {
     ri[i] = 0.0;
@@ -1068,6 +1090,7 @@ __global__ __launch_bounds__(SurrTile<V>::NW * 64, SurrTile<V>::OCC) void k_surr
 // the constant bank, counters cleared (on CX.stream)
 int surr_prepare(long n, const char* who) {
   if (!CX.ready || !CX.surr_ready) return cfail(std::string(who) + ": mechanism or surrogate not set");
+  if (!CX.rev_ready) return cfail(std::string(who) + ": ReReac 1/2 needs ocfd_chem_set_reverse_rates");
   const MlpDevice* net = ocfd_mlp_device_view();
   if (!net) return cfail(std::string(who) + ": no network (ocfd_mlp_create)");
   const int L = net->n_layers;
@@ -1273,6 +1296,7 @@ int launch_adaptive(long n, double* rho, double* E, const double* dt, const Chem
 int chem_adaptive_batch(long n, double* rho, double* E, const double* dt, double rtol_y, double y_floor, double rtol_t,
                         int max_sub, int* nsub_out, int scheduler, bool order) {
   if (!CX.ready) return cfail("ocfd_chem_adaptive_batch: call ocfd_chem_set_mechanism first");
+  if (!CX.rev_ready) return cfail("ocfd_chem_adaptive_batch: ReReac 1/2 needs ocfd_chem_set_reverse_rates");
   if (max_sub < 1 || !(rtol_y > 0.0) || !(y_floor >= 0.0) || !(rtol_t > 0.0))
     return cfail("ocfd_chem_adaptive_batch: bad controller parameters");
   if (scheduler < 0 || scheduler > 2) return cfail("ocfd_chem_adaptive_batch: scheduler 0, 1 or 2");
@@ -1321,10 +1345,12 @@ int32_t ocfd_chem_set_mechanism(int32_t ns, int32_t nr, int32_t rereac, const in
                                 const double* tct, double tct_max, const double* mi, const double* coef) {
   if (ns != CH_NS) return cfail("ocfd_chem_set_mechanism: this build is compiled for 9 species");
   if (nr > CH_NR_MAX || n_tct > CH_NT_MAX || n_tct < 1) return cfail("ocfd_chem_set_mechanism: too many reactions or ranges");
-  if (rereac != 0) return cfail("ocfd_chem_set_mechanism: only ReReac=0 (Gibbs equilibrium constants) is supported");
+  if (rereac < 0 || rereac > 2) return cfail("ocfd_chem_set_mechanism: ReReac must be 0, 1 or 2");
   ChemMech& h = CX.h;
   h = ChemMech{};
   h.ns = ns;
+  h.rereac = rereac;
+  CX.rev_ready = rereac == 0;
   h.nr = nr;
   h.n_tct = n_tct;
   for (int j = 0; j < nr; j++) {
@@ -1374,10 +1400,25 @@ int32_t ocfd_chem_set_mechanism(int32_t ns, int32_t nr, int32_t rereac, const in
   return 0;
 }
 
+// Reverse-rate data of ReReac = 1 or 2, rev[j*5 .. j*5+4] per reaction (the
+// line read_Reac reads after the Arrhenius lines, CHEM_reactions.f90:46-47).
+int32_t ocfd_chem_set_reverse_rates(const double* rev) {
+  if (!CX.ready) return cfail("ocfd_chem_set_reverse_rates: call ocfd_chem_set_mechanism first");
+  if (!rev) return cfail("ocfd_chem_set_reverse_rates: null data");
+  ChemMech& h = CX.h;
+  for (int j = 0; j < h.nr; j++)
+    for (int a = 0; a < 5; a++) h.rev[j][a] = rev[j * 5 + a];
+  if (hipDeviceSynchronize() != hipSuccess) return cfail("ocfd_chem_set_reverse_rates: device error");
+  if (hipMemcpyToSymbol(HIP_SYMBOL(cm), &h, sizeof(ChemMech)) != hipSuccess) return cfail("hipMemcpyToSymbol failed");
+  CX.rev_ready = true;
+  return 0;
+}
+
 // Chemical_time_advance over a batch of independent cells (OCFD_Time_Adv.f90:169-196
 // calls chemical_timeAdv per cell); device pointers; nsub substeps of dt/nsub.
 int32_t ocfd_chem_timeadv_batch(int64_t n, double* rho, double* E, const double* dt, int32_t mode, int32_t nsub) {
   if (!CX.ready) return cfail("ocfd_chem_timeadv_batch: call ocfd_chem_set_mechanism first");
+  if (!CX.rev_ready) return cfail("ocfd_chem_timeadv_batch: ReReac 1/2 needs ocfd_chem_set_reverse_rates");
   if (n <= 0) return 0;
   if (nsub < 1) return cfail("ocfd_chem_timeadv_batch: nsub must be >= 1");
   dim3 g((unsigned)((n + 127) / 128)), b(128);

@@ -1983,13 +1983,13 @@ void enq_stage(int krk, double dt1) {
 }
 
 template <int DIR>
-void launch_species_sweep(hipStream_t st) {
+void launch_species_sweep(hipStream_t st, int m0, int m1) {
   const DevState& S = G.S;
   constexpr int NP = SpTile<DIR>::NP, TS = SpTile<DIR>::TS;
   const long npen = DIR == 0 ? (long)S.ny * S.nz : DIR == 1 ? (long)S.nx * S.nz : (long)S.nx * S.ny;
   dim3 g((unsigned)((S.n[DIR] + TS - 1) / TS), (unsigned)((npen + NP - 1) / NP));
   static const char* names[3] = {"k_species_sweep_x", "k_species_sweep_y", "k_species_sweep_z"};
-  launch_s(st, names[DIR], k_species_sweep<DIR>, g, dim3(256), S);
+  launch_s(st, names[DIR], k_species_sweep<DIR>, g, dim3(256), S, m0, m1);
 }
 
 // species right-hand sides (krk = 0: leave them in dus) or the species RK3
@@ -1999,17 +1999,42 @@ void launch_species_sweep(hipStream_t st) {
 void enq_species(int krk, double dt1, hipStream_t st = nullptr, hipEvent_t after_prep = nullptr) {
   const DevState& S = G.S;
   if (!st) st = G.stream;
-  launch_species_sweep<0>(st);
-  launch_species_sweep<1>(st);
-  launch_species_sweep<2>(st);
+  launch_species_sweep<0>(st, 0, S.ns);
+  launch_species_sweep<1>(st, 0, S.ns);
+  launch_species_sweep<2>(st, 0, S.ns);
   if (S.viscous) {
-    launch_s(st, "k_species_conc", k_species_conc, grid1(S.ni), dim3(256), S);
+    launch_s(st, "k_species_conc", k_species_conc, grid1(S.ni), dim3(256), S, 0, S.ns);
     launch_s(st, "k_species_visc_prep", S.cd8 ? k_species_visc_prep<1> : k_species_visc_prep<0>, grid1(S.ni),
-             dim3(256), S);
+             dim3(256), S, 0, S.ns);
   }
   if (after_prep) hipEventRecord(after_prep, st);
   launch_s(st, "k_species_rk", S.cd8 ? k_species_rk<1> : k_species_rk<0>, grid1(S.ni), dim3(256), S, krk, dt1,
-           S.viscous);
+           S.viscous, 0, S.ns);
+}
+
+// One species m (0-based) of the reference's species loop
+// (OCFD_Time_Adv.f90:64-71): part 0 du_scaler_invis (Dus(m) = inviscid
+// right-hand side), part 1 du_scaler_viscous (Dus(m) += div of the diffusive
+// flux; Qp zeroed at m = 0, accumulated after), part 2 Scalar_time_adv_RK3
+// (di(m) from din(m) and Dus(m)).  In that order over every species they
+// give enq_species bit for bit.
+void enq_species_one(int part, int m, int krk, double dt1) {
+  const DevState& S = G.S;
+  const hipStream_t st = G.stream;
+  if (part == 0) {
+    launch_species_sweep<0>(st, m, m + 1);
+    launch_species_sweep<1>(st, m, m + 1);
+    launch_species_sweep<2>(st, m, m + 1);
+  } else if (part == 1) {
+    launch_s(st, "k_species_conc", k_species_conc, grid1(S.ni), dim3(256), S, m, m + 1);
+    launch_s(st, "k_species_visc_prep", S.cd8 ? k_species_visc_prep<1> : k_species_visc_prep<0>, grid1(S.ni),
+             dim3(256), S, m, m + 1);
+    launch_s(st, "k_species_rk", S.cd8 ? k_species_rk<1> : k_species_rk<0>, grid1(S.ni), dim3(256), S, 0, 0.0, 1,
+             m, m + 1);
+  } else {
+    launch_s(st, "k_species_rk", S.cd8 ? k_species_rk<1> : k_species_rk<0>, grid1(S.ni), dim3(256), S, krk, dt1, 0,
+             m, m + 1);
+  }
 }
 
 // one RK3 stage for N_SPEC > 1 (OCFD_Time_Adv.f90:33-91): transport, sensor,
@@ -2656,6 +2681,32 @@ int32_t ocfd_species_time_adv(int32_t krk, double dt) {
   if (G.S.ns < 2) return fail("ocfd_species_time_adv: needs N_SPEC > 1");
   if (krk < 1 || krk > 3) return fail("ocfd_species_time_adv: KRK must be 1, 2 or 3");
   enq_species(krk, dt);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+int32_t ocfd_du_scaler_invis(int32_t m) {
+  if (check_ready()) return 1;
+  if (G.S.ns < 2) return fail("ocfd_du_scaler_invis: needs N_SPEC > 1");
+  if (m < 1 || m > G.S.ns) return fail("ocfd_du_scaler_invis: species index must be 1..N_SPEC");
+  enq_species_one(0, m - 1, 0, 0.0);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+int32_t ocfd_du_scaler_viscous(int32_t m) {
+  if (check_ready()) return 1;
+  if (G.S.ns < 2) return fail("ocfd_du_scaler_viscous: needs N_SPEC > 1");
+  if (m < 1 || m > G.S.ns) return fail("ocfd_du_scaler_viscous: species index must be 1..N_SPEC");
+  if (!G.S.viscous) return fail("ocfd_du_scaler_viscous: the run is inviscid (IF_Viscous = 0)");
+  enq_species_one(1, m - 1, 0, 0.0);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+int32_t ocfd_scalar_time_adv_rk3(int32_t krk, int32_t m, double dt) {
+  if (check_ready()) return 1;
+  if (G.S.ns < 2) return fail("ocfd_scalar_time_adv_rk3: needs N_SPEC > 1");
+  if (krk < 1 || krk > 3) return fail("ocfd_scalar_time_adv_rk3: KRK must be 1, 2 or 3");
+  if (m < 1 || m > G.S.ns) return fail("ocfd_scalar_time_adv_rk3: species index must be 1..N_SPEC");
+  enq_species_one(2, m - 1, krk, dt);
   HIPCHK(hipGetLastError());
   return 0;
 }

@@ -306,7 +306,7 @@ template <> struct SpTile<1> { static constexpr int NP = 8, TS = 32; };
 template <> struct SpTile<2> { static constexpr int NP = 8, TS = 32; };
 
 template <int DIR>
-__global__ __launch_bounds__(256) void k_species_sweep(DevState S) {
+__global__ __launch_bounds__(256) void k_species_sweep(DevState S, int m0, int m1) {
   constexpr int NP = SpTile<DIR>::NP, TS = SpTile<DIR>::TS;
   constexpr int NPT = TS + 8, NIF = TS + 1;
   // one species at a time through small LDS buffers: the split velocity,
@@ -357,7 +357,8 @@ __global__ __launch_bounds__(256) void k_species_sweep(DevState S) {
     sK[t] = (uint8_t)recon_kind(S, DIR, s_if, side, mask);
   }
   const double h1 = S.h_1[DIR];
-  for (int m = 0; m < ns; m++) {
+  (void)ns;
+  for (int m = m0; m < m1; m++) {
     const double* Di = S.di + (long)m * S.nh;
     for (int t = tid; t < NP * NPT; t += blockDim.x) {
       const int pl = DIR == 0 ? t / NPT : t % NP;
@@ -410,20 +411,24 @@ __global__ __launch_bounds__(256) void k_species_sweep(DevState S) {
 // order (zeroed at the first species).  c_m is formed once per cell by
 // k_species_conc (the reference's Ev1 = fs/d, :25-31).
 // ===========================================================================
-__global__ __launch_bounds__(256) void k_species_conc(DevState S) {
+__global__ __launch_bounds__(256) void k_species_conc(DevState S, int m0, int m1) {
   const long c = cell_index();
   if (c >= S.ni) return;
   int i, j, k;
   cell_ijk(S, c, i, j, k);
   const long q = hq(S, i, j, k);
   const double d = S.d[q];
-  for (int m = 0; m < S.ns; m++) S.cs[c + (long)m * S.ni] = S.di[q + (long)m * S.nh] / d;
+  for (int m = m0; m < m1; m++) S.cs[c + (long)m * S.ni] = S.di[q + (long)m * S.nh] / d;
 }
 
 // INT: every stencil of the cell is central (no wrap, no closure), so each
 // species' derivatives are straight-line code whose loads issue together.
+// Species m0 .. m1-1; Qp starts from zero at the first species (m0 = 0) and
+// from its stored value otherwise, so species taken one at a time add up in
+// the same order as all at once.
 template <int CD, bool INT>
-__device__ __forceinline__ void species_visc_prep_cell(const DevState& S, long c, long q, int i, int j, int k) {
+__device__ __forceinline__ void species_visc_prep_cell(const DevState& S, long c, long q, int i, int j, int k, int m0,
+                                                       int m1) {
   const double T0 = S.T[q], d0 = S.d[q];
   const double Akx = S.Akx[q], Aky = S.Aky[q], Akz = S.Akz[q];
   const double Aix = S.Aix[q], Aiy = S.Aiy[q], Aiz = S.Aiz[q];
@@ -431,9 +436,14 @@ __device__ __forceinline__ void species_visc_prep_cell(const DevState& S, long c
   const double Akx1 = S.Akx1[q], Aky1 = S.Aky1[q], Akz1 = S.Akz1[q];
   const double Aix1 = S.Aix1[q], Aiy1 = S.Aiy1[q], Aiz1 = S.Aiz1[q];
   const double Asx1 = S.Asx1[q], Asy1 = S.Asy1[q], Asz1 = S.Asz1[q];
-  double Qpx = 0.0, Qpy = 0.0, Qpz = 0.0;
   const long ni = S.ni;
-  for (int m = 0; m < S.ns; m++) {
+  double Qpx = 0.0, Qpy = 0.0, Qpz = 0.0;
+  if (m0 > 0) {
+    Qpx = S.Qp[c];
+    Qpy = S.Qp[c + ni];
+    Qpz = S.Qp[c + 2 * ni];
+  }
+  for (int m = m0; m < m1; m++) {
     const double* C = S.cs + (long)m * ni + c;
     auto dc = [&](int dir, int s) {
       const long st = S.si[dir];
@@ -466,7 +476,7 @@ __device__ __forceinline__ void species_visc_prep_cell(const DevState& S, long c
 }
 
 template <int CD>
-__global__ __launch_bounds__(256) void k_species_visc_prep(DevState S) {
+__global__ __launch_bounds__(256) void k_species_visc_prep(DevState S, int m0, int m1) {
   constexpr int R = CD == 1 ? 4 : 3;
   const long c = cell_index();
   if (c >= S.ni) return;
@@ -474,9 +484,9 @@ __global__ __launch_bounds__(256) void k_species_visc_prep(DevState S) {
   cell_ijk(S, c, i, j, k);
   const long q = hq(S, i, j, k);
   if (wave_all(central_cell(i, S.nx, R) && central_cell(j, S.ny, R) && central_cell(k, S.nz, R)))
-    species_visc_prep_cell<CD, true>(S, c, q, i, j, k);
+    species_visc_prep_cell<CD, true>(S, c, q, i, j, k, m0, m1);
   else
-    species_visc_prep_cell<CD, false>(S, c, q, i, j, k);
+    species_visc_prep_cell<CD, false>(S, c, q, i, j, k, m0, m1);
 }
 
 // Species update: Dus += div(diffusive flux) Ajac (OCFD_scaler_viscous.f90:67-77)
@@ -484,7 +494,7 @@ __global__ __launch_bounds__(256) void k_species_visc_prep(DevState S) {
 // krk = 0 stores the complete right-hand side in dus instead.
 template <int CD, bool INT>
 __device__ __forceinline__ void species_rk_cell(const DevState& S, long c, long q, int i, int j, int k, int krk,
-                                                double dt, int visc) {
+                                                double dt, int visc, int m0, int m1) {
   const long ni = S.ni;
   const double ajac = S.Ajac[q];
   auto dv = [&](const double* F, int dir, int s) {
@@ -501,7 +511,7 @@ __device__ __forceinline__ void species_rk_cell(const DevState& S, long c, long 
   };
   const double ra = krk == 2 ? 3.0 / 4.0 : 1.0 / 3.0;
   const double rb = krk == 2 ? 1.0 / 4.0 : 2.0 / 3.0;
-  for (int m = 0; m < S.ns; m++) {
+  for (int m = m0; m < m1; m++) {
     const long qm = q + (long)m * S.nh, cm = c + (long)m * ni;
     // the stage's old values, read with the stencil loads
     const double dn = krk >= 1 ? S.din[cm] : 0.0;
@@ -523,7 +533,7 @@ __device__ __forceinline__ void species_rk_cell(const DevState& S, long c, long 
 }
 
 template <int CD>
-__global__ __launch_bounds__(256) void k_species_rk(DevState S, int krk, double dt, int visc) {
+__global__ __launch_bounds__(256) void k_species_rk(DevState S, int krk, double dt, int visc, int m0, int m1) {
   constexpr int R = CD == 1 ? 4 : 3;
   const long c = cell_index();
   if (c >= S.ni) return;
@@ -531,9 +541,9 @@ __global__ __launch_bounds__(256) void k_species_rk(DevState S, int krk, double 
   cell_ijk(S, c, i, j, k);
   const long q = hq(S, i, j, k);
   if (wave_all(central_cell(i, S.nx, R) && central_cell(j, S.ny, R) && central_cell(k, S.nz, R)))
-    species_rk_cell<CD, true>(S, c, q, i, j, k, krk, dt, visc);
+    species_rk_cell<CD, true>(S, c, q, i, j, k, krk, dt, visc, m0, m1);
   else
-    species_rk_cell<CD, false>(S, c, q, i, j, k, krk, dt, visc);
+    species_rk_cell<CD, false>(S, c, q, i, j, k, krk, dt, visc, m0, m1);
 }
 
 // Rescaling_di (Comput_flow_variables.f90:101-121) and comput_T_P (:32-46)

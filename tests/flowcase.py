@@ -90,3 +90,40 @@ def setup_jet(engine, case):
 
 
 SPECIES_PRIMS = PRIMS + ("DI",)
+
+
+def curvilinear_box(n, a=0.05, L=2.0):
+    """A periodic box of side L under the non-separable smooth map
+    X = L (s_i + a sin 2pi s_j sin 2pi s_k), Y = L (s_j + a sin 2pi s_k sin 2pi s_i),
+    Z = L (s_k + a sin 2pi s_i sin 2pi s_j), s = (index - 1)/n, given as 3-D
+    coordinate arrays (the GRID3D input of init_mesh).  Returns the mesh, the
+    span and the exact d(X, Y, Z)/d(xi, eta, zeta) with xi = (i - 1) h,
+    h = 1/(n - 1) (OCFD_readpara.f90:351-353)."""
+    load_package()
+    from ocfd_amd import cases
+    s = np.arange(n) / n
+    S1, S2, S3 = np.meshgrid(s, s, s, indexing="ij")
+    w = 2 * np.pi
+    sn = lambda t: np.sin(w * t)
+    cs = lambda t: np.cos(w * t)
+    X = L * (S1 + a * sn(S2) * sn(S3))
+    Y = L * (S2 + a * sn(S3) * sn(S1))
+    Z = L * (S3 + a * sn(S1) * sn(S2))
+    ds = (n - 1) / n            # d s / d xi
+    Jm = np.empty((3, 3) + X.shape)   # Jm[c][e] = d coord c / d index-coordinate e
+    Jm[0, 0] = L * ds * np.ones_like(X)
+    Jm[0, 1] = L * a * w * cs(S2) * sn(S3) * ds
+    Jm[0, 2] = L * a * w * sn(S2) * cs(S3) * ds
+    Jm[1, 0] = L * a * w * sn(S3) * cs(S1) * ds
+    Jm[1, 1] = L * ds * np.ones_like(X)
+    Jm[1, 2] = L * a * w * cs(S3) * sn(S1) * ds
+    Jm[2, 0] = L * a * w * cs(S1) * sn(S2) * ds
+    Jm[2, 1] = L * a * w * sn(S1) * cs(S2) * ds
+    Jm[2, 2] = L * ds * np.ones_like(X)
+    mesh = []
+    for A in (X, Y, Z):
+        h = cases.halo_zeros(n, n, n, 1.0)
+        cases.interior(h)[...] = A
+        mesh.append(h)
+    span = np.array([L, 0, 0, 0, L, 0, 0, 0, L], dtype=np.float64)
+    return mesh, span, Jm

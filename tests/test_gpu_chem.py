@@ -223,3 +223,85 @@ def test_conservation_at_scale(env):
     scale = torch.abs(e0) + (torch.abs(r0 * hf)).sum(0)
     assert torch.max(torch.abs(et1 - et0) / scale).item() <= 1e-12
     assert torch.isfinite(rho).all() and int(nsub.min()) >= 1
+
+
+def _reverse_data(M, orc, rereac):
+    """Reverse-rate data that reproduce the Gibbs equilibrium of this
+    mechanism: ReReac = 2, a Park fit of ln Kc in z = 10000/T; ReReac = 1, an
+    Arrhenius fit of k_r = k_f / Kc (without third bodies) over 800..2800 K."""
+    Ts = np.linspace(800.0, 2800.0, 60)
+    X = np.zeros(M.n_spec)
+    X[M.index("O2")], X[M.index("N2")], X[M.index("H2")] = 0.15, 0.55, 0.3
+    ci = X * 101300.0 / (8.314 * 1500.0) * 1e-6
+    kc, kr = [], []
+    for T in Ts:
+        wf, wr, kj = orc.rate_coef(ci, T)
+        ctrb = np.where((M.third_body == 1) & (M.af_type != 1), M.eff @ ci, 1.0)
+        kc.append(kj)
+        kr.append(wr / ctrb)
+    kc, kr = np.log(np.array(kc)), np.log(np.array(kr))
+    rev = np.zeros((M.n_reac, 5))
+    if rereac == 2:
+        V = np.stack([(1e4 / Ts) ** p for p in range(5)], axis=1)
+        rev[:] = np.linalg.lstsq(V, kc, rcond=None)[0].T
+    else:
+        V = np.stack([np.ones_like(Ts), np.log(Ts), -1.0 / Ts], axis=1)
+        c = np.linalg.lstsq(V, kr, rcond=None)[0].T
+        rev[:, 0], rev[:, 1], rev[:, 2] = np.exp(c[:, 0]), c[:, 1], c[:, 2]
+    return rev
+
+
+@pytest.mark.parametrize("rereac", [1, 2])
+def test_reverse_rate_models_match_oracle(env, rereac):
+    """ReReac = 1 (Arrhenius reverse rates) and 2 (Park fit of Kc, floored at
+    1e-30), Reaction_rate_coef CHEM_reactions.f90:176-188: the point-implicit
+    integrator (5 substeps) and the adaptive one (scheduler 2, the lane queue
+    plus the cooperative kernel) against the oracle with the same data."""
+    import copy
+    import oracle
+    from ocfd_amd import chemistry
+    pkg, chem, b, M, orc = env
+    M2 = copy.deepcopy(M)
+    M2.rereac = rereac
+    M2.reverse = _reverse_data(M, orc, rereac)
+    try:
+        ch2 = chemistry.ChemBatch(M2)
+        o2 = oracle.ChemOracle(pkg, M2)
+        rho, E, d = to_dev(b["rho"], b["E"], b["dt"])
+        ch2.chemical_timeAdv(rho, E, d, mode=-1, nsub=5)
+        ch2.synchronize()
+        ro, eo = o2.batch(-1, b["rho"].T, b["E"], b["dt"], 5)
+        dr, de = report(f"ReReac {rereac} point-implicit", rho.cpu().numpy(), E.cpu().numpy(), ro.T, eo)
+        assert dr <= RTOL_RHO and de <= RTOL_E
+        rho, E, d = to_dev(b["rho"], b["E"], b["dt"] * 20.0)
+        ns = ch2.adaptive(rho, E, d, rtol_y=0.05, y_floor=1e-6, rtol_t=0.01, max_sub=2000, scheduler=2)
+        ch2.synchronize()
+        ro, eo, no = o2.adaptive_batch(b["rho"].T, b["E"], b["dt"] * 20.0, 0.05, 1e-6, 0.01, 2000)
+        same = ns.cpu().numpy() == no
+        assert same.mean() >= 0.999
+        dr, de = report(f"ReReac {rereac} adaptive", rho.cpu().numpy()[:, same], E.cpu().numpy()[same],
+                        ro.T[:, same], eo[same])
+        assert dr <= 1e3 * RTOL_RHO and de <= 1e3 * RTOL_E
+    finally:
+        chemistry.ChemBatch(M)          # the module's other tests use ReReac = 0
+        oracle.ChemOracle(pkg, M)
+
+
+def test_reverse_rate_data_required(env):
+    """ReReac = 1 or 2 without reverse data: the batch calls refuse."""
+    import copy
+    from ocfd_amd import chemistry, OcfdError
+    pkg, chem, b, M, orc = env
+    lib = chem.lib
+    k = chem._keep
+    try:
+        rc = lib.ocfd_chem_set_mechanism(M.n_spec, M.n_reac, 2, *[chemistry._vp(k[n]) for n in
+                                         ("af", "tb", "fc", "arr", "vf", "vr", "eff")], M.n_tct,
+                                         chemistry._vp(k["tct"]), float(M.tct_max), chemistry._vp(k["mi"]),
+                                         chemistry._vp(k["coef"]))
+        assert rc == 0
+        rho, E, d = to_dev(b["rho"][:, :8], b["E"][:8], b["dt"][:8])
+        with pytest.raises(OcfdError, match="reverse"):
+            chem.chemical_timeAdv(rho, E, d, mode=-1, nsub=1)
+    finally:
+        chemistry.ChemBatch(M)

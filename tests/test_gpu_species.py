@@ -254,3 +254,49 @@ def test_rk3_step_with_ghost_cells(pkg):
     o.flow_time_advance(dt)
     compare_all(g, o)
     g.close()
+
+
+def test_per_species_entries_equal_fused_step(pkg):
+    """The reference's species loop through the per-species entries
+    (du_scaler_invis, du_scaler_viscous, Scalar_time_adv_RK3 for m = 1..N_SPEC,
+    OCFD_Time_Adv.f90:64-71) gives the fused step bit for bit; after the
+    first species' pieces, Dus(1) and the partial Qp equal the oracle's."""
+    import oracle
+    case = fc.small_jet()
+    dt = case[5]["dt"]
+    ns = case[0].n_spec
+    a = pkg.FlowSolver(case[0])
+    fc.setup_jet(a, case)
+    a.flow_time_advance(dt)
+    fa = {n: a.download(n) for n in ("F", "DI", "T", "P")}
+    a.close()
+    o = oracle.Oracle(pkg, case[0])
+    fc.setup_jet(o, case)
+    o.comput_Amu_Amk_AMD()
+    o.exchange_all()
+    dus_o, _ = o.du_species()
+    b = pkg.FlowSolver(case[0])
+    fc.setup_jet(b, case)
+    b.save_stage0()
+    for krk in (1, 2, 3):
+        b.stage_begin(krk)
+        for m in range(1, ns + 1):
+            b.du_scaler_invis(m)
+            b.du_scaler_viscous(m)
+            if krk == 1:
+                dus = b.download("DUS")[..., m - 1]
+                d, same = fc.compare(dus, dus_o[..., m - 1])
+                assert d <= 1e-12, (m, d)
+            b.scalar_time_adv_rk3(krk, m, dt)
+        b.du_inviscous()
+        b.du_viscous()
+        b.OCFD_time_adv_RK3(krk, dt)
+        b.comput_flow_variables()
+        b.OCFD_bc()
+    for n, v in fa.items():
+        assert np.array_equal(v.view(np.uint64), b.download(n).view(np.uint64)), n
+    with pytest.raises(pkg.OcfdError):
+        b.du_scaler_invis(ns + 1)
+    with pytest.raises(pkg.OcfdError):
+        b.scalar_time_adv_rk3(4, 1, dt)
+    b.close()
