@@ -303,6 +303,22 @@ int32_t ocfd_negative_et_count(void);
  * use.  Any pointer may be NULL.  Fails when no certified mean has run. */
 int32_t ocfd_hybrid_certificate(int64_t* uncertain, double* margin, double* eta);
 
+/* ---- diagnostics (device reductions of the resident state) --------------
+ * show_flow_msg (code/flow-solver/OCFD_Time_Adv.f90:200-239): E0[0] = mean
+ * over the interior of f5/Ajac (total energy), E0[1] = mean of
+ * d(u^2+v^2+w^2)/2/Ajac (kinetic energy).
+ * ana_box (code/others/OCFD_ANA.f90:197-303), the statistics of isotropic
+ * turbulence from the current primitives and Amu (call
+ * ocfd_comput_amu_amk first if no stage has run since the state changed):
+ * out[0..17] = Re_lamda, Re_lamdax, Amt, vv_rms, u_rms, v_rms, w_rms,
+ * u_skewness, u_flatness, ux_skewness, ux_flatness, E_kinetic, d_rms,
+ * T_aver, Amu_aver, d_aver, Ux_rms, Alamdax (the reference's
+ * statistics.dat columns after tt).  The sums are deterministic trees, so
+ * they differ from the reference's sequential sums by summation order only.
+ * Both synchronise. */
+int32_t ocfd_show_flow_msg(double* E0);
+int32_t ocfd_ana_box(double* out);
+
 /* ---- tile autotuner (no reference counterpart) ---------------------------
  * The inviscid sweeps come in several workgroup tile shapes (pencils per
  * workgroup x cells per pencil).  ocfd_autotune times each shape per

@@ -125,6 +125,7 @@ def load_library(path=None):
         "ocfd_time_adv_rk3": (i32, [i32, dbl]), "ocfd_comput_flow_variables": (i32, []),
         "ocfd_bc": (i32, []), "ocfd_negative_et_count": (i32, []),
         "ocfd_hybrid_certificate": (i32, [P(ctypes.c_int64), P(dbl), P(dbl)]),
+        "ocfd_show_flow_msg": (i32, [vp]), "ocfd_ana_box": (i32, [vp]),
         "ocfd_flux1d_batch": (i32, [i32, i32, ctypes.c_int64, i32, vp, vp, vp, P(i32), P(i32)]), "ocfd_synchronize": (i32, []),
         "ocfd_stream": (vp, []), "ocfd_kernel_count": (i32, []), "ocfd_kernel_name": (ctypes.c_char_p, [i32]),
         "ocfd_kernel_stats": (i32, [i32, P(dbl), P(ctypes.c_int64)]), "ocfd_kernel_stats_reset": (i32, []),
@@ -375,6 +376,24 @@ class FlowSolver:
         self._check(self.lib.ocfd_flux1d_batch(int(side), int(scheme), npen, n, v.data_ptr(), hh.data_ptr(), sp, b, sb))
         self._check(self.lib.ocfd_synchronize())
         return hh
+
+    ANA_BOX_NAMES = ("Re_lamda", "Re_lamdax", "Amt", "vv_rms", "u_rms", "v_rms", "w_rms", "u_skewness",
+                     "u_flatness", "ux_skewness", "ux_flatness", "E_kinetic", "d_rms", "T_aver", "Amu_aver",
+                     "d_aver", "Ux_rms", "Alamdax")
+
+    def show_flow_msg(self):
+        """(mean total energy f5/J, mean kinetic energy) over the interior
+        (show_flow_msg, OCFD_Time_Adv.f90:200-239)."""
+        e = np.zeros(2)
+        self._check(self.lib.ocfd_show_flow_msg(_ptr(e)))
+        return float(e[0]), float(e[1])
+
+    def ana_box(self):
+        """Isotropic-turbulence statistics of ana_box (OCFD_ANA.f90:197-303)
+        as a dict keyed by the reference's statistics.dat column names."""
+        out = np.zeros(18)
+        self._check(self.lib.ocfd_ana_box(_ptr(out)))
+        return dict(zip(self.ANA_BOX_NAMES, (float(x) for x in out)))
 
     def hybrid_certificate(self):
         """(uncertain interfaces, smallest |R/theta - 1|, band eta) of the

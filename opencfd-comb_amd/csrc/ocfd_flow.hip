@@ -578,6 +578,7 @@ __global__ __launch_bounds__(256) void k_sweep_char(DevState S) {
 }
 
 #include "ocfd_species.h"
+#include "ocfd_diag.h"
 
 // ===========================================================================
 // Viscous terms, part 1 (du_viscous, OCFD_viscous.f90:27-95, with the
@@ -1667,6 +1668,8 @@ struct Ctx {
   hipStream_t stream = nullptr;
   std::vector<void*> allocs;
   double* part = nullptr;  // fast-sum partials
+  double* diag = nullptr;  // diagnostics: block partials [ndiag][DIAG_SLOTS], then the DIAG_SLOTS results
+  int ndiag = 512;
   int nparts = 0;
   int hybrid_sum_mode = 1;
   double hyb_eta0 = 0.0;      // certificate band before OCFD_OPT_HYBRID_CERT_WIDEN
@@ -2122,6 +2125,21 @@ double* field_ptr(int id, long* n) {
 // ===========================================================================
 // C-ABI
 // ===========================================================================
+// One diagnostics pass: block partials, then the means into G.diag's result
+// slots, copied to out[0 .. N).  Synchronises.
+template <int PASS>
+static int diag_pass(double d_aver, double* out) {
+  const DevState& S = G.S;
+  constexpr int N = DiagN<PASS>::N;
+  double* res = G.diag + (long)G.ndiag * DIAG_SLOTS;
+  launch("k_diag_partial", S.cd8 ? k_diag_partial<PASS, 1> : k_diag_partial<PASS, 0>, dim3(G.ndiag), dim3(256), S,
+         G.diag, d_aver);
+  launch("k_diag_final", k_diag_final, dim3(N), dim3(64), S, (const double*)G.diag, G.ndiag, N, res);
+  HIPCHK(hipMemcpyAsync(out, res, sizeof(double) * N, hipMemcpyDeviceToHost, G.stream));
+  HIPCHK(hipStreamSynchronize(G.stream));
+  return 0;
+}
+
 extern "C" {
 
 const char* ocfd_last_error(void) { return G.err.c_str(); }
@@ -2138,6 +2156,7 @@ int32_t ocfd_finalize(void) {
   G.allocs.clear();
   G.chem_rho = G.chem_E = G.chem_dt = nullptr;
   G.part = nullptr;
+  G.diag = nullptr;
   for (int d = 0; d < 3; d++) {
     G.kc[d] = nullptr;
     G.kct[d] = nullptr;
@@ -2311,6 +2330,8 @@ int32_t ocfd_init(const ocfd_cfg* cfg) {
   }
   G.nparts = 1024;
   G.part = dalloc(G.nparts);
+  G.diag = dalloc((long)G.ndiag * DIAG_SLOTS + DIAG_SLOTS);
+  if (!G.diag) return fail("ocfd_init: hipMalloc failed");
   {
     // Certificate of the fast sensor mean (k_sum_partial/k_sum_final against
     // the reference's sequential sum, k_sum_ordered).  Both sums of the same
@@ -2875,6 +2896,44 @@ int32_t ocfd_negative_et_count(void) {
   if (!G.inited) return 0;
   hipStreamSynchronize(G.stream);
   return *G.h_negt;
+}
+
+// show_flow_msg (OCFD_Time_Adv.f90:200-239)
+int32_t ocfd_show_flow_msg(double* E0) {
+  if (check_ready()) return 1;
+  if (!E0) return fail("ocfd_show_flow_msg: null output");
+  return diag_pass<0>(0.0, E0);
+}
+
+// ana_box (OCFD_ANA.f90:197-303): the device means, then the reference's
+// derived statistics (:243-270) and the density rms (:272-281)
+int32_t ocfd_ana_box(double* out) {
+  if (check_ready()) return 1;
+  if (!out) return fail("ocfd_ana_box: null output");
+  double a[DIAG_SLOTS];
+  if (diag_pass<1>(0.0, a)) return 1;
+  double u_rms = a[0], v_rms = a[1], w_rms = a[2], Ux_rms = a[3], Ux_sk = a[4], Ux_fl = a[5], d_aver = a[6];
+  const double c_aver = a[7], Amu_aver = a[8], E_k = a[9], T_aver = a[10];
+  double u_sk = a[11], u_fl = a[12];
+  const double VV_rms = std::sqrt((u_rms + v_rms + w_rms) / 3.0);
+  const double Amt = std::sqrt(u_rms + v_rms + w_rms) / c_aver;
+  u_rms = std::sqrt(u_rms);
+  v_rms = std::sqrt(v_rms);
+  w_rms = std::sqrt(w_rms);
+  Ux_rms = std::sqrt(Ux_rms);
+  const double Alamda = VV_rms / Ux_rms, Alamdax = u_rms / Ux_rms;
+  const double Re_lamda = VV_rms * Alamda * d_aver / Amu_aver, Re_lamdax = VV_rms * Alamdax * d_aver / Amu_aver;
+  Ux_sk = Ux_sk / ((Ux_rms * Ux_rms) * Ux_rms);
+  Ux_fl = Ux_fl / ((Ux_rms * Ux_rms) * (Ux_rms * Ux_rms));
+  u_sk = u_sk / ((u_rms * u_rms) * u_rms);
+  u_fl = u_fl / ((u_rms * u_rms) * (u_rms * u_rms));
+  double drms[DIAG_SLOTS];
+  if (diag_pass<2>(d_aver, drms)) return 1;
+  const double d_rms = std::sqrt(drms[0]);
+  const double r[18] = {Re_lamda, Re_lamdax, Amt,   VV_rms, u_rms,  v_rms,    w_rms,    u_sk,   u_fl,
+                        Ux_sk,    Ux_fl,     E_k,   d_rms,  T_aver, Amu_aver, d_aver,   Ux_rms, Alamdax};
+  for (int m = 0; m < 18; m++) out[m] = r[m];
+  return 0;
 }
 
 int32_t ocfd_hybrid_certificate(int64_t* uncertain, double* margin, double* eta) {

@@ -45,6 +45,7 @@ def load(ocfd_module):
         "orc_du_inviscous": (i32, []), "orc_du_viscous": (i32, []), "orc_time_adv_rk3": (i32, [i32, dbl]),
         "orc_comput_flow_variables": (i32, []), "orc_bc": (i32, []), "orc_comput_rhybrid": (i32, []),
         "orc_exchange_all": (i32, []), "orc_negative_et_count": (i32, []), "orc_sensor_mean": (dbl, []),
+        "orc_show_flow_msg": (i32, [vp]), "orc_ana_box": (i32, [vp]),
         "orc_dx0": (None, [vp, vp]), "orc_dy0": (None, [vp, vp]), "orc_dz0": (None, [vp, vp]),
         "orc_recon_flux1": (i32, [i32, vp, vp, i32, vp, vp, vp]),
         "orc_recon_flux2": (i32, [i32, vp, vp, i32, vp, vp, vp]),
@@ -157,6 +158,16 @@ class Oracle:
 
     def negative_et_count(self):
         return int(self.lib.orc_negative_et_count())
+
+    def show_flow_msg(self):
+        e = np.zeros(2)
+        self._check(self.lib.orc_show_flow_msg(_ptr(e)))
+        return float(e[0]), float(e[1])
+
+    def ana_box(self):
+        out = np.zeros(18)
+        self._check(self.lib.orc_ana_box(_ptr(out)))
+        return dict(zip(self.m.FlowSolver.ANA_BOX_NAMES, (float(x) for x in out)))
 
     def sensor_mean(self):
         return float(self.lib.orc_sensor_mean())

@@ -1647,6 +1647,86 @@ int orc_exchange_all(void) {
   for (int m = 0; m < G.nspec; m++) exchange_xyz(G.di + (long)m * G.nh);
   return 0;
 }
+/* show_flow_msg, OCFD_Time_Adv.f90:200-239: interior means of the total
+ * energy f5/J and of the kinetic energy d|u|^2/2/J (sequential sums in
+ * (k,j,i) order, as one rank's loop before the MPI_REDUCE). */
+int orc_show_flow_msg(double* E0) {
+  double e1 = 0.0, e2 = 0.0;
+  for (int k = 1; k <= G.nz; k++)
+    for (int j = 1; j <= G.ny; j++)
+      for (int i = 1; i <= G.nx; i++) {
+        long q = H(i, j, k), c = I(i, j, k);
+        double tmp0 = 1.0 / G.Ajac[q];
+        e1 = e1 + G.f[c + 4 * G.ni] * tmp0;
+        e2 = e2 + G.d[q] * (G.u[q] * G.u[q] + G.v[q] * G.v[q] + G.w[q] * G.w[q]) * 0.5 * tmp0;
+      }
+  double nt = (double)G.nx * G.ny * G.nz;
+  E0[0] = e1 / nt;
+  E0[1] = e2 / nt;
+  return 0;
+}
+
+/* ana_box, OCFD_ANA.f90:197-303: statistics of isotropic turbulence from the
+ * current primitives and Amu (u exchanged, its derivatives by OCFD_dx0/dy0/dz0
+ * with Scheme_Vis).  out[0..17] = Re_lamda, Re_lamdax, Amt, vv_rms, u_rms,
+ * v_rms, w_rms, u_skewness, u_flatness, ux_skewness, ux_flatness, E_kinetic,
+ * d_rms, T_aver, Amu_aver, d_aver, Ux_rms, Alamdax (the order of the
+ * reference's statistics.dat line after tt). */
+int orc_ana_box(double* out) {
+  exchange_xyz(G.u);
+  orc_dx0(G.u, G.uk);
+  orc_dy0(G.u, G.ui);
+  orc_dz0(G.u, G.us);
+  double a[13] = {0};
+  for (int k = 1; k <= G.nz; k++)
+    for (int j = 1; j <= G.ny; j++)
+      for (int i = 1; i <= G.nx; i++) {
+        long q = H(i, j, k), c = I(i, j, k);
+        double ux = G.uk[c] * G.Akx[q] + G.ui[c] * G.Aix[q] + G.us[c] * G.Asx[q];
+        double gamma = G.p[q] / G.Et[q] + 1.0;
+        double c1 = sqrt(gamma * G.p[q] / G.d[q]);
+        double uu = G.u[q];
+        a[7] = a[7] + c1;
+        a[6] = a[6] + G.d[q];
+        a[8] = a[8] + G.Amu[c];
+        a[10] = a[10] + G.T[q];
+        a[0] = a[0] + uu * uu;
+        a[1] = a[1] + G.v[q] * G.v[q];
+        a[2] = a[2] + G.w[q] * G.w[q];
+        a[3] = a[3] + ux * ux;
+        a[4] = a[4] + (ux * ux) * ux;
+        a[5] = a[5] + (ux * ux) * (ux * ux);
+        a[11] = a[11] + (uu * uu) * uu;
+        a[12] = a[12] + (uu * uu) * (uu * uu);
+        a[9] = a[9] + (uu * uu + G.v[q] * G.v[q] + G.w[q] * G.w[q]) * G.d[q] * 0.5;
+      }
+  double nt = (double)G.nx * G.ny * G.nz;
+  for (int m = 0; m < 13; m++) a[m] = a[m] / nt;
+  double u_rms = a[0], v_rms = a[1], w_rms = a[2], Ux_rms = a[3], Ux_sk = a[4], Ux_fl = a[5], d_aver = a[6];
+  double c_aver = a[7], Amu_aver = a[8], E_k = a[9], T_aver = a[10], u_sk = a[11], u_fl = a[12];
+  double VV_rms = sqrt((u_rms + v_rms + w_rms) / 3.0);
+  double Amt = sqrt(u_rms + v_rms + w_rms) / c_aver;
+  u_rms = sqrt(u_rms); v_rms = sqrt(v_rms); w_rms = sqrt(w_rms); Ux_rms = sqrt(Ux_rms);
+  double Alamda = VV_rms / Ux_rms, Alamdax = u_rms / Ux_rms;
+  double Re_lamda = VV_rms * Alamda * d_aver / Amu_aver, Re_lamdax = VV_rms * Alamdax * d_aver / Amu_aver;
+  Ux_sk = Ux_sk / ((Ux_rms * Ux_rms) * Ux_rms);
+  Ux_fl = Ux_fl / ((Ux_rms * Ux_rms) * (Ux_rms * Ux_rms));
+  u_sk = u_sk / ((u_rms * u_rms) * u_rms);
+  u_fl = u_fl / ((u_rms * u_rms) * (u_rms * u_rms));
+  double d_rms = 0.0;
+  for (int k = 1; k <= G.nz; k++)
+    for (int j = 1; j <= G.ny; j++)
+      for (int i = 1; i <= G.nx; i++) {
+        double x = G.d[H(i, j, k)] - d_aver;
+        d_rms = d_rms + x * x;
+      }
+  d_rms = sqrt(d_rms / nt);
+  double r[18] = {Re_lamda, Re_lamdax, Amt, VV_rms, u_rms, v_rms, w_rms, u_sk, u_fl, Ux_sk, Ux_fl, E_k, d_rms,
+                  T_aver, Amu_aver, d_aver, Ux_rms, Alamdax};
+  for (int m = 0; m < 18; m++) out[m] = r[m];
+  return 0;
+}
+
 int orc_negative_et_count(void) { return G.num_negt; }
 double orc_sensor_mean(void) { return G.sensor_dp_av; }
 

@@ -509,3 +509,51 @@ def test_grid3d_curvilinear_metrics_and_step(pkg):
     print(rep)
     assert all(v[1] == 1.0 for v in rep.values()), rep
     g.close()
+
+
+@pytest.mark.parametrize("which", ["isotropic", "flat_plate"])
+def test_diagnostics_match_oracle(pkg, which):
+    """show_flow_msg and ana_box as device reductions against the oracle's
+    sequential sums after one RK3 step (Amu from the step's last stage): the
+    tree and sequential sums differ by summation order only (<= 1e-13
+    relative to each statistic's scale)."""
+    case = fc.small_isotropic(32) if which == "isotropic" else fc.small_flat_plate(64, 40, 16)
+    g, o = pair(pkg, case)
+    dt1 = 0.5 * case[5]["dt"]
+    g.flow_time_advance(dt1)
+    o.flow_time_advance(dt1)
+    eg, eo = g.show_flow_msg(), o.show_flow_msg()
+    for a, b in zip(eg, eo):
+        assert abs(a - b) <= 1e-13 * abs(b), (a, b)
+    sg, so = g.ana_box(), o.ana_box()
+    print({k: (sg[k], so[k]) for k in sg})
+    for k in sg:
+        scale = max(abs(so[k]), 1e-300)
+        if k in ("u_skewness", "ux_skewness"):
+            scale = max(scale, 1.0)
+        if k == "d_rms":
+            scale = so["d_aver"]
+        assert abs(sg[k] - so[k]) <= 1e-12 * scale, (k, sg[k], so[k])
+    g.close()
+
+
+def test_isotropic_initial_statistics(pkg):
+    """The 64^3 decaying-turbulence start: ana_box on the device recovers the
+    turbulent Mach number 0.5 the generator imposes (to rounding) and
+    Re_lambda = 72 to within the CD6-vs-spectral derivative difference
+    (the generator scales with spectral du/dx, ana_box differentiates with
+    CD6)."""
+    from ocfd_amd import cases
+    cfg, mesh, span, st, info = cases.isotropic(64)
+    g = pkg.FlowSolver(cfg)
+    g.set_mesh(*mesh, span)
+    g.upload_state(st)
+    g.comput_flow_variables()
+    g.OCFD_bc()
+    g.comput_Amu_Amk_AMD()
+    s = g.ana_box()
+    print(s)
+    assert abs(s["Amt"] - 0.5) <= 1e-12
+    assert abs(s["Re_lamda"] - 72.0) <= 0.03 * 72.0
+    assert abs(s["u_skewness"]) < 0.2 and abs(s["d_rms"]) <= 1e-12 * s["d_aver"]
+    g.close()

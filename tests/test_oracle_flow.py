@@ -761,3 +761,63 @@ def test_bc_boundarylayer_known_answers(outlet, Tw):
             expect = pre[k][-1, 1:-1, :]
         assert close(got, expect), k
     o.close()
+
+
+# ------------------------------------------------------------- diagnostics
+def _sine_box(n=24, U0=40.0, V0=25.0):
+    """Periodic 2pi box, uniform d and T, u = U0 sin x, v = V0 sin y, w = 0."""
+    m = load_package()
+    from ocfd_amd import cases
+    cfg, mesh, span, st, info = cases.isotropic(n)
+    x = np.arange(n) * (2 * np.pi / n)
+    X, Y, _ = np.meshgrid(x, x, x, indexing="ij")
+    d = np.full((n, n, n), 1.2)
+    T = np.full((n, n, n), 300.0)
+    st = cases.state_from_primitives(d, U0 * np.sin(X), V0 * np.sin(Y), np.zeros_like(X), T)
+    return m, cfg, mesh, span, st, (n, U0, V0, 1.2, 300.0)
+
+
+def test_show_flow_msg_and_ana_box_closed_forms():
+    """show_flow_msg (OCFD_Time_Adv.f90:200-239) and ana_box (OCFD_ANA.f90:197-303)
+    on u = U0 sin x, v = V0 sin y: the discrete means of sin^2, sin^3, sin^4
+    are 1/2, 0, 3/8 on a periodic grid, and OCFD_dx0 (CD6) turns sin x into
+    k' cos x with the CD6 modified wavenumber
+    k' = (3/2 sin D - 3/10 sin 2D + 1/30 sin 3D)/D, D = 2pi/n.  From these:
+    u_rms = U0/sqrt2, Ux_rms = U0 k'/sqrt2, flatness 3/2, skewness 0,
+    vv_rms = sqrt((U0^2 + V0^2)/6), lambda = vv_rms / Ux_rms,
+    Re_lambda = vv_rms lambda d / mu(T) (Sutherland), Mt = sqrt((U0^2+V0^2)/2)/c
+    with c = sqrt(1.4 R T), E_kinetic = d (U0^2 + V0^2)/4, d_rms = 0;
+    show_flow_msg gives (Et + E_kinetic)/Ajac and E_kinetic/Ajac with
+    Et = 2.5 R d T and the uniform Ajac = (h/D)^3, h = 1/(n-1)."""
+    m, cfg, mesh, span, st, (n, U0, V0, d, T) = _sine_box()
+    import oracle
+    from ocfd_amd import cases
+    o = oracle.Oracle(m, cfg)
+    o.set_mesh(*mesh, span)
+    o.upload_state(st)
+    o.comput_flow_variables()
+    o.OCFD_bc()
+    o.comput_Amu_Amk_AMD()
+    s = o.ana_box()
+    D = 2 * np.pi / n
+    kp = (1.5 * np.sin(D) - 0.3 * np.sin(2 * D) + np.sin(3 * D) / 30.0) / D
+    vv = math.sqrt((U0 ** 2 + V0 ** 2) / 6.0)
+    uxr = U0 * kp / math.sqrt(2.0)
+    mu = float(cases.sutherland(T))
+    c = math.sqrt(1.4 * cases.R_AIR * T)
+    expect = dict(u_rms=U0 / math.sqrt(2.0), v_rms=V0 / math.sqrt(2.0), w_rms=0.0, Ux_rms=uxr, vv_rms=vv,
+                  u_flatness=1.5, ux_flatness=1.5, Re_lamda=vv * (vv / uxr) * d / mu,
+                  Re_lamdax=vv * ((U0 / math.sqrt(2.0)) / uxr) * d / mu, Amt=math.sqrt((U0 ** 2 + V0 ** 2) / 2.0) / c,
+                  E_kinetic=d * (U0 ** 2 + V0 ** 2) / 4.0, T_aver=T, Amu_aver=mu, d_aver=d,
+                  Alamdax=(U0 / math.sqrt(2.0)) / uxr)
+    for k, v in expect.items():
+        assert abs(s[k] - v) <= 1e-12 * max(abs(v), 1.0), (k, s[k], v)
+    for k in ("u_skewness", "ux_skewness", "d_rms"):
+        assert abs(s[k]) <= 1e-12, (k, s[k])
+    h = 1.0 / (n - 1)
+    ajac = (h / D) ** 3
+    e_tot, e_kin = o.show_flow_msg()
+    et = 2.5 * cases.R_AIR * d * T
+    assert abs(e_kin - d * (U0 ** 2 + V0 ** 2) / 4.0 / ajac) <= 1e-12 * e_kin
+    assert abs(e_tot - (et + d * (U0 ** 2 + V0 ** 2) / 4.0) / ajac) <= 1e-12 * e_tot
+    o.close()
