@@ -303,6 +303,29 @@ int32_t ocfd_negative_et_count(void);
  * use.  Any pointer may be NULL.  Fails when no certified mean has run. */
 int32_t ocfd_hybrid_certificate(int64_t* uncertain, double* margin, double* eta);
 
+/* ---- the reference's files ----------------------------------------------
+ * Fortran unformatted sequential files (each record framed by a 4-byte
+ * length before and after it), as the reference writes them with one rank.
+ * ocfd_read_flow_data: read_flow_data + init3d (code/mpi-io/OCFD_IO.f90:5-33,
+ *   code/flow-solver/OCFD_init.f90:1-25): a record (Istep int32, tt real8),
+ *   then d, u, v, w, T, di(1..N_SPEC), each as nz records of nx*ny reals,
+ *   into the interior of the resident arrays; Rescaling_di; f from the
+ *   primitives with Et = comput_E(di, T).  As in NS_solver, call
+ *   ocfd_comput_flow_variables and ocfd_bc afterwards.
+ * ocfd_save_flow_data: save_flow_data (OCFD_IO.f90:38-64), the same layout
+ *   (the reference names the file OCFD-comb%08d.dat after Istep).
+ * ocfd_read_mesh: init_mesh (code/flow-solver/OCFD_mesh.f90:1-57) reading
+ *   OCFD-grid.dat for Iflag_Gridtype 10 (GRID1D: x(nx), y(ny), z(nz)),
+ *   20 (GRID2D_PLANE: x(nx,ny), y(nx,ny), z(nz)), 21 (GRID2D_AXIAL_SYMM:
+ *   x(nx,ny), R(nx,ny), seta(nz); y = R cos seta, z = R sin seta),
+ *   30 (GRID3D: x, y, z as nz records each) or 31 (GRID_AND_JACOBIAN3D:
+ *   GRID3D plus Akx, Aky, Akz, Aix, Aiy, Aiz, Asx, Asy, Asz, Ajac, used
+ *   instead of comput_Jacobian3d); span[9] = Periodic_ISpan, _JSpan, _KSpan.
+ *   Then the metrics as ocfd_set_mesh. */
+int32_t ocfd_read_flow_data(const char* path, int32_t* istep, double* tt);
+int32_t ocfd_save_flow_data(const char* path, int32_t istep, double tt);
+int32_t ocfd_read_mesh(const char* path, int32_t gridtype, const double* span);
+
 /* ---- diagnostics (device reductions of the resident state) --------------
  * show_flow_msg (code/flow-solver/OCFD_Time_Adv.f90:200-239): E0[0] = mean
  * over the interior of f5/Ajac (total energy), E0[1] = mean of

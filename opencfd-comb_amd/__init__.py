@@ -126,6 +126,9 @@ def load_library(path=None):
         "ocfd_bc": (i32, []), "ocfd_negative_et_count": (i32, []),
         "ocfd_hybrid_certificate": (i32, [P(ctypes.c_int64), P(dbl), P(dbl)]),
         "ocfd_show_flow_msg": (i32, [vp]), "ocfd_ana_box": (i32, [vp]),
+        "ocfd_read_flow_data": (i32, [ctypes.c_char_p, P(i32), P(dbl)]),
+        "ocfd_save_flow_data": (i32, [ctypes.c_char_p, i32, dbl]),
+        "ocfd_read_mesh": (i32, [ctypes.c_char_p, i32, vp]),
         "ocfd_flux1d_batch": (i32, [i32, i32, ctypes.c_int64, i32, vp, vp, vp, P(i32), P(i32)]), "ocfd_synchronize": (i32, []),
         "ocfd_stream": (vp, []), "ocfd_kernel_count": (i32, []), "ocfd_kernel_name": (ctypes.c_char_p, [i32]),
         "ocfd_kernel_stats": (i32, [i32, P(dbl), P(ctypes.c_int64)]), "ocfd_kernel_stats_reset": (i32, []),
@@ -266,6 +269,23 @@ class FlowSolver:
         if a.shape != (self.ny, self.nz, 5 + self.cfg.n_spec):
             raise ValueError(f"inlet2d: expected shape {(self.ny, self.nz, 5 + self.cfg.n_spec)}, got {a.shape}")
         self._check(self.lib.ocfd_set_bc_jet(_ptr(a)))
+
+    def read_flow_data(self, path):
+        """read_flow_data + init3d from a restart file of the reference's
+        layout; returns (Istep, tt).  Follow with comput_flow_variables and
+        OCFD_bc, as NS_solver does."""
+        istep, tt = ctypes.c_int32(0), ctypes.c_double(0.0)
+        self._check(self.lib.ocfd_read_flow_data(os.fsencode(path), ctypes.byref(istep), ctypes.byref(tt)))
+        return int(istep.value), float(tt.value)
+
+    def save_flow_data(self, path, istep, tt):
+        """save_flow_data: d, u, v, w, T, di of the resident state."""
+        self._check(self.lib.ocfd_save_flow_data(os.fsencode(path), int(istep), float(tt)))
+
+    def read_mesh(self, path, gridtype, span=None):
+        """init_mesh from an OCFD-grid.dat of Iflag_Gridtype 10, 20, 21, 30 or 31."""
+        sp = np.zeros(9) if span is None else np.ascontiguousarray(span, dtype=np.float64)
+        self._check(self.lib.ocfd_read_mesh(os.fsencode(path), int(gridtype), _ptr(sp)))
 
     def upload_state(self, state):
         """state: dict with F, DI, D, U, V, W, T, P, ET (reference layouts)."""

@@ -579,6 +579,7 @@ __global__ __launch_bounds__(256) void k_sweep_char(DevState S) {
 
 #include "ocfd_species.h"
 #include "ocfd_diag.h"
+#include "ocfd_io.h"
 
 // ===========================================================================
 // Viscous terms, part 1 (du_viscous, OCFD_viscous.f90:27-95, with the
@@ -2471,8 +2472,14 @@ int32_t ocfd_set_bc_jet(const double* inlet2d) {
   return 0;
 }
 
-int32_t ocfd_set_mesh(const double* axx, const double* ayy, const double* azz, const double* span) {
-  if (check_ready()) return 1;
+// init_mesh (OCFD_mesh.f90:1-57) from halo-shaped host coordinate arrays:
+// the coordinates' interior, periodic ghosts with the span shift
+// (updata_Axyz_periodic), comput_Jacobian3d -- or, when `metrics` is given
+// (GRID_AND_JACOBIAN3D), the ten interior arrays Akx..Asz, Ajac read from
+// the grid file instead -- their periodic ghosts, Jac_Ghost_boundary and the
+// A*1 = A*/Ajac arrays.
+static int mesh_setup(const double* axx, const double* ayy, const double* azz, const double* span,
+                      const double* metrics) {
   DevState& S = G.S;
   const size_t bytes = sizeof(double) * (size_t)S.nh;
   // staging buffer, freed on every return path (hipFree waits for the device)
@@ -2495,8 +2502,17 @@ int32_t ocfd_set_mesh(const double* axx, const double* ayy, const double* azz, c
     if (S.per[1]) hipLaunchKernelGGL(k_fill_faces<1>, grid1(npy * 8), dim3(256), 0, G.stream, S, dst[a], 1, span ? span[3 + a] : 0.0);
     if (S.per[2]) hipLaunchKernelGGL(k_fill_faces<2>, grid1(npz * 8), dim3(256), 0, G.stream, S, dst[a], 1, span ? span[6 + a] : 0.0);
   }
-  hipLaunchKernelGGL(k_metrics, grid1(S.ni), dim3(256), 0, G.stream, S);
   double* M[10] = {S.Akx, S.Aky, S.Akz, S.Aix, S.Aiy, S.Aiz, S.Asx, S.Asy, S.Asz, S.Ajac};
+  if (!metrics) {
+    hipLaunchKernelGGL(k_metrics, grid1(S.ni), dim3(256), 0, G.stream, S);
+  } else {
+    for (int a = 0; a < 10; a++) {
+      HIPCHK(hipMemcpyAsync(tmp, metrics + (size_t)a * S.ni, sizeof(double) * (size_t)S.ni, hipMemcpyHostToDevice,
+                            G.stream));
+      hipLaunchKernelGGL(k_interior_to_halo, grid1(S.ni), dim3(256), 0, G.stream, S, (const double*)tmp, M[a]);
+      HIPCHK(hipStreamSynchronize(G.stream));
+    }
+  }
   for (int a = 0; a < 10; a++) {
     const long npx = (long)S.ny * S.nz, npy = (long)S.nx * S.nz, npz = (long)S.nx * S.ny;
     if (S.per[0]) hipLaunchKernelGGL(k_fill_faces<0>, grid1(npx * 8), dim3(256), 0, G.stream, S, M[a], 0, 0.0);
@@ -2517,6 +2533,191 @@ int32_t ocfd_set_mesh(const double* axx, const double* ayy, const double* azz, c
   HIPCHK(hipGetLastError());
   drop_graph();
   return 0;
+}
+
+int32_t ocfd_set_mesh(const double* axx, const double* ayy, const double* azz, const double* span) {
+  if (check_ready()) return 1;
+  return mesh_setup(axx, ayy, azz, span, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// The reference's files: Fortran unformatted sequential records, each framed
+// by a 4-byte length before and after the payload (records of 2 GiB or more,
+// which compilers split, are refused).
+// ---------------------------------------------------------------------------
+struct FortranFile {
+  FILE* f = nullptr;
+  std::string path;
+  ~FortranFile() { if (f) fclose(f); }
+  int open(const char* p, const char* mode, const char* who) {
+    if (!p) return fail(std::string(who) + ": null path");
+    path = p;
+    f = fopen(p, mode);
+    if (!f) return fail(std::string(who) + ": cannot open " + path);
+    return 0;
+  }
+  // one record; its length must be `bytes` (or at least `bytes` if `prefix`,
+  // the rest of the record is then skipped)
+  int read(void* buf, size_t bytes, const char* who, bool prefix = false) {
+    int32_t a = 0, b = 0;
+    if (fread(&a, 4, 1, f) != 1) return fail(std::string(who) + ": " + path + ": unexpected end of file");
+    if (a < 0 || (prefix ? (size_t)a < bytes : (size_t)a != bytes))
+      return fail(std::string(who) + ": " + path + ": record of " + std::to_string(a) + " bytes where " +
+                  std::to_string(bytes) + " were expected");
+    if (bytes && fread(buf, 1, bytes, f) != bytes) return fail(std::string(who) + ": " + path + ": short record");
+    if ((size_t)a > bytes && fseek(f, (long)((size_t)a - bytes), SEEK_CUR) != 0)
+      return fail(std::string(who) + ": " + path + ": seek failed");
+    if (fread(&b, 4, 1, f) != 1 || b != a) return fail(std::string(who) + ": " + path + ": bad record marker");
+    return 0;
+  }
+  int write(const void* buf, size_t bytes, const char* who) {
+    if (bytes > 0x7fffffffu) return fail(std::string(who) + ": record too large");
+    const int32_t a = (int32_t)bytes;
+    if (fwrite(&a, 4, 1, f) != 1 || (bytes && fwrite(buf, 1, bytes, f) != bytes) || fwrite(&a, 4, 1, f) != 1)
+      return fail(std::string(who) + ": " + path + ": write failed");
+    return 0;
+  }
+};
+
+// read_3d1 / write_3d1 of one field: nz records of nx*ny reals, k plane by k
+// plane (OCFD_IO_mpi.f90:151-307 with one rank), between the file and a
+// contiguous host array of nx*ny*nz reals
+static int read_field3d(FortranFile& F, double* host, const char* who) {
+  const DevState& S = G.S;
+  const size_t plane = (size_t)S.nx * S.ny;
+  for (int k = 0; k < S.nz; k++)
+    if (F.read(host + plane * k, sizeof(double) * plane, who)) return 1;
+  return 0;
+}
+static int write_field3d(FortranFile& F, const double* host, const char* who) {
+  const DevState& S = G.S;
+  const size_t plane = (size_t)S.nx * S.ny;
+  for (int k = 0; k < S.nz; k++)
+    if (F.write(host + plane * k, sizeof(double) * plane, who)) return 1;
+  return 0;
+}
+
+// read_flow_data + init3d (OCFD_IO.f90:5-33, OCFD_init.f90:1-25)
+int32_t ocfd_read_flow_data(const char* path, int32_t* istep, double* tt) {
+  if (check_ready()) return 1;
+  const DevState& S = G.S;
+  FortranFile F;
+  if (F.open(path, "rb", "ocfd_read_flow_data")) return 1;
+  unsigned char head[12];
+  if (F.read(head, 12, "ocfd_read_flow_data", true)) return 1;
+  int32_t is;
+  double t;
+  std::memcpy(&is, head, 4);
+  std::memcpy(&t, head + 4, 8);
+  std::vector<double> host((size_t)S.ni);
+  struct Staging {
+    double* p = nullptr;
+    ~Staging() { if (p) hipFree(p); }
+  } stage;
+  if (hipMalloc(&stage.p, sizeof(double) * (size_t)S.ni) != hipSuccess) return fail("ocfd_read_flow_data: hipMalloc failed");
+  double* dst[5] = {S.d, S.u, S.v, S.w, S.T};
+  for (int a = 0; a < 5 + S.ns; a++) {
+    if (read_field3d(F, host.data(), "ocfd_read_flow_data")) return 1;
+    double* to = a < 5 ? dst[a] : S.di + (long)(a - 5) * S.nh;
+    HIPCHK(hipMemcpyAsync(stage.p, host.data(), sizeof(double) * (size_t)S.ni, hipMemcpyHostToDevice, G.stream));
+    hipLaunchKernelGGL(k_interior_to_halo, grid1(S.ni), dim3(256), 0, G.stream, S, (const double*)stage.p, to);
+    HIPCHK(hipStreamSynchronize(G.stream));
+  }
+  hipLaunchKernelGGL(k_init3d, grid1(S.ni), dim3(256), 0, G.stream, S);
+  HIPCHK(hipStreamSynchronize(G.stream));
+  HIPCHK(hipGetLastError());
+  if (istep) *istep = is;
+  if (tt) *tt = t;
+  return 0;
+}
+
+// save_flow_data (OCFD_IO.f90:38-64): the caller names the file
+// (the reference uses OCFD-comb%08d.dat with Istep)
+int32_t ocfd_save_flow_data(const char* path, int32_t istep, double tt) {
+  if (check_ready()) return 1;
+  const DevState& S = G.S;
+  FortranFile F;
+  if (F.open(path, "wb", "ocfd_save_flow_data")) return 1;
+  unsigned char head[12];
+  std::memcpy(head, &istep, 4);
+  std::memcpy(head + 4, &tt, 8);
+  if (F.write(head, 12, "ocfd_save_flow_data")) return 1;
+  std::vector<double> host((size_t)S.ni);
+  struct Staging {
+    double* p = nullptr;
+    ~Staging() { if (p) hipFree(p); }
+  } stage;
+  if (hipMalloc(&stage.p, sizeof(double) * (size_t)S.ni) != hipSuccess) return fail("ocfd_save_flow_data: hipMalloc failed");
+  const double* srcs[5] = {S.d, S.u, S.v, S.w, S.T};
+  for (int a = 0; a < 5 + S.ns; a++) {
+    const double* from = a < 5 ? srcs[a] : S.di + (long)(a - 5) * S.nh;
+    hipLaunchKernelGGL(k_halo_to_interior, grid1(S.ni), dim3(256), 0, G.stream, S, from, stage.p);
+    HIPCHK(hipMemcpyAsync(host.data(), stage.p, sizeof(double) * (size_t)S.ni, hipMemcpyDeviceToHost, G.stream));
+    HIPCHK(hipStreamSynchronize(G.stream));
+    if (write_field3d(F, host.data(), "ocfd_save_flow_data")) return 1;
+  }
+  return 0;
+}
+
+// init_mesh from OCFD-grid.dat (OCFD_mesh.f90:1-57; read_mesh1d,
+// read_mesh2d_plane, read_mesh2d_AxialSymm, read_mesh3d: OCFD_IO_mpi.f90
+// and OCFD_mesh.f90:187-317) for Iflag_Gridtype = 10, 20, 21, 30 or 31
+int32_t ocfd_read_mesh(const char* path, int32_t gridtype, const double* span) {
+  if (check_ready()) return 1;
+  const DevState& S = G.S;
+  const char* who = "ocfd_read_mesh";
+  if (gridtype != 10 && gridtype != 20 && gridtype != 21 && gridtype != 30 && gridtype != 31)
+    return fail("ocfd_read_mesh: Iflag_Gridtype must be 10, 20, 21, 30 or 31");
+  FortranFile F;
+  if (F.open(path, "rb", who)) return 1;
+  const long nx = S.nx, ny = S.ny, nz = S.nz;
+  std::vector<double> X((size_t)S.nh, 1.0), Y((size_t)S.nh, 1.0), Z((size_t)S.nh, 1.0);
+  auto hidx = [&](long i, long j, long k) {  // 1-based interior index into a halo array
+    return (size_t)((i + OCFD_LAP - 1) + S.nxh * ((j + OCFD_LAP - 1) + (long)S.nyh * (k + OCFD_LAP - 1)));
+  };
+  std::vector<double> metrics;
+  if (gridtype == 10 || gridtype == 20 || gridtype == 21) {
+    const bool one = gridtype == 10;
+    std::vector<double> a(one ? nx : nx * ny), b(one ? ny : nx * ny), c(nz);
+    if (F.read(a.data(), sizeof(double) * a.size(), who) || F.read(b.data(), sizeof(double) * b.size(), who) ||
+        F.read(c.data(), sizeof(double) * c.size(), who))
+      return 1;
+    for (long k = 1; k <= nz; k++)
+      for (long j = 1; j <= ny; j++)
+        for (long i = 1; i <= nx; i++) {
+          const size_t h = hidx(i, j, k);
+          if (one) {
+            X[h] = a[i - 1];
+            Y[h] = b[j - 1];
+            Z[h] = c[k - 1];
+          } else {
+            const size_t p = (size_t)((i - 1) + nx * (j - 1));
+            X[h] = a[p];
+            if (gridtype == 20) {
+              Y[h] = b[p];
+              Z[h] = c[k - 1];
+            } else {  // y horizontal, z up; seta the angle from the y axis
+              Y[h] = b[p] * std::cos(c[k - 1]);
+              Z[h] = b[p] * std::sin(c[k - 1]);
+            }
+          }
+        }
+  } else {
+    std::vector<double> v((size_t)S.ni);
+    std::vector<double>* xyz[3] = {&X, &Y, &Z};
+    for (int a = 0; a < 3; a++) {
+      if (read_field3d(F, v.data(), who)) return 1;
+      for (long k = 1; k <= nz; k++)
+        for (long j = 1; j <= ny; j++)
+          for (long i = 1; i <= nx; i++) (*xyz[a])[hidx(i, j, k)] = v[(size_t)((i - 1) + nx * ((j - 1) + ny * (k - 1)))];
+    }
+    if (gridtype == 31) {
+      metrics.resize((size_t)10 * S.ni);
+      for (int a = 0; a < 10; a++)
+        if (read_field3d(F, metrics.data() + (size_t)a * S.ni, who)) return 1;
+    }
+  }
+  return mesh_setup(X.data(), Y.data(), Z.data(), span, metrics.empty() ? nullptr : metrics.data());
 }
 
 int32_t ocfd_upload_field(int32_t field, const double* host) {
