@@ -21,6 +21,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 METRIC = "Mcell-steps/s per RK3 step (FP64); chem cells/s surrogate vs plain at ≤1% ΔT"
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBPS = 8000.0
+PEAK_BF16_TFLOPS = 2500.0
 
 
 def load_pkg():
@@ -56,23 +57,30 @@ def run_steps(engine, info, nsteps, t0_step):
     return t0_step + nsteps
 
 
-def cpu_baseline(m, nsteps=20):
-    """The oracle (single-threaded C restatement) on a bounded sample of the
-    same workload: the flat plate at 160 x 120 x 32 (same wall-normal grid),
-    nsteps RK3 steps."""
+def cpu_baseline(m, nx, ny, nz, budget_s=15.0, max_steps=40):
+    """The oracle (the C restatement, OpenMP over every host thread that
+    OMP_NUM_THREADS allows) on the same workload: the whole flat plate at
+    nx x ny x nz, as many RK3 steps as fit in about budget_s after a first
+    (timed, discarded) step."""
     sys.path.insert(0, ROOT)
     import oracle
     from ocfd_amd import cases
-    case = cases.flat_plate(160, 120, 32, lx=0.2 * 160 / 640, lz=0.004445 * 0.5)
+    case = cases.flat_plate(nx, ny, nz)
     o = oracle.Oracle(m, case[0])
+    threads = o.set_threads(0)
     setup(o, case)
     t = time.perf_counter()
-    run_steps(o, case[5], nsteps, 0)
+    step = run_steps(o, case[5], 1, 0)
+    first = time.perf_counter() - t
+    nsteps = int(max(1, min(max_steps, budget_s / max(first, 1e-3))))
+    t = time.perf_counter()
+    run_steps(o, case[5], nsteps, step)
     el = time.perf_counter() - t
     o.close()
-    ncell = 160 * 120 * 32
-    return dict(value=ncell * nsteps / el / 1e6, unit="Mcell-steps/s", cores=1, kind="port",
-                sample=f"oracle (plain C, 1 thread) on the flat plate at 160x120x32, {nsteps} RK3 steps, {el:.1f} s")
+    ncell = nx * ny * nz
+    return dict(value=ncell * nsteps / el / 1e6, unit="Mcell-steps/s", cores=threads, kind="port",
+                sample=f"oracle (C restatement, OpenMP, {threads} threads) on the whole flat plate "
+                       f"{nx}x{ny}x{nz}, {nsteps} RK3 steps after one untimed, {el:.1f} s")
 
 
 SURR_TOL = 5e-4           # mass-fraction error the increment bound is calibrated to
@@ -112,14 +120,21 @@ def chem_bench(m, ncell=1 << 21, cpu=True, surrogate_cells=1 << 24, surrogate_ep
             best = ms if best is None else min(best, ms)
         return best, out
 
+    from ocfd_amd import flops
     res = {}
     ms, ns = timed(lambda: ch.adaptive(rho, E, dt, scheduler=1))
     k = ns.cpu().numpy()
     cnt = ch.counters()
+    # FP64 roofline of the integrator: about 7e3 flops per point-implicit
+    # substep, times the substeps the batch took
+    chem_tf = flops.CHEM_SUBSTEP * float(k.sum()) / (ms / 1e3) / 1e12
     res["plain_adaptive"] = dict(ms=ms, cells_per_s=ncell / ms * 1e3, unfinished=cnt["unfinished"],
                                  scheduler="persistent work queue (one cell per lane)",
                                  substeps=dict(mean=float(k.mean()), p50=float(np.median(k)),
-                                               p99=float(np.percentile(k, 99)), max=int(k.max())))
+                                               p99=float(np.percentile(k, 99)), max=int(k.max())),
+                                 roofline=dict(bound="fp64", achieved=chem_tf, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
+                                               frac=chem_tf / PEAK_FP64_TFLOPS, flops_per_substep=flops.CHEM_SUBSTEP,
+                                               substeps_total=int(k.sum())))
     ms, _ = timed(lambda: ch.adaptive(rho, E, dt, scheduler=2))
     res["plain_adaptive_queue_plus_cooperative"] = dict(ms=ms, cells_per_s=ncell / ms * 1e3, lane_budget=128,
                                                         cooperative_cells=ch.counters()["cooperative"])
@@ -171,6 +186,9 @@ def chem_bench(m, ncell=1 << 21, cpu=True, surrogate_cells=1 << 24, surrogate_ep
             ms_s, _ = timed(lambda: S.step(rho, E, dt, fallback=True, flags=flags))
             nfb = S.fallbacks()
             ms_nofb, _ = timed(lambda: S.step(rho, E, dt, fallback=False))
+            # BF16 MFMA roofline of the fused inference (network + checks,
+            # no fallback): 535,040 flops per cell
+            mlp_tf = flops.MLP_CELL * ncell / (ms_nofb / 1e3) / 1e12
             rho.copy_(r0)
             E.copy_(e0)
             S.step(rho, E, dt, fallback=True, flags=flags)
@@ -186,6 +204,9 @@ def chem_bench(m, ncell=1 << 21, cpu=True, surrogate_cells=1 << 24, surrogate_ep
                                   (("domain", 1), ("element_balance", 2), ("negative_Y", 4), ("T_bounds", 8),
                                    ("increment_bound", 16))},
                 ms_without_fallback=ms_nofb,
+                roofline_inference=dict(bound="mfma", pipe="bf16 dense MFMA", achieved=mlp_tf,
+                                        peak=PEAK_BF16_TFLOPS, unit="TFLOP/s", frac=mlp_tf / PEAK_BF16_TFLOPS,
+                                        flops_per_cell=flops.MLP_CELL),
                 speedup_vs_plain=best_plain / ms_s, plain_ms=best_plain,
                 error_vs_plain=dict(dT_max_K=float(dT.max()), dT_p99_K=float(np.percentile(dT, 99)),
                                     dT_rel_max=float(np.max(dT / Tp)),
@@ -202,7 +223,8 @@ def chem_bench(m, ncell=1 << 21, cpu=True, surrogate_cells=1 << 24, surrogate_ep
         res_u = surrogate_run()
         S.set_config(cfg_b)
         res_b = surrogate_run()
-        keep = ("ms", "cells_per_s", "fallbacks", "fallback_reasons", "speedup_vs_plain", "error_vs_plain")
+        keep = ("ms", "cells_per_s", "fallbacks", "fallback_reasons", "speedup_vs_plain", "error_vs_plain",
+                "roofline_inference")
         out["surrogate"] = dict(
             network="12 -> 256 x5 -> 9 GELU, BF16 MFMA (FP32 accumulation), fused kernel",
             tile=dict(cells_per_workgroup=tile_rows, waves=tile_waves,
@@ -289,7 +311,15 @@ def jet_bench(m, S, nx=256, ny=192, nz=128, steps=10, warmup=2, tau=2e-5, cfl=0.
                step="NS_solver: flow RK3 dt/2, chemistry dt, comput_T_P, flow RK3 dt/2")
     ms_flow = run(m.CHEM_FIXED, reaction=False)
     out["flow_only_ms_per_step"] = ms_flow
-    out["flow_Mcell_steps_per_s"] = ncell * 2 / ms_flow / 1e3
+    rate = ncell * 2 / ms_flow / 1e3
+    out["flow_Mcell_steps_per_s"] = rate
+    # step roofline of the reacting flow (two RK3 steps per NS step):
+    # min(HBM / 896 B, FP64 / 2.9e4 flops) per cell-step
+    from ocfd_amd import flops
+    b_alg, f_alg = flops.jet_step_bytes_per_cell(9), flops.JET_STEP
+    bh, bf = PEAK_HBM_GBPS * 1e9 / b_alg / 1e6, PEAK_FP64_TFLOPS * 1e12 / f_alg / 1e6
+    out["flow_step_roofline"] = dict(F_alg_per_cell=f_alg, B_alg_per_cell=b_alg, hbm_bound=bh, fp64_bound=bf,
+                                     unit="Mcell-steps/s", frac=rate / min(bh, bf), hbm_frac=rate / bh)
     ms_plain = run(m.CHEM_ADAPTIVE)
     Tp, dip = g.download("T"), g.download("DI")
     unfinished = None
@@ -327,23 +357,28 @@ def jet_bench(m, S, nx=256, ny=192, nz=128, steps=10, warmup=2, tau=2e-5, cfl=0.
     return out
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy), or None."""
-    path = os.path.join(ROOT, "profiles", "flat_plate_640x120x64_traffic.md")
-    if not os.path.exists(path):
-        return None
-    col = None
-    for line in open(path):
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "flat_plate_640x120x64_traffic.md")
+
+
+def traffic_table():
+    """{kernel: (HBM bytes per launch, launches per RK3 step)} from the
+    committed rocprofv3 counter summary of the flat plate (FETCH_SIZE +
+    WRITE_SIZE, each scaled by what it reads for a plain copy)."""
+    out = {}
+    if not os.path.exists(TRAFFIC_PROFILE):
+        return out
+    col = per = None
+    for line in open(TRAFFIC_PROFILE):
         cells = [c.strip() for c in line.split("|")]
         if col is None and "traffic (MiB)" in cells:
             col = cells.index("traffic (MiB)")
-        elif col is not None and len(cells) > col and cells[1] == kernel:
+            per = cells.index("launches per step") if "launches per step" in cells else None
+        elif col is not None and len(cells) > col and cells[1] and not cells[1].startswith("-"):
             try:
-                return float(cells[col]) * 2 ** 20
+                out[cells[1]] = (float(cells[col]) * 2 ** 20, float(cells[per]) if per is not None else None)
             except ValueError:
-                return None
-    return None
+                pass
+    return out
 
 
 def main():
@@ -461,19 +496,29 @@ def main():
         # u,v,w,T,d,di + 18 metrics and writes Amu,Amk,Ev(12)
         bytes_cell = {"k_sweep_x": 73 + 40, "k_sweep_y": 73 + 80, "k_sweep_z": 73 + 88,
                       "k_rk": 240 + 112, "k_visc_prep": 192 + 112}.get(name)
-        traffic = measured_traffic(name)
+        traffic = traffic_table().get(name, (None, None))[0]
         if per_cell is not None:
             fp64 = per_cell * ncell / avg_s / 1e12
-            hbm = bytes_cell * ncell / avg_s / 1e9
+            hbm_own = bytes_cell * ncell / avg_s / 1e9
             memory_bound = name in ("k_rk", "k_visc_prep")
             if memory_bound:
+                # the compulsory state bytes of a step (320 B per cell-step for
+                # 5 conserved FP64 variables), one stage launch = 1/3 step
+                b_stage = flops.step_bytes_per_cell(5) / 3.0
+                hbm = b_stage * ncell / avg_s / 1e9
                 roofline = dict(bound="hbm", kernel=name, achieved=hbm, peak=PEAK_HBM_GBPS, unit="GB/s",
-                                frac=hbm / PEAK_HBM_GBPS, traffic=traffic, bytes_per_cell=bytes_cell,
+                                frac=hbm / PEAK_HBM_GBPS, traffic=traffic,
+                                bytes_per_cell=b_stage,
+                                bytes_rule="compulsory state bytes: 320 B per cell per RK3 step, one stage "
+                                           "launch = 1/3 step",
+                                kernel_own=dict(bytes_per_cell=bytes_cell, achieved_GBps=hbm_own,
+                                                frac=hbm_own / PEAK_HBM_GBPS,
+                                                rule="every input field read once, every output written once"),
                                 fp64_achieved_TFLOPs=fp64)
             else:
                 roofline = dict(bound="mfma", pipe="fp64 vector (VALU); MI355X FP64 peak", kernel=name,
                                 achieved=fp64, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s", frac=fp64 / PEAK_FP64_TFLOPS,
-                                traffic=traffic, flops_per_cell=per_cell, hbm_achieved_GBps=hbm)
+                                traffic=traffic, flops_per_cell=per_cell, hbm_achieved_GBps=hbm_own)
             roofline.update(cells_per_launch=ncell, avg_launch_ms=avg_s * 1e3,
                             timing="HIP events around every launch on the library stream, second pass of "
                                    f"{nprof} steps (graph replay off, one kernel at a time)")
@@ -485,9 +530,35 @@ def main():
     bound_hbm = PEAK_HBM_GBPS * 1e9 / b_alg / 1e6
     bound_fp64 = PEAK_FP64_TFLOPS * 1e12 / f_alg / 1e6
     step_roofline = dict(F_alg_per_cell=f_alg, B_alg_per_cell=b_alg, hbm_bound=bound_hbm, fp64_bound=bound_fp64,
-                         unit="Mcell-steps/s", frac=value / min(bound_hbm, bound_fp64))
+                         unit="Mcell-steps/s", frac=value / min(bound_hbm, bound_fp64),
+                         hbm_frac=value / bound_hbm)
+    # HBM bytes of a whole RK3 step from the counter summary, against the
+    # compulsory 320 B per cell
+    tt = traffic_table()
+    if tt and all(v[1] is not None for v in tt.values()):
+        step_bytes = sum(b * n for b, n in tt.values())
+        step_roofline["step_traffic_ratio"] = step_bytes / (b_alg * ncell)
+        step_roofline["step_traffic_bytes"] = step_bytes
+        step_roofline["step_traffic_source"] = os.path.relpath(TRAFFIC_PROFILE, ROOT)
 
-    cpu = None if args.no_cpu_baseline else cpu_baseline(m)
+    # the type-2 sensor's fast mean is certified: report the certificate of
+    # the last step, then time the ordered (single-wavefront) mean
+    unc, margin, eta = g.hybrid_certificate()
+    g.set_option(m.OPT_HYBRID_SUM_MODE, 0)
+    nord = min(args.steps, 10)
+    step = run_steps(g, info, 2, step)
+    g.synchronize()
+    t = time.perf_counter()
+    step = run_steps(g, info, nord, step)
+    g.synchronize()
+    ordered_ms = (time.perf_counter() - t) / nord * 1e3
+    g.set_option(m.OPT_HYBRID_SUM_MODE, 1)
+    mask = dict(mode="fast tree mean, certified", uncertain_interfaces=unc, margin=margin, eta=eta,
+                mask_certified=unc == 0,
+                masks_equal_ordered="always: uncertain interfaces trigger the ordered mean on the device",
+                ordered_mode_ms_per_step=ordered_ms, ordered_mode_Mcell_steps_per_s=ncell / ordered_ms / 1e3)
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline(m, cfg.nx, cfg.ny, cfg.nz)
     g.close()
     chem, S = (None, None) if args.no_chem else chem_bench(m, args.chem_cells, cpu=not args.no_cpu_baseline,
                                                             surrogate_cells=args.surrogate_train_cells,
@@ -500,13 +571,13 @@ def main():
         "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"Mach 2.25 flat-plate boundary layer {cfg.nx}x{cfg.ny}x{cfg.nz}, one TVD-RK3 step",
-                   "scheme": "HYBRID UD7L/WENO7/WENO5, pressure-gradient sensor (fast mean)",
+                   "scheme": "HYBRID UD7L/WENO7/WENO5, pressure-gradient sensor (certified fast mean)",
                    "splitting": "Steger-Warming", "viscous": "CD6", "gas": "perfect gas, Sutherland",
                    "graph": "hipGraph replay", "parallelism": "single GPU",
                    "streams": 1 if args.serial else 2,
                    "sweep_tiles": {"xyz"[d]: {"NP": tiles[d][1], "TS": tiles[d][2]} for d in range(3)},
                    "mask_fractions": {"x": frac[0], "y": frac[1], "z": frac[2]}},
-        "roofline": roofline, "step_roofline": step_roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "step_roofline": step_roofline, "hybrid_mask": mask, "cpu_baseline": cpu,
         "kernels_ms_per_step": {k: v[0] / max(1, min(args.steps, 20)) for k, v in kstats.items()},
         "chem": chem,
         "jet": jet,

@@ -81,3 +81,37 @@ def step_flops_per_cell(scheme, split=1, mask_fractions=None, viscous=True):
 
 def step_bytes_per_cell(n_var=5):
     return 64 * n_var
+
+
+# Reacting 9-species jet (WENO5, Steger-Warming, CD6, Chemkin/Wilke transport,
+# mixture-averaged diffusion, NASA-fit real gas), per cell per RK3 step, same
+# counting rule.  Per stage: flow sweeps 3 * (SPLIT_SW + CC + 10 * WENO5 + 5 *
+# DIFF) = 2457; species sweeps 9 species * 3 directions * (scalar split 11 +
+# 2 * WENO5 + DIFF) = 3906; species diffusion 9 * (3 CD6 gradients 24, chain
+# rule 15, rho*D*grad c 9, metric projection 15, enthalpy h_k 12 and Qp 6,
+# divergence 27) = 972; transport (mole fractions 27, 9 Chemkin mu fits 63,
+# Wilke 81 Phi * 8 = 648, conductivity fits and mixing 103, 45 binary D fits
+# 315 and the 9 mixture sums 162) = 1318; viscous VISC_PREP + VISC_DIV = 386;
+# updates RK + 9 species * 5 + primitives with Rescaling_di and the Newton
+# temperature (about 10 iterations of a 9-species NASA energy) = 260.
+# 3 * 9299 = 27,897, i.e. about 2.9e4 (+ about 200 exp/log per step).
+JET_STEP = 2.9e4
+
+
+def jet_step_bytes_per_cell(n_spec=9):
+    """Compulsory state bytes of a reacting RK3 step: 5 flow + N_SPEC species
+    variables, 8 reads/writes of each."""
+    return step_bytes_per_cell(5 + n_spec)
+
+
+# Point-implicit chemistry substep of the 9-species / 19-reaction mechanism
+# (CHEM_reactions.f90:230-272, 473-567): forward rates with third-body and
+# Troe falloff terms, Gibbs K_c of 9 species, rates of progress, the analytic
+# Jacobian of the 9 production rates, the 9x9 implicit matrix and its Gauss
+# elimination with partial pivoting, the temperature Newton iteration and
+# the energy update: about 7e3 flops (+ about 70 exp/log/pow).
+CHEM_SUBSTEP = 7.0e3
+
+# Surrogate inference, 12 -> 256 x5 -> 9: 2 * (12*256 + 4*256*256 + 256*9)
+# multiply-adds as flops, per cell (GELU and the checks not counted).
+MLP_CELL = 2 * (12 * 256 + 4 * 256 * 256 + 256 * 9)

@@ -56,6 +56,8 @@ def load(ocfd_module):
         "orc_comput_T_P": (i32, []), "orc_du_species": (i32, [vp]), "orc_download_qp": (i32, [vp]),
         "orc_char_matrices": (i32, [vp, vp, vp]),
         "orc_comput_T": (dbl, [vp, dbl]), "orc_comput_E": (dbl, [vp, dbl]), "orc_comput_Cp": (dbl, [dbl, vp, dbl]),
+        "orc_set_threads": (i32, [i32]),
+        "orc_filtering": (i32, [i32, i32, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -83,6 +85,18 @@ class Oracle:
     def _check(self, rc):
         if rc != 0:
             raise RuntimeError(self.lib.orc_last_error().decode())
+
+    def filtering(self, istep, filters):
+        """filtering (OCFD_filtering.f90:4-76) after Istep = istep; filters as
+        the product's FlowSolver.filtering."""
+        F = self.m.filter_array(filters)
+        self._check(self.lib.orc_filtering(int(istep), len(filters), _ptr(F)))
+
+    def set_threads(self, n):
+        """OpenMP threads of the flow loops (process-wide); returns the count
+        in use.  Results do not depend on it: the loops that run in parallel
+        are per cell or per pencil, and the sums stay sequential."""
+        return self.lib.orc_set_threads(int(n))
 
     field_shape = None  # bound below
 

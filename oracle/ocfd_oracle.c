@@ -32,6 +32,9 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include "../include/ocfd_hip.h"
 
 #define NTCT_MAX 10
@@ -649,6 +652,7 @@ static void dxyz0(int dir, const double* f, double* out) {
   double c1 = 0.8 / h, c2 = -0.2 / h, c3 = 3.80952380952380952e-2 / h, c4 = -3.571428571428571428e-3 / h;
   int cd8 = G.cfg.scheme_vis == OCFD_SCHEME_CD8;
   long st = dir == 0 ? 1 : dir == 1 ? G.nxh : (long)G.nxh * G.nyh;
+#pragma omp parallel for schedule(static)
   for (int k = 1; k <= G.nz; k++)
     for (int j = 1; j <= G.ny; j++)
       for (int i = 1; i <= G.nx; i++) {
@@ -679,6 +683,7 @@ void orc_dz0(const double* f, double* out) { dxyz0(2, f, out); }
 /* ------------------------------------------------------------------ */
 /* comput_cc, Comput_flow_variables.f90:50-63 */
 static void comput_cc(void) {
+#pragma omp parallel for schedule(static)
   for (long q = 0; q < G.nh; q++) {
     double gamma = G.p[q] / G.Et[q] + 1.0;
     G.cc[q] = sqrt(gamma * G.p[q] / G.d[q]);
@@ -692,17 +697,14 @@ static unsigned char set_hybrid_scheme(double R0) {
   return 3;
 }
 
-/* du_inviscous, OCFD_inviscous.f90:47-202 (non-characteristic form) */
+/* du_inviscous, OCFD_inviscous.f90:47-202 (non-characteristic form).
+ * Pencils of one direction are independent (each writes its own cells and
+ * interfaces), so they are shared out over OpenMP threads, each with its own
+ * pencil buffers; the arithmetic per pencil is unchanged. */
 static void du_inviscous(void) {
   int nmax = G.nx > G.ny ? G.nx : G.ny;
   if (G.nz > nmax) nmax = G.nz;
   int lap = G.lap, npm = nmax + 2 * lap;
-  double *Et1 = dalloc(npm, 0), *p1 = dalloc(npm, 0), *c1 = dalloc(npm, 0), *d1 = dalloc(npm, 0);
-  double *u1 = dalloc(npm, 0), *v1 = dalloc(npm, 0), *w1 = dalloc(npm, 0);
-  double *A1 = dalloc(npm, 0), *A2 = dalloc(npm, 0), *A3 = dalloc(npm, 0);
-  double *fp = dalloc(5L * npm, 0), *fm = dalloc(5L * npm, 0);
-  double *hh1 = dalloc(nmax + 1, 0), *hh2 = dalloc(nmax + 1, 0);
-  unsigned char* scm = (unsigned char*)calloc((size_t)nmax + 1, 1);
   int hyb = G.cfg.scheme_invis == OCFD_SCHEME_HYBRID;
   double hx_1 = 1.0 / G.hx, hy_1 = 1.0 / G.hy, hz_1 = 1.0 / G.hz;
   comput_cc();
@@ -715,8 +717,17 @@ static void du_inviscous(void) {
     const int* sb = &G.cfg.scheme_boundary[2 * dir];
     int na = dir == 0 ? G.ny : G.nx, nb = dir == 2 ? G.ny : G.nz;
     unsigned char* mask = dir == 0 ? G.maskx : dir == 1 ? G.masky : G.maskz;
-    for (int b = 1; b <= nb; b++)
-      for (int a = 1; a <= na; a++) {
+#pragma omp parallel
+    {
+      double *Et1 = dalloc(npm, 0), *p1 = dalloc(npm, 0), *c1 = dalloc(npm, 0), *d1 = dalloc(npm, 0);
+      double *u1 = dalloc(npm, 0), *v1 = dalloc(npm, 0), *w1 = dalloc(npm, 0);
+      double *A1 = dalloc(npm, 0), *A2 = dalloc(npm, 0), *A3 = dalloc(npm, 0);
+      double *fp = dalloc(5L * npm, 0), *fm = dalloc(5L * npm, 0);
+      double *hh1 = dalloc(nmax + 1, 0), *hh2 = dalloc(nmax + 1, 0);
+      unsigned char* scm = (unsigned char*)calloc((size_t)nmax + 1, 1);
+#pragma omp for schedule(static)
+      for (long pen = 0; pen < (long)na * nb; pen++) {
+        int a = (int)(pen % na) + 1, b = (int)(pen / na) + 1;
         int i0 = dir == 0 ? 0 : a, j0 = dir == 0 ? a : dir == 1 ? 0 : b, k0 = dir == 2 ? 0 : b;
         long base = H(i0, j0, k0); /* point at s=0 of the pencil */
         if (hyb) {
@@ -752,6 +763,9 @@ static void du_inviscous(void) {
           }
         }
       }
+      free(Et1); free(p1); free(c1); free(d1); free(u1); free(v1); free(w1);
+      free(A1); free(A2); free(A3); free(fp); free(fm); free(hh1); free(hh2); free(scm);
+    }
   }
   if (G.cfg.if_mass_force) { /* OCFD_inviscous.f90:187-199 */
     const double* mf = G.cfg.mass_force;
@@ -765,8 +779,6 @@ static void du_inviscous(void) {
           G.du[c + 4 * G.ni] = G.du[c + 4 * G.ni] + G.d[q] * (G.u[q] * mf[0] + G.v[q] * mf[1] + G.w[q] * mf[2]);
         }
   }
-  free(Et1); free(p1); free(c1); free(d1); free(u1); free(v1); free(w1);
-  free(A1); free(A2); free(A3); free(fp); free(fm); free(hh1); free(hh2); free(scm);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1030,11 +1042,14 @@ static double chemkin_fit(const double* a, double lT) {
 static void comput_Amu_Amk_AMD(void) {
   const double TA = 288.15, TC = 110.4;
   int ns = G.nspec;
+#pragma omp parallel
+  {
   double* di0 = dalloc(ns, 0);
   double* Xi = dalloc(ns, 0);
   double* mui = dalloc(ns, 0);
   double* ki = dalloc(ns, 0);
   double* D2 = dalloc((long)ns * ns, 0);
+#pragma omp for schedule(static)
   for (int k = 1; k <= G.nz; k++)
     for (int j = 1; j <= G.ny; j++)
       for (int i = 1; i <= G.nx; i++) {
@@ -1099,6 +1114,7 @@ static void comput_Amu_Amk_AMD(void) {
         }
       }
   free(di0); free(Xi); free(mui); free(ki); free(D2);
+  }
 }
 
 /* du_viscous, OCFD_viscous.f90:27-131 */
@@ -1108,6 +1124,7 @@ static void du_viscous(void) {
   orc_dy0(G.u, G.ui); orc_dy0(G.v, G.vi); orc_dy0(G.w, G.wi); orc_dy0(G.T, G.Ti);
   orc_dz0(G.u, G.us); orc_dz0(G.v, G.vs); orc_dz0(G.w, G.ws); orc_dz0(G.T, G.Ts);
   long nh = G.nh;
+#pragma omp parallel for schedule(static)
   for (int k = 1; k <= G.nz; k++)
     for (int j = 1; j <= G.ny; j++)
       for (int i = 1; i <= G.nx; i++) {
@@ -1154,6 +1171,7 @@ static void du_viscous(void) {
     orc_dx0(G.Ev1 + m * nh, G.uk);
     orc_dy0(G.Ev2 + m * nh, G.ui);
     orc_dz0(G.Ev3 + m * nh, G.us);
+#pragma omp parallel for schedule(static)
     for (int k = 1; k <= G.nz; k++)
       for (int j = 1; j <= G.ny; j++)
         for (int i = 1; i <= G.nx; i++) {
@@ -1283,8 +1301,13 @@ static void scalar_time_adv_rk3(int krk, int ks, const double* Dus, double dt) {
 static void time_adv_rk3(int krk, double dt) {
   const double Ralfa[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0}, Rbeta[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
   long n = 5 * G.ni;
-  if (krk == 1) for (long c = 0; c < n; c++) G.f[c] = G.fn[c] + dt * G.du[c];
-  else for (long c = 0; c < n; c++) G.f[c] = Ralfa[krk - 1] * G.fn[c] + Rbeta[krk - 1] * (G.f[c] + dt * G.du[c]);
+  if (krk == 1) {
+#pragma omp parallel for schedule(static)
+    for (long c = 0; c < n; c++) G.f[c] = G.fn[c] + dt * G.du[c];
+  } else {
+#pragma omp parallel for schedule(static)
+    for (long c = 0; c < n; c++) G.f[c] = Ralfa[krk - 1] * G.fn[c] + Rbeta[krk - 1] * (G.f[c] + dt * G.du[c]);
+  }
 }
 
 /* handle_NegativeEt, OCFD_handle_NegativeT.f90:1-20 */
@@ -1300,34 +1323,67 @@ static int handle_negative_et(int i, int j, int k) {
 
 /* comput_T_P, Comput_flow_variables.f90:32-46 */
 static void comput_T_P(void) {
-  double* di0 = dalloc(G.nspec, 0);
-  for (int k = 1; k <= G.nz; k++)
-    for (int j = 1; j <= G.ny; j++)
-      for (int i = 1; i <= G.nx; i++) {
-        long q = H(i, j, k);
-        for (int m = 0; m < G.nspec; m++) di0[m] = G.di[q + m * G.nh];
-        G.T[q] = comput_T(di0, G.Et[q]);
-        G.p[q] = comput_P(di0, G.T[q]);
-      }
-  free(di0);
+#pragma omp parallel
+  {
+    double* di0 = dalloc(G.nspec, 0);
+#pragma omp for schedule(static)
+    for (int k = 1; k <= G.nz; k++)
+      for (int j = 1; j <= G.ny; j++)
+        for (int i = 1; i <= G.nx; i++) {
+          long q = H(i, j, k);
+          for (int m = 0; m < G.nspec; m++) di0[m] = G.di[q + m * G.nh];
+          G.T[q] = comput_T(di0, G.Et[q]);
+          G.p[q] = comput_P(di0, G.T[q]);
+        }
+    free(di0);
+  }
 }
 
 /* comput_flow_variables + Rescaling_di, Comput_flow_variables.f90:3-28,101-121 */
 static int comput_flow_variables(void) {
-  int stop = 0;
+  int stop = 0, negative = 0;
   G.num_negt = 0;
+  /* The repair of a cell with Et <= 0 reads its neighbours' Et, new for
+   * those earlier in (k,j,i) order and old for later ones, so the loop runs
+   * in order whenever any cell needs it; a parallel pre-scan with the same
+   * arithmetic finds out. */
+#pragma omp parallel for schedule(static) reduction(| : negative)
   for (int k = 1; k <= G.nz; k++)
     for (int j = 1; j <= G.ny; j++)
       for (int i = 1; i <= G.nx; i++) {
-        long c = I(i, j, k), q = H(i, j, k);
+        long c = I(i, j, k);
         double f1 = G.f[c], f2 = G.f[c + G.ni], f3 = G.f[c + 2 * G.ni], f4 = G.f[c + 3 * G.ni], f5 = G.f[c + 4 * G.ni];
-        G.d[q] = f1;
-        G.u[q] = f2 / f1;
-        G.v[q] = f3 / f1;
-        G.w[q] = f4 / f1;
-        G.Et[q] = f5 - (f2 * G.u[q] + f3 * G.v[q] + f4 * G.w[q]) * 0.5;
-        if (G.Et[q] <= 0) stop |= handle_negative_et(i, j, k);
+        double u = f2 / f1, v = f3 / f1, w = f4 / f1;
+        if (f5 - (f2 * u + f3 * v + f4 * w) * 0.5 <= 0) negative |= 1;
       }
+  if (!negative) {
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= G.nz; k++)
+      for (int j = 1; j <= G.ny; j++)
+        for (int i = 1; i <= G.nx; i++) {
+          long c = I(i, j, k), q = H(i, j, k);
+          double f1 = G.f[c], f2 = G.f[c + G.ni], f3 = G.f[c + 2 * G.ni], f4 = G.f[c + 3 * G.ni], f5 = G.f[c + 4 * G.ni];
+          G.d[q] = f1;
+          G.u[q] = f2 / f1;
+          G.v[q] = f3 / f1;
+          G.w[q] = f4 / f1;
+          G.Et[q] = f5 - (f2 * G.u[q] + f3 * G.v[q] + f4 * G.w[q]) * 0.5;
+        }
+  } else {
+    for (int k = 1; k <= G.nz; k++)
+      for (int j = 1; j <= G.ny; j++)
+        for (int i = 1; i <= G.nx; i++) {
+          long c = I(i, j, k), q = H(i, j, k);
+          double f1 = G.f[c], f2 = G.f[c + G.ni], f3 = G.f[c + 2 * G.ni], f4 = G.f[c + 3 * G.ni], f5 = G.f[c + 4 * G.ni];
+          G.d[q] = f1;
+          G.u[q] = f2 / f1;
+          G.v[q] = f3 / f1;
+          G.w[q] = f4 / f1;
+          G.Et[q] = f5 - (f2 * G.u[q] + f3 * G.v[q] + f4 * G.w[q]) * 0.5;
+          if (G.Et[q] <= 0) stop |= handle_negative_et(i, j, k);
+        }
+  }
+#pragma omp parallel for schedule(static)
   for (int k = 1; k <= G.nz; k++)
     for (int j = 1; j <= G.ny; j++)
       for (int i = 1; i <= G.nx; i++) {
@@ -1518,6 +1574,7 @@ static int comput_rhybrid(int ordered) {
     double *pk = dalloc(G.ni, 0), *pi = dalloc(G.ni, 0), *ps = dalloc(G.ni, 0);
     orc_dx0(G.p, pk); orc_dy0(G.p, pi); orc_dz0(G.p, ps);
     double dp0 = 0.0;
+#pragma omp parallel for schedule(static)
     for (int k = 1; k <= G.nz; k++)
       for (int j = 1; j <= G.ny; j++)
         for (int i = 1; i <= G.nx; i++) {
@@ -1526,8 +1583,11 @@ static int comput_rhybrid(int ordered) {
           double py = pk[c] * G.Aky[q] + pi[c] * G.Aiy[q] + ps[c] * G.Asy[q];
           double pz = pk[c] * G.Akz[q] + pi[c] * G.Aiz[q] + ps[c] * G.Asz[q];
           G.Rhybrid[q] = sqrt(px * px + py * py + pz * pz);
-          dp0 = dp0 + G.Rhybrid[q];
         }
+    /* the global sum in the reference's sequential (k,j,i) order */
+    for (int k = 1; k <= G.nz; k++)
+      for (int j = 1; j <= G.ny; j++)
+        for (int i = 1; i <= G.nx; i++) dp0 = dp0 + G.Rhybrid[H(i, j, k)];
     double dp_av = dp0 / (1.0 * G.nx * G.ny * G.nz);
     double dp_av1 = 1.0 / (dp_av + 1.0e-30);
     G.sensor_dp_av = dp_av;
@@ -1725,6 +1785,121 @@ int orc_ana_box(double* out) {
                   T_aver, Amu_aver, d_aver, Ux_rms, Alamdax};
   for (int m = 0; m < 18; m++) out[m] = r[m];
   return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* filtering, OCFD_filtering.f90:4-362 (Bogey & Bailly 9-point filter and
+ * the shock-capturing filter), called after Istep has been advanced
+ * (OCFD_NS_Solver.f90:45-48).  filter = Para%Filter(100,10) column-major:
+ * Filter(k,m) at filter[(m-1)*100 + (k-1)]. One rank: the global index
+ * ranges are the local ones, clipped to the box. */
+static void filter_fo9p(int dir, double* f0, double s0, const int* lo, const int* hi) {
+  const double d0 = 0.243527493120, d1 = -0.204788880640, d2 = 0.120007591680, d3 = -0.045211119360,
+               d4 = 0.008228661760;
+  int n = dir == 0 ? G.nx : dir == 1 ? G.ny : G.nz;
+  int b[3] = {lo[0], lo[1], lo[2]}, e[3] = {hi[0], hi[1], hi[2]};
+  if (!G.cfg.periodic[dir]) { /* filter_x3d :91-93 */
+    if (b[dir] < 6) b[dir] = 6;
+    if (e[dir] > n - 5) e[dir] = n - 5;
+  }
+  long st = dir == 0 ? 1 : dir == 1 ? G.nxh : (long)G.nxh * G.nyh;
+  for (int m = 0; m < 5; m++) {
+    for (int k = 1; k <= G.nz; k++)
+      for (int j = 1; j <= G.ny; j++)
+        for (int i = 1; i <= G.nx; i++) f0[H(i, j, k)] = G.f[I(i, j, k) + (long)m * G.ni];
+    if (dir == 0) exchange_x(f0); else if (dir == 1) exchange_y(f0); else exchange_z(f0);
+    for (int k = b[2]; k <= e[2]; k++)
+      for (int j = b[1]; j <= e[1]; j++)
+        for (int i = b[0]; i <= e[0]; i++) {
+          const double* F = f0 + H(i, j, k);
+#define F0(o) F[(long)(o) * st]
+          G.f[I(i, j, k) + (long)m * G.ni] =
+              F0(0) - s0 * (d0 * F0(0) + d1 * (F0(-1) + F0(1)) + d2 * (F0(-2) + F0(2)) + d3 * (F0(-3) + F0(3)) +
+                            d4 * (F0(-4) + F0(4)));
+#undef F0
+        }
+  }
+}
+
+static int filter_shock(int dir, double* f0, double s0, double rth, const int* lo, const int* hi) {
+  const double c1 = -0.210383, c2 = 0.039617;
+  int n = dir == 0 ? G.nx : dir == 1 ? G.ny : G.nz;
+  int b[3] = {lo[0], lo[1], lo[2]}, e[3] = {hi[0], hi[1], hi[2]};
+  if (!G.cfg.periodic[dir]) { /* filter_x3d_shock :204-206 */
+    if (b[dir] < 4) b[dir] = 4;
+    if (e[dir] > n - 3) e[dir] = n - 3;
+  }
+  if (comput_flow_variables()) return set_err("oracle: more than 10 negative-Et cells");
+  if (dir == 0) exchange_x(G.p); else if (dir == 1) exchange_y(G.p); else exchange_z(G.p);
+  long st = dir == 0 ? 1 : dir == 1 ? G.nxh : (long)G.nxh * G.nyh;
+  double* sc0 = dalloc(n + 2, 0.0); /* sc0(0:n+1) */
+  for (int m = 0; m < 5; m++) {
+    for (int k = 1; k <= G.nz; k++)
+      for (int j = 1; j <= G.ny; j++)
+        for (int i = 1; i <= G.nx; i++) f0[H(i, j, k)] = G.f[I(i, j, k) + (long)m * G.ni];
+    if (dir == 0) exchange_x(f0); else if (dir == 1) exchange_y(f0); else exchange_z(f0);
+    /* the two directions across the pencil, in the reference's loop order */
+    int a0 = dir == 0 ? 1 : 0, a1 = dir == 2 ? 1 : 2;
+    for (int pb = b[a1]; pb <= e[a1]; pb++)
+      for (int pa = b[a0]; pa <= e[a0]; pa++) {
+        int ijk[3];
+        ijk[a0] = pa; ijk[a1] = pb;
+        for (int s = b[dir] - 1; s <= e[dir] + 1; s++) {
+          ijk[dir] = s;
+          const double* P = G.p + H(ijk[0], ijk[1], ijk[2]);
+#define PP(o) P[(long)(o) * st]
+          double dp0 = 0.25 * (-PP(1) + 2.0 * PP(0) - PP(-1));
+          double dp1 = 0.25 * (-PP(2) + 2.0 * PP(1) - PP(0));
+          double dp2 = 0.25 * (-PP(0) + 2.0 * PP(-1) - PP(-2));
+          double ri = 0.5 * (SQ(dp0 - dp1) + SQ(dp0 - dp2)) / (PP(0) * PP(0)) + 1.0e-16;
+#undef PP
+          sc0[s] = 0.5 * (1.0 - rth / ri + fabs(1.0 - rth / ri));
+        }
+        for (int s = b[dir]; s <= e[dir]; s++) {
+          ijk[dir] = s;
+          double sc1 = 0.5 * (sc0[s] + sc0[s + 1]);
+          double sc2 = 0.5 * (sc0[s] + sc0[s - 1]);
+          const double* F = f0 + H(ijk[0], ijk[1], ijk[2]);
+#define F0(o) F[(long)(o) * st]
+          G.f[I(ijk[0], ijk[1], ijk[2]) + (long)m * G.ni] =
+              F0(0) - s0 * (sc1 * (c1 * (F0(1) - F0(0)) + c2 * (F0(2) - F0(-1))) -
+                            sc2 * (c1 * (F0(0) - F0(-1)) + c2 * (F0(1) - F0(-2))));
+#undef F0
+        }
+      }
+  }
+  free(sc0);
+  return 0;
+}
+
+int orc_filtering(int istep, int nfiltering, const double* filter) {
+  int any = 0;
+  for (int m = 0; m < nfiltering; m++)
+    if (istep % (int)lround(filter[m * 100]) == 0) any = 1;
+  if (!any) return 0;
+  double* f0 = dalloc(G.nh, 0.0);
+  int rc = 0;
+  for (int m = 0; m < nfiltering && !rc; m++) {
+    const double* F = filter + m * 100;
+    if (istep % (int)lround(F[0]) != 0) continue;
+    int on[3] = {(int)lround(F[1]), (int)lround(F[2]), (int)lround(F[3])};
+    int lo[3], hi[3], n[3] = {G.nx, G.ny, G.nz};
+    for (int d = 0; d < 3; d++) {
+      lo[d] = (int)lround(F[4 + 2 * d]);
+      hi[d] = (int)lround(F[5 + 2 * d]);
+      if (lo[d] < 1) lo[d] = 1;
+      if (hi[d] > n[d]) hi[d] = n[d];
+    }
+    int scheme = (int)lround(F[10]);
+    double s0 = F[11], rth = F[12];
+    for (int d = 0; d < 3 && !rc; d++) {
+      if (on[d] != 1) continue;
+      if (scheme == 1) filter_fo9p(d, f0, s0, lo, hi);
+      else if (scheme == 2) rc = filter_shock(d, f0, s0, rth, lo, hi);
+    }
+  }
+  free(f0);
+  return rc;
 }
 
 int orc_negative_et_count(void) { return G.num_negt; }

@@ -430,6 +430,26 @@ def test_step_conserves_on_periodic_box():
     o.close()
 
 
+def test_threads_do_not_change_the_step():
+    # the OpenMP loops are per cell or per pencil and the sensor mean stays a
+    # sequential sum, so one RK3 step of the hybrid flat plate is bit for bit
+    # the same on 1 thread and on several
+    m = load_package()
+    import oracle
+    case = fc.small_flat_plate(40, 24, 12)
+    out = []
+    for threads in (1, 4):
+        o = oracle.Oracle(m, case[0])
+        o.set_threads(threads)
+        fc.setup(o, case)
+        o.flow_time_advance(0.5 * case[5]["dt"])
+        out.append([o.download(k) for k in ("F", "T", "P", "RHYBRID")] + [o.download_mask(d) for d in range(3)])
+        o.close()
+    o.set_threads(0)
+    for a, b in zip(*out):
+        assert a.tobytes() == b.tobytes()
+
+
 def test_hybrid_sensor_closed_forms():
     # type 1: linear pressure -> R = 0; type 2: |grad p| uniform -> R = 1
     m = load_package()
@@ -820,4 +840,109 @@ def test_show_flow_msg_and_ana_box_closed_forms():
     et = 2.5 * cases.R_AIR * d * T
     assert abs(e_kin - d * (U0 ** 2 + V0 ** 2) / 4.0 / ajac) <= 1e-12 * e_kin
     assert abs(e_tot - (et + d * (U0 ** 2 + V0 ** 2) / 4.0) / ajac) <= 1e-12 * e_tot
+    o.close()
+
+
+# --------------------------------------------------------------- filtering
+def _filter_engine(m, n, periodic, F):
+    """An oracle context on a uniform n^3 box holding the conserved field F
+    (n, n, n, 5) with d = F[...,0] and di = d."""
+    import oracle
+    from ocfd_amd import cases
+    cfg = m.make_cfg(n, n, n, periodic=periodic)
+    o = oracle.Oracle(m, cfg)
+    x = np.arange(n) * 0.1
+    o.set_mesh(*cases.mesh_1d(x, x, x), np.array([n * 0.1, 0, 0, 0, n * 0.1, 0, 0, 0, n * 0.1]))
+    o.upload("F", F)
+    di = cases.halo_zeros(n, n, n, 1.0)
+    fc.interior(di)[...] = F[..., 0]
+    o.upload("DI", di[..., None])
+    return o
+
+
+def _row(every=1, xyz=(1, 0, 0), box=(1, 999, 1, 999, 1, 999), scheme=1, s0=1.0, rth=1e-5):
+    return [every, *xyz, *box, scheme, s0, rth]
+
+
+def test_filter_fo9p_closed_forms():
+    """Bogey & Bailly 9-point filter (OCFD_filtering.f90:78-116): the
+    coefficients annihilate constants, linear and quadratic data (d0 + 2 sum
+    d_j = 0, sum j^2 d_j = 0) and remove the odd-even mode entirely at s0 = 1
+    (d0 - 2 d1 + 2 d2 - 2 d3 + 2 d4 = 1); cells outside the box and within 5
+    of a non-periodic face keep their values bit for bit."""
+    m = load_package()
+    n = 16
+    i = np.arange(1, n + 1, dtype=np.float64)[:, None, None] * np.ones((1, n, n))
+    F = np.zeros((n, n, n, 5), order="F")
+    F[..., 0] = 1.0 + 0.01 * i + 0.001 * i * i      # quadratic in x
+    F[..., 1] = 2.0
+    F[..., 2] = 0.5 * (-1.0) ** i                     # odd-even mode in x
+    F[..., 3] = 0.25 * (-1.0) ** i + 3.0
+    F[..., 4] = 2.5e5 + 100.0 * (-1.0) ** i
+    for periodic in ((1, 1, 1), (0, 1, 1)):
+        o = _filter_engine(m, n, periodic, F)
+        o.filtering(4, [_row(every=2)])
+        G = o.download("F")
+        o.close()
+        lo, hi = (0, n) if periodic[0] else (5, n - 5)        # 0-based filtered range in x
+        inner = slice(lo, hi)
+        if not periodic[0]:
+            # the odd-even mode of the quadratic does not exist here: only
+            # the non-periodic case keeps a polynomial through the box
+            assert np.allclose(G[inner, :, :, 0], F[inner, :, :, 0], rtol=0, atol=1e-14)
+        assert np.allclose(G[inner, :, :, 1], 2.0, rtol=0, atol=1e-14)
+        assert np.max(np.abs(G[inner, :, :, 2])) < 1e-15
+        assert np.allclose(G[inner, :, :, 3], 3.0, rtol=0, atol=1e-14)
+        assert np.allclose(G[inner, :, :, 4], 2.5e5, rtol=1e-14, atol=0)
+        if not periodic[0]:
+            for sl in (slice(0, 5), slice(n - 5, n)):
+                assert G[sl].tobytes() == np.asfortranarray(F[sl]).tobytes()
+    # box restriction and the step interval
+    o = _filter_engine(m, n, (1, 1, 1), F)
+    o.filtering(3, [_row(every=2)])                          # mod(3, 2) != 0: nothing
+    assert o.download("F").tobytes() == np.asfortranarray(F).tobytes()
+    o.filtering(6, [_row(every=2, xyz=(1, 0, 0), box=(3, 8, 2, 5, 4, 4))])
+    G = o.download("F")
+    o.close()
+    keep = np.ones((n, n, n), dtype=bool)
+    keep[2:8, 1:5, 3:4] = False
+    assert np.array_equal(G[keep], F[keep])
+    assert np.max(np.abs(G[~keep][:, 2])) < 1e-15
+
+
+def test_filter_directions_and_shock_closed_forms():
+    """Directions run x, y, z in turn, each on the result of the one before;
+    the shock-capturing filter (:196-246) with rth = 0 has sc0 = 1 and removes
+    the odd-even mode at s0 = 1 (4 (c2 - c1) = 1); on a periodic box its flux
+    form conserves sum f; with rth > 0 on a linear pressure (no curvature) it
+    changes nothing, bit for bit; it renews the primitives first."""
+    m = load_package()
+    n = 16
+    I = np.arange(1, n + 1, dtype=np.float64)
+    i, j, k = np.meshgrid(I, I, I, indexing="ij")
+    F = np.zeros((n, n, n, 5), order="F")
+    F[..., 0] = 1.2
+    F[..., 1] = 0.3 * (-1.0) ** (i + j)
+    F[..., 4] = 2.5e5 * (1.0 + 0.01 * np.sin(2 * np.pi * i / n) * np.cos(2 * np.pi * k / n))
+    # x then y removes the odd-even mode in x+y; either direction alone too
+    for xyz in ((1, 1, 0), (1, 0, 0), (0, 1, 0)):
+        o = _filter_engine(m, n, (1, 1, 1), F)
+        o.filtering(1, [_row(xyz=xyz)])
+        assert np.max(np.abs(o.download("F")[..., 1])) < 1e-15
+        o.close()
+    o = _filter_engine(m, n, (1, 1, 1), F)
+    o.filtering(1, [_row(xyz=(1, 1, 1), scheme=2, rth=0.0)])
+    G = o.download("F")
+    assert np.max(np.abs(G[..., 1])) < 1e-15
+    for var in (0, 4):
+        assert abs(G[..., var].sum() - F[..., var].sum()) <= 1e-13 * np.abs(F[..., var]).sum()
+    assert np.array_equal(fc.interior(o.download("D")), F[..., 0])   # comput_flow_variables ran
+    o.close()
+    # linear pressure along a non-periodic x: no curvature, sc0 = 0, no change
+    FL = np.zeros((n, n, n, 5), order="F")
+    FL[..., 0] = 1.0
+    FL[..., 4] = 2.5e5 * (1.0 + 0.02 * i)
+    o = _filter_engine(m, n, (0, 1, 1), FL)
+    o.filtering(1, [_row(xyz=(1, 0, 0), scheme=2, s0=0.7, rth=1e-5)])
+    assert o.download("F").tobytes() == np.asfortranarray(FL).tobytes()
     o.close()
