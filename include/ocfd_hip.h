@@ -128,7 +128,14 @@ enum {
   /* Widens the certificate band of OCFD_OPT_HYBRID_SUM_MODE = 1 to
    * eta * 2^value (0 <= value <= 60; default 0).  Tests use it to force the
    * ordered recomputation; the masks stay the ordered ones for any value. */
-  OCFD_OPT_HYBRID_CERT_WIDEN = 6
+  OCFD_OPT_HYBRID_CERT_WIDEN = 6,
+  /* 1 (default) = the stepping kernels (viscous fluxes, update, pressure
+   * sensor) evaluate the metrics of comput_Jacobian3d from per-row tables of
+   * the nine coordinate derivatives instead of reading the 19 metric arrays,
+   * when ocfd_set_mesh found that the tables give every interior metric bit
+   * for bit (GRID1D meshes); 0 = always read the arrays.  Results are
+   * bitwise independent of the value. */
+  OCFD_OPT_METRIC_TABLES = 7
 };
 
 /* ---- lifetime ---------------------------------------------------------- */
@@ -171,6 +178,9 @@ int32_t ocfd_set_bc_jet(const double* inlet2d);
  * Periodic_JSpan(1:3), Periodic_KSpan(1:3). */
 int32_t ocfd_set_mesh(const double* axx, const double* ayy, const double* azz,
                       const double* span);
+/* 1 when the stepping kernels use the metric tables (OCFD_OPT_METRIC_TABLES),
+ * 0 when they read the metric arrays.  No counterpart in the reference. */
+int32_t ocfd_metric_tables_in_use(void);
 
 /* ---- state transfer ---------------------------------------------------- */
 int32_t ocfd_upload_field(int32_t field, const double* host);
@@ -268,6 +278,18 @@ int32_t ocfd_du_viscous(void);              /* OCFD_viscous.f90:27-131 */
 int32_t ocfd_time_adv_rk3(int32_t krk, double dt); /* OCFD_Time_Adv.f90:98-130 */
 int32_t ocfd_comput_flow_variables(void);   /* Comput_flow_variables.f90:3-46 */
 int32_t ocfd_bc(void);                      /* OCFD_BC.f90:2-188 */
+/* filtering (code/others/OCFD_filtering.f90:4-76), which NS_solver calls
+ * after advancing Istep (code/flow-solver/OCFD_NS_Solver.f90:45-48): every
+ * filter m (1..NFiltering) with mod(istep, Filter(1,m)) == 0 is applied in x,
+ * y, z order where Filter(2..4,m) = 1, over the index box Filter(5..10,m)
+ * (clipped to the domain), with Filter(11,m) = 1 the Bogey & Bailly 9-point
+ * filter (filter_x3d, :78-116) or 2 the shock-capturing filter
+ * (filter_x3d_shock, :196-246, which first renews the primitives by
+ * comput_flow_variables), amplitude s0 = Filter(12,m) and threshold
+ * rth = Filter(13,m).  filter = Para%Filter(100,10), column-major, as the
+ * reference's module holds it.  Only f changes (and, for the shock filter,
+ * the primitives), as in the reference. */
+int32_t ocfd_filtering(int32_t istep, int32_t nfiltering, const double* filter);
 
 /* OCFD2d_flux1 (side 0, flux+) and OCFD2d_flux2 (side 1, flux-) over npen
  * pencils at once (code/flow-solver/OCFD_inviscous.f90:209-281, 286-362).

@@ -41,6 +41,7 @@ METRICS = FIELDS[FIELD["AKX"]:FIELD["ASZ1"] + 1]
 
 OPT_HYBRID_SUM_MODE, OPT_USE_GRAPH, OPT_KERNEL_TIMING, OPT_COLUMN_PLANES, OPT_CONCURRENT = 1, 2, 3, 4, 5
 OPT_HYBRID_CERT_WIDEN = 6
+OPT_METRIC_TABLES = 7
 CHEM_FIXED, CHEM_ADAPTIVE, CHEM_SURROGATE = 0, 1, 2
 
 
@@ -124,7 +125,9 @@ def load_library(path=None):
         "ocfd_comput_rhybrid": (i32, []), "ocfd_du_inviscous": (i32, []), "ocfd_du_viscous": (i32, []),
         "ocfd_time_adv_rk3": (i32, [i32, dbl]), "ocfd_comput_flow_variables": (i32, []),
         "ocfd_bc": (i32, []), "ocfd_negative_et_count": (i32, []),
+        "ocfd_filtering": (i32, [i32, i32, vp]),
         "ocfd_hybrid_certificate": (i32, [P(ctypes.c_int64), P(dbl), P(dbl)]),
+        "ocfd_metric_tables_in_use": (i32, []),
         "ocfd_show_flow_msg": (i32, [vp]), "ocfd_ana_box": (i32, [vp]),
         "ocfd_read_flow_data": (i32, [ctypes.c_char_p, P(i32), P(dbl)]),
         "ocfd_save_flow_data": (i32, [ctypes.c_char_p, i32, dbl]),
@@ -170,6 +173,20 @@ def load_library(path=None):
     if path is None:
         _LIB = lib
     return lib
+
+
+def filter_array(filters):
+    """Para%Filter(100,10) (column-major) holding the given rows of
+    Filter(1:13,m); unused entries are zero."""
+    if len(filters) > 10:
+        raise ValueError("at most 10 filters (NFiltering <= 10)")
+    F = np.zeros((100, 10), dtype=np.float64, order="F")
+    for m, row in enumerate(filters):
+        row = np.asarray(row, dtype=np.float64)
+        if row.shape != (13,):
+            raise ValueError("a filter row holds the 13 values Filter(1:13,m)")
+        F[:13, m] = row
+    return F
 
 
 def _ptr(a):
@@ -367,6 +384,15 @@ class FlowSolver:
     def OCFD_bc(self):
         self._check(self.lib.ocfd_bc())
 
+    def filtering(self, istep, filters):
+        """filtering (OCFD_filtering.f90:4-76) after Istep = istep.  filters:
+        a sequence of up to 10 rows of the 13 Filter(1:13,m) values (step
+        interval, X/Y/Z flags, ib, ie, jb, je, kb, ke, scheme 1 = 9-point or
+        2 = shock-capturing, s0, rth), passed as the reference's
+        Para%Filter(100,10)."""
+        F = filter_array(filters)
+        self._check(self.lib.ocfd_filtering(int(istep), len(filters), _ptr(F)))
+
     def OCFD2d_flux(self, side, scheme, v, bound_index=(0, 0), scheme_boundary=(0, 0), scm=None):
         """OCFD2d_flux1 (side 0) / OCFD2d_flux2 (side 1) over a batch of
         pencils (OCFD_inviscous.f90:209-362).  v is a float64 CUDA tensor of
@@ -426,6 +452,11 @@ class FlowSolver:
 
     def negative_et_count(self):
         return int(self.lib.ocfd_negative_et_count())
+
+    def metric_tables_in_use(self):
+        """True when the stepping kernels evaluate the metrics from the
+        per-row coordinate-derivative tables (OPT_METRIC_TABLES)."""
+        return bool(self.lib.ocfd_metric_tables_in_use())
 
     def synchronize(self):
         self._check(self.lib.ocfd_synchronize())

@@ -79,6 +79,15 @@ struct DevState {
   int jet_iflag;       // BC_Para(1) of the jet
   double* red;         // reduction scratch
   int* negt;           // [0] = count, [1..16] = flagged cell linear indices
+  // Metric tables (mt_on = 1 when they reproduce every interior metric bit
+  // for bit, checked at set-up): the nine coordinate derivatives of
+  // comput_Jacobian3d (OCFD_mesh.f90:74-82) for a mesh whose x depends on i
+  // only, y on j only and z on k only (Iflag_Gridtype = GRID1D).  Diagonal
+  // ones are 1-D; an off-diagonal one is a one-sided difference of a constant,
+  // zero away from non-periodic faces, and depends on two indices.
+  //   xi[nx], yj[ny], zk[nz]; xj, yi [ny][nx]; xk, zi [nz][nx]; yk, zj [nz][ny]
+  int mt_on;
+  const double *mt_xi, *mt_yj, *mt_zk, *mt_xj, *mt_yi, *mt_xk, *mt_zi, *mt_yk, *mt_zj;
 };
 
 __device__ __forceinline__ long hq(const DevState& S, int i, int j, int k) {
@@ -94,6 +103,61 @@ __device__ __forceinline__ int wrap(int s, int n, int per) {
 }
 
 __device__ __forceinline__ double sqr(double x) { return x * x; }
+
+// ---------------------------------------------------------------------------
+// Metrics from the nine coordinate derivatives, comput_Jacobian3d
+// (OCFD_mesh.f90:84-100) and A*1 = A*/Ajac (:39-51), in the reference's
+// expression shapes; k_metrics and the table evaluation share them, so both
+// give the same bits.  Order of m[]: Akx, Aky, Akz, Aix, Aiy, Aiz, Asx, Asy,
+// Asz, Ajac.
+struct Metrics {
+  double kx, ky, kz, ix, iy, iz, sx, sy, sz, jac;
+};
+__device__ __forceinline__ Metrics metrics_from_derivs(double xi1, double yi1, double zi1, double xj1, double yj1,
+                                                       double zj1, double xk1, double yk1, double zk1) {
+  Metrics m;
+  const double Jac1 = 1.0 / (xi1 * yj1 * zk1 + yi1 * zj1 * xk1 + zi1 * xj1 * yk1 - zi1 * yj1 * xk1 - yi1 * xj1 * zk1 -
+                             xi1 * zj1 * yk1);
+  m.jac = Jac1;
+  m.kx = Jac1 * (yj1 * zk1 - zj1 * yk1);
+  m.ky = Jac1 * (zj1 * xk1 - xj1 * zk1);
+  m.kz = Jac1 * (xj1 * yk1 - yj1 * xk1);
+  m.ix = Jac1 * (yk1 * zi1 - zk1 * yi1);
+  m.iy = Jac1 * (zk1 * xi1 - xk1 * zi1);
+  m.iz = Jac1 * (xk1 * yi1 - yk1 * xi1);
+  m.sx = Jac1 * (yi1 * zj1 - zi1 * yj1);
+  m.sy = Jac1 * (zi1 * xj1 - xi1 * zj1);
+  m.sz = Jac1 * (xi1 * yj1 - yi1 * xj1);
+  return m;
+}
+// the table derivatives at interior cell (i, j, k)
+__device__ __forceinline__ Metrics metrics_table(const DevState& S, int i, int j, int k) {
+  const long a = i - 1, b = j - 1, c = k - 1;
+  const long ab = a + (long)S.nx * b, ac = a + (long)S.nx * c, bc = b + (long)S.ny * c;
+  return metrics_from_derivs(S.mt_xi[a], S.mt_yi[ab], S.mt_zi[ac], S.mt_xj[ab], S.mt_yj[b], S.mt_zj[bc], S.mt_xk[ac],
+                             S.mt_yk[bc], S.mt_zk[c]);
+}
+// the interior cell's metrics: evaluated from the tables when they are on,
+// else read from the halo arrays at q
+__device__ __forceinline__ Metrics metrics_at(const DevState& S, long q, int i, int j, int k) {
+  if (S.mt_on) return metrics_table(S, i, j, k);
+  Metrics m;
+  m.kx = S.Akx[q]; m.ky = S.Aky[q]; m.kz = S.Akz[q];
+  m.ix = S.Aix[q]; m.iy = S.Aiy[q]; m.iz = S.Aiz[q];
+  m.sx = S.Asx[q]; m.sy = S.Asy[q]; m.sz = S.Asz[q];
+  m.jac = S.Ajac[q];
+  return m;
+}
+__device__ __forceinline__ double jac_at(const DevState& S, long q, int i, int j, int k) {
+  if (!S.mt_on) return S.Ajac[q];
+  const long a = i - 1, b = j - 1, c = k - 1;
+  const long ab = a + (long)S.nx * b, ac = a + (long)S.nx * c, bc = b + (long)S.ny * c;
+  const double xi1 = S.mt_xi[a], yi1 = S.mt_yi[ab], zi1 = S.mt_zi[ac];
+  const double xj1 = S.mt_xj[ab], yj1 = S.mt_yj[b], zj1 = S.mt_zj[bc];
+  const double xk1 = S.mt_xk[ac], yk1 = S.mt_yk[bc], zk1 = S.mt_zk[c];
+  return 1.0 / (xi1 * yj1 * zk1 + yi1 * zj1 * xk1 + zi1 * xj1 * yk1 - zi1 * yj1 * xk1 - yi1 * xj1 * zk1 -
+                xi1 * zj1 * yk1);
+}
 
 // XCD-aware block order.  The hardware hands workgroup b to XCD b % 8; each
 // XCD has its own L2.  Remapping so that XCD x works on one contiguous range

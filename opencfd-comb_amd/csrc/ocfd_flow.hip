@@ -626,8 +626,8 @@ inline dim3 col_grid(const DevState& S, int kc) {
 // (us, vs, ws, Ts) are the zeta derivatives of u, v, w, T at the cell.
 // INT: the cell's xi and eta stencils are central (no wrap, no closure).
 template <int CD, bool MS, bool INT>
-__device__ __forceinline__ void visc_prep_cell(const DevState& S, long c, long q, int i, int j, double us, double vs,
-                                               double ws, double Ts) {
+__device__ __forceinline__ void visc_prep_cell(const DevState& S, long c, long q, int i, int j, int k, double us,
+                                               double vs, double ws, double Ts) {
   double amu, amk;
   if (MS) {
     amu = S.Amu[c];
@@ -659,9 +659,10 @@ __device__ __forceinline__ void visc_prep_cell(const DevState& S, long c, long q
   };
   const double uk = dh(S.u, 0, i), vk = dh(S.v, 0, i), wk = dh(S.w, 0, i), Tk = dh(S.T, 0, i);
   const double ui = dh(S.u, 1, j), vi = dh(S.v, 1, j), wi = dh(S.w, 1, j), Ti = dh(S.T, 1, j);
-  const double Akx = S.Akx[q], Aky = S.Aky[q], Akz = S.Akz[q];
-  const double Aix = S.Aix[q], Aiy = S.Aiy[q], Aiz = S.Aiz[q];
-  const double Asx = S.Asx[q], Asy = S.Asy[q], Asz = S.Asz[q];
+  const Metrics M = metrics_at(S, q, i, j, k);
+  const double Akx = M.kx, Aky = M.ky, Akz = M.kz;
+  const double Aix = M.ix, Aiy = M.iy, Aiz = M.iz;
+  const double Asx = M.sx, Asy = M.sy, Asz = M.sz;
   const double ux = uk * Akx + ui * Aix + us * Asx;
   const double vx = vk * Akx + vi * Aix + vs * Asx;
   const double wx = wk * Akx + wi * Aix + ws * Asx;
@@ -686,9 +687,16 @@ __device__ __forceinline__ void visc_prep_cell(const DevState& S, long c, long q
   const double E1 = uu * s11 + vv * s12 + ww * s13 + amk * Tx + qx;
   const double E2 = uu * s12 + vv * s22 + ww * s23 + amk * Ty + qy;
   const double E3 = uu * s13 + vv * s23 + ww * s33 + amk * Tz + qz;
-  const double Akx1 = S.Akx1[q], Aky1 = S.Aky1[q], Akz1 = S.Akz1[q];
-  const double Aix1 = S.Aix1[q], Aiy1 = S.Aiy1[q], Aiz1 = S.Aiz1[q];
-  const double Asx1 = S.Asx1[q], Asy1 = S.Asy1[q], Asz1 = S.Asz1[q];
+  double Akx1, Aky1, Akz1, Aix1, Aiy1, Aiz1, Asx1, Asy1, Asz1;
+  if (S.mt_on) {  // Akx1 = Akx/Ajac etc. (OCFD_mesh.f90:39-51)
+    Akx1 = M.kx / M.jac; Aky1 = M.ky / M.jac; Akz1 = M.kz / M.jac;
+    Aix1 = M.ix / M.jac; Aiy1 = M.iy / M.jac; Aiz1 = M.iz / M.jac;
+    Asx1 = M.sx / M.jac; Asy1 = M.sy / M.jac; Asz1 = M.sz / M.jac;
+  } else {
+    Akx1 = S.Akx1[q]; Aky1 = S.Aky1[q]; Akz1 = S.Akz1[q];
+    Aix1 = S.Aix1[q]; Aiy1 = S.Aiy1[q]; Aiz1 = S.Aiz1[q];
+    Asx1 = S.Asx1[q]; Asy1 = S.Asy1[q]; Asz1 = S.Asz1[q];
+  }
   double* Ev = S.Ev + c;
   const long ni = S.ni;
   Ev[0 * ni] = Akx1 * s11 + Aky1 * s12 + Akz1 * s13;
@@ -718,7 +726,7 @@ __global__ __launch_bounds__(256) void k_visc_prep(DevState S) {
   const long plh = (long)S.nxh * S.nyh;
   if (wave_all(central_cell(i, S.nx, R) && central_cell(j, S.ny, R) && central_cell(k, S.nz, R))) {
     auto dz = [&](const double* F) { return deriv1<CD, true>(S, 2, k, [&](int o) { return F[q + o * plh]; }); };
-    visc_prep_cell<CD, MS, true>(S, c, q, i, j, dz(S.u), dz(S.v), dz(S.w), dz(S.T));
+    visc_prep_cell<CD, MS, true>(S, c, q, i, j, k, dz(S.u), dz(S.v), dz(S.w), dz(S.T));
   } else {
     auto dz = [&](const double* F) {
       return deriv1<CD>(S, 2, k, [&](int o) {
@@ -726,7 +734,7 @@ __global__ __launch_bounds__(256) void k_visc_prep(DevState S) {
         return F[q + (long)(t - k) * plh];
       });
     };
-    visc_prep_cell<CD, MS, false>(S, c, q, i, j, dz(S.u), dz(S.v), dz(S.w), dz(S.T));
+    visc_prep_cell<CD, MS, false>(S, c, q, i, j, k, dz(S.u), dz(S.v), dz(S.w), dz(S.T));
   }
 }
 
@@ -755,10 +763,10 @@ __global__ __launch_bounds__(256) void k_visc_prep_col(DevState S, int kc) {
     const long c = ccol + (long)(k - 1) * pli, q = qcol + (long)k * plh;
     if (xy_int && central_cell(k, S.nz, R)) {
       auto dz = [&](const double* z) { return deriv1<CD, true>(S, 2, k, [&](int o) { return z[o + R]; }); };
-      visc_prep_cell<CD, MS, true>(S, c, q, P.i, P.j, dz(zu), dz(zv), dz(zw), dz(zT));
+      visc_prep_cell<CD, MS, true>(S, c, q, P.i, P.j, k, dz(zu), dz(zv), dz(zw), dz(zT));
     } else {
       auto dz = [&](const double* z) { return deriv1<CD>(S, 2, k, [&](int o) { return z[o + R]; }); };
-      visc_prep_cell<CD, MS, false>(S, c, q, P.i, P.j, dz(zu), dz(zv), dz(zw), dz(zT));
+      visc_prep_cell<CD, MS, false>(S, c, q, P.i, P.j, k, dz(zu), dz(zv), dz(zw), dz(zT));
     }
 #pragma unroll
     for (int o = 0; o < W - 1; o++) {
@@ -805,7 +813,7 @@ enum { RK_MASS = 1, RK_VISC = 2, RK_WRITE_DU = 4, RK_UPDATE_F = 8, RK_PRIMS = 16
 // zs[m] is the zeta derivative of Ev3(m+1) at the cell (read only with RK_VISC).
 // INT: the cell's xi and eta stencils are central (no wrap, no closure).
 template <int CD, int F, bool INT>
-__device__ __forceinline__ void rk_cell(const DevState& S, long c, long q, int i, int j, int krk, double dt,
+__device__ __forceinline__ void rk_cell(const DevState& S, long c, long q, int i, int j, int k, int krk, double dt,
                                         int flags_rt, const double* zs) {
   const int flags = F >= 0 ? F : flags_rt;
   const long ni = S.ni;
@@ -833,7 +841,7 @@ __device__ __forceinline__ void rk_cell(const DevState& S, long c, long q, int i
     du[4] = du[4] + d_old * (S.u[q] * S.mf[0] + S.v[q] * S.mf[1] + S.w[q] * S.mf[2]);
   }
   if (flags & RK_VISC) {
-    const double ajac = S.Ajac[q];
+    const double ajac = jac_at(S, q, i, j, k);
     auto dv = [&](const double* Fv, int dir, int s) {
       const double* Fc = Fv + c;
       const int st = dir == 0 ? 1 : S.nx;
@@ -935,7 +943,7 @@ __global__ __launch_bounds__(256) void k_rk(DevState S, int krk, double dt, int 
         zs[m] = deriv1<CD, true>(S, 2, k, [&](int o) { return Fv[o * pli]; });
       }
     }
-    rk_cell<CD, F, true>(S, c, q, i, j, krk, dt, flags_rt, zs);
+    rk_cell<CD, F, true>(S, c, q, i, j, k, krk, dt, flags_rt, zs);
   } else {
     if (flags & RK_VISC) {
 #pragma unroll
@@ -947,7 +955,7 @@ __global__ __launch_bounds__(256) void k_rk(DevState S, int krk, double dt, int 
         });
       }
     }
-    rk_cell<CD, F, false>(S, c, q, i, j, krk, dt, flags_rt, zs);
+    rk_cell<CD, F, false>(S, c, q, i, j, k, krk, dt, flags_rt, zs);
   }
 }
 
@@ -985,11 +993,11 @@ __global__ __launch_bounds__(256) void k_rk_col(DevState S, int kc, int krk, dou
     if (xy_int && central_cell(k, S.nz, R)) {
 #pragma unroll
       for (int m = 0; m < 4; m++) zs[m] = visc ? deriv1<CD, true>(S, 2, k, [&](int o) { return z[m][o + R]; }) : 0.0;
-      rk_cell<CD, F, true>(S, c, q, P.i, P.j, krk, dt, flags_rt, zs);
+      rk_cell<CD, F, true>(S, c, q, P.i, P.j, k, krk, dt, flags_rt, zs);
     } else {
 #pragma unroll
       for (int m = 0; m < 4; m++) zs[m] = visc ? deriv1<CD>(S, 2, k, [&](int o) { return z[m][o + R]; }) : 0.0;
-      rk_cell<CD, F, false>(S, c, q, P.i, P.j, krk, dt, flags_rt, zs);
+      rk_cell<CD, F, false>(S, c, q, P.i, P.j, k, krk, dt, flags_rt, zs);
     }
 #pragma unroll
     for (int o = 0; o < W - 1; o++)
@@ -1371,9 +1379,10 @@ __global__ __launch_bounds__(256) void k_sensor2(DevState S) {
     });
   };
   const double pk = dh(0, i), pi = dh(1, j), ps = dh(2, k);
-  const double px = pk * S.Akx[q] + pi * S.Aix[q] + ps * S.Asx[q];
-  const double py = pk * S.Aky[q] + pi * S.Aiy[q] + ps * S.Asy[q];
-  const double pz = pk * S.Akz[q] + pi * S.Aiz[q] + ps * S.Asz[q];
+  const Metrics M = metrics_at(S, q, i, j, k);
+  const double px = pk * M.kx + pi * M.ix + ps * M.sx;
+  const double py = pk * M.ky + pi * M.iy + ps * M.sy;
+  const double pz = pk * M.kz + pi * M.iz + ps * M.sz;
   S.Rh[q] = sqrt(px * px + py * py + pz * pz);
 }
 
@@ -1608,18 +1617,81 @@ __global__ __launch_bounds__(256) void k_metrics(DevState S) {
   const double xi1 = dh(S.Axx, 0, i), yi1 = dh(S.Ayy, 0, i), zi1 = dh(S.Azz, 0, i);
   const double xj1 = dh(S.Axx, 1, j), yj1 = dh(S.Ayy, 1, j), zj1 = dh(S.Azz, 1, j);
   const double xk1 = dh(S.Axx, 2, k), yk1 = dh(S.Ayy, 2, k), zk1 = dh(S.Azz, 2, k);
-  const double Jac1 = 1.0 / (xi1 * yj1 * zk1 + yi1 * zj1 * xk1 + zi1 * xj1 * yk1 - zi1 * yj1 * xk1 - yi1 * xj1 * zk1 -
-                             xi1 * zj1 * yk1);
-  S.Ajac[q] = Jac1;
-  S.Akx[q] = Jac1 * (yj1 * zk1 - zj1 * yk1);
-  S.Aky[q] = Jac1 * (zj1 * xk1 - xj1 * zk1);
-  S.Akz[q] = Jac1 * (xj1 * yk1 - yj1 * xk1);
-  S.Aix[q] = Jac1 * (yk1 * zi1 - zk1 * yi1);
-  S.Aiy[q] = Jac1 * (zk1 * xi1 - xk1 * zi1);
-  S.Aiz[q] = Jac1 * (xk1 * yi1 - yk1 * xi1);
-  S.Asx[q] = Jac1 * (yi1 * zj1 - zi1 * yj1);
-  S.Asy[q] = Jac1 * (zi1 * xj1 - xi1 * zj1);
-  S.Asz[q] = Jac1 * (xi1 * yj1 - yi1 * xj1);
+  const Metrics m = metrics_from_derivs(xi1, yi1, zi1, xj1, yj1, zj1, xk1, yk1, zk1);
+  S.Ajac[q] = m.jac;
+  S.Akx[q] = m.kx;
+  S.Aky[q] = m.ky;
+  S.Akz[q] = m.kz;
+  S.Aix[q] = m.ix;
+  S.Aiy[q] = m.iy;
+  S.Aiz[q] = m.iz;
+  S.Asx[q] = m.sx;
+  S.Asy[q] = m.sy;
+  S.Asz[q] = m.sz;
+}
+
+// Metric tables: the coordinate derivatives of k_metrics taken once per row
+// (diagonal: along the row's own direction; off-diagonal: on the face plane
+// they depend on).  Thread t of a flat launch writes one table entry.
+__global__ __launch_bounds__(256) void k_mettab_build(DevState S, double* xi, double* yj, double* zk, double* xj,
+                                                      double* yi, double* xk, double* zi, double* yk, double* zj) {
+  const long nxy = (long)S.nx * S.ny, nxz = (long)S.nx * S.nz, nyz = (long)S.ny * S.nz;
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  auto d = [&](const double* F, int dir, int i, int j, int k) {
+    const long q = hq(S, i, j, k);
+    const int s = dir == 0 ? i : dir == 1 ? j : k;
+    return deriv1(S, dir, s, [&](int o) { return F[q + (long)o * S.sh[dir]]; });
+  };
+  if (t < S.nx) { xi[t] = d(S.Axx, 0, (int)t + 1, 1, 1); return; }
+  t -= S.nx;
+  if (t < S.ny) { yj[t] = d(S.Ayy, 1, 1, (int)t + 1, 1); return; }
+  t -= S.ny;
+  if (t < S.nz) { zk[t] = d(S.Azz, 2, 1, 1, (int)t + 1); return; }
+  t -= S.nz;
+  if (t < nxy) {
+    const int i = (int)(t % S.nx) + 1, j = (int)(t / S.nx) + 1;
+    xj[t] = d(S.Axx, 1, i, j, 1);
+    yi[t] = d(S.Ayy, 0, i, j, 1);
+    return;
+  }
+  t -= nxy;
+  if (t < nxz) {
+    const int i = (int)(t % S.nx) + 1, k = (int)(t / S.nx) + 1;
+    xk[t] = d(S.Axx, 2, i, 1, k);
+    zi[t] = d(S.Azz, 0, i, 1, k);
+    return;
+  }
+  t -= nxz;
+  if (t < nyz) {
+    const int j = (int)(t % S.ny) + 1, k = (int)(t / S.ny) + 1;
+    yk[t] = d(S.Ayy, 2, 1, j, k);
+    zj[t] = d(S.Azz, 1, 1, j, k);
+  }
+}
+
+// Counts the interior cells where a table-evaluated metric (the ten of
+// comput_Jacobian3d and the nine A*1) differs in any bit from the stored one.
+__device__ __forceinline__ bool same_bits(double a, double b) { return __double_as_longlong(a) == __double_as_longlong(b); }
+__global__ __launch_bounds__(256) void k_mettab_check(DevState S, unsigned long long* bad) {
+  const long c = cell_index();
+  bool ok = true;
+  if (c < S.ni) {
+    const int i = (int)(c % S.nx) + 1;
+    const int j = (int)((c / S.nx) % S.ny) + 1;
+    const int k = (int)(c / ((long)S.nx * S.ny)) + 1;
+    const long q = hq(S, i, j, k);
+    const Metrics m = metrics_table(S, i, j, k);
+    ok = same_bits(m.kx, S.Akx[q]) && same_bits(m.ky, S.Aky[q]) && same_bits(m.kz, S.Akz[q]) &&
+         same_bits(m.ix, S.Aix[q]) && same_bits(m.iy, S.Aiy[q]) && same_bits(m.iz, S.Aiz[q]) &&
+         same_bits(m.sx, S.Asx[q]) && same_bits(m.sy, S.Asy[q]) && same_bits(m.sz, S.Asz[q]) &&
+         same_bits(m.jac, S.Ajac[q]) && same_bits(m.kx / m.jac, S.Akx1[q]) && same_bits(m.ky / m.jac, S.Aky1[q]) &&
+         same_bits(m.kz / m.jac, S.Akz1[q]) && same_bits(m.ix / m.jac, S.Aix1[q]) &&
+         same_bits(m.iy / m.jac, S.Aiy1[q]) && same_bits(m.iz / m.jac, S.Aiz1[q]) &&
+         same_bits(m.sx / m.jac, S.Asx1[q]) && same_bits(m.sy / m.jac, S.Asy1[q]) &&
+         same_bits(m.sz / m.jac, S.Asz1[q]) && same_bits(jac_at(S, q, i, j, k), S.Ajac[q]);
+  }
+  const unsigned long long nbad = __popcll(__ballot(!ok));
+  if ((threadIdx.x & 63) == 0 && nbad) atomicAdd(bad, nbad);
 }
 
 __global__ __launch_bounds__(256) void k_metrics_over_jac(DevState S) {
@@ -1645,6 +1717,95 @@ __global__ __launch_bounds__(256) void k_copy_interior(DevState S, const double*
   const int k = (int)(c / ((long)S.nx * S.ny)) + 1;
   const long q = hq(S, i, j, k);
   dst_halo[q] = src_halo[q];
+}
+
+// ===========================================================================
+// filtering (OCFD_filtering.f90:4-362): the Bogey & Bailly 9-point filter and
+// the shock-capturing filter along one direction over the index box [b, e]
+// (already clipped to the faces the stencil may reach), reading a copy ft of
+// f taken before the direction's pass, writing f.  One thread per cell of the
+// box, all five conserved variables.
+// ===========================================================================
+struct FiltBox {
+  int b[3], e[3];
+  double s0, rth;
+};
+
+__global__ __launch_bounds__(256) void k_copy5(const double* __restrict__ src, double* __restrict__ dst, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[t] = src[t];
+}
+
+template <int DIR>
+__device__ __forceinline__ bool filt_cell(const DevState& S, const FiltBox& B, int& i, int& j, int& k) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long ni = B.e[0] - B.b[0] + 1, nj = B.e[1] - B.b[1] + 1, nk = B.e[2] - B.b[2] + 1;
+  if (t >= ni * nj * nk) return false;
+  i = B.b[0] + (int)(t % ni);
+  j = B.b[1] + (int)((t / ni) % nj);
+  k = B.b[2] + (int)(t / (ni * nj));
+  return true;
+}
+
+// filter_x3d / filter_y3d / filter_z3d (:78-190)
+template <int DIR>
+__global__ __launch_bounds__(256) void k_filter9(DevState S, FiltBox B, const double* __restrict__ ft) {
+  int i, j, k;
+  if (!filt_cell<DIR>(S, B, i, j, k)) return;
+  const double d0 = 0.243527493120, d1 = -0.204788880640, d2 = 0.120007591680, d3 = -0.045211119360,
+               d4 = 0.008228661760;
+  const int s = DIR == 0 ? i : DIR == 1 ? j : k;
+  const int n = S.n[DIR], per = S.per[DIR];
+  const long c = iq(S, i, j, k);
+  auto at = [&](const double* F, int o) { return F[c + (long)(wrap(s + o, n, per) - s) * S.si[DIR]]; };
+#pragma unroll
+  for (int m = 0; m < 5; m++) {
+    const double* F = ft + (long)m * S.ni;
+    S.f[c + (long)m * S.ni] =
+        at(F, 0) - B.s0 * (d0 * at(F, 0) + d1 * (at(F, -1) + at(F, 1)) + d2 * (at(F, -2) + at(F, 2)) +
+                           d3 * (at(F, -3) + at(F, 3)) + d4 * (at(F, -4) + at(F, 4)));
+  }
+}
+
+// sc0 of filter_x3d_shock (:226-232) at position s of the cell's pencil
+template <int DIR>
+__device__ __forceinline__ double filt_sc0(const DevState& S, long q0, int s0pos, int s, double rth) {
+  const int n = S.n[DIR], per = S.per[DIR];
+  auto P = [&](int o) { return S.p[q0 + (long)(wrap(s + o, n, per) - s0pos) * S.sh[DIR]]; };
+  const double dp0 = 0.25 * (-P(1) + 2.0 * P(0) - P(-1));
+  const double dp1 = 0.25 * (-P(2) + 2.0 * P(1) - P(0));
+  const double dp2 = 0.25 * (-P(0) + 2.0 * P(-1) - P(-2));
+  const double ri = 0.5 * (sqr(dp0 - dp1) + sqr(dp0 - dp2)) / (P(0) * P(0)) + 1.0e-16;
+  return 0.5 * (1.0 - rth / ri + fabs(1.0 - rth / ri));
+}
+
+// filter_x3d_shock / filter_y3d_shock / filter_z3d_shock (:196-360), after
+// comput_flow_variables has renewed p
+template <int DIR>
+__global__ __launch_bounds__(256) void k_filter_shock(DevState S, FiltBox B, const double* __restrict__ ft) {
+  int i, j, k;
+  if (!filt_cell<DIR>(S, B, i, j, k)) return;
+  const double c1 = -0.210383, c2 = 0.039617;
+  const int s = DIR == 0 ? i : DIR == 1 ? j : k;
+  const int n = S.n[DIR], per = S.per[DIR];
+  const long q = hq(S, i, j, k), c = iq(S, i, j, k);
+  const double sm = filt_sc0<DIR>(S, q, s, s - 1, B.rth);
+  const double sc = filt_sc0<DIR>(S, q, s, s, B.rth);
+  const double sp = filt_sc0<DIR>(S, q, s, s + 1, B.rth);
+  const double sc1 = 0.5 * (sc + sp);
+  const double sc2 = 0.5 * (sc + sm);
+  auto at = [&](const double* F, int o) { return F[c + (long)(wrap(s + o, n, per) - s) * S.si[DIR]]; };
+#pragma unroll
+  for (int m = 0; m < 5; m++) {
+    const double* F = ft + (long)m * S.ni;
+    S.f[c + (long)m * S.ni] =
+        at(F, 0) - B.s0 * (sc1 * (c1 * (at(F, 1) - at(F, 0)) + c2 * (at(F, 2) - at(F, -1))) -
+                           sc2 * (c1 * (at(F, 0) - at(F, -1)) + c2 * (at(F, 1) - at(F, -2))));
+  }
+}
+
+__global__ void k_zero_u64(unsigned long long* a) {
+  if (threadIdx.x == 0) *a = 0ull;
 }
 
 __global__ void k_fill(double* a, long n, double v) {
@@ -1685,6 +1846,10 @@ struct Ctx {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_qp = nullptr;
   int concurrent = 1;
   int use_graph = 0;
+  int metric_tables = 1;     // OCFD_OPT_METRIC_TABLES
+  double* filt_tmp = nullptr;  // ocfd_filtering's copy of f
+  bool mt_valid = false;     // the tables reproduce every interior metric
+  unsigned long long* mt_bad = nullptr;
   int timing = 0;
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
@@ -2158,6 +2323,9 @@ int32_t ocfd_finalize(void) {
   G.chem_rho = G.chem_E = G.chem_dt = nullptr;
   G.part = nullptr;
   G.diag = nullptr;
+  G.filt_tmp = nullptr;
+  G.mt_bad = nullptr;
+  G.mt_valid = false;
   for (int d = 0; d < 3; d++) {
     G.kc[d] = nullptr;
     G.kct[d] = nullptr;
@@ -2368,6 +2536,27 @@ int32_t ocfd_init(const ocfd_cfg* cfg) {
                      (long)S.ny * 6 + 6 + (long)S.nx * S.nz, 0.0);
   if (hipHostMalloc((void**)&G.h_negt, sizeof(int)) != hipSuccess) return fail("ocfd_init: hipHostMalloc failed");
   *G.h_negt = 0;
+  {
+    // metric tables (off until a mesh has been set and checked)
+    const long nxy = (long)S.nx * S.ny, nxz = (long)S.nx * S.nz, nyz = (long)S.ny * S.nz;
+    double* t = dalloc((long)S.nx + S.ny + S.nz + 2 * (nxy + nxz + nyz));
+    void* bad = nullptr;
+    if (!t || hipMalloc(&bad, sizeof(unsigned long long)) != hipSuccess) return fail("ocfd_init: hipMalloc failed");
+    G.allocs.push_back(bad);
+    G.mt_bad = (unsigned long long*)bad;
+    S.mt_xi = t;
+    S.mt_yj = t + S.nx;
+    S.mt_zk = t + S.nx + S.ny;
+    t += S.nx + S.ny + S.nz;
+    S.mt_xj = t;
+    S.mt_yi = t + nxy;
+    S.mt_xk = t + 2 * nxy;
+    S.mt_zi = t + 2 * nxy + nxz;
+    S.mt_yk = t + 2 * nxy + 2 * nxz;
+    S.mt_zj = t + 2 * nxy + 2 * nxz + nyz;
+    S.mt_on = 0;
+    G.mt_valid = false;
+  }
   HIPCHK(hipStreamSynchronize(G.stream));
   return 0;
 }
@@ -2386,6 +2575,11 @@ int32_t ocfd_set_option(int32_t key, int32_t value) {
     case OCFD_OPT_COLUMN_PLANES:
       if (value < -1) return fail("ocfd_set_option: column planes must be >= -1");
       G.col_planes = value;
+      drop_graph();
+      return 0;
+    case OCFD_OPT_METRIC_TABLES:
+      G.metric_tables = value ? 1 : 0;
+      G.S.mt_on = G.metric_tables && G.mt_valid;
       drop_graph();
       return 0;
   }
@@ -2529,6 +2723,25 @@ static int mesh_setup(const double* axx, const double* ayy, const double* azz, c
       hipLaunchKernelGGL(k_jac_ghost2_metrics, grid1(ghost_count(f)), dim3(256), 0, G.stream, S, f);
     }
   hipLaunchKernelGGL(k_metrics_over_jac, grid1(S.nh), dim3(256), 0, G.stream, S);
+  // Metric tables: built from the coordinates as k_metrics differentiates
+  // them, then compared with every stored interior metric; they replace the
+  // arrays in the stepping kernels only if no bit differs (a mesh that is not
+  // GRID1D, or metrics given by the caller, fail the check).
+  S.mt_on = 0;
+  G.mt_valid = false;
+  if (!metrics) {
+    const long nt = (long)S.nx + S.ny + S.nz + (long)S.nx * S.ny + (long)S.nx * S.nz + (long)S.ny * S.nz;
+    hipLaunchKernelGGL(k_mettab_build, grid1(nt), dim3(256), 0, G.stream, S, (double*)S.mt_xi, (double*)S.mt_yj,
+                       (double*)S.mt_zk, (double*)S.mt_xj, (double*)S.mt_yi, (double*)S.mt_xk, (double*)S.mt_zi,
+                       (double*)S.mt_yk, (double*)S.mt_zj);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, G.stream, G.mt_bad);
+    hipLaunchKernelGGL(k_mettab_check, grid1(S.ni), dim3(256), 0, G.stream, S, G.mt_bad);
+    unsigned long long bad = 1;
+    HIPCHK(hipMemcpyAsync(&bad, G.mt_bad, sizeof bad, hipMemcpyDeviceToHost, G.stream));
+    HIPCHK(hipStreamSynchronize(G.stream));
+    G.mt_valid = bad == 0;
+    S.mt_on = G.metric_tables && G.mt_valid;
+  }
   HIPCHK(hipStreamSynchronize(G.stream));
   HIPCHK(hipGetLastError());
   drop_graph();
@@ -2539,6 +2752,8 @@ int32_t ocfd_set_mesh(const double* axx, const double* ayy, const double* azz, c
   if (check_ready()) return 1;
   return mesh_setup(axx, ayy, azz, span, nullptr);
 }
+
+int32_t ocfd_metric_tables_in_use(void) { return G.S.mt_on ? 1 : 0; }
 
 // ---------------------------------------------------------------------------
 // The reference's files: Fortran unformatted sequential records, each framed
@@ -2883,6 +3098,64 @@ int32_t ocfd_comput_flow_variables(void) {
   HIPCHK(hipGetLastError());
   return 0;
 }
+// filtering (OCFD_filtering.f90:4-76), called by NS_solver after Istep has
+// been advanced (OCFD_NS_Solver.f90:45-48)
+int32_t ocfd_filtering(int32_t istep, int32_t nfiltering, const double* filter) {
+  if (check_ready()) return 1;
+  if (nfiltering < 0 || nfiltering > 10) return fail("ocfd_filtering: NFiltering must be 0..10");
+  if (nfiltering > 0 && !filter) return fail("ocfd_filtering: null Filter array");
+  const DevState& S = G.S;
+  bool any = false;
+  for (int m = 0; m < nfiltering; m++) {
+    const long every = std::lround(filter[m * 100]);
+    if (every < 1) return fail("ocfd_filtering: Filter(1,m) (the step interval) must be >= 1");
+    if (istep % every == 0) any = true;
+  }
+  if (!any) return 0;
+  if (!G.filt_tmp) {
+    G.filt_tmp = dalloc(5 * S.ni);
+    if (!G.filt_tmp) return fail("ocfd_filtering: hipMalloc failed");
+  }
+  const int n[3] = {S.nx, S.ny, S.nz};
+  for (int m = 0; m < nfiltering; m++) {
+    const double* F = filter + m * 100;
+    if (istep % std::lround(F[0]) != 0) continue;
+    const int scheme = (int)std::lround(F[10]);
+    FiltBox B;
+    for (int d = 0; d < 3; d++) {
+      B.b[d] = std::max((int)std::lround(F[4 + 2 * d]), 1);
+      B.e[d] = std::min((int)std::lround(F[5 + 2 * d]), n[d]);
+    }
+    B.s0 = F[11];
+    B.rth = F[12];
+    for (int d = 0; d < 3; d++) {
+      if (std::lround(F[1 + d]) != 1 || (scheme != 1 && scheme != 2)) continue;
+      FiltBox Bd = B;
+      if (!S.per[d]) {  // the stencil stays inside the box (:91-93, :204-206)
+        Bd.b[d] = std::max(Bd.b[d], scheme == 1 ? 6 : 4);
+        Bd.e[d] = std::min(Bd.e[d], n[d] - (scheme == 1 ? 5 : 3));
+      }
+      const long cnt = (long)std::max(0, Bd.e[0] - Bd.b[0] + 1) * std::max(0, Bd.e[1] - Bd.b[1] + 1) *
+                       std::max(0, Bd.e[2] - Bd.b[2] + 1);
+      if (scheme == 2 && ocfd_comput_flow_variables()) return 1;
+      if (cnt <= 0) continue;
+      launch("k_copy5", k_copy5, grid1(5 * S.ni), dim3(256), (const double*)S.f, G.filt_tmp, 5 * S.ni);
+      const double* ft = G.filt_tmp;
+      if (scheme == 1) {
+        if (d == 0) launch("k_filter9", k_filter9<0>, grid1(cnt), dim3(256), S, Bd, ft);
+        if (d == 1) launch("k_filter9", k_filter9<1>, grid1(cnt), dim3(256), S, Bd, ft);
+        if (d == 2) launch("k_filter9", k_filter9<2>, grid1(cnt), dim3(256), S, Bd, ft);
+      } else {
+        if (d == 0) launch("k_filter_shock", k_filter_shock<0>, grid1(cnt), dim3(256), S, Bd, ft);
+        if (d == 1) launch("k_filter_shock", k_filter_shock<1>, grid1(cnt), dim3(256), S, Bd, ft);
+        if (d == 2) launch("k_filter_shock", k_filter_shock<2>, grid1(cnt), dim3(256), S, Bd, ft);
+      }
+    }
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 int32_t ocfd_bc(void) {
   if (check_ready()) return 1;
   enq_bc();

@@ -124,6 +124,30 @@ def test_rk3_step_flat_plate_hybrid(pkg, sum_mode):
     g.close()
 
 
+def test_metric_tables_bitwise(pkg):
+    """GRID1D meshes (the periodic box; the flat plate, stretched in y with
+    walls in x and y) switch the stepping kernels to the per-row metric
+    tables; three RK3 steps with the tables equal three with the 19 metric
+    arrays bit for bit, eagerly and under graph replay."""
+    for case in (fc.small_isotropic(16), fc.small_flat_plate(48, 32, 16)):
+        dt1 = 0.5 * case[5]["dt"]
+        out = []
+        for tables, graph in ((0, 0), (1, 0), (1, 1)):
+            g = pkg.FlowSolver(case[0])
+            g.set_option(pkg.OPT_METRIC_TABLES, tables)
+            g.set_option(pkg.OPT_USE_GRAPH, graph)
+            fc.setup(g, case)
+            assert g.metric_tables_in_use() == bool(tables)
+            for _ in range(3):
+                g.flow_time_advance(dt1)
+            g.synchronize()
+            out.append({n: g.download(n) for n in ("F", "RHYBRID") + fc.PRIMS})
+            g.close()
+        for k, o in enumerate(out[1:], 1):
+            for n in o:
+                assert np.array_equal(o[n].view(np.uint64), out[0][n].view(np.uint64)), (k, n)
+
+
 def test_concurrent_streams_bitwise(pkg):
     """The viscous preparation on a second stream beside the sweeps
     (OPT_CONCURRENT = 1, the default) gives the one-stream run bit for bit,
@@ -502,6 +526,7 @@ def test_grid3d_curvilinear_metrics_and_step(pkg):
         a, b = g.download(name), o.download(name)
         assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), name
     assert np.max(np.abs(fc.interior(o.download("AKY")))) > 1e-3   # off-diagonal metrics are not zero
+    assert not g.metric_tables_in_use()   # a non-separable mesh keeps the metric arrays
     dt1 = 0.25 * info["dt"]
     g.flow_time_advance(dt1)
     o.flow_time_advance(dt1)
