@@ -102,6 +102,9 @@ module ocfd_hip
     integer(c_int32_t) function ocfd_bc() bind(C, name="ocfd_bc")
       import
     end function
+    integer(c_int32_t) function ocfd_filtering(istep, nfiltering, filter) bind(C, name="ocfd_filtering")
+      import; integer(c_int32_t), value :: istep, nfiltering; real(c_double), intent(in) :: filter(100, 10)
+    end function
     integer(c_int32_t) function ocfd_du_scaler_invis(m) bind(C, name="ocfd_du_scaler_invis")
       import; integer(c_int32_t), value :: m
     end function
