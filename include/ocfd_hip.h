@@ -129,12 +129,15 @@ enum {
    * eta * 2^value (0 <= value <= 60; default 0).  Tests use it to force the
    * ordered recomputation; the masks stay the ordered ones for any value. */
   OCFD_OPT_HYBRID_CERT_WIDEN = 6,
-  /* 1 (default) = the stepping kernels (viscous fluxes, update, pressure
-   * sensor) evaluate the metrics of comput_Jacobian3d from per-row tables of
-   * the nine coordinate derivatives instead of reading the 19 metric arrays,
+  /* 1 = the stepping kernels (viscous fluxes, update, pressure sensor)
+   * evaluate the metrics of comput_Jacobian3d from per-row tables of the
+   * nine coordinate derivatives instead of reading the 19 metric arrays,
    * when ocfd_set_mesh found that the tables give every interior metric bit
-   * for bit (GRID1D meshes); 0 = always read the arrays.  Results are
-   * bitwise independent of the value. */
+   * for bit (GRID1D meshes); 0 (default) = always read the arrays.  The
+   * tables move fewer bytes but cost ten FP64 divisions per cell in the
+   * viscous preparation, which made the stepping kernels slower on MI355X
+   * (flat plate 640x120x64: 558/578 us vs 487/484 us per launch).  Results
+   * are bitwise independent of the value. */
   OCFD_OPT_METRIC_TABLES = 7
 };
 

@@ -137,10 +137,12 @@ __device__ __forceinline__ Metrics metrics_table(const DevState& S, int i, int j
   return metrics_from_derivs(S.mt_xi[a], S.mt_yi[ab], S.mt_zi[ac], S.mt_xj[ab], S.mt_yj[b], S.mt_zj[bc], S.mt_xk[ac],
                              S.mt_yk[bc], S.mt_zk[c]);
 }
-// the interior cell's metrics: evaluated from the tables when they are on,
-// else read from the halo arrays at q
+// the interior cell's metrics: evaluated from the tables (MT, a compile-time
+// choice so that a kernel carries one form only), else read from the halo
+// arrays at q
+template <bool MT>
 __device__ __forceinline__ Metrics metrics_at(const DevState& S, long q, int i, int j, int k) {
-  if (S.mt_on) return metrics_table(S, i, j, k);
+  if constexpr (MT) return metrics_table(S, i, j, k);
   Metrics m;
   m.kx = S.Akx[q]; m.ky = S.Aky[q]; m.kz = S.Akz[q];
   m.ix = S.Aix[q]; m.iy = S.Aiy[q]; m.iz = S.Aiz[q];
@@ -148,8 +150,9 @@ __device__ __forceinline__ Metrics metrics_at(const DevState& S, long q, int i, 
   m.jac = S.Ajac[q];
   return m;
 }
+template <bool MT>
 __device__ __forceinline__ double jac_at(const DevState& S, long q, int i, int j, int k) {
-  if (!S.mt_on) return S.Ajac[q];
+  if constexpr (!MT) return S.Ajac[q];
   const long a = i - 1, b = j - 1, c = k - 1;
   const long ab = a + (long)S.nx * b, ac = a + (long)S.nx * c, bc = b + (long)S.ny * c;
   const double xi1 = S.mt_xi[a], yi1 = S.mt_yi[ab], zi1 = S.mt_zi[ac];

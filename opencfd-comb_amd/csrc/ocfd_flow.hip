@@ -625,7 +625,7 @@ inline dim3 col_grid(const DevState& S, int kc) {
 // the species-diffusion enthalpy flux Qp from k_species_visc_prep.
 // (us, vs, ws, Ts) are the zeta derivatives of u, v, w, T at the cell.
 // INT: the cell's xi and eta stencils are central (no wrap, no closure).
-template <int CD, bool MS, bool INT>
+template <int CD, bool MS, bool INT, bool MT>
 __device__ __forceinline__ void visc_prep_cell(const DevState& S, long c, long q, int i, int j, int k, double us,
                                                double vs, double ws, double Ts) {
   double amu, amk;
@@ -659,7 +659,7 @@ __device__ __forceinline__ void visc_prep_cell(const DevState& S, long c, long q
   };
   const double uk = dh(S.u, 0, i), vk = dh(S.v, 0, i), wk = dh(S.w, 0, i), Tk = dh(S.T, 0, i);
   const double ui = dh(S.u, 1, j), vi = dh(S.v, 1, j), wi = dh(S.w, 1, j), Ti = dh(S.T, 1, j);
-  const Metrics M = metrics_at(S, q, i, j, k);
+  const Metrics M = metrics_at<MT>(S, q, i, j, k);
   const double Akx = M.kx, Aky = M.ky, Akz = M.kz;
   const double Aix = M.ix, Aiy = M.iy, Aiz = M.iz;
   const double Asx = M.sx, Asy = M.sy, Asz = M.sz;
@@ -688,7 +688,7 @@ __device__ __forceinline__ void visc_prep_cell(const DevState& S, long c, long q
   const double E2 = uu * s12 + vv * s22 + ww * s23 + amk * Ty + qy;
   const double E3 = uu * s13 + vv * s23 + ww * s33 + amk * Tz + qz;
   double Akx1, Aky1, Akz1, Aix1, Aiy1, Aiz1, Asx1, Asy1, Asz1;
-  if (S.mt_on) {  // Akx1 = Akx/Ajac etc. (OCFD_mesh.f90:39-51)
+  if constexpr (MT) {  // Akx1 = Akx/Ajac etc. (OCFD_mesh.f90:39-51)
     Akx1 = M.kx / M.jac; Aky1 = M.ky / M.jac; Akz1 = M.kz / M.jac;
     Aix1 = M.ix / M.jac; Aiy1 = M.iy / M.jac; Aiz1 = M.iz / M.jac;
     Asx1 = M.sx / M.jac; Asy1 = M.sy / M.jac; Asz1 = M.sz / M.jac;
@@ -714,7 +714,7 @@ __device__ __forceinline__ void visc_prep_cell(const DevState& S, long c, long q
 }
 
 // one thread per cell
-template <int CD, bool MS>
+template <int CD, bool MS, bool MT>
 __global__ __launch_bounds__(256) void k_visc_prep(DevState S) {
   constexpr int R = CD == 1 ? 4 : 3;
   const long c = cell_index();
@@ -726,7 +726,7 @@ __global__ __launch_bounds__(256) void k_visc_prep(DevState S) {
   const long plh = (long)S.nxh * S.nyh;
   if (wave_all(central_cell(i, S.nx, R) && central_cell(j, S.ny, R) && central_cell(k, S.nz, R))) {
     auto dz = [&](const double* F) { return deriv1<CD, true>(S, 2, k, [&](int o) { return F[q + o * plh]; }); };
-    visc_prep_cell<CD, MS, true>(S, c, q, i, j, k, dz(S.u), dz(S.v), dz(S.w), dz(S.T));
+    visc_prep_cell<CD, MS, true, MT>(S, c, q, i, j, k, dz(S.u), dz(S.v), dz(S.w), dz(S.T));
   } else {
     auto dz = [&](const double* F) {
       return deriv1<CD>(S, 2, k, [&](int o) {
@@ -734,12 +734,12 @@ __global__ __launch_bounds__(256) void k_visc_prep(DevState S) {
         return F[q + (long)(t - k) * plh];
       });
     };
-    visc_prep_cell<CD, MS, false>(S, c, q, i, j, k, dz(S.u), dz(S.v), dz(S.w), dz(S.T));
+    visc_prep_cell<CD, MS, false, MT>(S, c, q, i, j, k, dz(S.u), dz(S.v), dz(S.w), dz(S.T));
   }
 }
 
 // marching over k planes with the z windows of u, v, w, T in registers
-template <int CD, bool MS>
+template <int CD, bool MS, bool MT>
 __global__ __launch_bounds__(256) void k_visc_prep_col(DevState S, int kc) {
   constexpr int R = CD == 1 ? 4 : 3, W = 2 * R + 1;
   ColPos P;
@@ -763,10 +763,10 @@ __global__ __launch_bounds__(256) void k_visc_prep_col(DevState S, int kc) {
     const long c = ccol + (long)(k - 1) * pli, q = qcol + (long)k * plh;
     if (xy_int && central_cell(k, S.nz, R)) {
       auto dz = [&](const double* z) { return deriv1<CD, true>(S, 2, k, [&](int o) { return z[o + R]; }); };
-      visc_prep_cell<CD, MS, true>(S, c, q, P.i, P.j, k, dz(zu), dz(zv), dz(zw), dz(zT));
+      visc_prep_cell<CD, MS, true, MT>(S, c, q, P.i, P.j, k, dz(zu), dz(zv), dz(zw), dz(zT));
     } else {
       auto dz = [&](const double* z) { return deriv1<CD>(S, 2, k, [&](int o) { return z[o + R]; }); };
-      visc_prep_cell<CD, MS, false>(S, c, q, P.i, P.j, k, dz(zu), dz(zv), dz(zw), dz(zT));
+      visc_prep_cell<CD, MS, false, MT>(S, c, q, P.i, P.j, k, dz(zu), dz(zv), dz(zw), dz(zT));
     }
 #pragma unroll
     for (int o = 0; o < W - 1; o++) {
@@ -812,7 +812,7 @@ enum { RK_MASS = 1, RK_VISC = 2, RK_WRITE_DU = 4, RK_UPDATE_F = 8, RK_PRIMS = 16
 // the flags at run time (per-stage entry points).
 // zs[m] is the zeta derivative of Ev3(m+1) at the cell (read only with RK_VISC).
 // INT: the cell's xi and eta stencils are central (no wrap, no closure).
-template <int CD, int F, bool INT>
+template <int CD, int F, bool INT, bool MT>
 __device__ __forceinline__ void rk_cell(const DevState& S, long c, long q, int i, int j, int k, int krk, double dt,
                                         int flags_rt, const double* zs) {
   const int flags = F >= 0 ? F : flags_rt;
@@ -841,7 +841,7 @@ __device__ __forceinline__ void rk_cell(const DevState& S, long c, long q, int i
     du[4] = du[4] + d_old * (S.u[q] * S.mf[0] + S.v[q] * S.mf[1] + S.w[q] * S.mf[2]);
   }
   if (flags & RK_VISC) {
-    const double ajac = jac_at(S, q, i, j, k);
+    const double ajac = jac_at<MT>(S, q, i, j, k);
     auto dv = [&](const double* Fv, int dir, int s) {
       const double* Fc = Fv + c;
       const int st = dir == 0 ? 1 : S.nx;
@@ -923,7 +923,7 @@ __device__ __forceinline__ void rk_cell(const DevState& S, long c, long q, int i
 }
 
 // one thread per cell
-template <int CD, int F>
+template <int CD, int F, bool MT>
 __global__ __launch_bounds__(256) void k_rk(DevState S, int krk, double dt, int flags_rt) {
   constexpr int R = CD == 1 ? 4 : 3;
   const int flags = F >= 0 ? F : flags_rt;
@@ -943,7 +943,7 @@ __global__ __launch_bounds__(256) void k_rk(DevState S, int krk, double dt, int 
         zs[m] = deriv1<CD, true>(S, 2, k, [&](int o) { return Fv[o * pli]; });
       }
     }
-    rk_cell<CD, F, true>(S, c, q, i, j, k, krk, dt, flags_rt, zs);
+    rk_cell<CD, F, true, MT>(S, c, q, i, j, k, krk, dt, flags_rt, zs);
   } else {
     if (flags & RK_VISC) {
 #pragma unroll
@@ -955,12 +955,12 @@ __global__ __launch_bounds__(256) void k_rk(DevState S, int krk, double dt, int 
         });
       }
     }
-    rk_cell<CD, F, false>(S, c, q, i, j, k, krk, dt, flags_rt, zs);
+    rk_cell<CD, F, false, MT>(S, c, q, i, j, k, krk, dt, flags_rt, zs);
   }
 }
 
 // marching over k planes with the z windows of Ev3(1:4) in registers
-template <int CD, int F>
+template <int CD, int F, bool MT>
 __global__ __launch_bounds__(256) void k_rk_col(DevState S, int kc, int krk, double dt, int flags_rt) {
   constexpr int R = CD == 1 ? 4 : 3, W = 2 * R + 1;
   const int flags = F >= 0 ? F : flags_rt;
@@ -993,11 +993,11 @@ __global__ __launch_bounds__(256) void k_rk_col(DevState S, int kc, int krk, dou
     if (xy_int && central_cell(k, S.nz, R)) {
 #pragma unroll
       for (int m = 0; m < 4; m++) zs[m] = visc ? deriv1<CD, true>(S, 2, k, [&](int o) { return z[m][o + R]; }) : 0.0;
-      rk_cell<CD, F, true>(S, c, q, P.i, P.j, k, krk, dt, flags_rt, zs);
+      rk_cell<CD, F, true, MT>(S, c, q, P.i, P.j, k, krk, dt, flags_rt, zs);
     } else {
 #pragma unroll
       for (int m = 0; m < 4; m++) zs[m] = visc ? deriv1<CD>(S, 2, k, [&](int o) { return z[m][o + R]; }) : 0.0;
-      rk_cell<CD, F, false>(S, c, q, P.i, P.j, k, krk, dt, flags_rt, zs);
+      rk_cell<CD, F, false, MT>(S, c, q, P.i, P.j, k, krk, dt, flags_rt, zs);
     }
 #pragma unroll
     for (int o = 0; o < W - 1; o++)
@@ -1365,6 +1365,7 @@ __global__ __launch_bounds__(256) void k_sensor1(DevState S) {
             fabs((zm - 2.0 * p0 + zp) / (zm + 2.0 * p0 + zp));
 }
 
+template <bool MT>
 __global__ __launch_bounds__(256) void k_sensor2(DevState S) {
   const long c = cell_index();
   if (c >= S.ni) return;
@@ -1379,36 +1380,67 @@ __global__ __launch_bounds__(256) void k_sensor2(DevState S) {
     });
   };
   const double pk = dh(0, i), pi = dh(1, j), ps = dh(2, k);
-  const Metrics M = metrics_at(S, q, i, j, k);
+  const Metrics M = metrics_at<MT>(S, q, i, j, k);
   const double px = pk * M.kx + pi * M.ix + ps * M.sx;
   const double py = pk * M.ky + pi * M.iy + ps * M.sy;
   const double pz = pk * M.kz + pi * M.iz + ps * M.sz;
   S.Rh[q] = sqrt(px * px + py * py + pz * pz);
 }
 
-// Ordered mean: one lane sums in the reference's (k,j,i) order, so the
-// result is bit-identical to the single-rank MPI_ALLREDUCE of :64.
+// Ordered mean: one lane adds in the reference's (k,j,i) order, so the
+// result is bit-identical to the single-rank MPI_ALLREDUCE of :64.  The
+// other 63 lanes of the (single) wave fetch the next chunk of SUMC values
+// while lane 0 adds the current one from LDS, so the add chain, not the
+// load latency, sets the time.
+constexpr int SUMC = 1024;
 __device__ void k_sum_ordered_body(DevState S) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double dp0 = 0.0;
-  for (int k = 1; k <= S.nz; k++)
-    for (int j = 1; j <= S.ny; j++) {
-      const double* row = S.Rh + hq(S, 1, j, k);
-      int i = 0;
-      for (; i + 8 <= S.nx; i += 8) {
-        double r[8];
+  __shared__ double buf[SUMC];
+  if (blockIdx.x != 0) return;
+  const int lane = threadIdx.x;
+  const long N = S.ni;
+  double r[SUMC / 64];
+  auto fetch = [&](long base) {
 #pragma unroll
-        for (int a = 0; a < 8; a++) r[a] = row[i + a];
-#pragma unroll
-        for (int a = 0; a < 8; a++) dp0 = dp0 + r[a];
+    for (int a = 0; a < SUMC / 64; a++) {
+      const long c = base + a * 64 + lane;
+      if (c < N) {
+        const int i = (int)(c % S.nx) + 1;
+        const int j = (int)((c / S.nx) % S.ny) + 1;
+        const int k = (int)(c / ((long)S.nx * S.ny)) + 1;
+        r[a] = S.Rh[hq(S, i, j, k)];
+      } else {
+        r[a] = 0.0;
       }
-      for (; i < S.nx; i++) dp0 = dp0 + row[i];
     }
-  const double dp_av = dp0 / (1.0 * S.nx * S.ny * S.nz);
-  S.red[0] = 1.0 / (dp_av + 1.0e-30);
-  S.red[1] = dp_av;
+  };
+  double dp0 = 0.0;
+  fetch(0);
+  for (long base = 0; base < N; base += SUMC) {
+#pragma unroll
+    for (int a = 0; a < SUMC / 64; a++) buf[a * 64 + lane] = r[a];
+    __syncthreads();
+    if (base + SUMC < N) fetch(base + SUMC);
+    if (lane == 0) {
+      const int m = (int)min((long)SUMC, N - base);
+      int a = 0;
+      for (; a + 8 <= m; a += 8) {
+        double t[8];
+#pragma unroll
+        for (int b = 0; b < 8; b++) t[b] = buf[a + b];
+#pragma unroll
+        for (int b = 0; b < 8; b++) dp0 = dp0 + t[b];
+      }
+      for (; a < m; a++) dp0 = dp0 + buf[a];
+    }
+    __syncthreads();
+  }
+  if (lane == 0) {
+    const double dp_av = dp0 / (1.0 * S.nx * S.ny * S.nz);
+    S.red[0] = 1.0 / (dp_av + 1.0e-30);
+    S.red[1] = dp_av;
+  }
 }
-__global__ void k_sum_ordered(DevState S) { k_sum_ordered_body(S); }
+__global__ __launch_bounds__(64) void k_sum_ordered(DevState S) { k_sum_ordered_body(S); }
 
 // Fast (non-ordered) mean of the type-2 sensor: per-thread strided sums, a
 // wavefront tree by __shfl_xor, the four wave sums in fixed order; then one
@@ -1449,7 +1481,7 @@ __global__ __launch_bounds__(64) void k_sum_final(DevState S, const double* part
 // The sequential mean again, only when the certificate of the fast one
 // failed (k_masks counted an interface inside the rounding band); otherwise
 // the lane exits at once.  Launched on one wave so it costs nothing when idle.
-__global__ void k_sum_ordered_if(DevState S) {
+__global__ __launch_bounds__(64) void k_sum_ordered_if(DevState S) {
   if (reinterpret_cast<const unsigned long long*>(S.red)[4] == 0ull) return;
   k_sum_ordered_body(S);
 }
@@ -1518,63 +1550,85 @@ __global__ __launch_bounds__(256) void k_sensor_norm(DevState S) {
 // relative band hyb_eta of a threshold could flip between the fast and the
 // ordered mean; such interfaces are counted (red[4]) and the smallest
 // relative distance to a threshold is kept (red[5]) for the report.
-template <int DIR, int SCALE>
-__device__ __forceinline__ void mask_one(const DevState& S, long t, bool& unsure, double& dist) {
-  const int n = S.n[DIR];
-  const long npen = DIR == 0 ? (long)S.ny * S.nz : DIR == 1 ? (long)S.nx * S.nz : (long)S.nx * S.ny;
-  long pen;
-  int s;
-  if (DIR == 0) { s = (int)(t % (n + 1)); pen = t / (n + 1); }
-  else { pen = t % npen; s = (int)(t / npen); }
-  int a0, a1;
-  pencil_ijk<DIR>(S, pen, a0, a1);
-  // exchanged Rhybrid: periodic faces wrap; non-periodic ghosts keep 0
-  const double r0 = S.Rh[sweep_hq<DIR>(S, a0, a1, s)];
-  const double r1 = S.Rh[sweep_hq<DIR>(S, a0, a1, s + 1)];
-  const double R0 = SCALE == 0 ? fmax(r0, r1) : fmax(r0, r1) * S.red[0];
-  const uint8_t m = R0 <= S.hyb_t1 ? 1 : (R0 <= S.hyb_t2 ? 2 : 3);
-  S.mask[DIR][mask_index<DIR>(S, pen, s)] = m;
+// One kernel builds the masks of all three directions: thread (i, j, k), 0 <=
+// i <= nx, 0 <= j <= ny, 0 <= k <= nz, owns interface s = i of the x-pencil
+// (j, k), s = j of the y-pencil (i, k) and s = k of the z-pencil (i, j), each
+// where that pencil exists (its other two indices >= 1).  Workgroups take
+// 64 x 4 (i, j) tiles of one k plane from a fixed grid, so the index
+// arithmetic is 32-bit and per tile; Rh(i, j, k) is read once for the three
+// interfaces.  REDO: the masks again with the ordered mean, only when the
+// certified launch counted an uncertain interface (otherwise every workgroup
+// reads one counter and exits).
+constexpr int MASK_BLOCKS = 2048, MK_TX = 64, MK_TY = 4;
+template <int SCALE>
+__device__ __forceinline__ uint8_t mask_of(const DevState& S, double r0, double r1, double inv, long& nunsure,
+                                           double& dist) {
+  const double R0 = SCALE == 0 ? fmax(r0, r1) : fmax(r0, r1) * inv;
   if (SCALE == 2) {
     const double e = S.hyb_eta;
-    unsure = (R0 >= S.hyb_t1 * (1.0 - e) && R0 <= S.hyb_t1 * (1.0 + e)) ||
-             (R0 >= S.hyb_t2 * (1.0 - e) && R0 <= S.hyb_t2 * (1.0 + e));
-    dist = fmin(fabs(R0 * S.hyb_inv_t1 - 1.0), fabs(R0 * S.hyb_inv_t2 - 1.0));
+    const bool unsure = (R0 >= S.hyb_t1 * (1.0 - e) && R0 <= S.hyb_t1 * (1.0 + e)) ||
+                        (R0 >= S.hyb_t2 * (1.0 - e) && R0 <= S.hyb_t2 * (1.0 + e));
+    nunsure += unsure ? 1 : 0;
+    dist = fmin(dist, fmin(fabs(R0 * S.hyb_inv_t1 - 1.0), fabs(R0 * S.hyb_inv_t2 - 1.0)));
   }
+  return R0 <= S.hyb_t1 ? 1 : (R0 <= S.hyb_t2 ? 2 : 3);
 }
 
-template <int DIR, int SCALE>
+template <int SCALE, bool REDO>
 __global__ __launch_bounds__(256) void k_masks(DevState S) {
-  const int n = S.n[DIR];
-  const long npen = DIR == 0 ? (long)S.ny * S.nz : DIR == 1 ? (long)S.nx * S.nz : (long)S.nx * S.ny;
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = t < npen * (n + 1);
-  bool unsure = false;
+  if (REDO && reinterpret_cast<const unsigned long long*>(S.red)[4] == 0ull) return;
+  const int nx = S.nx, ny = S.ny, nz = S.nz;
+  const int nti = (nx + MK_TX) / MK_TX, ntj = (ny + MK_TY) / MK_TY;  // tiles over 0..nx, 0..ny
+  const int ntiles = nti * ntj * (nz + 1);
+  const double inv = SCALE == 0 ? 1.0 : S.red[0];
+  long nunsure = 0;
   double dist = __builtin_inf();
-  if (live) mask_one<DIR, SCALE>(S, t, unsure, dist);
-  if (SCALE == 2) {
-    const unsigned long long b = __ballot(unsure);
-    for (int o = 32; o > 0; o >>= 1) dist = fmin(dist, __shfl_xor(dist, o, 64));
-    if ((threadIdx.x & 63) == 0) {
-      unsigned long long* cnt = reinterpret_cast<unsigned long long*>(S.red);
-      if (b) atomicAdd(cnt + 4, (unsigned long long)__popcll(b));
-      // non-negative doubles order as their bit patterns
-      if (dist < __builtin_inf()) atomicMin(cnt + 5, (unsigned long long)__double_as_longlong(dist));
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int ti = tile % nti, r = tile / nti;
+    const int tj = r % ntj, k = r / ntj;
+    const int i = ti * MK_TX + (int)(threadIdx.x % MK_TX);
+    const int j = tj * MK_TY + (int)(threadIdx.x / MK_TX);
+    if (i > nx || j > ny) continue;
+    // the exchanged Rhybrid: periodic faces wrap; non-periodic ghosts keep 0
+    const int iw = wrap(i, nx, S.per[0]), jw = wrap(j, ny, S.per[1]), kw = wrap(k, nz, S.per[2]);
+    const long q = hq(S, iw, jw, kw);
+    const double r0 = S.Rh[q];
+    if (j >= 1 && k >= 1) {
+      const double r1 = S.Rh[q + (long)(wrap(i + 1, nx, S.per[0]) - iw)];
+      S.mask[0][i + (long)(nx + 1) * ((j - 1) + (long)ny * (k - 1))] = mask_of<SCALE>(S, r0, r1, inv, nunsure, dist);
+    }
+    if (i >= 1 && k >= 1) {
+      const double r1 = S.Rh[q + (long)(wrap(j + 1, ny, S.per[1]) - jw) * S.sh[1]];
+      S.mask[1][(i - 1) + (long)nx * (j + (long)(ny + 1) * (k - 1))] = mask_of<SCALE>(S, r0, r1, inv, nunsure, dist);
+    }
+    if (i >= 1 && j >= 1) {
+      const double r1 = S.Rh[q + (long)(wrap(k + 1, nz, S.per[2]) - kw) * S.sh[2]];
+      S.mask[2][(i - 1) + (long)nx * ((j - 1) + (long)ny * k)] = mask_of<SCALE>(S, r0, r1, inv, nunsure, dist);
     }
   }
-}
-
-// Masks again from the ordered mean, only when k_masks<.., 2> counted an
-// uncertain interface (a fixed grid striding over the interfaces, so the
-// idle case is a few hundred waves that read one counter and exit).
-template <int DIR>
-__global__ __launch_bounds__(256) void k_masks_redo(DevState S) {
-  if (reinterpret_cast<const unsigned long long*>(S.red)[4] == 0ull) return;
-  const int n = S.n[DIR];
-  const long npen = DIR == 0 ? (long)S.ny * S.nz : DIR == 1 ? (long)S.nx * S.nz : (long)S.nx * S.ny;
-  bool unsure;
-  double dist;
-  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < npen * (n + 1); t += (long)gridDim.x * blockDim.x)
-    mask_one<DIR, 1>(S, t, unsure, dist);
+  if (SCALE == 2) {
+    // certificate counters, reduced in the workgroup first so that the
+    // counters see one atomic per workgroup
+    __shared__ unsigned long long sU[4];
+    __shared__ double sD[4];
+    for (int o = 32; o > 0; o >>= 1) {
+      dist = fmin(dist, __shfl_xor(dist, o, 64));
+      nunsure += __shfl_xor(nunsure, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      sU[threadIdx.x >> 6] = (unsigned long long)nunsure;
+      sD[threadIdx.x >> 6] = dist;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const unsigned long long u = sU[0] + sU[1] + sU[2] + sU[3];
+      const double d = fmin(fmin(sD[0], sD[1]), fmin(sD[2], sD[3]));
+      unsigned long long* cnt = reinterpret_cast<unsigned long long*>(S.red);
+      if (u) atomicAdd(cnt + 4, u);
+      // non-negative doubles order as their bit patterns
+      if (d < __builtin_inf()) atomicMin(cnt + 5, (unsigned long long)__double_as_longlong(d));
+    }
+  }
 }
 
 // ===========================================================================
@@ -1688,7 +1742,7 @@ __global__ __launch_bounds__(256) void k_mettab_check(DevState S, unsigned long 
          same_bits(m.kz / m.jac, S.Akz1[q]) && same_bits(m.ix / m.jac, S.Aix1[q]) &&
          same_bits(m.iy / m.jac, S.Aiy1[q]) && same_bits(m.iz / m.jac, S.Aiz1[q]) &&
          same_bits(m.sx / m.jac, S.Asx1[q]) && same_bits(m.sy / m.jac, S.Asy1[q]) &&
-         same_bits(m.sz / m.jac, S.Asz1[q]) && same_bits(jac_at(S, q, i, j, k), S.Ajac[q]);
+         same_bits(m.sz / m.jac, S.Asz1[q]) && same_bits(jac_at<true>(S, q, i, j, k), S.Ajac[q]);
   }
   const unsigned long long nbad = __popcll(__ballot(!ok));
   if ((threadIdx.x & 63) == 0 && nbad) atomicAdd(bad, nbad);
@@ -1846,7 +1900,7 @@ struct Ctx {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_qp = nullptr;
   int concurrent = 1;
   int use_graph = 0;
-  int metric_tables = 1;     // OCFD_OPT_METRIC_TABLES
+  int metric_tables = 0;     // OCFD_OPT_METRIC_TABLES (off: the divisions cost more than the loads)
   double* filt_tmp = nullptr;  // ocfd_filtering's copy of f
   bool mt_valid = false;     // the tables reproduce every interior metric
   unsigned long long* mt_bad = nullptr;
@@ -1962,8 +2016,13 @@ template <int CD, int F>
 void launch_rk_f(int krk, double dt, int flags) {
   const DevState& S = G.S;
   const int kc = col_planes();
-  if (kc > 0) launch("k_rk", k_rk_col<CD, F>, col_grid(S, kc), dim3(256), S, kc, krk, dt, flags);
-  else launch("k_rk", k_rk<CD, F>, grid1(S.ni), dim3(256), S, krk, dt, flags);
+  if (S.mt_on) {
+    if (kc > 0) launch("k_rk", k_rk_col<CD, F, true>, col_grid(S, kc), dim3(256), S, kc, krk, dt, flags);
+    else launch("k_rk", k_rk<CD, F, true>, grid1(S.ni), dim3(256), S, krk, dt, flags);
+  } else {
+    if (kc > 0) launch("k_rk", k_rk_col<CD, F, false>, col_grid(S, kc), dim3(256), S, kc, krk, dt, flags);
+    else launch("k_rk", k_rk<CD, F, false>, grid1(S.ni), dim3(256), S, krk, dt, flags);
+  }
 }
 
 template <int CD>
@@ -1988,13 +2047,18 @@ void launch_visc_prep(bool ms, hipStream_t st = nullptr) {
   const DevState& S = G.S;
   if (!st) st = G.stream;
   const int kc = col_planes();
+  const bool mt = S.mt_on;
   if (kc > 0) {
-    auto kern = ms ? (S.cd8 ? k_visc_prep_col<1, true> : k_visc_prep_col<0, true>)
-                   : (S.cd8 ? k_visc_prep_col<1, false> : k_visc_prep_col<0, false>);
+    auto kern = mt ? (ms ? (S.cd8 ? k_visc_prep_col<1, true, true> : k_visc_prep_col<0, true, true>)
+                         : (S.cd8 ? k_visc_prep_col<1, false, true> : k_visc_prep_col<0, false, true>))
+                   : (ms ? (S.cd8 ? k_visc_prep_col<1, true, false> : k_visc_prep_col<0, true, false>)
+                         : (S.cd8 ? k_visc_prep_col<1, false, false> : k_visc_prep_col<0, false, false>));
     launch_s(st, "k_visc_prep", kern, col_grid(S, kc), dim3(256), S, kc);
   } else {
-    auto kern = ms ? (S.cd8 ? k_visc_prep<1, true> : k_visc_prep<0, true>)
-                   : (S.cd8 ? k_visc_prep<1, false> : k_visc_prep<0, false>);
+    auto kern = mt ? (ms ? (S.cd8 ? k_visc_prep<1, true, true> : k_visc_prep<0, true, true>)
+                         : (S.cd8 ? k_visc_prep<1, false, true> : k_visc_prep<0, false, true>))
+                   : (ms ? (S.cd8 ? k_visc_prep<1, true, false> : k_visc_prep<0, true, false>)
+                         : (S.cd8 ? k_visc_prep<1, false, false> : k_visc_prep<0, false, false>));
     launch_s(st, "k_visc_prep", kern, grid1(S.ni), dim3(256), S);
   }
 }
@@ -2074,11 +2138,12 @@ void launch_sweep() {
   launch_sweep_variant<DIR>(G.sweep_variant[DIR]);
 }
 
-template <int DIR, int SCALE>
+template <int SCALE, bool REDO = false>
 void launch_masks() {
   const DevState& S = G.S;
-  const long npen = DIR == 0 ? (long)S.ny * S.nz : DIR == 1 ? (long)S.nx * S.nz : (long)S.nx * S.ny;
-  launch("k_masks", k_masks<DIR, SCALE>, grid1(npen * (S.n[DIR] + 1)), dim3(256), S);
+  const long nt = (long)((S.nx + MK_TX) / MK_TX) * ((S.ny + MK_TY) / MK_TY) * (S.nz + 1);
+  launch(REDO ? "k_masks_redo" : "k_masks", k_masks<SCALE, REDO>, dim3((unsigned)std::min<long>(nt, MASK_BLOCKS)),
+         dim3(256), S);
 }
 
 void enq_rhybrid() {
@@ -2087,29 +2152,22 @@ void enq_rhybrid() {
   const int sensor = (int)std::lround(G.cfg.hybrid_para[0]);
   if (sensor == 1) {
     launch("k_sensor1", k_sensor1, grid1(S.ni), dim3(256), S);
-    launch_masks<0, 0>();
-    launch_masks<1, 0>();
-    launch_masks<2, 0>();
+    launch_masks<0>();
     return;
   }
-  launch("k_sensor2", k_sensor2, grid1(S.ni), dim3(256), S);
+  if (S.mt_on) launch("k_sensor2", k_sensor2<true>, grid1(S.ni), dim3(256), S);
+  else launch("k_sensor2", k_sensor2<false>, grid1(S.ni), dim3(256), S);
   if (G.hybrid_sum_mode == 0) {
     launch("k_sum_ordered", k_sum_ordered, dim3(1), dim3(64), S);
-    launch_masks<0, 1>();
-    launch_masks<1, 1>();
-    launch_masks<2, 1>();
+    launch_masks<1>();
   } else {
     // fast mean, certified: the masks are the ordered mean's in every case
     G.hyb_cert_ran = true;
     launch("k_sum_partial", k_sum_partial, dim3(G.nparts), dim3(256), S, G.part);
     launch("k_sum_final", k_sum_final, dim3(1), dim3(64), S, (const double*)G.part, G.nparts);
-    launch_masks<0, 2>();
-    launch_masks<1, 2>();
-    launch_masks<2, 2>();
+    launch_masks<2>();
     launch("k_sum_ordered_if", k_sum_ordered_if, dim3(1), dim3(64), S);
-    launch("k_masks_redo", k_masks_redo<0>, dim3(512), dim3(256), S);
-    launch("k_masks_redo", k_masks_redo<1>, dim3(512), dim3(256), S);
-    launch("k_masks_redo", k_masks_redo<2>, dim3(512), dim3(256), S);
+    launch_masks<1, true>();
   }
   launch("k_sensor_norm", k_sensor_norm, grid1(S.ni), dim3(256), S);
 }
