@@ -580,7 +580,9 @@ def main():
                    "splitting": "Steger-Warming", "viscous": "CD6", "gas": "perfect gas, Sutherland",
                    "graph": "hipGraph replay", "parallelism": "single GPU",
                    "streams": 1 if args.serial else 2,
-                   "sweep_tiles": {"xyz"[d]: {"NP": tiles[d][1], "TS": tiles[d][2]} for d in range(3)},
+                   "sweep_tiles": {"xyz"[d]: {"NP": tiles[d][1], "TS": tiles[d][2],
+                                              "ms_per_variant": [round(x, 4) for x in tiles[d][3]]}
+                                   for d in range(3)},
                    "mask_fractions": {"x": frac[0], "y": frac[1], "z": frac[2]}},
         "roofline": roofline, "step_roofline": step_roofline, "hybrid_mask": mask, "cpu_baseline": cpu,
         "kernels_ms_per_step": {k: v[0] / max(1, min(args.steps, 20)) for k, v in kstats.items()},

@@ -37,25 +37,39 @@ int chem_surrogate_batch(long n, double* rho, double* E, const double* dt, bool 
 // are i-rows (coalesced along the sweep); for DIR 1/2 the NP pencils are
 // adjacent in i, so loads coalesce across pencils.
 // ===========================================================================
-// Tile shapes (pencils per workgroup NP x cells per pencil TS) offered to
-// the autotuner; variant 0 is the default, the shape that measured fastest
-// inside steps of the flat plate, and the autotuner leaves it only for a
-// shape more than 3% faster.  Every variant computes every
-// interface with the same arithmetic, so results do not depend on the choice.
+// Tile shapes offered to the autotuner: NP pencils per workgroup x TS cells
+// per pencil, on NT threads.  NT is 4 or 8 wavefronts: a workgroup of 5
+// wavefronts puts two of them on one SIMD, which then holds back every
+// barrier (measured 1.5x slower on the flat plate).  NP (TS+8) <= NT makes
+// the flux splitting of the tile's points one pass; for DIR 1/2, NP >= 8
+// keeps the loads at 64 B or more per row.  Variant 0 is the default, the
+// shape that measured fastest inside steps of the flat plate; the autotuner
+// leaves it only for a shape more than 3% faster.  Every variant computes
+// every interface with the same arithmetic, so results do not depend on the
+// choice.
 template <int DIR, int V> struct SweepTile;
-template <> struct SweepTile<0, 0> { static constexpr int NP = 4, TS = 64; };
-template <> struct SweepTile<0, 1> { static constexpr int NP = 8, TS = 32; };
-template <> struct SweepTile<0, 2> { static constexpr int NP = 2, TS = 128; };
-template <> struct SweepTile<0, 3> { static constexpr int NP = 8, TS = 64; };
-template <> struct SweepTile<1, 0> { static constexpr int NP = 8, TS = 32; };
-template <> struct SweepTile<1, 1> { static constexpr int NP = 16, TS = 16; };
-template <> struct SweepTile<1, 2> { static constexpr int NP = 16, TS = 32; };
-template <> struct SweepTile<1, 3> { static constexpr int NP = 32, TS = 16; };
-template <> struct SweepTile<2, 0> { static constexpr int NP = 8, TS = 32; };
-template <> struct SweepTile<2, 1> { static constexpr int NP = 16, TS = 16; };
-template <> struct SweepTile<2, 2> { static constexpr int NP = 16, TS = 32; };
-template <> struct SweepTile<2, 3> { static constexpr int NP = 32, TS = 16; };
+template <> struct SweepTile<0, 0> { static constexpr int NP = 4, TS = 56, NT = 256; };
+template <> struct SweepTile<0, 1> { static constexpr int NP = 4, TS = 64, NT = 256; };
+template <> struct SweepTile<0, 2> { static constexpr int NP = 2, TS = 120, NT = 256; };
+template <> struct SweepTile<0, 3> { static constexpr int NP = 8, TS = 56, NT = 512; };
+template <> struct SweepTile<1, 0> { static constexpr int NP = 8, TS = 24, NT = 256; };
+template <> struct SweepTile<1, 1> { static constexpr int NP = 8, TS = 32, NT = 256; };
+template <> struct SweepTile<1, 2> { static constexpr int NP = 16, TS = 24, NT = 512; };
+template <> struct SweepTile<1, 3> { static constexpr int NP = 8, TS = 56, NT = 512; };
+template <> struct SweepTile<2, 0> { static constexpr int NP = 8, TS = 24, NT = 256; };
+template <> struct SweepTile<2, 1> { static constexpr int NP = 8, TS = 32, NT = 256; };
+template <> struct SweepTile<2, 2> { static constexpr int NP = 16, TS = 24, NT = 512; };
+template <> struct SweepTile<2, 3> { static constexpr int NP = 8, TS = 56, NT = 512; };
 #define OCFD_SWEEP_VARIANTS 4
+
+// Interfaces per thread of the linear pass: the smallest run length L for
+// which the 10 NP lines (5 components x 2 sides x NP pencils) of NIF
+// interfaces, cut into runs of L, fit in NT threads.
+constexpr int sweep_run(int NP, int NIF, int NT) {
+  int L = 1;
+  while (10 * NP * ((NIF + L - 1) / L) > NT) L++;
+  return L;
+}
 
 template <int DIR>
 __device__ __forceinline__ void pencil_ijk(const DevState& S, long pen, int& a0, int& a1) {
@@ -175,18 +189,19 @@ __device__ __forceinline__ void split_point(int kind, double Et1, double p1, dou
 enum { KC_NONE = 0, KC_WRITE = 1, KC_READ = 2 };
 
 template <int DIR, int V>
-__global__ __launch_bounds__(256) void k_sweep(DevState S, int kmode, unsigned short* kc, int* kct) {
-  constexpr int NP = SweepTile<DIR, V>::NP, TS = SweepTile<DIR, V>::TS;
+__global__ __launch_bounds__((SweepTile<DIR, V>::NT)) void k_sweep(DevState S, int kmode, unsigned short* kc,
+                                                                   int* kct) {
+  constexpr int NP = SweepTile<DIR, V>::NP, TS = SweepTile<DIR, V>::TS, NT = SweepTile<DIR, V>::NT;
+  constexpr int NW = NT / 64;
   constexpr int NPT = TS + 8, NIF = TS + 1, NE = 2 * NP * NIF;
-  constexpr int MAXT = (5 * NE + 255) / 256;
-  static_assert(10 * NP * NIF <= 10 * NP * NPT, "interface values must fit in the split buffer");
+  static_assert(NT % 64 == 0, "whole wavefronts");
   static_assert(NE < 0x1000, "entry index must fit in 12 bits");
-  // one LDS buffer: split fluxes (phase 1-2), then interface values (phase 3)
+  // split fluxes (phase 1-2) and interface values (phase 2-3)
   __shared__ double sF[10 * NP * NPT];
-  double* sH = sF;
+  __shared__ double sH[10 * NP * NIF];
   __shared__ unsigned short sList[NE];
   __shared__ unsigned char sKind[NE];
-  __shared__ int sCnt[4][10];
+  __shared__ int sCnt[NW][10];
   __shared__ int sTotal;
   __shared__ PencilBase<DIR, NP> PB;
   const int n = S.n[DIR];
@@ -209,7 +224,7 @@ __global__ __launch_bounds__(256) void k_sweep(DevState S, int kmode, unsigned s
   __syncthreads();
 
   // phase 1: split fluxes at the tile's points s = c0-4 .. c0+ncell+3
-  for (int t = tid; t < NP * NPT; t += blockDim.x) {
+  for (int t = tid; t < NP * NPT; t += NT) {
     const int pl = DIR == 0 ? t / NPT : t % NP;
     const int sl = DIR == 0 ? t % NPT : t / NP;
     const long pen = pen0 + pl;
@@ -231,14 +246,18 @@ __global__ __launch_bounds__(256) void k_sweep(DevState S, int kmode, unsigned s
   const bool hyb = S.scheme == OCFD_SCHEME_HYBRID;
   const bool nearb = !S.per[DIR] && (c0 - 1 <= 4 || c0 + ncell - 1 >= n - 4);
   const bool uniform = !hyb && !nearb;
+  // linear mode (hybrid or UD7L schemes): a first pass evaluates UD7L at
+  // every interface with sliding windows, and the sorted list holds only the
+  // entries of the other kinds, whose values overwrite it
+  const bool lin = hyb || S.scheme == 4;
+  const int kfirst = lin ? 2 : 1;
   const int wave = tid >> 6, lane = tid & 63;
-  constexpr int PERW = (NE + 3) / 4;
+  constexpr int PERW = (NE + NW - 1) / NW;
   const int e0 = wave * PERW, e1 = min(NE, e0 + PERW);
-  int total;
   if (!uniform && kmode == KC_READ) {
     const long base = lb * NE;
     const int cnt = kct[lb];
-    for (int i = tid; i < cnt; i += blockDim.x) sList[i] = kc[base + i];
+    for (int i = tid; i < cnt; i += NT) sList[i] = kc[base + i];
     if (tid == 0) sTotal = cnt;
   } else if (!uniform) {
     int cnt[10];
@@ -261,7 +280,7 @@ __global__ __launch_bounds__(256) void k_sweep(DevState S, int kmode, unsigned s
         sKind[e] = (unsigned char)kind;
       }
 #pragma unroll
-      for (int k = 1; k < 10; k++) cnt[k] += __popcll(__ballot(kind == k));
+      for (int k = 1; k < 10; k++) cnt[k] += k < kfirst ? 0 : __popcll(__ballot(kind == k));
     }
     if (lane == 0) {
 #pragma unroll
@@ -274,7 +293,8 @@ __global__ __launch_bounds__(256) void k_sweep(DevState S, int kmode, unsigned s
     for (int k = 1; k < 10; k++) {
       int before = 0;
       ktot[k] = 0;
-      for (int w = 0; w < 4; w++) {
+      if (k < kfirst) continue;
+      for (int w = 0; w < NW; w++) {
         if (w < wave) before += sCnt[w][k];
         ktot[k] += sCnt[w][k];
       }
@@ -288,7 +308,7 @@ __global__ __launch_bounds__(256) void k_sweep(DevState S, int kmode, unsigned s
       const int kind = e < e1 ? (int)sKind[e] : 0;
 #pragma unroll
       for (int k = 1; k < 10; k++) {
-        if (ktot[k] == 0) continue;  // kind absent from the tile (uniform)
+        if (k < kfirst || ktot[k] == 0) continue;  // linear kind, or kind absent from the tile
         const unsigned long long bal = __ballot(kind == k);
         if (kind == k) sList[off[k] + __popcll(bal & lt)] = (unsigned short)((k << 12) | e);
         off[k] += __popcll(bal);
@@ -297,41 +317,70 @@ __global__ __launch_bounds__(256) void k_sweep(DevState S, int kmode, unsigned s
     if (kmode == KC_WRITE) {
       __syncthreads();
       const long base = lb * NE;
-      for (int i = tid; i < acc; i += blockDim.x) kc[base + i] = sList[i];
+      for (int i = tid; i < acc; i += NT) kc[base + i] = sList[i];
       if (tid == 0) kct[lb] = acc;
     }
   }
   __syncthreads();
-  total = uniform ? NE : sTotal;
+  const int total = uniform ? (lin ? 0 : NE) : sTotal;
 
-  // phase 2b: reconstruct; consecutive lanes take consecutive sorted entries.
-  // Results are held in registers until every lane has read its windows,
-  // then written over the split buffer.
-  const int inner = S.scheme == 1 ? K_WENO5 : S.scheme == 2 ? K_WENO7 : S.scheme == 3 ? K_OMP6 : K_UD7L;
-  // item t = tid + it*256 of 5*total is (comp, idx) = divmod(t, total),
-  // advanced by 256 per iteration without a division
+  // phase 2a': UD7L (OCFD_UD7L_P, OCFD_Schemes.f90:302-313) at every
+  // interface of the tile.  Thread = (line, run): a line is (side, component,
+  // pencil), a run L consecutive interfaces; the window slides by one point
+  // per interface (flux+: w[q] = v(l-3+q); flux-: w[q] = v(l+4-q)), one LDS
+  // read each, the same products summed in the same order as recon().
   constexpr int LPS = DIR == 0 ? 1 : NP;  // LDS stride between neighbouring points of a pencil
-  double res[MAXT];
-  int dst[MAXT];
-  int comp = 0, idx = tid;
-  while (idx >= total && comp < 5) {
-    idx -= total;
-    comp++;
-  }
+  if (lin) {
+    constexpr int L = sweep_run(NP, NIF, NT), G = (NIF + L - 1) / L;
+    if (tid < 10 * NP * G) {
+      const int g = tid % G, line = tid / G;
+      const int pl = line % NP, sc = line / NP;  // sc = side*5 + comp
+      const int side = sc >= 5 ? 1 : 0;
+      const int l0 = g * L, l1 = min(l0 + L, ncell + 1);
+      if (pen0 + pl < npen && l0 < l1) {
+        const double c0_ = S.ud7l[0], c1_ = S.ud7l[1], c2_ = S.ud7l[2], c3_ = S.ud7l[3];
+        const double c4_ = S.ud7l[4], c5_ = S.ud7l[5], c6_ = S.ud7l[6], c7_ = S.ud7l[7];
+        const double* F = sF + sc * NP * NPT;
+        double* H = sH + sc * NP * NIF;
+        double w[8];
+        if (!side) {
 #pragma unroll
-  for (int it = 0; it < MAXT; it++) {
-    dst[it] = -1;
-    if (it > 0) {
-      idx += 256;
-      while (idx >= total && comp < 5) {
-        idx -= total;
-        comp++;
+          for (int q = 0; q < 7; q++) w[q + 1] = F[lp(pl, l0 + q)];
+        } else {
+#pragma unroll
+          for (int q = 0; q < 7; q++) w[q] = F[lp(pl, l0 + 7 - q - 1)];
+        }
+#pragma unroll 1
+        for (int l = l0; l < l1; l++) {
+          if (!side) {
+#pragma unroll
+            for (int q = 0; q < 7; q++) w[q] = w[q + 1];
+            w[7] = F[lp(pl, l + 7)];
+          } else {
+#pragma unroll
+            for (int q = 7; q > 0; q--) w[q] = w[q - 1];
+            w[0] = F[lp(pl, l + 7)];
+          }
+          H[lh(pl, l)] = c0_ * w[0] + c1_ * w[1] + c2_ * w[2] + c3_ * w[3] + c4_ * w[4] + c5_ * w[5] +
+                         c6_ * w[6] + c7_ * w[7];
+        }
       }
     }
-    if (comp >= 5) continue;
+    __syncthreads();
+  }
+
+  // phase 2b: reconstruct.  Item t is component t % 5 of sorted entry t / 5,
+  // so the kind changes only where the sorted list does (a few wavefronts
+  // per tile run two kinds).  The loop is kept rolled: the reconstruction
+  // switch appears once in the code, which keeps the kernel inside the
+  // instruction cache.
+  const int inner = S.scheme == 1 ? K_WENO5 : S.scheme == 2 ? K_WENO7 : S.scheme == 3 ? K_OMP6 : K_UD7L;
+#pragma unroll 1
+  for (int t = tid; t < 5 * total; t += NT) {
+    const int idx = t / 5, comp = t - 5 * idx;
     const int u = uniform ? idx : (int)sList[idx];
     const int e = u & 0xFFF;
-    const int side = e / (NP * NIF);
+    const int side = e >= NP * NIF ? 1 : 0;
     const int r = e - side * (NP * NIF);
     const int pl = DIR == 0 ? r / NIF : r % NP;
     const int l = DIR == 0 ? r % NIF : r / NP;
@@ -343,18 +392,13 @@ __global__ __launch_bounds__(256) void k_sweep(DevState S, int kmode, unsigned s
     double w[8];
 #pragma unroll
     for (int qq = 0; qq < 8; qq++) w[qq] = F[qq * step];
-    res[it] = recon(kind, w, S.ud7l);
-    dst[it] = (side * 5 + comp) * NP * NIF + r;
+    sH[(side * 5 + comp) * NP * NIF + r] = recon(kind, w, S.ud7l);
   }
-  __syncthreads();
-#pragma unroll
-  for (int it = 0; it < MAXT; it++)
-    if (dst[it] >= 0) sH[dst[it]] = res[it];
   __syncthreads();
 
   // phase 3: du from interface differences
   const double h1 = S.h_1[DIR];
-  for (int t = tid; t < 5 * NP * TS; t += blockDim.x) {
+  for (int t = tid; t < 5 * NP * TS; t += NT) {
     const int m = t / (NP * TS);
     const int r = t % (NP * TS);
     const int pl = DIR == 0 ? r / TS : r % NP;
@@ -2093,7 +2137,7 @@ void launch_sweep_v() {
   int mode = KC_WRITE;
   if (G.kc_var[DIR] == V) mode = KC_READ;
   G.kc_var[DIR] = V;
-  launch(names[DIR], k_sweep<DIR, V>, g, dim3(256), S, mode, G.kc[DIR], G.kct[DIR]);
+  launch(names[DIR], k_sweep<DIR, V>, g, dim3(SweepTile<DIR, V>::NT), S, mode, G.kc[DIR], G.kct[DIR]);
 }
 
 // entries and tiles of the largest kind-list layout over the tile variants
@@ -3358,6 +3402,11 @@ int32_t ocfd_autotune(int32_t reps) {
   drop_graph();
   const int saved_timing = G.timing;
   G.timing = 0;
+  // time the sweeps on the masks a step builds from the current state: most
+  // interfaces of a hybrid scheme take the linear reconstruction and the
+  // rest WENO, and masks not yet built would time neither (the masks are
+  // rebuilt at the next step's first stage anyway)
+  if (G.cfg.scheme_invis == OCFD_SCHEME_HYBRID) enq_rhybrid();
   hipEvent_t a, b;
   HIPCHK(hipEventCreate(&a));
   HIPCHK(hipEventCreate(&b));
