@@ -448,6 +448,21 @@ int32_t ocfd_chem_timeadv_batch(int64_t n, double* rho, double* E, const double*
 int32_t ocfd_chem_adaptive_batch(int64_t n, double* rho, double* E, const double* dt, double rtol_y, double y_floor,
                                  double rtol_t, int32_t max_sub, int32_t* nsub_out, int32_t scheduler);
 
+/* ocfd_chem_adaptive_batch with scheduler 1 (the per-lane work queue) and
+ * the longest-first cell order, allocating nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Its work counters, sort keys, index lists and radix-sort workspace live in
+ * a static device array of about 32.5 MiB that each GPU the library runs on
+ * holds until the process ends; n above 2^20 cells is refused with an
+ * error.  Results are bit-identical to ocfd_chem_adaptive_batch.  Every call
+ * on one GPU shares that array (the call clears its counters first), so calls
+ * that overlap on one GPU race.  The chemistry stream and mechanism are those
+ * of ocfd_chem_set_stream / ocfd_chem_set_mechanism (which allocates, at
+ * set-up). */
+int32_t ocfd_chem_adaptive_batch_noalloc(int64_t n, double* rho, double* E, const double* dt, double rtol_y,
+                                         double y_floor, double rtol_t, int32_t max_sub, int32_t* nsub_out);
+
 /* Chemistry options: OCFD_CHEM_OPT_COOP_BUDGET = attempts a cell may take on
  * one lane before scheduler 2 moves it to the cooperative kernel (default
  * 128, the fastest of 16..1024 on a 2^21-cell jet-flame batch);

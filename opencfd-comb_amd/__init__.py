@@ -147,6 +147,7 @@ def load_library(path=None):
         "ocfd_chem_set_reverse_rates": (i32, [vp]),
         "ocfd_chem_timeadv_batch": (i32, [ctypes.c_int64, vp, vp, vp, i32, i32]),
         "ocfd_chem_adaptive_batch": (i32, [ctypes.c_int64, vp, vp, vp, dbl, dbl, dbl, i32, vp, i32]),
+        "ocfd_chem_adaptive_batch_noalloc": (i32, [ctypes.c_int64, vp, vp, vp, dbl, dbl, dbl, i32, vp]),
         "ocfd_chem_synchronize": (i32, []), "ocfd_chem_stream": (vp, []), "ocfd_chem_set_stream": (i32, [vp]),
         "ocfd_chem_last_error": (ctypes.c_char_p, []), "ocfd_chem_unfinished": (ctypes.c_int64, []),
         "ocfd_chem_counters": (i32, [vp, i32]), "ocfd_chem_set_option": (i32, [i32, i32]),

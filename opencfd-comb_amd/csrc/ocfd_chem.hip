@@ -1258,10 +1258,12 @@ int order_list_longest_first(long n, long cap, const int* list, const unsigned l
 // list of at most CX.coop_direct_max cells goes to the cooperative kernel
 // after one attempt per cell (the surrogate's rejects).
 int launch_adaptive(long n, double* rho, double* E, const double* dt, const ChemCtl& ctl, int* nsub_out, int scheduler,
-                    const int* list, const unsigned long long* list_count, bool direct = true) {
+                    const int* list, const unsigned long long* list_count, bool direct = true,
+                    unsigned long long* counter = nullptr) {
+  if (!counter) counter = CX.counter;
   if (scheduler == 0 && !list) {
     hipLaunchKernelGGL(k_chem_adaptive, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, CX.stream, n, rho, E, dt, ctl,
-                       nsub_out, CX.counter);
+                       nsub_out, counter);
     return hipGetLastError() == hipSuccess ? 0 : cfail("chemistry: launch failed");
   }
   int dev = 0, ncu = 0, per = 0;
@@ -1276,8 +1278,8 @@ int launch_adaptive(long n, double* rho, double* E, const double* dt, const Chem
   const bool coop = scheduler == 2;
   if (coop && ensure_heavy(n)) return 1;
   hipLaunchKernelGGL(k_chem_queue, dim3((unsigned)blocks), dim3(128), 0, CX.stream, n, rho, E, dt, ctl, nsub_out,
-                     CX.counter, list, list_count, coop ? CX.coop_budget : 0, direct ? CX.coop_direct_max : 0,
-                     CX.heavy);
+                     counter, list, list_count, coop ? CX.coop_budget : 0, direct ? CX.coop_direct_max : 0,
+                     coop ? CX.heavy : HeavyBuf{});
   if (coop) {
     int per2 = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per2, k_chem_coop, 64, 0) != hipSuccess)
@@ -1285,7 +1287,7 @@ int launch_adaptive(long n, double* rho, double* E, const double* dt, const Chem
     long b2 = (long)ncu * (per2 < 1 ? 1 : per2);
     if (b2 > n) b2 = n;
     hipLaunchKernelGGL(k_chem_coop, dim3((unsigned)b2), dim3(64), 0, CX.stream, n, rho, E, dt, ctl, nsub_out,
-                       CX.counter, CX.heavy);
+                       counter, CX.heavy);
   }
   return hipGetLastError() == hipSuccess ? 0 : cfail("chemistry: launch failed");
 }
@@ -1332,6 +1334,42 @@ int chem_surrogate_batch(long n, double* rho, double* E, const double* dt, bool 
     if (launch_adaptive(n, rho, E, dt, ctl, nullptr, CX.fallback_scheduler, list, CX.counter + 2)) return 1;
   }
   if (hipGetLastError() != hipSuccess) return cfail("ocfd_chem_surrogate_batch: launch failed");
+  return 0;
+}
+
+// ===========================================================================
+// Workspace of ocfd_chem_adaptive_batch_noalloc (no reference counterpart):
+// the work counters, the cell-order keys and index lists and the radix-sort
+// workspace of one batch of at most NA_MAX_CELLS cells, in one file-scope
+// device array.  Each GPU holds its own copy (about 32.5 MiB) from module
+// load until the process ends; host code reaches it only through the
+// per-device address from hipGetSymbolAddress, never through the symbol.
+constexpr long NA_MAX_CELLS = 1L << 20;
+constexpr size_t NA_COUNTERS = 256;  // 8 u64 work counters (k_chem_reset), then padding
+constexpr size_t NA_COUNT = 256;     // length of the ordered list (u64)
+constexpr size_t NA_KEYS = sizeof(unsigned int) * NA_MAX_CELLS;
+constexpr size_t NA_LIST = sizeof(int) * NA_MAX_CELLS;
+constexpr size_t NA_TEMP = size_t(16) << 20;  // radix-sort workspace
+constexpr size_t NA_BYTES = NA_COUNTERS + NA_COUNT + 2 * NA_KEYS + 2 * NA_LIST + NA_TEMP;
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+constexpr int NA_MAX_DEVICES = 64;
+unsigned char* na_base[NA_MAX_DEVICES] = {};
+
+// The current device's copy of scratch, resolved once per device (with that
+// device current) on the first call there.
+int na_workspace(unsigned char** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NA_MAX_DEVICES)
+    return cfail("ocfd_chem_adaptive_batch_noalloc: no usable current device");
+  if (!na_base[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return cfail(std::string("ocfd_chem_adaptive_batch_noalloc: hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+    na_base[dev] = (unsigned char*)p;
+  }
+  *out = na_base[dev];
   return 0;
 }
 
@@ -1437,6 +1475,53 @@ int32_t ocfd_chem_adaptive_batch(int64_t n, double* rho, double* E, const double
                                  double rtol_t, int32_t max_sub, int32_t* nsub_out, int32_t scheduler) {
   return chem_adaptive_batch((long)n, rho, E, dt, rtol_y, y_floor, rtol_t, (int)max_sub, (int*)nsub_out,
                              (int)scheduler, CX.order != 0);
+}
+
+int32_t ocfd_chem_adaptive_batch_noalloc(int64_t n, double* rho, double* E, const double* dt, double rtol_y,
+                                         double y_floor, double rtol_t, int32_t max_sub, int32_t* nsub_out) {
+  const char* who = "ocfd_chem_adaptive_batch_noalloc";
+  if (!CX.ready) return cfail(std::string(who) + ": call ocfd_chem_set_mechanism first");
+  if (!CX.rev_ready) return cfail(std::string(who) + ": ReReac 1/2 needs ocfd_chem_set_reverse_rates");
+  if (max_sub < 1 || !(rtol_y > 0.0) || !(y_floor >= 0.0) || !(rtol_t > 0.0))
+    return cfail(std::string(who) + ": bad controller parameters");
+  if (n < 0 || n > NA_MAX_CELLS)
+    return cfail(std::string(who) + ": n must be 0 .. 2^20 (" + std::to_string(NA_MAX_CELLS) +
+                 ", the capacity of its static workspace), got " + std::to_string((long long)n));
+  unsigned char* base = nullptr;
+  if (na_workspace(&base)) return 1;
+  unsigned long long* counter = (unsigned long long*)base;
+  unsigned long long* count = (unsigned long long*)(base + NA_COUNTERS);
+  unsigned char* p = base + NA_COUNTERS + NA_COUNT;
+  unsigned int* keys_in = (unsigned int*)p;
+  unsigned int* keys_out = (unsigned int*)(p + NA_KEYS);
+  int* idx_in = (int*)(p + 2 * NA_KEYS);
+  int* idx_out = (int*)(p + 2 * NA_KEYS + NA_LIST);
+  void* temp = p + 2 * NA_KEYS + 2 * NA_LIST;
+  const ChemCtl ctl{rtol_y, y_floor, rtol_t, max_sub};
+  // the previous call's counters are still there: clear them (a kernel)
+  hipLaunchKernelGGL(k_chem_reset, dim3(1), dim3(64), 0, CX.stream, counter);
+  if (n == 0) return hipGetLastError() == hipSuccess ? 0 : cfail(std::string(who) + ": launch failed");
+  const int* list = nullptr;
+  const unsigned long long* list_count = nullptr;
+  if (n > 1) {
+    // the longest-first order of ocfd_chem_adaptive_batch, on the workspace
+    size_t need = 0;
+    if (hipcub::DeviceRadixSort::SortPairsDescending(nullptr, need, (const unsigned int*)nullptr, (unsigned int*)nullptr,
+                                                     (const int*)nullptr, (int*)nullptr, (int)n) != hipSuccess)
+      return cfail(std::string(who) + ": radix sort query failed");
+    if (need > NA_TEMP)
+      return cfail(std::string(who) + ": the radix sort needs " + std::to_string(need) + " bytes of workspace, more than " +
+                   std::to_string(NA_TEMP));
+    hipLaunchKernelGGL(k_order_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, CX.stream, (long)n, rho, E, dt,
+                       keys_in, idx_in, count);
+    size_t tb = need;
+    const hipError_t e = hipcub::DeviceRadixSort::SortPairsDescending(temp, tb, keys_in, keys_out, idx_in, idx_out,
+                                                                      (int)n, 0, 32, CX.stream);
+    if (e != hipSuccess) return cfail(std::string(who) + ": radix sort failed: " + hipGetErrorString(e));
+    list = idx_out;
+    list_count = count;
+  }
+  return launch_adaptive((long)n, rho, E, dt, ctl, (int*)nsub_out, 1, list, list_count, false, counter);
 }
 
 int32_t ocfd_chem_surrogate_set(const double* cfg, int32_t count) {
