@@ -586,6 +586,7 @@ def main():
                    "mask_fractions": {"x": frac[0], "y": frac[1], "z": frac[2]}},
         "roofline": roofline, "step_roofline": step_roofline, "hybrid_mask": mask, "cpu_baseline": cpu,
         "kernels_ms_per_step": {k: v[0] / max(1, min(args.steps, 20)) for k, v in kstats.items()},
+        "kernels_launches_per_step": {k: v[1] / max(1, min(args.steps, 20)) for k, v in kstats.items()},
         "chem": chem,
         "jet": jet,
     }

@@ -275,12 +275,15 @@ __global__ __launch_bounds__((SweepTile<DIR, V>::NT)) void k_sweep(DevState S, i
         if (pen < npen && l <= ncell) {
           const int s_if = c0 - 1 + l;
           const int mask = hyb ? (int)S.mask[DIR][PB.mb[pl] + (long)s_if * mask_stride<DIR>(S)] : 0;
-          kind = recon_kind(S, DIR, s_if, side, mask);
+          // away from non-periodic faces the kind is the mask (hybrid: 1..3)
+          kind = nearb ? recon_kind(S, DIR, s_if, side, mask) : mask;
         }
         sKind[e] = (unsigned char)kind;
       }
+      // only kinds 2 and 3 occur in a hybrid tile away from the faces
 #pragma unroll
-      for (int k = 1; k < 10; k++) cnt[k] += k < kfirst ? 0 : __popcll(__ballot(kind == k));
+      for (int k = 1; k < 10; k++)
+        cnt[k] += (k < kfirst || (!nearb && k > 3)) ? 0 : __popcll(__ballot(kind == k));
     }
     if (lane == 0) {
 #pragma unroll
