@@ -339,7 +339,19 @@ def jet_bench(m, S, nx=256, ny=192, nz=128, steps=10, warmup=2, tau=2e-5, cfl=0.
             Ts_i = Ts[I]
             Ys = dis[I] / dis[I].sum(axis=3, keepdims=True)
             dY = np.abs(Ys[..., major] - Yp[..., major]).max()
+            # why cells fall back: the fused checks on the final field's
+            # interior cells, one chemistry step of dt, no fallback
+            rho_c = torch.tensor(np.ascontiguousarray(dis[I].reshape(-1, dis.shape[3]).T), device="cuda")
+            E_c = torch.tensor(np.ascontiguousarray(g.download("ET")[I].reshape(-1)), device="cuda")
+            fl = torch.zeros(E_c.shape[0], dtype=torch.int32, device="cuda")
+            S.step(rho_c, E_c, torch.full_like(E_c, dt), fallback=False, flags=fl)
+            fl = fl.cpu().numpy()
+            del rho_c, E_c
+            reasons = {k: int(np.sum((fl & b) != 0)) for k, b in (("domain", 1), ("element_balance", 2),
+                                                                   ("negative_Y", 4), ("T_bounds", 8),
+                                                                   ("increment_bound", 16))}
             return dict(ms_per_step=ms_s, chem_ms_per_step=ms_s - ms_flow, fallbacks_last_step=nfb,
+                        fallback_reasons_final_field=reasons,
                         fallback_fraction=nfb / ncell, speedup_step_vs_plain=ms_plain / ms_s,
                         speedup_chem_vs_plain=(ms_plain - ms_flow) / max(ms_s - ms_flow, 1e-9),
                         error_vs_plain=dict(peak_T_rel=float(abs(Ts_i.max() - Tp_i.max()) / Tp_i.max()),

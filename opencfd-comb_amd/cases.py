@@ -280,33 +280,40 @@ def flat_plate(nx=640, ny=120, nz=64, seed=2, x0=0.1, lx=0.2, ly=0.012, lz=0.004
 # fuel X_H2 = 0.85, X_N2 = 0.15 at 305 K; oxidiser air X_O2 = 0.21,
 # X_N2 = 0.79 at 1150 K; 1 atm.
 # ---------------------------------------------------------------------------
-def jet_streams(mech):
+def jet_streams(mech, kind="planar"):
+    """Mass fractions of the fuel and oxidiser streams: "planar" = the
+    planar jet (X_H2 = 0.10, X_N2 = 0.90 against air), "round" = the
+    reference's round-jet deck (X_H2 = 0.85, X_N2 = 0.15 against air)."""
     xf = np.zeros(mech.n_spec)
     xo = np.zeros(mech.n_spec)
-    xf[mech.index("H2")], xf[mech.index("N2")] = 0.85, 0.15
+    h2 = {"planar": 0.10, "round": 0.85}[kind]
+    xf[mech.index("H2")], xf[mech.index("N2")] = h2, 1.0 - h2
     xo[mech.index("O2")], xo[mech.index("N2")] = 0.21, 0.79
     return mech.mass_fractions(xf), mech.mass_fractions(xo)
 
 
 def jet_chem_states(n, seed=4, mech=None, T_range=(900.0, 2500.0), p_range_atm=(0.5, 5.0), dt_range=(1e-8, 1e-6),
-                    tau_range=(1e-7, 2e-3)):
+                    tau_range=(1e-7, 2e-3), streams="planar"):
     """Unreacted jet-mixture cells and the data to move them along their
     0-D ignition trajectories.
 
-    Z (mixture fraction of the fuel stream, by mass) ~ U[0, 1], T ~ U[T_range],
-    p ~ log-U[p_range_atm] atm, dt ~ log-U[dt_range] s (the chemistry step to
-    benchmark), tau ~ log-U[tau_range] s (how far along its constant-volume
-    trajectory each cell is taken before the benchmark; the caller advances
-    the cells by tau with an integrator).  Returns a dict of numpy arrays:
-    rho [n_spec, n] (kg/m3), E, T, p, Z, dt, tau [n]."""
+    Z (mixture fraction of the fuel stream, by mass) ~ U[0, 1] between the
+    planar jet's streams (``jet_streams``), T ~ U[T_range], p ~ U[p_range_atm]
+    atm, dt ~ log-U[dt_range] s (the chemistry step to benchmark), tau ~
+    log-U[tau_range] s (how far along its ignition trajectory each cell is
+    taken before the benchmark; the caller advances the cells by tau with the
+    0-D integrator, which holds density and internal energy fixed as the
+    reference's chemical_timeAdv does, so the trajectories are constant
+    volume).  Returns a dict of numpy arrays: rho [n_spec, n] (kg/m3), E, T,
+    p, Z, dt, tau [n]."""
     from .mechanism import Mechanism
     mech = mech or Mechanism()
     rng = np.random.default_rng(seed)
-    yf, yo = jet_streams(mech)
+    yf, yo = jet_streams(mech, streams)
     Z = rng.uniform(0.0, 1.0, n)
     Y = Z[None, :] * yf[:, None] + (1.0 - Z[None, :]) * yo[:, None]
     T = rng.uniform(T_range[0], T_range[1], n)
-    p = 101300.0 * np.exp(rng.uniform(math.log(p_range_atm[0]), math.log(p_range_atm[1]), n))
+    p = 101300.0 * rng.uniform(p_range_atm[0], p_range_atm[1], n)
     rho = Y * mech.density(Y, T, p)[None, :]
     E = mech.energy(rho, T)
     dt = np.exp(rng.uniform(math.log(dt_range[0]), math.log(dt_range[1]), n))

@@ -898,7 +898,8 @@ __global__ void k_chem_reset(unsigned long long* queue) {
 // the MLP (BF16 MFMA, FP32 accumulation) gives z, and in FP64
 //   dY = P (z * out_scale)          (P projects onto element conservation)
 // is accepted when the cell lies in the training domain, the H/O/N balance of
-// dY is within elem_tol of the element content, Y + dY >= -y_neg_tol, every
+// dY is within elem_tol of the cell's total element content (moles of H, O
+// and N per kg), Y + dY >= -y_neg_tol, every
 // |dY_k| <= dy_max (the increment size up to which the network's error was
 // measured to stay within tolerance) and the temperature after the update
 // (Newton on the new state, update_Et energy) is finite and within
@@ -1051,14 +1052,19 @@ __global__ __launch_bounds__(SurrTile<V>::NW * 64, SurrTile<V>::OCC) void k_surr
         for (int j = 0; j < CH_NS; j++) a = a + scfg.P[i][j] * dY[j];
         dYp[i] = a;
       }
+      // each element's balance to elem_tol of the cell's total element
+      // content (mol/kg): a cell without one element (pure air has no H)
+      // is held to the same absolute bound as the others, not to zero
+      double content = 0.0;
+#pragma unroll
+      for (int e = 0; e < 3; e++)
+#pragma unroll
+        for (int i = 0; i < CH_NS; i++) content = content + scfg.Cel[e][i] * fmax(di[i] / r, 0.0);
 #pragma unroll
       for (int e = 0; e < 3; e++) {
-        double res = 0.0, content = 0.0;
+        double res = 0.0;
 #pragma unroll
-        for (int i = 0; i < CH_NS; i++) {
-          res = res + scfg.Cel[e][i] * dYp[i];
-          content = content + scfg.Cel[e][i] * fmax(di[i] / r, 0.0);
-        }
+        for (int i = 0; i < CH_NS; i++) res = res + scfg.Cel[e][i] * dYp[i];
         if (!(fabs(res) <= scfg.elem_tol * content + 1e-300)) why |= 2;
       }
 #pragma unroll

@@ -168,15 +168,24 @@ class Surrogate:
             raise OcfdError(self.lib.ocfd_chem_last_error().decode())
 
     # -- data ----------------------------------------------------------------
-    def dataset(self, n, seed=0, controller=None):
+    def dataset(self, n, seed=0, controller=None, cold_fraction=0.125, T_cold=(280.0, 900.0)):
         """n jet-flame cells along their ignition trajectories (progress by
         the plain integrator) and their state after one chemistry step dt by
-        the plain adaptive FP64 integrator.  Device tensors."""
+        the plain adaptive FP64 integrator.  A fraction cold_fraction of the
+        cells is drawn with T in T_cold (the planar jet's 300 K fuel stream and
+        its cold mixing layer, where the chemistry is frozen), the rest with T
+        in 900-2500 K; the cells are shuffled so any prefix samples both.
+        Device tensors."""
         import torch
         from . import cases
         from .chemistry import RTOL_Y, Y_FLOOR, RTOL_T
         c = controller or (RTOL_Y, Y_FLOOR, RTOL_T)
-        s = cases.jet_chem_states(n, seed=seed, mech=self.mech)
+        n_cold = int(round(n * cold_fraction))
+        s = cases.jet_chem_states(n - n_cold, seed=seed, mech=self.mech)
+        if n_cold:
+            sc = cases.jet_chem_states(n_cold, seed=seed + 1000, mech=self.mech, T_range=T_cold)
+            perm = np.random.default_rng(seed).permutation(n)
+            s = {k: np.concatenate([s[k], sc[k]], axis=-1)[..., perm] for k in ("rho", "E", "dt", "tau")}
         dev = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda").contiguous()
         rho, E, tau, dt = dev(s["rho"]), dev(s["E"]), dev(s["tau"]), dev(s["dt"])
         self.chem.adaptive(rho, E, tau, *c, max_sub=100000)

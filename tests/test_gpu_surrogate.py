@@ -134,6 +134,31 @@ def test_domain_check_sends_cells_to_fallback(sur):
     assert np.all((f[:100] & 1) != 0)
 
 
+def test_single_element_streams_pass_the_balance_check(sur):
+    """Cells of one stream only (pure air: no H; pure H2/N2 fuel: no O) pass
+    the element-balance check: the projected increment conserves every
+    element to rounding, and the bound is relative to the cell's total
+    element content, so an absent element is not held to zero."""
+    import torch
+    from ocfd_amd import cases
+    S, ch, M, test = sur
+    yf, yo = cases.jet_streams(M)
+    n = 4096
+    rng = np.random.default_rng(5)
+    Y = np.where(np.arange(n)[None, :] % 2 == 0, yo[:, None], yf[:, None])
+    T = rng.uniform(950.0, 2400.0, n)
+    p = 101300.0 * rng.uniform(0.6, 4.5, n)
+    rho = np.ascontiguousarray(Y * M.density(Y, T, p)[None, :])
+    r = torch.tensor(rho, device="cuda")
+    E = torch.tensor(M.energy(rho, T), device="cuda")
+    dt = torch.tensor(np.exp(rng.uniform(np.log(2e-8), np.log(5e-7), n)), device="cuda")
+    flags = torch.zeros(n, dtype=torch.int32, device="cuda")
+    S.step(r, E, dt, fallback=False, flags=flags)
+    f = flags.cpu().numpy()
+    print("flags of pure-stream cells:", {b: int(np.sum((f & b) != 0)) for b in (1, 2, 4, 8, 16)})
+    assert not np.any((f & 2) != 0)
+
+
 def test_tile_shapes_bitwise_identical_and_autotune(sur):
     """Every tile shape of the fused kernel gives the same bits; the
     autotuner leaves the inputs untouched and keeps one of the shapes."""
