@@ -191,14 +191,19 @@ int32_t ocfd_download_field(int32_t field, double* host);
 int32_t ocfd_download_mask(int32_t dir, uint8_t* host);
 
 /* Boundary-layer BC data (ocfd_bc_BoundaryLayer and its init routines,
- * code/boundary-condition/OCFD_BC_Boundarylayers.f90:20-243,258-489):
- *   inlet1d[(5+N_SPEC) x ny]  (column-major (ny,5+N_SPEC): d,u,v,w,T,di..),
- *                               already rescaled so sum(di)=d;
- *   upper0d[5+N_SPEC];
+ * code/boundary-condition/OCFD_BC_Boundarylayers.f90:20-243,247-489; the
+ * boundary-layer BC runs with N_SPEC = 1, so 5+N_SPEC = 6 variables
+ * d,u,v,w,T,di), in the layout of BC%flow_inlet*d / flow_upper*d, column-major
+ * with the variable index slowest, already rescaled so sum(di)=d
+ * (init_bcdata_boundary, :247-430):
+ *   inlet  by BC_Para(1) (BcData_inlet): 0 -> [6] (flow_inlet0d),
+ *          1 -> (ny,6) (flow_inlet1d), 2 -> (ny,nz,6) (flow_inlet2d);
+ *   upper  by BC_Para(2) (BcData_upper): -1 -> none (may be NULL; the upper
+ *          face keeps its values), 0 -> [6], 1 -> (nx,6), 2 -> (nx,nz,6);
  *   nonreflect_upper[nx*nz]    (int32, 1 = non-reflecting, 0 = Dirichlet);
  *   wall_pertb[nx*nz]          (fx*gz of the Rai blow/suction).
  * The random phases of the reference are drawn by the caller. */
-int32_t ocfd_set_bc_boundarylayer(const double* inlet1d, const double* upper0d,
+int32_t ocfd_set_bc_boundarylayer(const double* inlet, const double* upper,
                                   const int32_t* nonreflect_upper,
                                   const double* wall_pertb);
 /* ht(tt) of get_ht_multifrequancy (OCFD_BC_Boundarylayers.f90:594-621),

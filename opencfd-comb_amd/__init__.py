@@ -198,6 +198,35 @@ class OcfdError(RuntimeError):
     pass
 
 
+def bl_data_shapes(cfg):
+    """Shapes of the boundary-layer inlet and upper data (Fortran order,
+    variable index last) for BC_Para(1) (BcData_inlet 0, 1, 2) and
+    BC_Para(2) (BcData_upper -1, 0, 1, 2): inlet (nv,), (ny, nv) or
+    (ny, nz, nv); upper None, (nv,), (nx, nv) or (nx, nz, nv), nv = 5 + N_SPEC."""
+    nv = 5 + cfg.n_spec
+    ki, ku = int(round(cfg.bc_para[0])), int(round(cfg.bc_para[1]))
+    inl = {0: (nv,), 1: (cfg.ny, nv), 2: (cfg.ny, cfg.nz, nv)}.get(ki)
+    up = {-1: None, 0: (nv,), 1: (cfg.nx, nv), 2: (cfg.nx, cfg.nz, nv)}.get(ku, "bad")
+    if inl is None or up == "bad":
+        raise ValueError(f"BC_Para(1) = {ki} / BC_Para(2) = {ku}: inlet data 0, 1, 2 and upper data -1, 0, 1, 2")
+    return inl, up
+
+
+def bl_data_arrays(cfg, inlet, upper):
+    """inlet and upper as Fortran-ordered float64 arrays of the shapes
+    ``bl_data_shapes`` gives (upper is None for BC_Para(2) = -1)."""
+    s_in, s_up = bl_data_shapes(cfg)
+    inl = np.asfortranarray(inlet, dtype=np.float64)
+    if inl.shape != s_in:
+        raise ValueError(f"inlet data: expected shape {s_in}, got {inl.shape}")
+    if s_up is None:
+        return inl, None
+    up = np.asfortranarray(upper, dtype=np.float64)
+    if up.shape != s_up:
+        raise ValueError(f"upper data: expected shape {s_up}, got {up.shape}")
+    return inl, up
+
+
 class FlowSolver:
     """Device-resident flow state and the hot-path entry points.
 
@@ -266,12 +295,17 @@ class FlowSolver:
         sp = np.zeros(9) if span is None else np.ascontiguousarray(span, dtype=np.float64)
         self._check(self.lib.ocfd_set_mesh(_ptr(arrs[0]), _ptr(arrs[1]), _ptr(arrs[2]), _ptr(sp)))
 
-    def set_bc_boundarylayer(self, inlet1d, upper0d, nonreflect_upper, wall_pertb):
-        inl = np.asfortranarray(inlet1d, dtype=np.float64)
-        up = np.ascontiguousarray(upper0d, dtype=np.float64)
+    def set_bc_boundarylayer(self, inlet, upper, nonreflect_upper, wall_pertb):
+        """BC%flow_inlet0d/1d/2d and BC%flow_upper0d/1d/2d
+        (init_bcdata_boundary, OCFD_BC_Boundarylayers.f90:247-430) by
+        BC_Para(1) / BC_Para(2); shapes in ``bl_data_shapes``."""
+        inl, up = bl_data_arrays(self.cfg, inlet, upper)
         nr = np.asfortranarray(nonreflect_upper, dtype=np.int32)
         wp = np.asfortranarray(wall_pertb, dtype=np.float64)
-        self._check(self.lib.ocfd_set_bc_boundarylayer(_ptr(inl), _ptr(up), _ptr(nr), _ptr(wp)))
+        if nr.shape != (self.nx, self.nz) or wp.shape != (self.nx, self.nz):
+            raise ValueError("nonreflect_upper and wall_pertb must have shape (nx, nz)")
+        self._check(self.lib.ocfd_set_bc_boundarylayer(_ptr(inl), None if up is None else _ptr(up), _ptr(nr),
+                                                       _ptr(wp)))
 
     def set_bc_time(self, ht):
         self._check(self.lib.ocfd_set_bc_time(float(ht)))

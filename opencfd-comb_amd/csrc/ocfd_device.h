@@ -52,11 +52,11 @@ struct DevState {
   double amu0, pr, sc;
   double mf[3];
   // BC (boundary layer)
-  int bl_inlet, bl_outlet;
+  int bl_inlet, bl_upper, bl_outlet;  // BcData_inlet (0/1/2), BcData_upper (-1/0/1/2), bc_outlet
   double bl_tw, bl_xinit, bl_disA;
   const double* bl_ht;   // device scalar: ht(tt) of the current time step
-  const double* bl_inlet1d;
-  const double* bl_upper0d;
+  const double* bl_in;   // flow_inlet0d/1d/2d: [6], [6][ny], [6][nz][ny] (variable slowest)
+  const double* bl_up;   // flow_upper0d/1d/2d: [6], [6][nx], [6][nz][nx]
   const int* bl_nonref;
   const double* bl_pertb;
   // flow arrays

@@ -1123,34 +1123,44 @@ __global__ void k_negt_repair(DevState S) {
 // Boundary conditions (OCFD_bc, OCFD_BC.f90:2-188; boundary layer case
 // ocfd_bc_BoundaryLayer, OCFD_BC_Boundarylayers.f90:20-243).
 // ===========================================================================
-__global__ void k_bl_inlet(DevState S) {  // :63-80 (BcData_inlet = 1)
+// Boundary data of one face point: variable m (d, u, v, w, T, di) of the
+// 0-D, 1-D or 2-D data (kind 0/1/2) at position a (1..na) along the face's
+// first direction and b (1..nb) along the second.
+__device__ __forceinline__ double bl_data(const double* r, int kind, int a, int b, int na, int nb, int m) {
+  if (kind == 0) return r[m];
+  if (kind == 1) return r[(a - 1) + (long)na * m];
+  return r[(a - 1) + (long)na * ((b - 1) + (long)nb * m)];
+}
+
+__global__ void k_bl_inlet(DevState S) {  // :50-97 (BcData_inlet = 0, 1, 2)
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)S.ny * S.nz) return;
   const int j = (int)(t % S.ny) + 1, k = (int)(t / S.ny) + 1;
   const long q = hq(S, 1, j, k);
-  const double* r = S.bl_inlet1d;
-  const long ny = S.ny;
-  S.d[q] = r[(j - 1) + 0 * ny];
-  S.u[q] = r[(j - 1) + 1 * ny];
-  S.v[q] = r[(j - 1) + 2 * ny];
-  S.w[q] = r[(j - 1) + 3 * ny];
-  S.T[q] = r[(j - 1) + 4 * ny];
-  S.di[q] = r[(j - 1) + 5 * ny];
+  const double* r = S.bl_in;
+  const int kind = S.bl_inlet;
+  S.d[q] = bl_data(r, kind, j, k, S.ny, S.nz, 0);
+  S.u[q] = bl_data(r, kind, j, k, S.ny, S.nz, 1);
+  S.v[q] = bl_data(r, kind, j, k, S.ny, S.nz, 2);
+  S.w[q] = bl_data(r, kind, j, k, S.ny, S.nz, 3);
+  S.T[q] = bl_data(r, kind, j, k, S.ny, S.nz, 4);
+  S.di[q] = bl_data(r, kind, j, k, S.ny, S.nz, 5);
 }
 
-__global__ void k_bl_upper(DevState S) {  // :100-140 (Dirichlet where not non-reflecting)
+__global__ void k_bl_upper(DevState S) {  // :100-140 (BcData_upper = 0, 1, 2; Dirichlet where not non-reflecting)
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)S.nx * S.nz) return;
   if (S.bl_nonref[t] != 0) return;
   const int i = (int)(t % S.nx) + 1, k = (int)(t / S.nx) + 1;
   const long q = hq(S, i, S.ny, k);
-  const double* u0 = S.bl_upper0d;
-  S.d[q] = u0[0];
-  S.u[q] = u0[1];
-  S.v[q] = u0[2];
-  S.w[q] = u0[3];
-  S.T[q] = u0[4];
-  S.di[q] = u0[5];
+  const double* r = S.bl_up;
+  const int kind = S.bl_upper;
+  S.d[q] = bl_data(r, kind, i, k, S.nx, S.nz, 0);
+  S.u[q] = bl_data(r, kind, i, k, S.nx, S.nz, 1);
+  S.v[q] = bl_data(r, kind, i, k, S.nx, S.nz, 2);
+  S.w[q] = bl_data(r, kind, i, k, S.nx, S.nz, 3);
+  S.T[q] = bl_data(r, kind, i, k, S.nx, S.nz, 4);
+  S.di[q] = bl_data(r, kind, i, k, S.nx, S.nz, 5);
 }
 
 __global__ void k_bl_wall(DevState S) {  // :143-201 (isothermal, non-catalytic, blow/suction)
@@ -1949,6 +1959,7 @@ struct Ctx {
   int use_graph = 0;
   int metric_tables = 0;     // OCFD_OPT_METRIC_TABLES (off: the divisions cost more than the loads)
   double* filt_tmp = nullptr;  // ocfd_filtering's copy of f
+  long bl_nin = 0, bl_nup = 0;  // lengths of the boundary-layer inlet and upper data
   bool mt_valid = false;     // the tables reproduce every interior metric
   unsigned long long* mt_bad = nullptr;
   int timing = 0;
@@ -2222,8 +2233,9 @@ void enq_rhybrid() {
 void enq_bc() {
   const DevState& S = G.S;
   if (G.cfg.ibc == OCFD_BC_BOUNDARYLAYER) {
-    if (S.bl_inlet == 1) launch("k_bl_inlet", k_bl_inlet, grid1((long)S.ny * S.nz), dim3(256), S);
-    launch("k_bl_upper", k_bl_upper, grid1((long)S.nx * S.nz), dim3(256), S);
+    launch("k_bl_inlet", k_bl_inlet, grid1((long)S.ny * S.nz), dim3(256), S);
+    // BcData_upper = -1: no upper data; the face keeps its values
+    if (S.bl_upper >= 0) launch("k_bl_upper", k_bl_upper, grid1((long)S.nx * S.nz), dim3(256), S);
     launch("k_bl_wall", k_bl_wall, grid1((long)S.nx * S.nz), dim3(256), S);
     if (S.bl_outlet == 1 || S.bl_outlet == 2) launch("k_bl_outlet", k_bl_outlet, grid1((long)S.ny * S.nz), dim3(256), S);
   } else if (G.cfg.ibc == OCFD_BC_JET) {
@@ -2537,14 +2549,18 @@ int32_t ocfd_init(const ocfd_cfg* cfg) {
   S.amu0 = c.amu0; S.pr = c.pr; S.sc = c.sc;
   for (int a = 0; a < 3; a++) S.mf[a] = c.mass_force[a];
   S.bl_inlet = (int)std::lround(c.bc_para[0]);
+  S.bl_upper = (int)std::lround(c.bc_para[1]);
   S.bl_outlet = (int)std::lround(c.bc_para[3]);
   S.bl_tw = c.bc_para[4];
   S.bl_xinit = c.bc_para[5];
   S.bl_disA = c.bc_para[7];
   S.jet_iflag = c.ibc == OCFD_BC_JET ? (int)std::lround(c.bc_para[0]) : 0;
   if (c.ibc == OCFD_BC_BOUNDARYLAYER) {
-    if (S.bl_inlet != 1) return fail("ocfd_init: boundary layer inlet must be 1-D data (BC_Para(1)=1)");
-    if (std::lround(c.bc_para[1]) != 0) return fail("ocfd_init: boundary layer upper data must be 0-D (BC_Para(2)=0)");
+    if (S.bl_inlet < 0 || S.bl_inlet > 2)
+      return fail("ocfd_init: boundary layer inlet data must be 0-D, 1-D or 2-D (BC_Para(1) = 0, 1, 2)");
+    if (S.bl_upper < -1 || S.bl_upper > 2)
+      return fail("ocfd_init: boundary layer upper data must be none, 0-D, 1-D or 2-D (BC_Para(2) = -1, 0, 1, 2)");
+    if (c.n_spec != 1) return fail("ocfd_init: the boundary-layer BC runs with N_SPEC = 1 (perfect gas)");
   }
 
   if (hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking) != hipSuccess) return fail("ocfd_init: no HIP device/stream");
@@ -2622,14 +2638,19 @@ int32_t ocfd_init(const ocfd_cfg* cfg) {
     S.hyb_inv_t1 = S.hyb_t1 > 0.0 ? 1.0 / S.hyb_t1 : 0.0;
     S.hyb_inv_t2 = S.hyb_t2 > 0.0 ? 1.0 / S.hyb_t2 : 0.0;
   }
-  double* bl = dalloc((long)S.ny * 6 + 6 + (long)S.nx * S.nz);
+  const bool blbc = c.ibc == OCFD_BC_BOUNDARYLAYER;
+  const long nin = 6L * (blbc && S.bl_inlet == 2 ? (long)S.ny * S.nz : blbc && S.bl_inlet == 1 ? (long)S.ny : 1L);
+  const long nup = 6L * (blbc && S.bl_upper == 2 ? (long)S.nx * S.nz : blbc && S.bl_upper == 1 ? (long)S.nx : 1L);
+  G.bl_nin = nin;
+  G.bl_nup = nup;
+  double* bl = dalloc(nin + nup + (long)S.nx * S.nz);
   void* nr = nullptr;
   if (!G.part || !bl || hipMalloc(&nr, sizeof(int) * (size_t)S.nx * S.nz) != hipSuccess) return fail("ocfd_init: hipMalloc failed");
   G.allocs.push_back(nr);
   hipMemsetAsync(nr, 0, sizeof(int) * (size_t)S.nx * S.nz, G.stream);
-  S.bl_inlet1d = bl;
-  S.bl_upper0d = bl + (long)S.ny * 6;
-  S.bl_pertb = bl + (long)S.ny * 6 + 6;
+  S.bl_in = bl;
+  S.bl_up = bl + nin;
+  S.bl_pertb = bl + nin + nup;
   S.bl_nonref = (const int*)nr;
   {
     double* htp = dalloc(1);
@@ -2637,7 +2658,7 @@ int32_t ocfd_init(const ocfd_cfg* cfg) {
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, G.stream, htp, 1L, 1.0);
     S.bl_ht = htp;
   }
-  hipLaunchKernelGGL(k_fill, grid1((long)S.ny * 6 + 6 + (long)S.nx * S.nz), dim3(256), 0, G.stream, bl,
+  hipLaunchKernelGGL(k_fill, grid1(nin + nup + (long)S.nx * S.nz), dim3(256), 0, G.stream, bl,
                      (long)S.ny * 6 + 6 + (long)S.nx * S.nz, 0.0);
   if (hipHostMalloc((void**)&G.h_negt, sizeof(int)) != hipSuccess) return fail("ocfd_init: hipHostMalloc failed");
   *G.h_negt = 0;
@@ -3070,13 +3091,16 @@ int32_t ocfd_download_mask(int32_t dir, uint8_t* host) {
   return 0;
 }
 
-int32_t ocfd_set_bc_boundarylayer(const double* inlet1d, const double* upper0d, const int32_t* nonreflect_upper,
+int32_t ocfd_set_bc_boundarylayer(const double* inlet, const double* upper, const int32_t* nonreflect_upper,
                                   const double* wall_pertb) {
   if (check_ready()) return 1;
   const DevState& S = G.S;
-  if (!inlet1d || !upper0d || !nonreflect_upper || !wall_pertb) return fail("ocfd_set_bc_boundarylayer: null pointer");
-  HIPCHK(hipMemcpyAsync((void*)S.bl_inlet1d, inlet1d, sizeof(double) * (size_t)S.ny * 6, hipMemcpyHostToDevice, G.stream));
-  HIPCHK(hipMemcpyAsync((void*)S.bl_upper0d, upper0d, sizeof(double) * 6, hipMemcpyHostToDevice, G.stream));
+  if (G.cfg.ibc != OCFD_BC_BOUNDARYLAYER) return fail("ocfd_set_bc_boundarylayer: the configuration has no boundary-layer BC");
+  if (!inlet || (!upper && S.bl_upper >= 0) || !nonreflect_upper || !wall_pertb)
+    return fail("ocfd_set_bc_boundarylayer: null pointer");
+  HIPCHK(hipMemcpyAsync((void*)S.bl_in, inlet, sizeof(double) * (size_t)G.bl_nin, hipMemcpyHostToDevice, G.stream));
+  if (S.bl_upper >= 0)
+    HIPCHK(hipMemcpyAsync((void*)S.bl_up, upper, sizeof(double) * (size_t)G.bl_nup, hipMemcpyHostToDevice, G.stream));
   HIPCHK(hipMemcpyAsync((void*)S.bl_nonref, nonreflect_upper, sizeof(int) * (size_t)S.nx * S.nz, hipMemcpyHostToDevice, G.stream));
   HIPCHK(hipMemcpyAsync((void*)S.bl_pertb, wall_pertb, sizeof(double) * (size_t)S.nx * S.nz, hipMemcpyHostToDevice, G.stream));
   HIPCHK(hipStreamSynchronize(G.stream));

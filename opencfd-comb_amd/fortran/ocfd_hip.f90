@@ -77,9 +77,9 @@ module ocfd_hip
     integer(c_int32_t) function ocfd_set_option(key, val) bind(C, name="ocfd_set_option")
       import; integer(c_int32_t), value :: key, val
     end function
-    integer(c_int32_t) function ocfd_set_bc_boundarylayer(inlet1d, upper0d, nonref, pertb) &
+    integer(c_int32_t) function ocfd_set_bc_boundarylayer(inlet, upper, nonref, pertb) &
         bind(C, name="ocfd_set_bc_boundarylayer")
-      import; real(c_double), intent(in) :: inlet1d(*), upper0d(*), pertb(*); integer(c_int32_t), intent(in) :: nonref(*)
+      import; real(c_double), intent(in) :: inlet(*), upper(*), pertb(*); integer(c_int32_t), intent(in) :: nonref(*)
     end function
     integer(c_int32_t) function ocfd_save_stage0() bind(C, name="ocfd_save_stage0")
       import

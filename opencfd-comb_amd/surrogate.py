@@ -244,6 +244,7 @@ class Surrogate:
         self.set_config(cfg)
         return cfg
 
+    T_LO_INDEX = 12 + 12 + N_SPEC          # lower end of the trained temperature domain
     DY_MAX_INDEX = 12 + 12 + N_SPEC + 6 + 4
 
     def calibrate(self, data, tol=5e-4):

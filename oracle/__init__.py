@@ -129,12 +129,14 @@ class Oracle:
         sp = np.zeros(9) if span is None else np.ascontiguousarray(span, dtype=np.float64)
         self._check(self.lib.orc_set_mesh(_ptr(arrs[0]), _ptr(arrs[1]), _ptr(arrs[2]), _ptr(sp)))
 
-    def set_bc_boundarylayer(self, inlet1d, upper0d, nonreflect_upper, wall_pertb):
-        inl = np.asfortranarray(inlet1d, dtype=np.float64)
-        up = np.ascontiguousarray(upper0d, dtype=np.float64)
+    def set_bc_boundarylayer(self, inlet, upper, nonreflect_upper, wall_pertb):
+        inl, up = self.m.bl_data_arrays(self.cfg, inlet, upper)
         nr = np.asfortranarray(nonreflect_upper, dtype=np.int32)
         wp = np.asfortranarray(wall_pertb, dtype=np.float64)
-        self._check(self.lib.orc_set_bc_boundarylayer(_ptr(inl), _ptr(up), _ptr(nr), _ptr(wp)))
+        if nr.shape != (self.nx, self.nz) or wp.shape != (self.nx, self.nz):
+            raise ValueError("nonreflect_upper and wall_pertb must have shape (nx, nz)")
+        self._check(self.lib.orc_set_bc_boundarylayer(_ptr(inl), None if up is None else _ptr(up), _ptr(nr),
+                                                      _ptr(wp)))
 
     def set_bc_time(self, ht):
         self._check(self.lib.orc_set_bc_time(float(ht)))

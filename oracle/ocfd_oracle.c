@@ -74,7 +74,8 @@ static struct {
   double *coef;                      /* [k][j][7] */
   double *Ect;                       /* [k][j]    */
   /* boundary-layer BC */
-  double *bl_inlet1d, *bl_upper0d, *bl_wall_pertb;
+  double *bl_inlet, *bl_upper, *bl_wall_pertb;  /* BC%flow_inlet*d, BC%flow_upper*d, BC%wall_pertb */
+  long bl_nin, bl_nup;
   int *bl_nonref;
   double bl_ht;
   int num_negt;
@@ -1401,32 +1402,43 @@ static int comput_flow_variables(void) {
 }
 
 /* ------------------------------------------------------------------ */
-/* ocfd_bc_BoundaryLayer, OCFD_BC_Boundarylayers.f90:20-243 (inlet 1-D data,
- * upper 0-D data where not non-reflecting, wall, outlet types 0/1/2). */
+/* element m (d,u,v,w,T,di(:)) of BC%flow_inlet0d/1d/2d or flow_upper0d/1d/2d
+ * at face position (a, b): flow_0d(m), flow_1d(a,m), flow_2d(a,b,m) */
+static double bl_value(const double* r, int kind, int a, int b, int na, int nb, int m) {
+  if (kind == 0) return r[m];
+  if (kind == 1) return r[(a - 1) + (long)na * m];
+  return r[(a - 1) + (long)na * ((b - 1) + (long)nb * m)];
+}
+
+/* ocfd_bc_BoundaryLayer, OCFD_BC_Boundarylayers.f90:20-243 (inlet 0-D, 1-D or
+ * 2-D data, :50-97; upper 0-D, 1-D or 2-D data where not non-reflecting,
+ * none for BcData_upper = -1, :100-140; wall; outlet types 0/1/2). */
 static void bc_boundarylayer(void) {
   int nx = G.nx, ny = G.ny, nz = G.nz, ns = G.nspec;
   long nh = G.nh;
   int inl = (int)lround(G.cfg.bc_para[0]), upp = (int)lround(G.cfg.bc_para[1]);
   int outlet = (int)lround(G.cfg.bc_para[3]);
   double Tw = G.cfg.bc_para[4], xinit = G.cfg.bc_para[5], disA = G.cfg.bc_para[7];
-  (void)upp;
-  if (inl == 1) {
+  if (inl >= 0 && inl <= 2) {
     for (int k = 1; k <= nz; k++)
       for (int j = 1; j <= ny; j++) {
         long q = H(1, j, k);
-        const double* row = G.bl_inlet1d;
-        G.d[q] = row[(j - 1) + 0L * ny]; G.u[q] = row[(j - 1) + 1L * ny]; G.v[q] = row[(j - 1) + 2L * ny];
-        G.w[q] = row[(j - 1) + 3L * ny]; G.T[q] = row[(j - 1) + 4L * ny];
-        for (int m = 0; m < ns; m++) G.di[q + m * nh] = row[(j - 1) + (5L + m) * ny];
+        const double* r = G.bl_inlet;
+        G.d[q] = bl_value(r, inl, j, k, ny, nz, 0); G.u[q] = bl_value(r, inl, j, k, ny, nz, 1);
+        G.v[q] = bl_value(r, inl, j, k, ny, nz, 2); G.w[q] = bl_value(r, inl, j, k, ny, nz, 3);
+        G.T[q] = bl_value(r, inl, j, k, ny, nz, 4);
+        for (int m = 0; m < ns; m++) G.di[q + m * nh] = bl_value(r, inl, j, k, ny, nz, 5 + m);
       }
   }
   for (int k = 1; k <= nz; k++)
     for (int i = 1; i <= nx; i++) {
-      if (G.bl_nonref[(i - 1) + (long)nx * (k - 1)] == 0) {
+      if (G.bl_nonref[(i - 1) + (long)nx * (k - 1)] == 0 && upp >= 0 && upp <= 2) {
         long q = H(i, ny, k);
-        const double* u0 = G.bl_upper0d;
-        G.d[q] = u0[0]; G.u[q] = u0[1]; G.v[q] = u0[2]; G.w[q] = u0[3]; G.T[q] = u0[4];
-        for (int m = 0; m < ns; m++) G.di[q + m * nh] = u0[5 + m];
+        const double* r = G.bl_upper;
+        G.d[q] = bl_value(r, upp, i, k, nx, nz, 0); G.u[q] = bl_value(r, upp, i, k, nx, nz, 1);
+        G.v[q] = bl_value(r, upp, i, k, nx, nz, 2); G.w[q] = bl_value(r, upp, i, k, nx, nz, 3);
+        G.T[q] = bl_value(r, upp, i, k, nx, nz, 4);
+        for (int m = 0; m < ns; m++) G.di[q + m * nh] = bl_value(r, upp, i, k, nx, nz, 5 + m);
       }
     }
   double ht = G.bl_ht;
@@ -2120,7 +2132,7 @@ static void free_all(void) {
                      &G.Aiz, &G.Asx, &G.Asy, &G.Asz, &G.Ajac, &G.Akx1, &G.Aky1, &G.Akz1, &G.Aix1, &G.Aiy1,
                      &G.Aiz1, &G.Asx1, &G.Asy1, &G.Asz1, &G.Rhybrid, &G.Ev1, &G.Ev2, &G.Ev3, &G.uk, &G.ui,
                      &G.us, &G.vk, &G.vi, &G.vs, &G.wk, &G.wi, &G.ws, &G.Tk, &G.Ti, &G.Ts, &G.Qpx, &G.Qpy,
-                     &G.Qpz, &G.Mi, &G.Ri, &G.coef, &G.Ect, &G.bl_inlet1d, &G.bl_upper0d, &G.bl_wall_pertb,
+                     &G.Qpz, &G.Mi, &G.Ri, &G.coef, &G.Ect, &G.bl_inlet, &G.bl_upper, &G.bl_wall_pertb,
                      &G.dus, &G.mu_an, &G.k_an, &G.D_an, &G.inlet2d};
   for (size_t a = 0; a < sizeof arrs / sizeof arrs[0]; a++) { free(*arrs[a]); *arrs[a] = NULL; }
   free(G.maskx); free(G.masky); free(G.maskz); free(G.bl_nonref);
@@ -2178,7 +2190,15 @@ int orc_init(const ocfd_cfg* cfg) {
   G.Mi = dalloc(G.nspec, 0.02897); G.Ri = dalloc(G.nspec, 0);
   for (int k = 0; k < G.nspec; k++) G.Ri[k] = R0_GAS / G.Mi[k];
   G.coef = dalloc((long)G.nspec * NTCT_MAX * 7, 0); G.Ect = dalloc((long)G.nspec * NTCT_MAX, 0);
-  G.bl_inlet1d = dalloc((long)G.ny * (5 + G.nspec), 0); G.bl_upper0d = dalloc(5 + G.nspec, 0);
+  {
+    /* init_bcdata_boundary (OCFD_BC_Boundarylayers.f90:247-430): 0-D, 1-D or
+     * 2-D inlet / upper data by BC_Para(1) / BC_Para(2), variable slowest */
+    int ki = (int)lround(G.cfg.bc_para[0]), ku = (int)lround(G.cfg.bc_para[1]);
+    int bl = G.cfg.ibc == OCFD_BC_BOUNDARYLAYER;
+    G.bl_nin = (5L + G.nspec) * (bl && ki == 2 ? (long)G.ny * G.nz : bl && ki == 1 ? (long)G.ny : 1L);
+    G.bl_nup = (5L + G.nspec) * (bl && ku == 2 ? (long)G.nx * G.nz : bl && ku == 1 ? (long)G.nx : 1L);
+    G.bl_inlet = dalloc(G.bl_nin, 0); G.bl_upper = dalloc(G.bl_nup, 0);
+  }
   G.bl_wall_pertb = dalloc((long)G.nx * G.nz, 0);
   G.bl_nonref = (int*)calloc((size_t)G.nx * G.nz, sizeof(int));
   G.bl_ht = 1.0;
@@ -2235,9 +2255,9 @@ int orc_set_species(int n_tct, const double* tct, double tct_max, const double* 
   return 0;
 }
 
-int orc_set_bc_boundarylayer(const double* inlet1d, const double* upper0d, const int* nonref, const double* pertb) {
-  memcpy(G.bl_inlet1d, inlet1d, sizeof(double) * (size_t)G.ny * (5 + G.nspec));
-  memcpy(G.bl_upper0d, upper0d, sizeof(double) * (size_t)(5 + G.nspec));
+int orc_set_bc_boundarylayer(const double* inlet, const double* upper, const int* nonref, const double* pertb) {
+  memcpy(G.bl_inlet, inlet, sizeof(double) * (size_t)G.bl_nin);
+  if (upper) memcpy(G.bl_upper, upper, sizeof(double) * (size_t)G.bl_nup);
   memcpy(G.bl_nonref, nonref, sizeof(int) * (size_t)G.nx * G.nz);
   memcpy(G.bl_wall_pertb, pertb, sizeof(double) * (size_t)G.nx * G.nz);
   return 0;

@@ -127,3 +127,26 @@ def curvilinear_box(n, a=0.05, L=2.0):
         mesh.append(h)
     span = np.array([L, 0, 0, 0, L, 0, 0, 0, L], dtype=np.float64)
     return mesh, span, Jm
+
+
+def bl_data_of_kinds(bc, ki, ku, nx, ny, nz, rng):
+    """Inlet data of BcData_inlet = ki and upper data of BcData_upper = ku
+    (OCFD_BC_Boundarylayers.f90:247-430 layouts) made from a deck's 1-D inlet
+    profile and 0-D upper state, with small seeded variations along the face
+    so that every position is distinguishable."""
+    inl1, up0 = np.asarray(bc["inlet1d"]), np.asarray(bc["upper0d"])
+    if ki == 0:
+        inl = inl1[ny // 2].copy()
+    elif ki == 1:
+        inl = inl1.copy(order="F")
+    else:
+        inl = np.asfortranarray(inl1[:, None, :] * (1.0 + 0.01 * rng.uniform(-1, 1, (ny, nz, 1))))
+    if ku == -1:
+        up = None
+    elif ku == 0:
+        up = up0.copy()
+    elif ku == 1:
+        up = np.asfortranarray(up0[None, :] * (1.0 + 0.01 * rng.uniform(-1, 1, (nx, 1))))
+    else:
+        up = np.asfortranarray(up0[None, None, :] * (1.0 + 0.01 * rng.uniform(-1, 1, (nx, nz, 1))))
+    return inl, up

@@ -582,3 +582,36 @@ def test_isotropic_initial_statistics(pkg):
     assert abs(s["Re_lamda"] - 72.0) <= 0.03 * 72.0
     assert abs(s["u_skewness"]) < 0.2 and abs(s["d_rms"]) <= 1e-12 * s["d_aver"]
     g.close()
+
+
+@pytest.mark.parametrize("kinds", [(0, 1), (2, 2), (2, -1), (0, -1)])
+def test_bl_data_kinds_bitwise(pkg, kinds):
+    """Boundary-layer inlet data 0-D / 2-D and upper data none / 1-D / 2-D
+    (BC_Para(1), BC_Para(2); OCFD_BC_Boundarylayers.f90:50-140, 247-430):
+    the state after OCFD_bc and after one RK3 step equals the oracle's bit
+    for bit, with the upper face split between Dirichlet and non-reflecting
+    points."""
+    import oracle
+    cfg, mesh, span, st, bc, info = fc.small_flat_plate(48, 32, 16)
+    cfg.bc_para[0], cfg.bc_para[1] = kinds
+    rng = np.random.default_rng(17)
+    bc = dict(bc)
+    bc["inlet1d"], bc["upper0d"] = fc.bl_data_of_kinds(bc, kinds[0], kinds[1], cfg.nx, cfg.ny, cfg.nz, rng)
+    nonref = np.ones((cfg.nx, cfg.nz), dtype=np.int32, order="F")
+    nonref[::2, :] = 0
+    bc["nonreflect_upper"] = nonref
+    case = (cfg, mesh, span, st, bc, info)
+    g = pkg.FlowSolver(cfg)
+    o = oracle.Oracle(pkg, cfg)
+    for eng in (g, o):
+        fc.setup(eng, case)
+    rep = fc.compare_state(g, o, tol=0.0)
+    assert all(v[1] == 1.0 for v in rep.values()), rep
+    dt1 = 0.5 * info["dt"]
+    for eng in (g, o):
+        eng.flow_time_advance(dt1)
+    rep = fc.compare_state(g, o, tol=0.0)
+    print(kinds, rep)
+    assert all(v[1] == 1.0 for v in rep.values()), rep
+    g.close()
+    o.close()

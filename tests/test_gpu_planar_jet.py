@@ -147,7 +147,20 @@ def test_surrogate_in_flow_step(pkg, surrogate):
     ns = case[0].n_spec
     dt = case[5]["dt"]
     ctl = (chemistry.RTOL_Y, chemistry.Y_FLOOR, chemistry.RTOL_T, 4000)
+    # the trained domain reaches the cold jet stream; a floor of 900 K sends
+    # the jet's 300 K cells to the fallback so both paths are exercised
+    saved = S.cfg.copy()
+    cfg = saved.copy()
+    cfg[S.T_LO_INDEX] = 900.0
+    S.set_config(cfg)
+    try:
+        _surrogate_in_flow_step(pkg, S, ch, mech, case, ns, dt, ctl)
+    finally:
+        S.set_config(saved)
 
+
+def _surrogate_in_flow_step(pkg, S, ch, mech, case, ns, dt, ctl):
+    import torch
     a = pkg.FlowSolver(case[0])
     fc.setup_jet(a, case)
     a.set_chem_method(pkg.CHEM_SURROGATE, *ctl, scheduler=2)

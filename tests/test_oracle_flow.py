@@ -677,11 +677,12 @@ def test_grid3d_metrics_converge_at_sixth_order():
 
 
 # ------------------------------------------------------ boundary-layer BC
-def _bl_case(outlet, Tw, nx=24, ny=16, nz=8, shear=(0.15, 0.1)):
+def _bl_case(outlet, Tw, nx=24, ny=16, nz=8, shear=(0.15, 0.1), kinds=(1, 0)):
     """A small flat-plate deck with a sheared mesh (the wall surface is the
     plane y = s x + t z, so its unit normal is (-s, 1, -t)/sqrt(1+s^2+t^2)),
     random perturbations of the state, a random blowing pattern, upper
-    points split between 0-D data and non-reflecting, and the given outlet
+    points split between Dirichlet data and non-reflecting, inlet and upper
+    data of the given kinds (BC_Para(1), BC_Para(2)), and the given outlet
     type and wall temperature (BC_Para(4), BC_Para(5))."""
     from ocfd_amd import cases
     case = list(fc.small_flat_plate(nx, ny, nz))
@@ -702,6 +703,8 @@ def _bl_case(outlet, Tw, nx=24, ny=16, nz=8, shear=(0.15, 0.1)):
     f *= 1.0 + 0.02 * rng.standard_normal(f.shape)
     st["F"] = f
     bc = dict(bc)
+    cfg.bc_para[0], cfg.bc_para[1] = kinds
+    bc["inlet1d"], bc["upper0d"] = fc.bl_data_of_kinds(bc, kinds[0], kinds[1], nx, ny, nz, rng)
     bc["wall_pertb"] = np.asfortranarray(rng.uniform(-1.0, 1.0, (nx, nz)))
     nonref = np.ones((nx, nz), dtype=np.int32, order="F")
     nonref[::3, :] = 0
@@ -709,13 +712,16 @@ def _bl_case(outlet, Tw, nx=24, ny=16, nz=8, shear=(0.15, 0.1)):
     return (cfg, (axx, ayy, azz), span, st, bc, info), (s, t)
 
 
-@pytest.mark.parametrize("outlet,Tw", [(1, 321.9), (2, 321.9), (0, -1.0)])
-def test_bc_boundarylayer_known_answers(outlet, Tw):
+@pytest.mark.parametrize("outlet,Tw,kinds", [(1, 321.9, (1, 0)), (2, 321.9, (1, 0)), (0, -1.0, (1, 0)),
+                                              (2, 321.9, (0, 1)), (1, -1.0, (2, 2)), (0, 321.9, (2, -1))])
+def test_bc_boundarylayer_known_answers(outlet, Tw, kinds):
     """ocfd_bc_BoundaryLayer (OCFD_BC_Boundarylayers.f90:20-243) followed by
     the face re-synthesis of OCFD_bc, against values the BC must produce:
-      inlet (i=1): the 1-D profile (d, u, v, w, T);
-      upper (j=ny): the 0-D data where the point is not non-reflecting, the
-        state before the call elsewhere;
+      inlet (i=1): the inlet data (d, u, v, w, T) at (j, k): the 0-D state,
+        the 1-D profile at j or the 2-D data at (j, k) (BC_Para(1) = 0/1/2);
+      upper (j=ny): the upper data at (i, k) (0-D, 1-D at i, 2-D at (i, k);
+        BC_Para(2) = 0/1/2) where the point is not non-reflecting, the state
+        before the call elsewhere and everywhere for BC_Para(2) = -1;
       wall (j=1), x < Wall_Xinit: a copy of j=2;
       wall, x >= Wall_Xinit: velocity A0 n with A0 = disA ht pertb and n the
         geometric unit normal of the sheared wall, T = T_w (T_w > 0) or
@@ -725,7 +731,7 @@ def test_bc_boundarylayer_known_answers(outlet, Tw):
     m = load_package()
     import oracle
     from ocfd_amd import cases
-    case, (s, t) = _bl_case(outlet, Tw)
+    case, (s, t) = _bl_case(outlet, Tw, kinds=kinds)
     cfg, mesh, span, st, bc, info = case
     ht = 0.37
     o = oracle.Oracle(m, cfg)
@@ -740,17 +746,23 @@ def test_bc_boundarylayer_known_answers(outlet, Tw):
     nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
     R = cases.R_AIR
     close = lambda a, b, tol=1e-13: np.max(np.abs(a - b)) <= tol * max(np.max(np.abs(b)), 1e-300)
-    # inlet
-    inl = bc["inlet1d"]
+    # inlet: data at (j, k) for every (j, k) of the face
+    ki, ku = kinds
+    inl = np.asarray(bc["inlet1d"])
+    inl3 = np.broadcast_to(inl[None, None, :] if ki == 0 else inl[:, None, :] if ki == 1 else inl, (ny, nz, inl.shape[-1]))
     for c, k in enumerate(("D", "U", "V", "W", "T")):
-        assert close(post[k][0, 1:-1, :], np.broadcast_to(inl[1:-1, c][:, None], (ny - 2, nz))), k
-    # upper
-    up = bc["upper0d"]
-    dat = bc["nonreflect_upper"] == 0
+        assert close(post[k][0, 1:-1, :], inl3[1:-1, :, c]), k
+    # upper: data at (i, k) where Dirichlet, the state before elsewhere
+    dat = (bc["nonreflect_upper"] == 0) & (ku >= 0)
+    if ku >= 0:
+        up = np.asarray(bc["upper0d"])
+        up3 = np.broadcast_to(up[None, None, :] if ku == 0 else up[:, None, :] if ku == 1 else up,
+                              (nx, nz, up.shape[-1]))
     for c, k in enumerate(("D", "U", "V", "W", "T")):
         top, before = post[k][1:-1, -1, :], pre[k][1:-1, -1, :]
         d1 = dat[1:-1]
-        assert close(top[d1], np.full(int(d1.sum()), up[c])), k
+        if ku >= 0:
+            assert close(top[d1], up3[1:-1, :, c][d1]), k
         assert close(top[~d1], before[~d1]), k
     # wall
     x = fc.interior(mesh[0])[:, 0, 0]
