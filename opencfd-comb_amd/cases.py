@@ -179,6 +179,101 @@ def stretched_y(ny, dy1, ly):
     return np.concatenate([[0.0], np.cumsum(dy1 * r ** np.arange(ny - 1))])
 
 
+# ---------------------------------------------------------------------------
+# Boundary-layer BC set-up (the init routines of OCFD_BC_Boundarylayers.f90).
+# The reference draws its random phases from an unseeded random_number; here
+# the caller passes a seeded numpy Generator, drawn in the reference's order.
+# ---------------------------------------------------------------------------
+PI_REF = 3.14159265358979  # the PI parameter of OCFD_BC_Boundarylayers.f90:493,599
+
+
+def init_bc_nonref(kind, axx_top, ayy_top, aoa=0.0):
+    """init_bc_nonRef (OCFD_BC_Boundarylayers.f90:406-443): the upper face's
+    BC%NonReflect_upper(i,k) (1 = non-reflecting, 0 = Dirichlet data).
+    kind = BC_Para(3): 1 every point non-reflecting; 2 every point Dirichlet;
+    0 (auto) Dirichlet where the top surface's slope sy = dy/dx (backward
+    difference, forward at i = 1) is >= tan(AoA) + 1e-6, non-reflecting
+    elsewhere.  axx_top, ayy_top: (nx, nz) coordinates of the plane j = ny.
+    Returns int32 (nx, nz), Fortran order."""
+    axx_top = np.asarray(axx_top, dtype=np.float64)
+    ayy_top = np.asarray(ayy_top, dtype=np.float64)
+    nx, nz = axx_top.shape
+    if kind == 1:
+        return np.ones((nx, nz), dtype=np.int32, order="F")
+    if kind == 2:
+        return np.zeros((nx, nz), dtype=np.int32, order="F")
+    if kind != 0:
+        raise ValueError("BC_Para(3) (upper boundary type) must be 0, 1 or 2")
+    sy0 = math.tan(aoa)
+    sy = np.empty((nx, nz))
+    sy[0] = (ayy_top[1] - ayy_top[0]) / (axx_top[1] - axx_top[0])
+    sy[1:] = (ayy_top[1:] - ayy_top[:-1]) / (axx_top[1:] - axx_top[:-1])
+    return np.asfortranarray(np.where(sy >= sy0 + 1.0e-6, 0, 1).astype(np.int32))
+
+
+def _mode_weights(m):
+    """Weights 1, 1/1.25, 1/1.25^2, ... normalised to sum 1 (Zl, TM)."""
+    w = np.empty(m)
+    for k in range(m):
+        w[k] = 1.0 if k == 0 else w[k - 1] / 1.25
+    return w / w.sum()
+
+
+def init_bc_wall_perturbation(dis_type, xbegin, xend, mz, zl_span, axx_wall, azz_wall, rng):
+    """init_bc_wall_perturbation (OCFD_BC_Boundarylayers.f90:446-562): the
+    wall's BC%wall_pertb(i,k) from the coordinates of the plane j = 1.
+    dis_type 0: zero.  1 (Rai's multi-wave blowing/suction): fx gz with
+    fx = 4/sqrt(27) sin(seta) (1 - cos(seta)), seta = 2 pi (x - xbegin) /
+    (xend - xbegin), inside [xbegin, xend] and 0 outside; gz = sum_m Zl(m)
+    sin(2 pi m (z/ZL + faiz(m))) for mz > 0 (random phases faiz), 1 for
+    mz = 0, sin(-2 pi mz z/ZL) for mz < 0.  2: uniform random in (-1, 1)
+    inside the strip, 0 outside.  Returns (nx, nz), Fortran order."""
+    X = np.asarray(axx_wall, dtype=np.float64)
+    Z = np.asarray(azz_wall, dtype=np.float64)
+    if dis_type == 0:
+        return np.zeros(X.shape, order="F")
+    inside = (X >= xbegin) & (X <= xend)
+    if dis_type == 2:
+        r = rng.random(X.shape[::-1]).T          # drawn in the reference's (k, i) loop order
+        return np.asfortranarray(np.where(inside, 2.0 * (r - 0.5), 0.0))
+    if dis_type != 1:
+        raise ValueError("BC_Para(7) (wall disturbance type) must be 0, 1 or 2")
+    if mz > 0:
+        faiz = rng.random(mz)
+        zl = _mode_weights(mz)
+    seta = 2.0 * PI_REF * (X - xbegin) / (xend - xbegin)
+    fx = np.where(inside, 4.0 / math.sqrt(27.0) * np.sin(seta) * (1.0 - np.cos(seta)), 0.0)
+    sz = Z / zl_span
+    if mz > 0:
+        gz = np.zeros_like(X)
+        for m in range(1, mz + 1):
+            gz = gz + zl[m - 1] * np.sin(2.0 * PI_REF * m * (sz + faiz[m - 1]))
+    elif mz == 0:
+        gz = np.ones_like(X)
+    else:
+        gz = np.sin(-2.0 * PI_REF * mz * sz)
+    return np.asfortranarray(fx * gz)
+
+
+def get_ht_multifrequancy(mt, beta, rng):
+    """get_ht_multifrequancy (OCFD_BC_Boundarylayers.f90:594-621): the
+    temporal factor of the wall disturbance, ht(tt) = sum_m TM(m)
+    sin(m beta tt 2 pi + 2 pi fait(m)) for beta > 0 (TM normalised weights,
+    random phases fait drawn once), 1 otherwise.  Returns the function ht."""
+    fait = rng.random(mt) if mt > 0 else np.zeros(0)
+    tm = _mode_weights(mt) if mt > 0 else np.zeros(0)
+
+    def ht(tt):
+        if not beta > 0.0:
+            return 1.0
+        h = 0.0
+        for m in range(1, mt + 1):
+            h = h + tm[m - 1] * math.sin(m * beta * tt * 2.0 * PI_REF + 2.0 * PI_REF * fait[m - 1])
+        return float(h)
+
+    return ht
+
+
 def flat_plate(nx=640, ny=120, nz=64, seed=2, x0=0.1, lx=0.2, ly=0.012, lz=0.004445, theta=(2.0, 10.0),
                cfl=0.5):
     """Mach 2.25 spatially developing flat-plate boundary layer (hybrid
@@ -231,32 +326,18 @@ def flat_plate(nx=640, ny=120, nz=64, seed=2, x0=0.1, lx=0.2, ly=0.012, lz=0.004
     inlet[:, 0], inlet[:, 1], inlet[:, 2], inlet[:, 4], inlet[:, 5] = dd, uu, vv, tt, dd
     inlet[:, 5] = inlet[:, 5] * inlet[:, 0] / inlet[:, 5]
     upper = np.array([rho_inf, U_inf, 0.0, 0.0, T_inf, rho_inf])
-    # upper boundary: auto type with a flat top -> non-reflecting everywhere
-    # (init_bc_nonRef, OCFD_BC_Boundarylayers.f90:406-443)
-    nonref = np.ones((nx, nz), dtype=np.int32, order="F")
-    # Rai blow/suction strip (init_bc_wall_perturbation, :446-489), seeded phases
+    X = x[:, None] * np.ones((1, nz))
+    Z = np.ones((nx, 1)) * z[None, :]
+    # upper boundary: auto type (BC_Para(3) = 0) with a flat top ->
+    # non-reflecting everywhere
+    nonref = init_bc_nonref(0, X, np.full((nx, nz), y[-1]))
+    # Rai blow/suction strip with 10 spanwise and 10 temporal modes, seeded phases
     rng = np.random.Generator(np.random.PCG64(seed))
     mz = mt = 10
     xb, xe = 0.1143, 0.1270
-    faiz = rng.random(mz)
-    zl = np.array([1.25 ** (-k) for k in range(mz)])
-    zl /= zl.sum()
-    X = x[:, None] * np.ones((1, nz))
-    Z = np.ones((nx, 1)) * z[None, :]
-    seta = 2.0 * math.pi * (X - xb) / (xe - xb)
-    fx = np.where((X >= xb) & (X <= xe), 4.0 / math.sqrt(27.0) * np.sin(seta) * (1.0 - np.cos(seta)), 0.0)
-    gz = np.zeros_like(X)
-    for m in range(1, mz + 1):
-        gz += zl[m - 1] * np.sin(2.0 * math.pi * m * (Z / lz + faiz[m - 1]))
-    pertb = np.asfortranarray(fx * gz)
-    fait = rng.random(mt)
-    tm = np.array([1.25 ** (-k) for k in range(mt)])
-    tm /= tm.sum()
     freq = 28861.58
-
-    def ht(tt):  # get_ht_multifrequancy, OCFD_BC_Boundarylayers.f90:594-621
-        return float(sum(tm[m - 1] * math.sin(m * freq * tt * 2.0 * math.pi + 2.0 * math.pi * fait[m - 1])
-                         for m in range(1, mt + 1)))
+    pertb = init_bc_wall_perturbation(1, xb, xe, mz, lz, X, Z, rng)
+    ht = get_ht_multifrequancy(mt, freq, rng)
 
     bc_para = [1, 0, 0, 0, T_wall, 0.0, 1, 0.01 * U_inf, xb, xe, mt, mz, lz, freq]
     hp = [2.0, theta[0], theta[1], 0.0]
