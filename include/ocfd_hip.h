@@ -544,7 +544,10 @@ int32_t ocfd_chem_surrogate_set(const double* cfg, int32_t count);
  * flag_out [n] (device, may be NULL): 0 surrogate, else the failed checks
  * (1 outside the training domain, 2 element balance, 4 negative mass
  * fraction, 8 temperature bounds, 16 an increment above dy_max) and the cell
- * went to the fallback.  The
+ * went to the fallback.  With OCFD_CHEM_OPT_ORDER = 1 (the default) the
+ * call reads the number of rejected cells back (it synchronises the
+ * chemistry stream once), launches nothing more when there is none and
+ * otherwise sorts only those cells longest first.  The
  * network must be 12 -> 256 x k -> 9. */
 int32_t ocfd_chem_surrogate_batch(int64_t n, double* rho, double* E, const double* dt, int32_t fallback,
                                   double rtol_y, double y_floor, double rtol_t, int32_t max_sub, int32_t* flag_out);

@@ -225,7 +225,13 @@ class Surrogate:
         (dy_max: no increment bound until calibrate sets one)."""
         n = data["E0"].shape[0]
         k = min(n, sample)
-        f = self.features(data["rho0"], data["E0"], data["dt"], mode=1)[:k].cpu().numpy()
+        fall = self.features(data["rho0"], data["E0"], data["dt"], mode=1)
+        # the acceptance domain spans every training cell (the normalisation
+        # uses a sample): cells just beyond a sample's extremes are states the
+        # network was trained on
+        fmin, fmax = fall.amin(0).cpu().numpy(), fall.amax(0).cpu().numpy()
+        f = fall[:k].cpu().numpy()
+        del fall
         r0 = data["rho0"][:, :k].cpu().numpy()
         r1 = data["rho1"][:, :k].cpu().numpy()
         dY = ((r1 - r0) / r0.sum(axis=0)[None, :]).T
@@ -234,11 +240,10 @@ class Surrogate:
         sigma = np.maximum(dY.std(axis=0), 1e-12)
         cel = element_matrix(self.mech)
         P = projection(cel, sigma)
-        T, lp, ldt = f[:, 0], f[:, 1], f[:, 11]
         cfg = np.concatenate([
             shift, scale, sigma,
-            [T.min(), T.max(), np.exp(lp.min()), np.exp(lp.max()), np.exp(ldt.min()), np.exp(ldt.max())],
-            [y_neg_tol, elem_tol, T_out[0], T_out[1] if T_out[1] is not None else 1.1 * T.max(), dy_max],
+            [fmin[0], fmax[0], np.exp(fmin[1]), np.exp(fmax[1]), np.exp(fmin[11]), np.exp(fmax[11])],
+            [y_neg_tol, elem_tol, T_out[0], T_out[1] if T_out[1] is not None else 1.1 * fmax[0], dy_max],
             P.ravel(), cel.ravel()])
         assert cfg.size == CFG_LEN
         self.set_config(cfg)
