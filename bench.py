@@ -186,7 +186,7 @@ def chem_bench(m, ncell=1 << 21, cpu=True, surrogate_cells=1 << 24, surrogate_ep
             ms_s, _ = timed(lambda: S.step(rho, E, dt, fallback=True, flags=flags))
             nfb = S.fallbacks()
             ms_nofb, _ = timed(lambda: S.step(rho, E, dt, fallback=False))
-            # BF16 MFMA roofline of the fused inference (network + checks,
+            # BF16 MFMA roofline of the inference (features, network, checks,
             # no fallback): 535,040 flops per cell
             mlp_tf = flops.MLP_CELL * ncell / (ms_nofb / 1e3) / 1e12
             rho.copy_(r0)
@@ -226,7 +226,7 @@ def chem_bench(m, ncell=1 << 21, cpu=True, surrogate_cells=1 << 24, surrogate_ep
         keep = ("ms", "cells_per_s", "fallbacks", "fallback_reasons", "speedup_vs_plain", "error_vs_plain",
                 "roofline_inference")
         out["surrogate"] = dict(
-            network="12 -> 256 x5 -> 9 GELU, BF16 MFMA (FP32 accumulation), fused kernel",
+            network="12 -> 256 x5 -> 9 GELU, BF16 MFMA (FP32 accumulation), features / network / checks kernels",
             tile=dict(cells_per_workgroup=tile_rows, waves=tile_waves,
                       ms_per_shape={"128x4": tile_ms[0], "128x8": tile_ms[1], "64x4": tile_ms[2],
                                     "96x4": tile_ms[3]}),

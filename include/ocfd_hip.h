@@ -551,7 +551,7 @@ int32_t ocfd_chem_surrogate_set(const double* cfg, int32_t count);
  * network must be 12 -> 256 x k -> 9. */
 int32_t ocfd_chem_surrogate_batch(int64_t n, double* rho, double* E, const double* dt, int32_t fallback,
                                   double rtol_y, double y_floor, double rtol_t, int32_t max_sub, int32_t* flag_out);
-/* Tile-shape autotuner of the fused surrogate kernel (no reference
+/* Tile-shape autotuner of the surrogate network kernel (no reference
  * counterpart): each shape (cells per workgroup x waves: 128x4, 128x8,
  * 64x4, 96x4) runs reps times on scratch copies of the batch (rho, E are not
  * modified), timed with HIP events; the fastest is kept.  Results are

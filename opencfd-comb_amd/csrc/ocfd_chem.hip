@@ -77,6 +77,7 @@ struct ChemCtx {
   hipStream_t stream = nullptr, own = nullptr;
   unsigned long long* counter = nullptr;  // [0] queue head, [1] unfinished, [2] surrogate fallbacks
   int* fb_list = nullptr;                  // cells handed to the fallback integrator
+  unsigned char* surr_buf = nullptr;       // surrogate phases: X[12][cap], O[9][cap] (float), ok[cap]
   long fb_cap = 0;
   HeavyBuf heavy{};                        // cells handed to the cooperative kernel
   int fallback_scheduler = 2;              // scheduler of the surrogate's fallback
@@ -93,7 +94,7 @@ struct ChemCtx {
   long ord_cap = 0;
   unsigned char* ord_mem = nullptr;
   bool surr_ready = false;
-  int surr_variant = 0;                    // tile shape of k_surrogate (SurrTile)
+  int surr_variant = 0;                    // tile shape of k_surr_mlp (SurrTile)
   float surr_tune_ms[8] = {};
   std::string err;
 };
@@ -975,123 +976,150 @@ __global__ void k_fill_u64(unsigned long long* p, int count) {
   if ((int)threadIdx.x < count && blockIdx.x == 0) p[threadIdx.x] = 0;
 }
 
-// Tile shapes of the fused surrogate kernel (cells per workgroup, waves,
+// Tile shapes of the surrogate network kernel (cells per workgroup, waves,
 // workgroups per CU the register budget aims at).  All give bit-identical
 // results; ocfd_chem_surrogate_autotune picks the fastest.
 template <int V> struct SurrTile;
 template <> struct SurrTile<0> { static constexpr int ROWS = 128, NW = 4, OCC = 2; };
 template <> struct SurrTile<1> { static constexpr int ROWS = 128, NW = 8, OCC = 1; };
 template <> struct SurrTile<2> { static constexpr int ROWS = 64, NW = 4, OCC = 3; };
-template <> struct SurrTile<3> { static constexpr int ROWS = 96, NW = 4, OCC = 3; };
+template <> struct SurrTile<3> { static constexpr int ROWS = 96, NW = 4, OCC = 2; };
 #define SURR_VARIANTS 4
 
-template <int V>
-__global__ __launch_bounds__(SurrTile<V>::NW * 64, SurrTile<V>::OCC) void k_surrogate(
-    long n, double* rho, double* Ecell, const double* dtcell, int* flag_out, int* fb_list,
-    unsigned long long* counters) {
-  constexpr int ROWS = SurrTile<V>::ROWS, NW = SurrTile<V>::NW;
-  __shared__ __attribute__((aligned(16))) tbf16 H[ROWS * MT_LDH];
-  __shared__ int ok_in[ROWS];
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const long c0 = (long)blockIdx.x * ROWS;
+// The surrogate step runs as three launches, so that every phase keeps all
+// of its threads busy: the per-cell FP64 work (features with the Newton
+// temperature; projection, checks, update_Et and a second Newton
+// temperature) takes one cell per thread, and the network takes whole
+// workgroups of MFMA waves.  (Fused into one kernel, the per-cell phases ran
+// on the ROWS threads of a workgroup of NW * 64 while the other waves waited
+// at its barriers.)
+//
+// Phase 1: normalised inputs X[12][n] (float) and the domain check ok[n].
+__global__ __launch_bounds__(256) void k_surr_features(long n, const double* __restrict__ rho,
+                                                       const double* __restrict__ Ecell,
+                                                       const double* __restrict__ dtcell, float* __restrict__ X,
+                                                       unsigned char* __restrict__ okv) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
   Mech m = (Mech)&cm;
-  if (t < ROWS) {
-    const long c = c0 + t;
-    float x[12];
+  float x[12];
+  double di[CH_NS];
+  double r = 0.0;
 #pragma unroll
-    for (int j = 0; j < 12; j++) x[j] = 0.f;
-    int ok = 0;
-    if (c < n) {
-      double di[CH_NS];
-      double r = 0.0;
-#pragma unroll
-      for (int i = 0; i < CH_NS; i++) {
-        di[i] = rho[(long)i * n + c];
-        r = r + di[i];
-      }
-      const double E = Ecell[c], dt = dtcell[c];
-      double f[12], T, p;
-      surr_raw_features(m, di, r, E, dt, f, T, p);
-      ok = (T >= scfg.T_lo && T <= scfg.T_hi && p >= scfg.p_lo && p <= scfg.p_hi && dt >= scfg.dt_lo &&
+  for (int i = 0; i < CH_NS; i++) {
+    di[i] = rho[(long)i * n + c];
+    r = r + di[i];
+  }
+  const double E = Ecell[c], dt = dtcell[c];
+  double f[12], T, p;
+  surr_raw_features(m, di, r, E, dt, f, T, p);
+  int ok = (T >= scfg.T_lo && T <= scfg.T_hi && p >= scfg.p_lo && p <= scfg.p_hi && dt >= scfg.dt_lo &&
             dt <= scfg.dt_hi && r > 0.0) ? 1 : 0;
 #pragma unroll
-      for (int j = 0; j < 12; j++) {
-        x[j] = (float)((f[j] - scfg.in_shift[j]) * scfg.in_scale[j]);
-        if (!(fabsf(x[j]) <= 1e6f)) ok = 0;  // also catches NaN
-      }
-      if (!ok) {
+  for (int j = 0; j < 12; j++) {
+    x[j] = (float)((f[j] - scfg.in_shift[j]) * scfg.in_scale[j]);
+    if (!(fabsf(x[j]) <= 1e6f)) ok = 0;  // also catches NaN
+  }
 #pragma unroll
-        for (int j = 0; j < 12; j++) x[j] = 0.f;
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < 64; j++) H[t * MT_LDH + j] = (tbf16)(j < 12 ? x[j] : 0.f);
-    ok_in[t] = ok;
+  for (int j = 0; j < 12; j++) X[(long)j * n + c] = ok ? x[j] : 0.f;
+  okv[c] = (unsigned char)ok;
+}
+
+// Phase 2: the network on tiles of ROWS cells, O[16][n] (float; rows 0..8
+// hold the 9 outputs) from X.
+template <int V>
+__global__ __launch_bounds__(SurrTile<V>::NW * 64, SurrTile<V>::OCC) void k_surr_mlp(long n,
+                                                                                       const float* __restrict__ X,
+                                                                                       float* __restrict__ Oout) {
+  constexpr int ROWS = SurrTile<V>::ROWS, NW = SurrTile<V>::NW, NT = NW * 64;
+  __shared__ __attribute__((aligned(16))) tbf16 H[ROWS * MT_LDH];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const long c0 = (long)blockIdx.x * ROWS;
+  for (int e = t; e < ROWS * 64; e += NT) {
+    const int row = e / 64, j = e % 64;
+    const long c = c0 + row;
+    H[row * MT_LDH + j] = (tbf16)(j < 12 && c < n ? X[(long)j * n + c] : 0.f);
   }
   __syncthreads();
   const int L = cnet.n_layers;
-  mt_layer256<true, ROWS, NW>(H, cnet.W[0], cnet.b[0], 64, cnet.dinp[0], lane, w);
-  for (int l = 1; l < L - 1; l++) mt_layer256<true, ROWS, NW>(H, cnet.W[l], cnet.b[l], 256, 256, lane, w);
+  mt_layer256<true, ROWS, NW, 64>(H, cnet.W[0], cnet.b[0], cnet.dinp[0], lane, w);
+  for (int l = 1; l < L - 1; l++) mt_layer256<true, ROWS, NW, 256>(H, cnet.W[l], cnet.b[l], 256, lane, w);
   float* O = (float*)H;
   mt_out16<ROWS, NW>(H, cnet.W[L - 1], cnet.b[L - 1], 256, 256, O, lane, w);
-  if (t < ROWS) {
-    const long c = c0 + t;
-    if (c < n) {
-      int why = ok_in[t] ? 0 : 1;
-      double di[CH_NS], dY[CH_NS], dYp[CH_NS], dinew[CH_NS];
-      double r = 0.0;
-#pragma unroll
-      for (int i = 0; i < CH_NS; i++) {
-        di[i] = rho[(long)i * n + c];
-        r = r + di[i];
-        dY[i] = (double)O[t * 16 + i] * scfg.out_scale[i];
-      }
-#pragma unroll
-      for (int i = 0; i < CH_NS; i++) {
-        double a = 0.0;
-#pragma unroll
-        for (int j = 0; j < CH_NS; j++) a = a + scfg.P[i][j] * dY[j];
-        dYp[i] = a;
-      }
-      // each element's balance to elem_tol of the cell's total element
-      // content (mol/kg): a cell without one element (pure air has no H)
-      // is held to the same absolute bound as the others, not to zero
-      double content = 0.0;
-#pragma unroll
-      for (int e = 0; e < 3; e++)
-#pragma unroll
-        for (int i = 0; i < CH_NS; i++) content = content + scfg.Cel[e][i] * fmax(di[i] / r, 0.0);
-#pragma unroll
-      for (int e = 0; e < 3; e++) {
-        double res = 0.0;
-#pragma unroll
-        for (int i = 0; i < CH_NS; i++) res = res + scfg.Cel[e][i] * dYp[i];
-        if (!(fabs(res) <= scfg.elem_tol * content + 1e-300)) why |= 2;
-      }
-#pragma unroll
-      for (int i = 0; i < CH_NS; i++) {
-        if (!(di[i] / r + dYp[i] >= -scfg.y_neg_tol)) why |= 4;
-        if (!(fabs(dYp[i]) <= scfg.dy_max)) why |= 16;  // also NaN
-        dinew[i] = di[i] + r * dYp[i];
-      }
-      double E = Ecell[c];
-      chem_update_et(m, di, dinew, E);
-      const double Tn = chem_T(m, dinew, E);
-      if (!(Tn >= scfg.T_out_lo && Tn <= scfg.T_out_hi)) why |= 8;  // also NaN
-      const bool ok = why == 0;
-      if (ok) {
-#pragma unroll
-        for (int i = 0; i < CH_NS; i++) rho[(long)i * n + c] = dinew[i];
-        Ecell[c] = E;
-      } else {
-        const unsigned long long k = atomicAdd(counters + 2, 1ull);
-        fb_list[k] = (int)c;
-      }
-      if (flag_out) flag_out[c] = why;
-    }
+  for (int e = t; e < ROWS * CH_NS; e += NT) {
+    const int row = e % ROWS, i = e / ROWS;
+    const long c = c0 + row;
+    if (c < n) Oout[(long)i * n + c] = O[row * 16 + i];
   }
 }
 
+// Phase 3: in FP64, dY = P (O * out_scale) (P projects onto element
+// conservation); the cell is accepted when it lies in the training domain,
+// the H/O/N balance of dY is within elem_tol of the cell's total element
+// content (moles of H, O and N per kg), Y + dY >= -y_neg_tol, every
+// |dY_k| <= dy_max (the increment size up to which the network's error was
+// measured to stay within tolerance) and the temperature after the update
+// (Newton on the new state, update_Et energy) is finite and within
+// [T_out_lo, T_out_hi].  Accepted cells are written (rho_k += rho dY_k, E by
+// update_Et); the others go to a list for the plain FP64 integrator.
+__global__ __launch_bounds__(256) void k_surr_finish(long n, double* rho, double* Ecell,
+                                                     const float* __restrict__ Oin,
+                                                     const unsigned char* __restrict__ okv, int* flag_out,
+                                                     int* fb_list, unsigned long long* counters) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  Mech m = (Mech)&cm;
+  int why = okv[c] ? 0 : 1;
+  double di[CH_NS], dY[CH_NS], dYp[CH_NS], dinew[CH_NS];
+  double r = 0.0;
+#pragma unroll
+  for (int i = 0; i < CH_NS; i++) {
+    di[i] = rho[(long)i * n + c];
+    r = r + di[i];
+    dY[i] = (double)Oin[(long)i * n + c] * scfg.out_scale[i];
+  }
+#pragma unroll
+  for (int i = 0; i < CH_NS; i++) {
+    double a = 0.0;
+#pragma unroll
+    for (int j = 0; j < CH_NS; j++) a = a + scfg.P[i][j] * dY[j];
+    dYp[i] = a;
+  }
+  // each element's balance to elem_tol of the cell's total element content
+  // (mol/kg): a cell without one element (pure air has no H) is held to the
+  // same absolute bound as the others, not to zero
+  double content = 0.0;
+#pragma unroll
+  for (int e = 0; e < 3; e++)
+#pragma unroll
+    for (int i = 0; i < CH_NS; i++) content = content + scfg.Cel[e][i] * fmax(di[i] / r, 0.0);
+#pragma unroll
+  for (int e = 0; e < 3; e++) {
+    double res = 0.0;
+#pragma unroll
+    for (int i = 0; i < CH_NS; i++) res = res + scfg.Cel[e][i] * dYp[i];
+    if (!(fabs(res) <= scfg.elem_tol * content + 1e-300)) why |= 2;
+  }
+#pragma unroll
+  for (int i = 0; i < CH_NS; i++) {
+    if (!(di[i] / r + dYp[i] >= -scfg.y_neg_tol)) why |= 4;
+    if (!(fabs(dYp[i]) <= scfg.dy_max)) why |= 16;  // also NaN
+    dinew[i] = di[i] + r * dYp[i];
+  }
+  double E = Ecell[c];
+  chem_update_et(m, di, dinew, E);
+  const double Tn = chem_T(m, dinew, E);
+  if (!(Tn >= scfg.T_out_lo && Tn <= scfg.T_out_hi)) why |= 8;  // also NaN
+  if (why == 0) {
+#pragma unroll
+    for (int i = 0; i < CH_NS; i++) rho[(long)i * n + c] = dinew[i];
+    Ecell[c] = E;
+  } else {
+    const unsigned long long k = atomicAdd(counters + 2, 1ull);
+    fb_list[k] = (int)c;
+  }
+  if (flag_out) flag_out[c] = why;
+}
 
 // checks of the surrogate network, fallback list capacity, network view into
 // the constant bank, counters cleared (on CX.stream)
@@ -1107,8 +1135,14 @@ int surr_prepare(long n, const char* who) {
   if (n > 0x7fffffffL) return cfail(std::string(who) + ": at most 2^31-1 cells");
   if (n > CX.fb_cap) {
     if (CX.fb_list) hipFree(CX.fb_list);
+    if (CX.surr_buf) hipFree(CX.surr_buf);
     CX.fb_list = nullptr;
-    if (hipMalloc(&CX.fb_list, (size_t)n * sizeof(int)) != hipSuccess) return cfail(std::string(who) + ": hipMalloc");
+    CX.surr_buf = nullptr;
+    CX.fb_cap = 0;
+    // the phases' buffers: inputs X[12][n] and outputs O[9][n] (float), ok[n]
+    if (hipMalloc(&CX.fb_list, (size_t)n * sizeof(int)) != hipSuccess ||
+        hipMalloc(&CX.surr_buf, (size_t)n * (21 * sizeof(float) + 1)) != hipSuccess)
+      return cfail(std::string(who) + ": hipMalloc");
     CX.fb_cap = n;
   }
   if (hipMemcpyToSymbolAsync(HIP_SYMBOL(cnet), net, sizeof(MlpDevice), 0, hipMemcpyHostToDevice, CX.stream) !=
@@ -1121,8 +1155,15 @@ int surr_prepare(long n, const char* who) {
 template <int V>
 void launch_surrogate_v(long n, double* rho, double* E, const double* dt, int* flag_out) {
   constexpr int R = SurrTile<V>::ROWS, NW = SurrTile<V>::NW;
-  hipLaunchKernelGGL(k_surrogate<V>, dim3((unsigned)((n + R - 1) / R)), dim3(NW * 64), 0, CX.stream, n, rho, E, dt,
-                     flag_out, CX.fb_list, CX.counter);
+  float* X = (float*)CX.surr_buf;
+  float* O = X + 12 * (size_t)CX.fb_cap;
+  unsigned char* ok = (unsigned char*)(O + 9 * (size_t)CX.fb_cap);
+  const dim3 g1((unsigned)((n + 255) / 256));
+  hipLaunchKernelGGL(k_surr_features, g1, dim3(256), 0, CX.stream, n, (const double*)rho, (const double*)E, dt, X, ok);
+  hipLaunchKernelGGL(k_surr_mlp<V>, dim3((unsigned)((n + R - 1) / R)), dim3(NW * 64), 0, CX.stream, n,
+                     (const float*)X, O);
+  hipLaunchKernelGGL(k_surr_finish, g1, dim3(256), 0, CX.stream, n, rho, E, (const float*)O,
+                     (const unsigned char*)ok, flag_out, CX.fb_list, CX.counter);
 }
 void launch_surrogate(int v, long n, double* rho, double* E, const double* dt, int* flag_out) {
   switch (v) {
@@ -1560,7 +1601,7 @@ int32_t ocfd_chem_surrogate_batch(int64_t n, double* rho, double* E, const doubl
                               (int*)flag_out, CX.order != 0);
 }
 
-// Autotuner of the fused surrogate kernel: every tile shape runs `reps`
+// Autotuner of the surrogate network kernel: every tile shape runs `reps`
 // times on scratch copies of the batch (the inputs are not modified), timed
 // with HIP events on the chemistry stream; the fastest is kept.  Results are
 // bit-identical for every shape.  ms_out (may be NULL) receives the mean ms

@@ -38,14 +38,15 @@ constexpr int TM = 64, TN = 64, TK = 32, LDT = TK + 8;  // LDS row: 40 bf16 = 80
 enum { EPI_STORE = 0, EPI_BIAS_GELU = 1, EPI_DGELU = 2 };
 
 // GELU in its tanh form, 0.5 z (1 + tanh u) with u = c (z + 0.044715 z^3),
-// evaluated as z * sigmoid(2u) = z * rcp(1 + exp2(-2 c log2(e) (z + 0.044715
-// z^3))) with the hardware exp2 and reciprocal (v_exp_f32, v_rcp_f32: 1 ulp,
-// far below the bfloat16 rounding of the activations; a correctly rounded
-// division cost a sixth of the fused inference kernel's time).  The inference
-// tile evaluates the same expression.
+// evaluated as z * sigmoid(2u) = z * rcp(1 + exp2(z (C0 + C1 z^2))), C0 =
+// -2 c log2(e), C1 = 0.044715 C0, with the hardware exp2 and reciprocal
+// (v_exp_f32, v_rcp_f32: 1 ulp, far below the bfloat16 rounding of the
+// activations; a correctly rounded division cost a sixth of the fused
+// inference kernel's time).  The inference tile evaluates the same
+// expression.
 __device__ __forceinline__ float gelu_sig(float z) {
-  const float w = fmaf(0.044715f * z * z, z, z);
-  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f((-2.0f * 0.7978845608028654f * 1.4426950408889634f) * w));
+  constexpr float C0 = -2.0f * 0.7978845608028654f * 1.4426950408889634f, C1 = C0 * 0.044715f;
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * fmaf(z * z, C1, C0)));
 }
 __device__ __forceinline__ float gelu_f(float z) { return z * gelu_sig(z); }
 __device__ __forceinline__ float dgelu_f(float z) {

@@ -18,62 +18,74 @@ typedef float tfloatx4 __attribute__((ext_vector_type(4)));
 constexpr int MT_ROWS = 128;  // largest number of cells per workgroup
 constexpr int MT_LDH = 264;   // LDS row pitch (bf16): 256 + 8 against bank conflicts
 
-// GELU (tanh form) as z * sigmoid(2u) with the hardware exp and reciprocal,
-// the same evaluation as training
+// GELU in its tanh form, z * sigmoid(2u), u = c (z + 0.044715 z^3), with
+// the hardware exp2 and reciprocal; 2u log2(e) is formed as
+// z (C0 + C1 z^2), the same evaluation as training
+constexpr float MT_G0 = -2.0f * 0.7978845608028654f * 1.4426950408889634f;
+constexpr float MT_G1 = MT_G0 * 0.044715f;
 __device__ __forceinline__ float mt_gelu(float z) {
-  const float w = fmaf(0.044715f * z * z, z, z);
-  return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f((-2.0f * 0.7978845608028654f * 1.4426950408889634f) * w));
+  const float t = z * fmaf(z * z, MT_G1, MT_G0);
+  return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(t));
 }
+
+typedef __bf16 tbf16x4 __attribute__((ext_vector_type(4)));
 
 // H[ROWS][0:K] (LDS) -> act(b + H . W^T) written back into H[ROWS][0:256].
 // W: [256][ldw] bf16 row-major (out x in).  NW waves; wave w computes output
-// columns [256/NW w, 256/NW (w+1)) for all ROWS rows.  Every output element
-// accumulates its K-chunks of 32 in increasing order whatever ROWS and NW
-// are, so all tile shapes give bit-identical results.
-template <bool GELU, int ROWS, int NW>
-__device__ __forceinline__ void mt_layer256(tbf16* H, const tbf16* __restrict__ W, const float* __restrict__ b, int K,
+// columns [256/NW w, 256/NW (w+1)) for all ROWS rows.  The MFMA takes the
+// weights as its A operand and the activations as B, so a lane's
+// accumulator holds 4 consecutive output columns of one cell, converted
+// and stored to LDS as one 8-byte write.  Every output element accumulates
+// its K-chunks of 32 in increasing order whatever ROWS and NW are, so all
+// tile shapes give bit-identical results.  K (64 or 256) is a template
+// argument: the K loop unrolls and the weight fragments are loaded two
+// chunks ahead (from L2, whose latency exceeds one chunk of MFMA work)
+// into a rotating set of registers.
+template <bool GELU, int ROWS, int NW, int K>
+__device__ __forceinline__ void mt_layer256(tbf16* H, const tbf16* __restrict__ W, const float* __restrict__ b,
                                             int ldw, int lane, int w) {
-  constexpr int IT = ROWS / 16, JT = 16 / NW;
-  const int lr = lane & 15, lk = (lane >> 4) * 8;
+  constexpr int IT = ROWS / 16, JT = 16 / NW, KS = K / 32;
+  static_assert(K % 32 == 0 && KS >= 1, "K in chunks of 32");
+  const int lr = lane & 15, lk = (lane >> 4) * 8, lq = (lane >> 4) * 4;
   const int cb = 16 * JT * w;
-  // the accumulators start from the bias of their column
-  tfloatx4 acc[IT][JT];
+  // the accumulators start from the bias of their columns (the first
+  // chunk's MFMA takes it as its C operand)
+  tfloatx4 acc[IT][JT], bias[JT];
 #pragma unroll
-  for (int j = 0; j < JT; j++) {
-    const float bias = b[cb + 16 * j + lr];
+  for (int j = 0; j < JT; j++) bias[j] = *(const tfloatx4*)&b[cb + 16 * j + lq];
+  const tbf16* Wl = W + (long)(cb + lr) * ldw + lk;
+  tbf16x8 wf[3][JT];
 #pragma unroll
-    for (int i = 0; i < IT; i++) acc[i][j] = tfloatx4{bias, bias, bias, bias};
-  }
-  tbf16x8 bn[JT];
+  for (int s = 0; s < 2 && s < KS; s++)
 #pragma unroll
-  for (int j = 0; j < JT; j++) bn[j] = *(const tbf16x8*)&W[(long)(cb + 16 * j + lr) * ldw + lk];
-  for (int k0 = 0; k0 < K; k0 += 32) {
-    tbf16x8 bc[JT];
+    for (int j = 0; j < JT; j++) wf[s][j] = *(const tbf16x8*)&Wl[(long)16 * j * ldw + 32 * s];
 #pragma unroll
-    for (int j = 0; j < JT; j++) bc[j] = bn[j];
-    if (k0 + 32 < K) {  // prefetch the next weight fragments
+  for (int s = 0; s < KS; s++) {
+    if (s + 2 < KS) {
 #pragma unroll
-      for (int j = 0; j < JT; j++) bn[j] = *(const tbf16x8*)&W[(long)(cb + 16 * j + lr) * ldw + k0 + 32 + lk];
+      for (int j = 0; j < JT; j++) wf[(s + 2) % 3][j] = *(const tbf16x8*)&Wl[(long)16 * j * ldw + 32 * (s + 2)];
     }
+    tbf16x8 a[IT];
 #pragma unroll
-    for (int i = 0; i < IT; i++) {
-      const tbf16x8 a = *(const tbf16x8*)&H[(16 * i + lr) * MT_LDH + k0 + lk];
-#pragma unroll
-      for (int j = 0; j < JT; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bc[j], acc[i][j], 0, 0, 0);
-    }
-  }
-  __syncthreads();  // every wave has read its input rows
-#pragma unroll
-  for (int j = 0; j < JT; j++) {
-    const int col = cb + 16 * j + lr;
+    for (int i = 0; i < IT; i++) a[i] = *(const tbf16x8*)&H[(16 * i + lr) * MT_LDH + 32 * s + lk];
 #pragma unroll
     for (int i = 0; i < IT; i++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const float v = acc[i][j][r];
-        H[(16 * i + (lane >> 4) * 4 + r) * MT_LDH + col] = (tbf16)(GELU ? mt_gelu(v) : v);
-      }
+      for (int j = 0; j < JT; j++)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[s % 3][j], a[i], s == 0 ? bias[j] : acc[i][j], 0, 0, 0);
   }
+  __syncthreads();  // every wave has read its input rows
+#pragma unroll
+  for (int i = 0; i < IT; i++)
+#pragma unroll
+    for (int j = 0; j < JT; j++) {
+      tfloatx4 v = acc[i][j];
+      if (GELU) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) v[r] = mt_gelu(v[r]);
+      }
+      *(tbf16x4*)&H[(16 * i + lr) * MT_LDH + cb + 16 * j + lq] = __builtin_convertvector(v, tbf16x4);
+    }
   __syncthreads();
 }
 

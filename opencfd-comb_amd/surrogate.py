@@ -254,7 +254,7 @@ class Surrogate:
 
     def calibrate(self, data, tol=5e-4):
         """Sets the increment bound dy_max from labelled cells the network
-        was not trained on: the fused kernel runs on them with no bound and
+        was not trained on: the inference runs on them with no bound and
         no fallback, and among the cells it accepts, dy_max is the largest
         increment size below which every cell's mass fractions (all species)
         are within tol of the plain integrator's (trust_bound).  Returns the
@@ -345,7 +345,7 @@ class Surrogate:
         return flags
 
     def autotune(self, rho, E, dt, reps=3):
-        """Time every tile shape of the fused kernel on scratch copies of the
+        """Time every tile shape of the network kernel on scratch copies of the
         batch and keep the fastest (bit-identical results).  Returns
         (rows, waves, ms per shape)."""
         n = self.chem._args(rho, E, dt)

@@ -1,4 +1,4 @@
-"""The learned chemistry surrogate on the GPU: fused kernel against the
+"""The learned chemistry surrogate on the GPU: inference kernels against the
 generic MLP path, conservation of accepted cells, fallback cells against the
 plain FP64 integrator, and accuracy against the plain integrator.
 
@@ -160,7 +160,7 @@ def test_single_element_streams_pass_the_balance_check(sur):
 
 
 def test_tile_shapes_bitwise_identical_and_autotune(sur):
-    """Every tile shape of the fused kernel gives the same bits; the
+    """Every tile shape of the network kernel gives the same bits; the
     autotuner leaves the inputs untouched and keeps one of the shapes."""
     import torch
     S, ch, M, test = sur
@@ -207,7 +207,7 @@ def test_training_gains_over_untrained_network(sur):
 def test_calibrated_increment_bound(sur):
     """calibrate() on labelled cells sets dy_max so that, on those same cells,
     every accepted cell is within tol of the plain integrator in every mass
-    fraction (the fused kernel is deterministic, so this is exact); exactly
+    fraction (the inference is deterministic, so this is exact); exactly
     the cells it reports go to the fallback with flag 16 alone, and there
     they match the plain integrator bit for bit.  On held-out cells the
     bound keeps the error within a few tol."""
