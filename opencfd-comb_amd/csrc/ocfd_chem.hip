@@ -77,7 +77,7 @@ struct ChemCtx {
   hipStream_t stream = nullptr, own = nullptr;
   unsigned long long* counter = nullptr;  // [0] queue head, [1] unfinished, [2] surrogate fallbacks
   int* fb_list = nullptr;                  // cells handed to the fallback integrator
-  unsigned char* surr_buf = nullptr;       // surrogate phases: X[12][cap], O[9][cap] (float), ok[cap]
+  unsigned char* surr_buf = nullptr;       // surrogate phases: X[cap][16] (bf16), O[9][cap] (float), ok[cap]
   long fb_cap = 0;
   HeavyBuf heavy{};                        // cells handed to the cooperative kernel
   int fallback_scheduler = 2;              // scheduler of the surrogate's fallback
@@ -994,10 +994,12 @@ template <> struct SurrTile<3> { static constexpr int ROWS = 96, NW = 4, OCC = 2
 // on the ROWS threads of a workgroup of NW * 64 while the other waves waited
 // at its barriers.)
 //
-// Phase 1: normalised inputs X[12][n] (float) and the domain check ok[n].
+// Phase 1: normalised inputs X[n][16] (bfloat16, rounded to nearest even;
+// features 12..15 zero), the network's first-layer operand rows, and the
+// domain check ok[n].
 __global__ __launch_bounds__(256) void k_surr_features(long n, const double* __restrict__ rho,
                                                        const double* __restrict__ Ecell,
-                                                       const double* __restrict__ dtcell, float* __restrict__ X,
+                                                       const double* __restrict__ dtcell, tbf16* __restrict__ X,
                                                        unsigned char* __restrict__ okv) {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
@@ -1020,37 +1022,40 @@ __global__ __launch_bounds__(256) void k_surr_features(long n, const double* __r
     x[j] = (float)((f[j] - scfg.in_shift[j]) * scfg.in_scale[j]);
     if (!(fabsf(x[j]) <= 1e6f)) ok = 0;  // also catches NaN
   }
+  tbf16x8 lo, hi;
 #pragma unroll
-  for (int j = 0; j < 12; j++) X[(long)j * n + c] = ok ? x[j] : 0.f;
+  for (int j = 0; j < 8; j++) {
+    lo[j] = (tbf16)(ok ? x[j] : 0.f);
+    hi[j] = (tbf16)(ok && j < 4 ? x[8 + j] : 0.f);
+  }
+  *(tbf16x8*)&X[c * 16] = lo;
+  *(tbf16x8*)&X[c * 16 + 8] = hi;
   okv[c] = (unsigned char)ok;
 }
 
-// Phase 2: the network on tiles of ROWS cells, O[16][n] (float; rows 0..8
-// hold the 9 outputs) from X.
+// Phase 2: the network on tiles of ROWS cells, O[9][n] (float) from X.  The
+// input rows are staged into LDS columns 0..15 with columns 16..31 zeroed:
+// the first layer runs over K = 32 (its weights beyond input 12 are zero).
 template <int V>
 __global__ __launch_bounds__(SurrTile<V>::NW * 64, SurrTile<V>::OCC) void k_surr_mlp(long n,
-                                                                                       const float* __restrict__ X,
+                                                                                       const tbf16* __restrict__ X,
                                                                                        float* __restrict__ Oout) {
   constexpr int ROWS = SurrTile<V>::ROWS, NW = SurrTile<V>::NW, NT = NW * 64;
   __shared__ __attribute__((aligned(16))) tbf16 H[ROWS * MT_LDH];
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int t = threadIdx.x, lane = t & 63, w = __builtin_amdgcn_readfirstlane(t >> 6);
   const long c0 = (long)blockIdx.x * ROWS;
-  for (int e = t; e < ROWS * 64; e += NT) {
-    const int row = e / 64, j = e % 64;
+  const tbf16x8 zero8 = {};
+  for (int e = t; e < ROWS * 2; e += NT) {
+    const int row = e >> 1, h = (e & 1) * 8;
     const long c = c0 + row;
-    H[row * MT_LDH + j] = (tbf16)(j < 12 && c < n ? X[(long)j * n + c] : 0.f);
+    *(tbf16x8*)&H[row * MT_LDH + h] = c < n ? *(const tbf16x8*)&X[c * 16 + h] : zero8;
+    *(tbf16x8*)&H[row * MT_LDH + 16 + h] = zero8;
   }
   __syncthreads();
   const int L = cnet.n_layers;
-  mt_layer256<true, ROWS, NW, 64>(H, cnet.W[0], cnet.b[0], cnet.dinp[0], lane, w);
+  mt_layer256<true, ROWS, NW, 32>(H, cnet.W[0], cnet.b[0], cnet.dinp[0], lane, w);
   for (int l = 1; l < L - 1; l++) mt_layer256<true, ROWS, NW, 256>(H, cnet.W[l], cnet.b[l], 256, lane, w);
-  float* O = (float*)H;
-  mt_out16<ROWS, NW>(H, cnet.W[L - 1], cnet.b[L - 1], 256, 256, O, lane, w);
-  for (int e = t; e < ROWS * CH_NS; e += NT) {
-    const int row = e % ROWS, i = e / ROWS;
-    const long c = c0 + row;
-    if (c < n) Oout[(long)i * n + c] = O[row * 16 + i];
-  }
+  mt_out_cols<ROWS, NW, 256, CH_NS>(H, cnet.W[L - 1], cnet.b[L - 1], 256, Oout, n, c0, lane, w);
 }
 
 // Phase 3: in FP64, dY = P (O * out_scale) (P projects onto element
@@ -1139,9 +1144,9 @@ int surr_prepare(long n, const char* who) {
     CX.fb_list = nullptr;
     CX.surr_buf = nullptr;
     CX.fb_cap = 0;
-    // the phases' buffers: inputs X[12][n] and outputs O[9][n] (float), ok[n]
+    // the phases' buffers: inputs X[n][16] (bf16), outputs O[9][n] (float), ok[n]
     if (hipMalloc(&CX.fb_list, (size_t)n * sizeof(int)) != hipSuccess ||
-        hipMalloc(&CX.surr_buf, (size_t)n * (21 * sizeof(float) + 1)) != hipSuccess)
+        hipMalloc(&CX.surr_buf, (size_t)n * (16 * sizeof(tbf16) + 9 * sizeof(float) + 1)) != hipSuccess)
       return cfail(std::string(who) + ": hipMalloc");
     CX.fb_cap = n;
   }
@@ -1155,13 +1160,13 @@ int surr_prepare(long n, const char* who) {
 template <int V>
 void launch_surrogate_v(long n, double* rho, double* E, const double* dt, int* flag_out) {
   constexpr int R = SurrTile<V>::ROWS, NW = SurrTile<V>::NW;
-  float* X = (float*)CX.surr_buf;
-  float* O = X + 12 * (size_t)CX.fb_cap;
+  tbf16* X = (tbf16*)CX.surr_buf;
+  float* O = (float*)(X + 16 * (size_t)CX.fb_cap);
   unsigned char* ok = (unsigned char*)(O + 9 * (size_t)CX.fb_cap);
   const dim3 g1((unsigned)((n + 255) / 256));
   hipLaunchKernelGGL(k_surr_features, g1, dim3(256), 0, CX.stream, n, (const double*)rho, (const double*)E, dt, X, ok);
   hipLaunchKernelGGL(k_surr_mlp<V>, dim3((unsigned)((n + R - 1) / R)), dim3(NW * 64), 0, CX.stream, n,
-                     (const float*)X, O);
+                     (const tbf16*)X, O);
   hipLaunchKernelGGL(k_surr_finish, g1, dim3(256), 0, CX.stream, n, rho, E, (const float*)O,
                      (const unsigned char*)ok, flag_out, CX.fb_list, CX.counter);
 }

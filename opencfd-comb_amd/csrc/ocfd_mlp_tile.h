@@ -4,8 +4,9 @@
 // staying in LDS as bfloat16 and each wave streaming its slice of the
 // weights from L2.
 //
-// Layer shapes this code takes: input padded to 64, hidden widths 256,
-// output padded; v_mfma_f32_16x16x32_bf16 with FP32 accumulation.  MFMA
+// Layer shapes this code takes: input of at most 32 features (zero-padded),
+// hidden widths 256, at most 16 outputs; v_mfma_f32_16x16x32_bf16 with FP32
+// accumulation.  MFMA
 // fragment layouts (16x16x32): lane l holds A[l%16][8(l/16) .. +8] and
 // B[8(l/16) .. +8][l%16]; the accumulator holds D[4(l/16) + r][l%16], r < 4.
 #pragma once
@@ -37,9 +38,9 @@ typedef __bf16 tbf16x4 __attribute__((ext_vector_type(4)));
 // accumulator holds 4 consecutive output columns of one cell, converted
 // and stored to LDS as one 8-byte write.  Every output element accumulates
 // its K-chunks of 32 in increasing order whatever ROWS and NW are, so all
-// tile shapes give bit-identical results.  K (64 or 256) is a template
-// argument: the K loop unrolls and the weight fragments are loaded two
-// chunks ahead (from L2, whose latency exceeds one chunk of MFMA work)
+// tile shapes give bit-identical results.  K (32 or 256) is a template
+// argument: the K loop unrolls and the weight fragments are loaded one or
+// two chunks ahead (from L2, whose latency exceeds one chunk of MFMA work)
 // into a rotating set of registers.
 template <bool GELU, int ROWS, int NW, int K>
 __device__ __forceinline__ void mt_layer256(tbf16* H, const tbf16* __restrict__ W, const float* __restrict__ b,
@@ -53,17 +54,21 @@ __device__ __forceinline__ void mt_layer256(tbf16* H, const tbf16* __restrict__ 
   tfloatx4 acc[IT][JT], bias[JT];
 #pragma unroll
   for (int j = 0; j < JT; j++) bias[j] = *(const tfloatx4*)&b[cb + 16 * j + lq];
+  // weight prefetch distance in K-chunks: 2, or 1 where the accumulators
+  // of 128 x 64 outputs leave no room
+  constexpr int PD = IT * JT >= 32 ? 1 : 2;
   const tbf16* Wl = W + (long)(cb + lr) * ldw + lk;
-  tbf16x8 wf[3][JT];
+  tbf16x8 wf[PD + 1][JT];
 #pragma unroll
-  for (int s = 0; s < 2 && s < KS; s++)
+  for (int s = 0; s < PD && s < KS; s++)
 #pragma unroll
     for (int j = 0; j < JT; j++) wf[s][j] = *(const tbf16x8*)&Wl[(long)16 * j * ldw + 32 * s];
 #pragma unroll
   for (int s = 0; s < KS; s++) {
-    if (s + 2 < KS) {
+    if (s + PD < KS) {
 #pragma unroll
-      for (int j = 0; j < JT; j++) wf[(s + 2) % 3][j] = *(const tbf16x8*)&Wl[(long)16 * j * ldw + 32 * (s + 2)];
+      for (int j = 0; j < JT; j++)
+        wf[(s + PD) % (PD + 1)][j] = *(const tbf16x8*)&Wl[(long)16 * j * ldw + 32 * (s + PD)];
     }
     tbf16x8 a[IT];
 #pragma unroll
@@ -72,7 +77,8 @@ __device__ __forceinline__ void mt_layer256(tbf16* H, const tbf16* __restrict__ 
     for (int i = 0; i < IT; i++)
 #pragma unroll
       for (int j = 0; j < JT; j++)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[s % 3][j], a[i], s == 0 ? bias[j] : acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[s % (PD + 1)][j], a[i], s == 0 ? bias[j] : acc[i][j],
+                                                            0, 0, 0);
   }
   __syncthreads();  // every wave has read its input rows
 #pragma unroll
@@ -89,39 +95,38 @@ __device__ __forceinline__ void mt_layer256(tbf16* H, const tbf16* __restrict__ 
   __syncthreads();
 }
 
-// Output layer, first 16 output columns: O[ROWS][16] (fp32, may alias H) =
-// b + H[ROWS][0:K] . W[0:16]^T.  The 16-row tiles are dealt to the waves in
-// turn (wave w takes tiles w, w + NW, ...), so ROWS need only be a multiple
-// of 16; each element accumulates its K-chunks in increasing order.
-template <int ROWS, int NW>
-__device__ __forceinline__ void mt_out16(const tbf16* H, const tbf16* __restrict__ W, const float* __restrict__ b,
-                                         int K, int ldw, float* O, int lane, int w) {
-  static_assert(ROWS % 16 == 0, "whole 16-row tiles");
-  constexpr int NT = ROWS / 16, TPW = (NT + NW - 1) / NW;
-  const int lr = lane & 15, lk = (lane >> 4) * 8;
-  const float bias = b[lr];
-  tfloatx4 acc[TPW];
+// Output layer: Oout[col][n] = b + H[ROWS][0:K] . W[col]^T for the first
+// NOUT (<= 16) columns, written from the accumulators (weights as the MFMA A
+// operand, so a lane holds 4 consecutive columns of one cell; 16 cells per
+// store).  The 16-row tiles are dealt to the waves in turn (wave w takes
+// tiles w, w + NW, ...), so ROWS need only be a multiple of 16; each element
+// accumulates its K-chunks in increasing order.  w must be wave-uniform.
+template <int ROWS, int NW, int K, int NOUT>
+__device__ __forceinline__ void mt_out_cols(const tbf16* H, const tbf16* __restrict__ W,
+                                            const float* __restrict__ b, int ldw, float* __restrict__ Oout, long n,
+                                            long c0, int lane, int w) {
+  static_assert(ROWS % 16 == 0 && K % 32 == 0 && NOUT <= 16, "whole tiles");
+  constexpr int NT = ROWS / 16, TPW = (NT + NW - 1) / NW, KS = K / 32;
+  const int lr = lane & 15, lk = (lane >> 4) * 8, lq = (lane >> 4) * 4;
+  const tfloatx4 bias = *(const tfloatx4*)&b[lq];
+  tbf16x8 wf[KS];
 #pragma unroll
-  for (int it = 0; it < TPW; it++) acc[it] = tfloatx4{bias, bias, bias, bias};
-  for (int k0 = 0; k0 < K; k0 += 32) {
-    const tbf16x8 bf = *(const tbf16x8*)&W[(long)lr * ldw + k0 + lk];
-#pragma unroll
-    for (int it = 0; it < TPW; it++) {
-      const int tt = w + it * NW;
-      if (tt < NT) {
-        const tbf16x8 a = *(const tbf16x8*)&H[(16 * tt + lr) * MT_LDH + k0 + lk];
-        acc[it] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bf, acc[it], 0, 0, 0);
-      }
-    }
-  }
-  __syncthreads();  // O may alias H
+  for (int s = 0; s < KS; s++) wf[s] = *(const tbf16x8*)&W[(long)lr * ldw + 32 * s + lk];
 #pragma unroll
   for (int it = 0; it < TPW; it++) {
     const int tt = w + it * NW;
-    if (tt < NT) {
+    if (NT % NW == 0 || tt < NT) {
+      tfloatx4 acc = bias;
 #pragma unroll
-      for (int r = 0; r < 4; r++) O[(16 * tt + (lane >> 4) * 4 + r) * 16 + lr] = acc[it][r];
+      for (int s = 0; s < KS; s++)
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[s], *(const tbf16x8*)&H[(16 * tt + lr) * MT_LDH + 32 * s + lk],
+                                                      acc, 0, 0, 0);
+      const long c = c0 + 16 * tt + lr;
+      if (c < n) {
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+          if (lq + r < NOUT) Oout[(long)(lq + r) * n + c] = acc[r];
+      }
     }
   }
-  __syncthreads();
 }
