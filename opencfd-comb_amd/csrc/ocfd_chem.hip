@@ -78,6 +78,10 @@ struct ChemCtx {
   unsigned long long* counter = nullptr;  // [0] queue head, [1] unfinished, [2] surrogate fallbacks
   int* fb_list = nullptr;                  // cells handed to the fallback integrator
   unsigned char* surr_buf = nullptr;       // surrogate phases: X[cap][16] (bf16), O[9][cap] (float), ok[cap]
+  tbf16* surr_wpack = nullptr;             // network weights in MFMA fragment order
+  long surr_wpack_cap = 0;
+  const tbf16* surr_wp[MLP_MAX_LAYERS] = {};  // per layer, into surr_wpack
+  unsigned long long surr_wp_version = ~0ull;  // network weights version packed
   long fb_cap = 0;
   HeavyBuf heavy{};                        // cells handed to the cooperative kernel
   int fallback_scheduler = 2;              // scheduler of the surrogate's fallback
@@ -918,6 +922,17 @@ struct SurrCfg {
 };
 __constant__ SurrCfg scfg;
 __constant__ MlpDevice cnet;
+// the network's weights in fragment order (mt_pack_src): layer 0 over K = 32,
+// hidden layers over 256, the output layer's first 16 rows
+struct SurrPacked {
+  const tbf16* W[MLP_MAX_LAYERS];
+};
+__constant__ SurrPacked cwp;
+
+__global__ void k_pack_frag(long count, int KS, int ldw, const tbf16* __restrict__ src, tbf16* __restrict__ dst) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < count) dst[e] = src[mt_pack_src(e, KS, ldw)];
+}
 
 // raw surrogate features of one cell: [T, ln p, cbrt(Y_0..Y_8), ln dt]
 __device__ __forceinline__ void surr_raw_features(Mech m, const double* di, double r, double E, double dt, double* f,
@@ -1053,9 +1068,9 @@ __global__ __launch_bounds__(SurrTile<V>::NW * 64, SurrTile<V>::OCC) void k_surr
   }
   __syncthreads();
   const int L = cnet.n_layers;
-  mt_layer256<true, ROWS, NW, 32>(H, cnet.W[0], cnet.b[0], cnet.dinp[0], lane, w);
-  for (int l = 1; l < L - 1; l++) mt_layer256<true, ROWS, NW, 256>(H, cnet.W[l], cnet.b[l], 256, lane, w);
-  mt_out_cols<ROWS, NW, 256, CH_NS>(H, cnet.W[L - 1], cnet.b[L - 1], 256, Oout, n, c0, lane, w);
+  mt_layer256<true, ROWS, NW, 32>(H, cwp.W[0], cnet.b[0], lane, w);
+  for (int l = 1; l < L - 1; l++) mt_layer256<true, ROWS, NW, 256>(H, cwp.W[l], cnet.b[l], lane, w);
+  mt_out_cols<ROWS, NW, 256, CH_NS>(H, cwp.W[L - 1], cnet.b[L - 1], Oout, n, c0, lane, w);
 }
 
 // Phase 3: in FP64, dY = P (O * out_scale) (P projects onto element
@@ -1150,8 +1165,36 @@ int surr_prepare(long n, const char* who) {
       return cfail(std::string(who) + ": hipMalloc");
     CX.fb_cap = n;
   }
+  // the weights in fragment order, repacked whenever the network's weights
+  // changed: 16 output blocks x 1 chunk for layer 0, 16 x 8 for the hidden
+  // layers, 1 x 8 for the output layer
+  long off[MLP_MAX_LAYERS], need = 0;
+  for (int l = 0; l < L; l++) {
+    off[l] = need;
+    need += (l == L - 1 ? 1 : 16) * (l == 0 ? 1 : 8) * 512L;
+  }
+  if (need > CX.surr_wpack_cap) {
+    CX.surr_wp_version = ~0ull;
+    if (CX.surr_wpack) hipFree(CX.surr_wpack);
+    CX.surr_wpack = nullptr;
+    CX.surr_wpack_cap = 0;
+    if (hipMalloc(&CX.surr_wpack, need * sizeof(tbf16)) != hipSuccess)
+      return cfail(std::string(who) + ": hipMalloc");
+    CX.surr_wpack_cap = need;
+  }
+  if (CX.surr_wp_version != ocfd_mlp_weights_version()) {
+    for (int l = 0; l < L; l++) {
+      const long cnt = (l == L - 1 ? 1 : 16) * (l == 0 ? 1 : 8) * 512L;
+      hipLaunchKernelGGL(k_pack_frag, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, CX.stream, cnt,
+                         l == 0 ? 1 : 8, net->dinp[l], (const tbf16*)net->W[l], CX.surr_wpack + off[l]);
+      CX.surr_wp[l] = CX.surr_wpack + off[l];
+    }
+    CX.surr_wp_version = ocfd_mlp_weights_version();
+  }
   if (hipMemcpyToSymbolAsync(HIP_SYMBOL(cnet), net, sizeof(MlpDevice), 0, hipMemcpyHostToDevice, CX.stream) !=
-      hipSuccess)
+          hipSuccess ||
+      hipMemcpyToSymbolAsync(HIP_SYMBOL(cwp), CX.surr_wp, sizeof(SurrPacked), 0, hipMemcpyHostToDevice,
+                             CX.stream) != hipSuccess)
     return cfail(std::string(who) + ": copy failed");
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, CX.stream, CX.counter, 8);
   return 0;

@@ -15,4 +15,6 @@ struct MlpDevice {
 
 // NULL until ocfd_mlp_create succeeded
 const MlpDevice* ocfd_mlp_device_view();
+// changes whenever the bf16 weights change (creation, set_params, train_step)
+unsigned long long ocfd_mlp_weights_version();
 hipStream_t ocfd_mlp_stream_handle();

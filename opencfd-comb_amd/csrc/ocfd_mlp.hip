@@ -266,6 +266,7 @@ struct MlpCtx {
   long lpart_cap = 0;
   float* loss = nullptr;  // device scalar
   long step = 0;
+  unsigned long long version = 0;  // bumped whenever the bf16 weights change
   hipStream_t stream = nullptr;
   std::string err;
   MlpDevice dev{};
@@ -438,6 +439,7 @@ void refresh_device_view() {
 }  // namespace
 
 const MlpDevice* ocfd_mlp_device_view() { return X.ready ? &X.dev : nullptr; }
+unsigned long long ocfd_mlp_weights_version() { return X.version; }
 hipStream_t ocfd_mlp_stream_handle() { return X.stream; }
 
 extern "C" {
@@ -472,6 +474,7 @@ int32_t ocfd_mlp_create(int32_t n_layers, const int32_t* dims) {
   X.step = 0;
   X.ready = true;
   refresh_device_view();
+  X.version++;
   return 0;
 }
 
@@ -508,6 +511,7 @@ int32_t ocfd_mlp_set_params(const float* flat) {
       return mfail("ocfd_mlp_set_params: memset failed");
   }
   X.step = 0;
+  X.version++;
   return hipStreamSynchronize(X.stream) == hipSuccess ? 0 : mfail("ocfd_mlp_set_params: stream error");
 }
 
@@ -579,6 +583,7 @@ int32_t ocfd_mlp_train_step(int64_t B, const float* x, const float* y, float lr,
   const long Bp = pad_to((int)B, 64);
   if (forward(Bp) || backward(B, Bp, y)) return 1;
   X.step++;
+  X.version++;
   const float c1 = 1.f - powf(beta1, (float)X.step), c2 = 1.f - powf(beta2, (float)X.step);
   for (auto& l : X.lay) {
     const long nw = (long)l.doutp * l.dinp;

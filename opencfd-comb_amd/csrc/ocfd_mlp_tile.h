@@ -31,6 +31,17 @@ __device__ __forceinline__ float mt_gelu(float z) {
 
 typedef __bf16 tbf16x4 __attribute__((ext_vector_type(4)));
 
+// Fragment order of a weight matrix W[out][ldw] (bf16, row-major) for the
+// MFMA A operand over K = 32 KS: element (block jb of 16 outputs, K-chunk
+// s, lane l, q < 8) = W[16 jb + l % 16][32 s + 8 (l / 16) + q] sits at
+// ((jb KS + s) 64 + l) 8 + q, so the 64 lanes' 16-byte fragment loads of
+// one (jb, s) cover 1 KB contiguously.
+__host__ __device__ inline long mt_pack_src(long e, int KS, int ldw) {
+  const long jb = e / (KS * 512);
+  const int rem = (int)(e % (KS * 512)), s = rem / 512, l = (rem % 512) / 8, q = rem % 8;
+  return (16 * jb + l % 16) * ldw + 32 * s + 8 * (l / 16) + q;
+}
+
 // H[ROWS][0:K] (LDS) -> act(b + H . W^T) written back into H[ROWS][0:256].
 // W: [256][ldw] bf16 row-major (out x in).  NW waves; wave w computes output
 // columns [256/NW w, 256/NW (w+1)) for all ROWS rows.  The MFMA takes the
@@ -41,10 +52,11 @@ typedef __bf16 tbf16x4 __attribute__((ext_vector_type(4)));
 // tile shapes give bit-identical results.  K (32 or 256) is a template
 // argument: the K loop unrolls and the weight fragments are loaded one or
 // two chunks ahead (from L2, whose latency exceeds one chunk of MFMA work)
-// into a rotating set of registers.
+// into a rotating set of registers.  Wp holds the weights in fragment order
+// (mt_pack_index): a wave's fragment load is 1 KB contiguous.
 template <bool GELU, int ROWS, int NW, int K>
-__device__ __forceinline__ void mt_layer256(tbf16* H, const tbf16* __restrict__ W, const float* __restrict__ b,
-                                            int ldw, int lane, int w) {
+__device__ __forceinline__ void mt_layer256(tbf16* H, const tbf16* __restrict__ Wp, const float* __restrict__ b,
+                                            int lane, int w) {
   constexpr int IT = ROWS / 16, JT = 16 / NW, KS = K / 32;
   static_assert(K % 32 == 0 && KS >= 1, "K in chunks of 32");
   const int lr = lane & 15, lk = (lane >> 4) * 8, lq = (lane >> 4) * 4;
@@ -57,18 +69,18 @@ __device__ __forceinline__ void mt_layer256(tbf16* H, const tbf16* __restrict__ 
   // weight prefetch distance in K-chunks: 2, or 1 where the accumulators
   // of 128 x 64 outputs leave no room
   constexpr int PD = IT * JT >= 32 ? 1 : 2;
-  const tbf16* Wl = W + (long)(cb + lr) * ldw + lk;
+  const tbf16* Wl = Wp + (long)(cb / 16) * KS * 512 + lane * 8;
   tbf16x8 wf[PD + 1][JT];
 #pragma unroll
   for (int s = 0; s < PD && s < KS; s++)
 #pragma unroll
-    for (int j = 0; j < JT; j++) wf[s][j] = *(const tbf16x8*)&Wl[(long)16 * j * ldw + 32 * s];
+    for (int j = 0; j < JT; j++) wf[s][j] = *(const tbf16x8*)&Wl[(j * KS + s) * 512];
 #pragma unroll
   for (int s = 0; s < KS; s++) {
     if (s + PD < KS) {
 #pragma unroll
       for (int j = 0; j < JT; j++)
-        wf[(s + PD) % (PD + 1)][j] = *(const tbf16x8*)&Wl[(long)16 * j * ldw + 32 * (s + PD)];
+        wf[(s + PD) % (PD + 1)][j] = *(const tbf16x8*)&Wl[(j * KS + s + PD) * 512];
     }
     tbf16x8 a[IT];
 #pragma unroll
@@ -102,16 +114,16 @@ __device__ __forceinline__ void mt_layer256(tbf16* H, const tbf16* __restrict__ 
 // tiles w, w + NW, ...), so ROWS need only be a multiple of 16; each element
 // accumulates its K-chunks in increasing order.  w must be wave-uniform.
 template <int ROWS, int NW, int K, int NOUT>
-__device__ __forceinline__ void mt_out_cols(const tbf16* H, const tbf16* __restrict__ W,
-                                            const float* __restrict__ b, int ldw, float* __restrict__ Oout, long n,
-                                            long c0, int lane, int w) {
+__device__ __forceinline__ void mt_out_cols(const tbf16* H, const tbf16* __restrict__ Wp,
+                                            const float* __restrict__ b, float* __restrict__ Oout, long n, long c0,
+                                            int lane, int w) {
   static_assert(ROWS % 16 == 0 && K % 32 == 0 && NOUT <= 16, "whole tiles");
   constexpr int NT = ROWS / 16, TPW = (NT + NW - 1) / NW, KS = K / 32;
   const int lr = lane & 15, lk = (lane >> 4) * 8, lq = (lane >> 4) * 4;
   const tfloatx4 bias = *(const tfloatx4*)&b[lq];
   tbf16x8 wf[KS];
 #pragma unroll
-  for (int s = 0; s < KS; s++) wf[s] = *(const tbf16x8*)&W[(long)lr * ldw + 32 * s + lk];
+  for (int s = 0; s < KS; s++) wf[s] = *(const tbf16x8*)&Wp[s * 512 + lane * 8];
 #pragma unroll
   for (int it = 0; it < TPW; it++) {
     const int tt = w + it * NW;
