@@ -63,12 +63,43 @@ template <> struct SweepTile<2, 3> { static constexpr int NP = 8, TS = 56, NT = 
 #define OCFD_SWEEP_VARIANTS 4
 
 // Interfaces per thread of the linear pass: the smallest run length L for
-// which the 10 NP lines (5 components x 2 sides x NP pencils) of NIF
-// interfaces, cut into runs of L, fit in NT threads.
+// which the 5 NP lines of one side (5 components x NP pencils) of NIF
+// interfaces, cut into runs of L, fit in the NT / 2 threads of that side.
 constexpr int sweep_run(int NP, int NIF, int NT) {
   int L = 1;
-  while (10 * NP * ((NIF + L - 1) / L) > NT) L++;
+  while (5 * NP * ((NIF + L - 1) / L) > NT / 2) L++;
   return L;
+}
+
+// One run of the linear pass: UD7L at interfaces l0 .. l1-1 (at most L) of
+// one line, the 8-point window sliding by one point per interface (flux+:
+// w[q] = v(l-3+q); flux-: w[q] = v(l+4-q)); F / H point at the line's
+// pencil in sF / sH, PS / IS are the LDS strides between neighbouring
+// points / interfaces.  Unrolled, so the window slides by renaming.
+template <int SIDE, int L, int PS, int IS>
+__device__ __forceinline__ void ud7l_run(const double* F, double* H, int l0, int l1, const double* cf) {
+  const double c0 = cf[0], c1 = cf[1], c2 = cf[2], c3 = cf[3], c4 = cf[4], c5 = cf[5], c6 = cf[6], c7 = cf[7];
+  double w[8];
+#pragma unroll
+  for (int q = 0; q < 7; q++) {
+    if (SIDE == 0) w[q + 1] = F[(l0 + q) * PS];
+    else w[q] = F[(l0 + 6 - q) * PS];
+  }
+#pragma unroll
+  for (int u = 0; u < L; u++) {
+    const int l = l0 + u;
+    if (l >= l1) break;
+    if (SIDE == 0) {
+#pragma unroll
+      for (int q = 0; q < 7; q++) w[q] = w[q + 1];
+      w[7] = F[(l + 7) * PS];
+    } else {
+#pragma unroll
+      for (int q = 7; q > 0; q--) w[q] = w[q - 1];
+      w[0] = F[(l + 7) * PS];
+    }
+    H[l * IS] = c0 * w[0] + c1 * w[1] + c2 * w[2] + c3 * w[3] + c4 * w[4] + c5 * w[5] + c6 * w[6] + c7 * w[7];
+  }
 }
 
 template <int DIR>
@@ -329,44 +360,26 @@ __global__ __launch_bounds__((SweepTile<DIR, V>::NT)) void k_sweep(DevState S, i
 
   // phase 2a': UD7L (OCFD_UD7L_P, OCFD_Schemes.f90:302-313) at every
   // interface of the tile.  Thread = (line, run): a line is (side, component,
-  // pencil), a run L consecutive interfaces; the window slides by one point
-  // per interface (flux+: w[q] = v(l-3+q); flux-: w[q] = v(l+4-q)), one LDS
-  // read each, the same products summed in the same order as recon().
+  // pencil), a run L consecutive interfaces (ud7l_run: one LDS read per
+  // interface, the same products summed in the same order as recon()).  The
+  // first NT / 2 threads take the flux+ lines and the others the flux-
+  // lines, so the side is uniform over each wavefront.
   constexpr int LPS = DIR == 0 ? 1 : NP;  // LDS stride between neighbouring points of a pencil
+  constexpr int LIS = DIR == 0 ? 1 : NP;  // LDS stride between neighbouring interfaces of a pencil
   if (lin) {
-    constexpr int L = sweep_run(NP, NIF, NT), G = (NIF + L - 1) / L;
-    if (tid < 10 * NP * G) {
-      const int g = tid % G, line = tid / G;
-      const int pl = line % NP, sc = line / NP;  // sc = side*5 + comp
-      const int side = sc >= 5 ? 1 : 0;
+    constexpr int HT = NT / 2, L = sweep_run(NP, NIF, NT), G = (NIF + L - 1) / L;
+    static_assert(HT % 64 == 0 && 5 * NP * G <= HT, "one side per wavefront");
+    const int side = __builtin_amdgcn_readfirstlane(tid >= HT ? 1 : 0);
+    const int rest = tid - side * HT;
+    if (rest < 5 * NP * G) {
+      const int g = rest % G, line = rest / G;
+      const int pl = line % NP, sc = side * 5 + line / NP;
       const int l0 = g * L, l1 = min(l0 + L, ncell + 1);
       if (pen0 + pl < npen && l0 < l1) {
-        const double c0_ = S.ud7l[0], c1_ = S.ud7l[1], c2_ = S.ud7l[2], c3_ = S.ud7l[3];
-        const double c4_ = S.ud7l[4], c5_ = S.ud7l[5], c6_ = S.ud7l[6], c7_ = S.ud7l[7];
-        const double* F = sF + sc * NP * NPT;
-        double* H = sH + sc * NP * NIF;
-        double w[8];
-        if (!side) {
-#pragma unroll
-          for (int q = 0; q < 7; q++) w[q + 1] = F[lp(pl, l0 + q)];
-        } else {
-#pragma unroll
-          for (int q = 0; q < 7; q++) w[q] = F[lp(pl, l0 + 7 - q - 1)];
-        }
-#pragma unroll 1
-        for (int l = l0; l < l1; l++) {
-          if (!side) {
-#pragma unroll
-            for (int q = 0; q < 7; q++) w[q] = w[q + 1];
-            w[7] = F[lp(pl, l + 7)];
-          } else {
-#pragma unroll
-            for (int q = 7; q > 0; q--) w[q] = w[q - 1];
-            w[0] = F[lp(pl, l + 7)];
-          }
-          H[lh(pl, l)] = c0_ * w[0] + c1_ * w[1] + c2_ * w[2] + c3_ * w[3] + c4_ * w[4] + c5_ * w[5] +
-                         c6_ * w[6] + c7_ * w[7];
-        }
+        const double* F = sF + sc * NP * NPT + lp(pl, 0);
+        double* H = sH + sc * NP * NIF + lh(pl, 0);
+        if (side == 0) ud7l_run<0, L, LPS, LIS>(F, H, l0, l1, S.ud7l);
+        else ud7l_run<1, L, LPS, LIS>(F, H, l0, l1, S.ud7l);
       }
     }
     __syncthreads();
