@@ -412,26 +412,34 @@ __global__ __launch_bounds__((SweepTile<DIR, V>::NT)) void k_sweep(DevState S, i
   }
   __syncthreads();
 
-  // phase 3: du from interface differences
+  // phase 3: du from interface differences, one thread per cell for the
+  // five components (the cell's index work done once)
   const double h1 = S.h_1[DIR];
-  for (int t = tid; t < 5 * NP * TS; t += NT) {
-    const int m = t / (NP * TS);
-    const int r = t % (NP * TS);
+  for (int r = tid; r < NP * TS; r += NT) {
     const int pl = DIR == 0 ? r / TS : r % NP;
     const int c = DIR == 0 ? r % TS : r / NP;
     const long pen = pen0 + pl;
     if (pen >= npen || c >= ncell) continue;
-    const double hp1 = sH[m * NP * NIF + lh(pl, c + 1)], hp0 = sH[m * NP * NIF + lh(pl, c)];
-    const double hm1 = sH[(5 + m) * NP * NIF + lh(pl, c + 1)], hm0 = sH[(5 + m) * NP * NIF + lh(pl, c)];
-    const double val = (hp1 - hp0 + hm1 - hm0) * h1;
-    const long ci = PB.iq1[pl] + (long)(c0 + c - 1) * S.si[DIR] + (long)m * S.ni;
+    const long ci = PB.iq1[pl] + (long)(c0 + c - 1) * S.si[DIR];
+    const double* Hc = sH + lh(pl, c);
+    double val[5], old[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      const double hp1 = Hc[m * NP * NIF + LIS], hp0 = Hc[m * NP * NIF];
+      const double hm1 = Hc[(5 + m) * NP * NIF + LIS], hm0 = Hc[(5 + m) * NP * NIF];
+      val[m] = (hp1 - hp0 + hm1 - hm0) * h1;
+      if (DIR != 0) old[m] = S.du[ci + (long)m * S.ni];
+    }
     if (DIR == 0) {
-      S.du[ci] = val;
+#pragma unroll
+      for (int m = 0; m < 5; m++) S.du[ci + (long)m * S.ni] = val[m];
     } else if (DIR == 1) {
-      S.du[ci] = S.du[ci] + val;
+#pragma unroll
+      for (int m = 0; m < 5; m++) S.du[ci + (long)m * S.ni] = old[m] + val[m];
     } else {
-      const long q = PB.hq1[pl] + (long)(c0 + c - 1) * S.sh[DIR];
-      S.du[ci] = -(S.du[ci] + val) * S.Ajac[q];
+      const double aj = S.Ajac[PB.hq1[pl] + (long)(c0 + c - 1) * S.sh[DIR]];
+#pragma unroll
+      for (int m = 0; m < 5; m++) S.du[ci + (long)m * S.ni] = -(old[m] + val[m]) * aj;
     }
   }
 }
