@@ -52,9 +52,10 @@ def run(S, test, fallback=True):
     return rho, E, flags
 
 
-def test_fused_kernel_matches_generic_forward(sur):
-    import torch
-    S, ch, M, test = sur
+def inference_vs_generic(S, test):
+    """Increments of the inference kernels and of the generic MLP forward
+    (features mode 2, scaling, element projection) on the accepted cells, in
+    units of each species' out_scale; also the inference increments."""
     rho, E, flags = run(S, test, fallback=False)
     x = S.features(test["rho0"], test["E0"], test["dt"], mode=2)
     z = S.net.forward(x).double().cpu().numpy()
@@ -68,7 +69,13 @@ def test_fused_kernel_matches_generic_forward(sur):
     acc = flags.cpu().numpy() == 0
     # N2 (index 8) is inert: its increment is zero and its scale a floor
     err = np.abs(dY_fused[acc] - dY_gen[acc])[:, :8] / sigma[None, :8]
-    print(f"fused vs generic: max {err.max():.2e}, median {np.median(err):.2e} (in units of out_scale)")
+    return err, dY_fused
+
+
+def test_fused_kernel_matches_generic_forward(sur):
+    S, ch, M, test = sur
+    err, _ = inference_vs_generic(S, test)
+    print(f"inference vs generic: max {err.max():.2e}, median {np.median(err):.2e} (in units of out_scale)")
     assert err.max() <= 2e-2 and np.median(err) <= 1e-3
 
 
@@ -264,3 +271,22 @@ def test_fallback_order_is_bitwise_identical(sur):
     assert out[0][3] > 0
     for a, b in zip(out[0], out[1]):
         assert np.array_equal(a, b)
+
+
+def test_inference_follows_new_weights(sur):
+    """The inference keeps the weights in its own (MFMA fragment) order; after
+    set_params it must use the new weights: it matches the generic forward
+    with them and differs from the result with the old ones."""
+    S, ch, M, test = sur
+    p0 = S.net.get_params()
+    _, d0 = inference_vs_generic(S, test)
+    try:
+        S.net.set_params(p0 * np.float32(0.9))
+        err, d1 = inference_vs_generic(S, test)
+        print(f"new weights: inference vs generic max {err.max():.2e}, median {np.median(err):.2e}")
+        assert err.max() <= 2e-2 and np.median(err) <= 1e-3
+        assert not np.array_equal(d0, d1)
+    finally:
+        S.net.set_params(p0)
+    _, d2 = inference_vs_generic(S, test)
+    assert np.array_equal(d0, d2)
