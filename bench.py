@@ -228,7 +228,7 @@ def chem_bench(m, ncell=1 << 21, cpu=True, surrogate_cells=1 << 24, surrogate_ep
         out["surrogate"] = dict(
             network="12 -> 256 x5 -> 9 GELU, BF16 MFMA (FP32 accumulation), features / network / checks kernels",
             tile=dict(cells_per_workgroup=tile_rows, waves=tile_waves,
-                      ms_per_shape={"128x4": tile_ms[0], "128x8": tile_ms[1], "64x4": tile_ms[2],
+                      ms_per_shape={"128x4": tile_ms[0], "128x4 (2x2 waves)": tile_ms[1], "64x4": tile_ms[2],
                                     "96x4": tile_ms[3]}),
             train=dict(cells=surrogate_cells, epochs=len(hist), batch=8192, data_s=t1 - t0, train_s=t2 - t1,
                        final_loss=hist[-1]),

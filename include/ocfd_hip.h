@@ -552,8 +552,9 @@ int32_t ocfd_chem_surrogate_set(const double* cfg, int32_t count);
 int32_t ocfd_chem_surrogate_batch(int64_t n, double* rho, double* E, const double* dt, int32_t fallback,
                                   double rtol_y, double y_floor, double rtol_t, int32_t max_sub, int32_t* flag_out);
 /* Tile-shape autotuner of the surrogate network kernel (no reference
- * counterpart): each shape (cells per workgroup x waves: 128x4, 128x8,
- * 64x4, 96x4) runs reps times on scratch copies of the batch (rho, E are not
+ * counterpart): each shape (cells per workgroup x waves: 128x4 with each
+ * wave on 64 output columns of all rows, 128x4 with the waves as 2 x 2
+ * blocks of 64 rows x 128 columns, 64x4, 96x4) runs reps times on scratch copies of the batch (rho, E are not
  * modified), timed with HIP events; the fastest is kept.  Results are
  * bit-identical for every shape.  ms_out (may be NULL) gets the mean ms per
  * launch of each shape (4 floats).  Synchronises. */

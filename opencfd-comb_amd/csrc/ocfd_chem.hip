@@ -992,13 +992,15 @@ __global__ void k_fill_u64(unsigned long long* p, int count) {
 }
 
 // Tile shapes of the surrogate network kernel (cells per workgroup, waves,
-// workgroups per CU the register budget aims at).  All give bit-identical
-// results; ocfd_chem_surrogate_autotune picks the fastest.
+// workgroups per CU the register budget aims at, row groups of waves: with
+// NWR = 2 the waves form 2 x 2 blocks of ROWS/2 rows x 128 columns, which
+// halves the LDS reads per MFMA and doubles the weight loads).  All give
+// bit-identical results; ocfd_chem_surrogate_autotune picks the fastest.
 template <int V> struct SurrTile;
-template <> struct SurrTile<0> { static constexpr int ROWS = 128, NW = 4, OCC = 2; };
-template <> struct SurrTile<1> { static constexpr int ROWS = 128, NW = 8, OCC = 1; };
-template <> struct SurrTile<2> { static constexpr int ROWS = 64, NW = 4, OCC = 3; };
-template <> struct SurrTile<3> { static constexpr int ROWS = 96, NW = 4, OCC = 2; };
+template <> struct SurrTile<0> { static constexpr int ROWS = 128, NW = 4, OCC = 2, NWR = 1; };
+template <> struct SurrTile<1> { static constexpr int ROWS = 128, NW = 4, OCC = 2, NWR = 2; };
+template <> struct SurrTile<2> { static constexpr int ROWS = 64, NW = 4, OCC = 3, NWR = 1; };
+template <> struct SurrTile<3> { static constexpr int ROWS = 96, NW = 4, OCC = 2, NWR = 1; };
 #define SURR_VARIANTS 4
 
 // The surrogate step runs as three launches, so that every phase keeps all
@@ -1068,8 +1070,9 @@ __global__ __launch_bounds__(SurrTile<V>::NW * 64, SurrTile<V>::OCC) void k_surr
   }
   __syncthreads();
   const int L = cnet.n_layers;
-  mt_layer256<true, ROWS, NW, 32>(H, cwp.W[0], cnet.b[0], lane, w);
-  for (int l = 1; l < L - 1; l++) mt_layer256<true, ROWS, NW, 256>(H, cwp.W[l], cnet.b[l], lane, w);
+  constexpr int NWR = SurrTile<V>::NWR;
+  mt_layer256<true, ROWS, NW, 32, NWR>(H, cwp.W[0], cnet.b[0], lane, w);
+  for (int l = 1; l < L - 1; l++) mt_layer256<true, ROWS, NW, 256, NWR>(H, cwp.W[l], cnet.b[l], lane, w);
   mt_out_cols<ROWS, NW, 256, CH_NS>(H, cwp.W[L - 1], cnet.b[L - 1], Oout, n, c0, lane, w);
 }
 

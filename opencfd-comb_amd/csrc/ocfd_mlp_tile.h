@@ -54,13 +54,14 @@ __host__ __device__ inline long mt_pack_src(long e, int KS, int ldw) {
 // two chunks ahead (from L2, whose latency exceeds one chunk of MFMA work)
 // into a rotating set of registers.  Wp holds the weights in fragment order
 // (mt_pack_index): a wave's fragment load is 1 KB contiguous.
-template <bool GELU, int ROWS, int NW, int K>
+template <bool GELU, int ROWS, int NW, int K, int NWR = 1>
 __device__ __forceinline__ void mt_layer256(tbf16* H, const tbf16* __restrict__ Wp, const float* __restrict__ b,
                                             int lane, int w) {
-  constexpr int IT = ROWS / 16, JT = 16 / NW, KS = K / 32;
+  constexpr int NWC = NW / NWR, IT = ROWS / NWR / 16, JT = 16 / NWC, KS = K / 32;
   static_assert(K % 32 == 0 && KS >= 1, "K in chunks of 32");
+  static_assert(NW % NWR == 0 && ROWS % (16 * NWR) == 0, "whole row groups");
   const int lr = lane & 15, lk = (lane >> 4) * 8, lq = (lane >> 4) * 4;
-  const int cb = 16 * JT * w;
+  const int cb = 16 * JT * (w % NWC), r0 = (ROWS / NWR) * (w / NWC);
   // the accumulators start from the bias of their columns (the first
   // chunk's MFMA takes it as its C operand)
   tfloatx4 acc[IT][JT], bias[JT];
@@ -84,7 +85,7 @@ __device__ __forceinline__ void mt_layer256(tbf16* H, const tbf16* __restrict__ 
     }
     tbf16x8 a[IT];
 #pragma unroll
-    for (int i = 0; i < IT; i++) a[i] = *(const tbf16x8*)&H[(16 * i + lr) * MT_LDH + 32 * s + lk];
+    for (int i = 0; i < IT; i++) a[i] = *(const tbf16x8*)&H[(r0 + 16 * i + lr) * MT_LDH + 32 * s + lk];
 #pragma unroll
     for (int i = 0; i < IT; i++)
 #pragma unroll
@@ -102,7 +103,7 @@ __device__ __forceinline__ void mt_layer256(tbf16* H, const tbf16* __restrict__ 
 #pragma unroll
         for (int r = 0; r < 4; r++) v[r] = mt_gelu(v[r]);
       }
-      *(tbf16x4*)&H[(16 * i + lr) * MT_LDH + cb + 16 * j + lq] = __builtin_convertvector(v, tbf16x4);
+      *(tbf16x4*)&H[(r0 + 16 * i + lr) * MT_LDH + cb + 16 * j + lq] = __builtin_convertvector(v, tbf16x4);
     }
   __syncthreads();
 }
