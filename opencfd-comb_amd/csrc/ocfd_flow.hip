@@ -689,6 +689,10 @@ inline dim3 col_grid(const DevState& S, int kc) {
   return dim3((unsigned)(nti * ntj * ((S.nz + kc - 1) / kc)));
 }
 
+// Ev1..3(1:4) layout: per direction g a block of ni cells x 4 components,
+// so a cell's four values of one direction are 32 contiguous bytes.
+__host__ __device__ __forceinline__ long ev_index(long ni, int g, long c, int m) { return (long)g * 4 * ni + 4 * c + m; }
+
 // MS (N_SPEC > 1): the transport coefficients come from k_transport_ms and
 // the species-diffusion enthalpy flux Qp from k_species_visc_prep.
 // (us, vs, ws, Ts) are the zeta derivatives of u, v, w, T at the cell.
@@ -765,20 +769,16 @@ __device__ __forceinline__ void visc_prep_cell(const DevState& S, long c, long q
     Aix1 = S.Aix1[q]; Aiy1 = S.Aiy1[q]; Aiz1 = S.Aiz1[q];
     Asx1 = S.Asx1[q]; Asy1 = S.Asy1[q]; Asz1 = S.Asz1[q];
   }
-  double* Ev = S.Ev + c;
-  const long ni = S.ni;
-  Ev[0 * ni] = Akx1 * s11 + Aky1 * s12 + Akz1 * s13;
-  Ev[1 * ni] = Akx1 * s12 + Aky1 * s22 + Akz1 * s23;
-  Ev[2 * ni] = Akx1 * s13 + Aky1 * s23 + Akz1 * s33;
-  Ev[3 * ni] = Akx1 * E1 + Aky1 * E2 + Akz1 * E3;
-  Ev[4 * ni] = Aix1 * s11 + Aiy1 * s12 + Aiz1 * s13;
-  Ev[5 * ni] = Aix1 * s12 + Aiy1 * s22 + Aiz1 * s23;
-  Ev[6 * ni] = Aix1 * s13 + Aiy1 * s23 + Aiz1 * s33;
-  Ev[7 * ni] = Aix1 * E1 + Aiy1 * E2 + Aiz1 * E3;
-  Ev[8 * ni] = Asx1 * s11 + Asy1 * s12 + Asz1 * s13;
-  Ev[9 * ni] = Asx1 * s12 + Asy1 * s22 + Asz1 * s23;
-  Ev[10 * ni] = Asx1 * s13 + Asy1 * s23 + Asz1 * s33;
-  Ev[11 * ni] = Asx1 * E1 + Asy1 * E2 + Asz1 * E3;
+  // Ev1..3(1:4) of the cell: 4 consecutive values per direction (ev_index),
+  // two 16-byte stores each
+  double2* E = (double2*)(S.Ev + ev_index(S.ni, 0, c, 0));
+  const long g2 = 2 * S.ni;  // one direction's block, in double2
+  E[0] = make_double2(Akx1 * s11 + Aky1 * s12 + Akz1 * s13, Akx1 * s12 + Aky1 * s22 + Akz1 * s23);
+  E[1] = make_double2(Akx1 * s13 + Aky1 * s23 + Akz1 * s33, Akx1 * E1 + Aky1 * E2 + Akz1 * E3);
+  E[g2] = make_double2(Aix1 * s11 + Aiy1 * s12 + Aiz1 * s13, Aix1 * s12 + Aiy1 * s22 + Aiz1 * s23);
+  E[g2 + 1] = make_double2(Aix1 * s13 + Aiy1 * s23 + Aiz1 * s33, Aix1 * E1 + Aiy1 * E2 + Aiz1 * E3);
+  E[2 * g2] = make_double2(Asx1 * s11 + Asy1 * s12 + Asz1 * s13, Asx1 * s12 + Asy1 * s22 + Asz1 * s23);
+  E[2 * g2 + 1] = make_double2(Asx1 * s13 + Asy1 * s23 + Asz1 * s33, Asx1 * E1 + Asy1 * E2 + Asz1 * E3);
 }
 
 // one thread per cell
@@ -910,24 +910,44 @@ __device__ __forceinline__ void rk_cell(const DevState& S, long c, long q, int i
   }
   if (flags & RK_VISC) {
     const double ajac = jac_at<MT>(S, q, i, j, k);
-    auto dv = [&](const double* Fv, int dir, int s) {
-      const double* Fc = Fv + c;
-      const int st = dir == 0 ? 1 : S.nx;
-      if constexpr (INT) {
-        return deriv1<CD, true>(S, dir, s, [&](int o) { return Fc[o * st]; });
-      } else {
+    double uk[4], ui[4];
+    if constexpr (INT) {
+      // central taps: the four components of a neighbour in two 16-byte loads
+      constexpr int R = CD == 1 ? 4 : 3;
+      const double* bx = S.Ev + ev_index(ni, 0, c, 0);
+      const double* by = S.Ev + ev_index(ni, 1, c, 0);
+      const long sy = 4L * S.nx;
+      double tx[2 * R + 1][4], ty[2 * R + 1][4];
+#pragma unroll
+      for (int o = -R; o <= R; o++) {
+        if (o == 0) continue;
+        const double2 xa = *(const double2*)(bx + 4 * o), xb = *(const double2*)(bx + 4 * o + 2);
+        const double2 ya = *(const double2*)(by + o * sy), yb = *(const double2*)(by + o * sy + 2);
+        tx[o + R][0] = xa.x; tx[o + R][1] = xa.y; tx[o + R][2] = xb.x; tx[o + R][3] = xb.y;
+        ty[o + R][0] = ya.x; ty[o + R][1] = ya.y; ty[o + R][2] = yb.x; ty[o + R][3] = yb.y;
+      }
+#pragma unroll
+      for (int m = 0; m < 4; m++) {
+        uk[m] = deriv1<CD, true>(S, 0, i, [&](int o) { return tx[o + R][m]; });
+        ui[m] = deriv1<CD, true>(S, 1, j, [&](int o) { return ty[o + R][m]; });
+      }
+    } else {
+      auto dv = [&](int g, int m, int dir, int s) {
+        const double* Fc = S.Ev + ev_index(ni, g, c, m);
+        const int st = dir == 0 ? 4 : 4 * S.nx;  // ev_index steps 4 per cell
         return deriv1<CD>(S, dir, s, [&](int o) {
           const int t = wrap(s + o, S.n[dir], S.per[dir]);
           return Fc[(t - s) * st];
         });
-      }
-    };
+      };
 #pragma unroll
-    for (int m = 0; m < 4; m++) {
-      const double uk = dv(S.Ev + (long)m * ni, 0, i);
-      const double ui = dv(S.Ev + (long)(4 + m) * ni, 1, j);
-      du[m + 1] = du[m + 1] + (uk + ui + zs[m]) * ajac;
+      for (int m = 0; m < 4; m++) {
+        uk[m] = dv(0, m, 0, i);
+        ui[m] = dv(1, m, 1, j);
+      }
     }
+#pragma unroll
+    for (int m = 0; m < 4; m++) du[m + 1] = du[m + 1] + (uk[m] + ui[m] + zs[m]) * ajac;
   }
   if (flags & RK_WRITE_DU) {
 #pragma unroll
@@ -1007,8 +1027,8 @@ __global__ __launch_bounds__(256) void k_rk(DevState S, int krk, double dt, int 
     if (flags & RK_VISC) {
 #pragma unroll
       for (int m = 0; m < 4; m++) {
-        const double* Fv = S.Ev + (long)(8 + m) * S.ni + c;
-        zs[m] = deriv1<CD, true>(S, 2, k, [&](int o) { return Fv[o * pli]; });
+        const double* Fv = S.Ev + ev_index(S.ni, 2, c, m);
+        zs[m] = deriv1<CD, true>(S, 2, k, [&](int o) { return Fv[4 * o * pli]; });
       }
     }
     rk_cell<CD, F, true, MT>(S, c, q, i, j, k, krk, dt, flags_rt, zs);
@@ -1016,10 +1036,10 @@ __global__ __launch_bounds__(256) void k_rk(DevState S, int krk, double dt, int 
     if (flags & RK_VISC) {
 #pragma unroll
       for (int m = 0; m < 4; m++) {
-        const double* Fv = S.Ev + (long)(8 + m) * S.ni;
+        const double* Fv = S.Ev + ev_index(S.ni, 2, c, m);
         zs[m] = deriv1<CD>(S, 2, k, [&](int o) {
           const int t = wrap(k + o, S.n[2], S.per[2]);
-          return Fv[c + (long)(t - k) * pli];
+          return Fv[4 * (long)(t - k) * pli];
         });
       }
     }
@@ -1044,9 +1064,9 @@ __global__ __launch_bounds__(256) void k_rk_col(DevState S, int kc, int krk, dou
   auto ld = [&](int slot, int p) {
     p = wrap(p, S.nz, S.per[2]);
     if (visc && p >= 1 && p <= S.nz) {
-      const double* Fv = S.Ev + 8 * S.ni + ccol + (long)(p - 1) * pli;
+      const double* Fv = S.Ev + ev_index(S.ni, 2, ccol + (long)(p - 1) * pli, 0);
 #pragma unroll
-      for (int m = 0; m < 4; m++) z[m][slot] = Fv[m * S.ni];
+      for (int m = 0; m < 4; m++) z[m][slot] = Fv[m];
     } else {
 #pragma unroll
       for (int m = 0; m < 4; m++) z[m][slot] = 0.0;
