@@ -695,11 +695,18 @@ __host__ __device__ __forceinline__ long ev_index(long ni, int g, long c, int m)
 
 // MS (N_SPEC > 1): the transport coefficients come from k_transport_ms and
 // the species-diffusion enthalpy flux Qp from k_species_visc_prep.
-// (us, vs, ws, Ts) are the zeta derivatives of u, v, w, T at the cell.
+// (us, vs, ws, Ts) are the zeta derivatives of u, v, w, T at the cell; ctr
+// (may be null) holds u, v, w, T at the cell when the caller has them; xt,
+// when given (interior form only), returns u, v, w, T (v = 0..3) at xi
+// offset o from the cell.
 // INT: the cell's xi and eta stencils are central (no wrap, no closure).
-template <int CD, bool MS, bool INT, bool MT>
+struct NoXTaps {
+  __device__ double operator()(int, int) const { return 0.0; }
+};
+template <int CD, bool MS, bool INT, bool MT, class XT = NoXTaps>
 __device__ __forceinline__ void visc_prep_cell(const DevState& S, long c, long q, int i, int j, int k, double us,
-                                               double vs, double ws, double Ts) {
+                                               double vs, double ws, double Ts, const double* ctr = nullptr,
+                                               const XT* xt = nullptr) {
   double amu, amk;
   if (MS) {
     amu = S.Amu[c];
@@ -708,7 +715,7 @@ __device__ __forceinline__ void visc_prep_cell(const DevState& S, long c, long q
     // transport (N_SPEC = 1): comput_vis_coeff Sutherland (TRANSPORT_modules.f90:265-269),
     // comput_Cp_pgas (CHEM_thermdyn.f90:125-136), Amk = Amu*Cp/Pr (:326-327)
     const double TA = 288.15, TC = 110.4;
-    const double T0 = S.T[q];
+    const double T0 = ctr ? ctr[3] : S.T[q];
     const double tr = T0 / TA;
     amu = S.amu0 * sqrt(pw3(tr)) * (TA + TC) / (TC + T0);
     const double Cp = (0.0 + 3.5 * S.Ri0 * S.di[q]) / S.d[q];
@@ -729,7 +736,13 @@ __device__ __forceinline__ void visc_prep_cell(const DevState& S, long c, long q
       });
     }
   };
-  const double uk = dh(S.u, 0, i), vk = dh(S.v, 0, i), wk = dh(S.w, 0, i), Tk = dh(S.T, 0, i);
+  double uk, vk, wk, Tk;
+  if (INT && xt) {
+    auto dx = [&](int v) { return deriv1<CD, true>(S, 0, i, [&](int o) { return (*xt)(v, o); }); };
+    uk = dx(0); vk = dx(1); wk = dx(2); Tk = dx(3);
+  } else {
+    uk = dh(S.u, 0, i); vk = dh(S.v, 0, i); wk = dh(S.w, 0, i); Tk = dh(S.T, 0, i);
+  }
   const double ui = dh(S.u, 1, j), vi = dh(S.v, 1, j), wi = dh(S.w, 1, j), Ti = dh(S.T, 1, j);
   const Metrics M = metrics_at<MT>(S, q, i, j, k);
   const double Akx = M.kx, Aky = M.ky, Akz = M.kz;
@@ -753,7 +766,7 @@ __device__ __forceinline__ void visc_prep_cell(const DevState& S, long c, long q
   const double s22 = (2.0 * vy - tmp2_3 * div) * amu;
   const double s33 = (2.0 * wz - tmp2_3 * div) * amu;
   const double s12 = (uy + vx) * amu, s13 = (uz + wx) * amu, s23 = (vz + wy) * amu;
-  const double uu = S.u[q], vv = S.v[q], ww = S.w[q];
+  const double uu = ctr ? ctr[0] : S.u[q], vv = ctr ? ctr[1] : S.v[q], ww = ctr ? ctr[2] : S.w[q];
   // Qpx/Qpy/Qpz are zero without species diffusion (OCFD_viscous.f90:13-15)
   const double qx = MS ? S.Qp[c] : 0.0, qy = MS ? S.Qp[c + S.ni] : 0.0, qz = MS ? S.Qp[c + 2 * S.ni] : 0.0;
   const double E1 = uu * s11 + vv * s12 + ww * s13 + amk * Tx + qx;
@@ -816,6 +829,18 @@ __global__ __launch_bounds__(256) void k_visc_prep_col(DevState S, int kc) {
   const long qcol = hq(S, P.i, P.j, 0), ccol = iq(S, P.i, P.j, 1);
   const bool xy_int = wave_all(central_cell(P.i, S.nx, R) && central_cell(P.j, S.ny, R));
   double zu[W], zv[W], zw[W], zT[W];
+  // xi taps of interior wavefronts through LDS: a wavefront is one row of
+  // COL_TX = 64 cells; each lane posts its cell's u, v, w, T of the current
+  // plane and the first / last R lanes the halo cells beside the row
+  static_assert(COL_TX == 64, "one row per wavefront");
+  __shared__ double xs[COL_TY][4][COL_TX + 2 * R];
+  const int lane = (int)threadIdx.x % COL_TX, wv = (int)threadIdx.x / COL_TX;
+  double (*xw)[COL_TX + 2 * R] = xs[wv];
+  struct XTaps {
+    const double (*xw)[COL_TX + 2 * R];
+    int lane;
+    __device__ double operator()(int v, int o) const { return xw[v][lane + R + o]; }
+  } xt{xw, lane};
   // plane p (-3 .. nz+4 before the periodic wrap) lies inside the halo box
   auto ld = [&](int slot, int p) {
     const long qp = qcol + (long)wrap(p, S.nz, S.per[2]) * plh;
@@ -829,12 +854,33 @@ __global__ __launch_bounds__(256) void k_visc_prep_col(DevState S, int kc) {
   for (int k = P.k0; k < P.k1; k++) {
     ld(W - 1, k + R);
     const long c = ccol + (long)(k - 1) * pli, q = qcol + (long)k * plh;
+    const double ctr[4] = {zu[R], zv[R], zw[R], zT[R]};  // the cell's values, plane k
+    if (xy_int) {
+      // the previous plane's reads of xs are done before these writes (a
+      // wavefront's LDS operations complete in order)
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int v = 0; v < 4; v++) xw[v][lane + R] = ctr[v];
+      if (lane < R || lane >= COL_TX - R) {
+        // halo cell P.i -+ R, clamped into the row (only interior rows read it)
+        const int ih = min(max(lane < R ? P.i - R : P.i + R, 1), S.nx);
+        const long qh = qcol + (ih - P.i) + (long)k * plh;
+        const int slot = lane < R ? lane : lane + 2 * R;
+        xw[0][slot] = S.u[qh];
+        xw[1][slot] = S.v[qh];
+        xw[2][slot] = S.w[qh];
+        xw[3][slot] = S.T[qh];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
     if (xy_int && central_cell(k, S.nz, R)) {
       auto dz = [&](const double* z) { return deriv1<CD, true>(S, 2, k, [&](int o) { return z[o + R]; }); };
-      visc_prep_cell<CD, MS, true, MT>(S, c, q, P.i, P.j, k, dz(zu), dz(zv), dz(zw), dz(zT));
+      visc_prep_cell<CD, MS, true, MT>(S, c, q, P.i, P.j, k, dz(zu), dz(zv), dz(zw), dz(zT), ctr, &xt);
     } else {
       auto dz = [&](const double* z) { return deriv1<CD>(S, 2, k, [&](int o) { return z[o + R]; }); };
-      visc_prep_cell<CD, MS, false, MT>(S, c, q, P.i, P.j, k, dz(zu), dz(zv), dz(zw), dz(zT));
+      visc_prep_cell<CD, MS, false, MT>(S, c, q, P.i, P.j, k, dz(zu), dz(zv), dz(zw), dz(zT), ctr);
     }
 #pragma unroll
     for (int o = 0; o < W - 1; o++) {
@@ -3504,9 +3550,11 @@ int32_t ocfd_autotune(int32_t reps) {
       float ms = 0.f;
       HIPCHK(hipEventElapsedTime(&ms, a, b));
       G.tune_ms[dir][v] = ms / (reps * per);
-      // isolated launches on the set-up state rank near-equal shapes by
-      // noise: keep the default unless another shape is clearly faster
-      if (v == 0 || ms < 0.97f * best) { best = ms; bestv = v; }
+      // isolated launches rank near-equal shapes by noise and differ from a
+      // launch inside the step (whose inputs come warm from the previous
+      // kernels, 10% apart for the x sweep): keep the default shape unless
+      // another is clearly faster
+      if (v == 0 || ms < 0.90f * best) { best = ms; bestv = v; }
     }
     G.sweep_variant[dir] = bestv;
   }
