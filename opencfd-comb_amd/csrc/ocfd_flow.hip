@@ -703,10 +703,10 @@ __host__ __device__ __forceinline__ long ev_index(long ni, int g, long c, int m)
 struct NoXTaps {
   __device__ double operator()(int, int) const { return 0.0; }
 };
-template <int CD, bool MS, bool INT, bool MT, class XT = NoXTaps>
+template <int CD, bool MS, bool INT, bool MT, class XT = NoXTaps, class YT = NoXTaps>
 __device__ __forceinline__ void visc_prep_cell(const DevState& S, long c, long q, int i, int j, int k, double us,
                                                double vs, double ws, double Ts, const double* ctr = nullptr,
-                                               const XT* xt = nullptr) {
+                                               const XT* xt = nullptr, const YT* yt = nullptr) {
   double amu, amk;
   if (MS) {
     amu = S.Amu[c];
@@ -743,7 +743,13 @@ __device__ __forceinline__ void visc_prep_cell(const DevState& S, long c, long q
   } else {
     uk = dh(S.u, 0, i); vk = dh(S.v, 0, i); wk = dh(S.w, 0, i); Tk = dh(S.T, 0, i);
   }
-  const double ui = dh(S.u, 1, j), vi = dh(S.v, 1, j), wi = dh(S.w, 1, j), Ti = dh(S.T, 1, j);
+  double ui, vi, wi, Ti;
+  if (INT && yt) {
+    auto dy = [&](int v) { return deriv1<CD, true>(S, 1, j, [&](int o) { return (*yt)(v, o); }); };
+    ui = dy(0); vi = dy(1); wi = dy(2); Ti = dy(3);
+  } else {
+    ui = dh(S.u, 1, j); vi = dh(S.v, 1, j); wi = dh(S.w, 1, j); Ti = dh(S.T, 1, j);
+  }
   const Metrics M = metrics_at<MT>(S, q, i, j, k);
   const double Akx = M.kx, Aky = M.ky, Akz = M.kz;
   const double Aix = M.ix, Aiy = M.iy, Aiz = M.iz;
@@ -819,28 +825,42 @@ __global__ __launch_bounds__(256) void k_visc_prep(DevState S) {
   }
 }
 
-// marching over k planes with the z windows of u, v, w, T in registers
+// marching over k planes with the z windows of u, v, w, T in registers.
+// The xi and eta taps of interior wavefronts come from LDS: a wavefront is
+// one row of COL_TX = 64 cells and posts its cells' u, v, w, T of the plane
+// in its own row buffer (the first / last R lanes add the halo cells beside
+// the row; no block barrier), and the block stages its COL_TY rows plus R
+// halo rows above and below in a plane buffer, double-buffered (the next
+// plane's own rows come from the z windows, its halo rows from memory; one
+// barrier per plane).  Halo cells are clamped into the grid: only interior
+// wavefronts read them, and theirs are real cells.
 template <int CD, bool MS, bool MT>
 __global__ __launch_bounds__(256) void k_visc_prep_col(DevState S, int kc) {
   constexpr int R = CD == 1 ? 4 : 3, W = 2 * R + 1;
+  constexpr int TX = COL_TX, TY = COL_TY, YH = TY + 2 * R;
+  static_assert(TX == 64, "one row per wavefront");
   ColPos P;
-  if (!col_pos(S, kc, P)) return;
+  const bool act = col_pos(S, kc, P);
+  const int lane = (int)threadIdx.x % TX, wv = (int)threadIdx.x / TX;
+  const int i0 = P.i - lane, j0 = P.j - wv;  // the tile's first column and row
+  const int ic = min(P.i, S.nx), jc = min(P.j, S.ny);  // inactive lanes read a real cell
   const long plh = (long)S.nxh * S.nyh, pli = (long)S.nx * S.ny;
-  const long qcol = hq(S, P.i, P.j, 0), ccol = iq(S, P.i, P.j, 1);
-  const bool xy_int = wave_all(central_cell(P.i, S.nx, R) && central_cell(P.j, S.ny, R));
+  const long qcol = hq(S, ic, jc, 0), ccol = iq(S, ic, jc, 1);
+  const bool xy_int = wave_all(!act || (central_cell(P.i, S.nx, R) && central_cell(P.j, S.ny, R)));
   double zu[W], zv[W], zw[W], zT[W];
-  // xi taps of interior wavefronts through LDS: a wavefront is one row of
-  // COL_TX = 64 cells; each lane posts its cell's u, v, w, T of the current
-  // plane and the first / last R lanes the halo cells beside the row
-  static_assert(COL_TX == 64, "one row per wavefront");
-  __shared__ double xs[COL_TY][4][COL_TX + 2 * R];
-  const int lane = (int)threadIdx.x % COL_TX, wv = (int)threadIdx.x / COL_TX;
-  double (*xw)[COL_TX + 2 * R] = xs[wv];
+  __shared__ double xs[TY][4][TX + 2 * R];
+  __shared__ double ys[2][4][YH][TX];
+  double (*xw)[TX + 2 * R] = xs[wv];
   struct XTaps {
-    const double (*xw)[COL_TX + 2 * R];
+    const double (*xw)[TX + 2 * R];
     int lane;
     __device__ double operator()(int v, int o) const { return xw[v][lane + R + o]; }
   } xt{xw, lane};
+  struct YTaps {
+    const double (*yb)[YH][TX];
+    int lane, wv;
+    __device__ double operator()(int v, int o) const { return yb[v][wv + R + o][lane]; }
+  };
   // plane p (-3 .. nz+4 before the periodic wrap) lies inside the halo box
   auto ld = [&](int slot, int p) {
     const long qp = qcol + (long)wrap(p, S.nz, S.per[2]) * plh;
@@ -849,10 +869,32 @@ __global__ __launch_bounds__(256) void k_visc_prep_col(DevState S, int kc) {
     zw[slot] = S.w[qp];
     zT[slot] = S.T[qp];
   };
+  // plane k into ys[b]: own rows from the window slot, halo rows from memory
+  auto stage = [&](int b, int k, int slot) {
+    ys[b][0][wv + R][lane] = zu[slot];
+    ys[b][1][wv + R][lane] = zv[slot];
+    ys[b][2][wv + R][lane] = zw[slot];
+    ys[b][3][wv + R][lane] = zT[slot];
+    const long qk = (long)wrap(k, S.nz, S.per[2]) * plh;
+    for (int e = threadIdx.x; e < 2 * R * TX; e += TX * TY) {
+      const int r = e / TX, col = e - r * TX;
+      const int row = r < R ? r : TY + r;  // rows 0..R-1 and TY+R..TY+2R-1
+      const int jj = min(max(j0 - R + row, 1), S.ny), ii = min(i0 + col, S.nx);
+      const long qh = hq(S, ii, jj, 0) + qk;
+      ys[b][0][row][col] = S.u[qh];
+      ys[b][1][row][col] = S.v[qh];
+      ys[b][2][row][col] = S.w[qh];
+      ys[b][3][row][col] = S.T[qh];
+    }
+  };
 #pragma unroll
   for (int o = 0; o < W - 1; o++) ld(o, P.k0 - R + o);
+  stage(0, P.k0, R);
+  __syncthreads();
+  int b = 0;
   for (int k = P.k0; k < P.k1; k++) {
     ld(W - 1, k + R);
+    if (k + 1 < P.k1) stage(b ^ 1, k + 1, R + 1);
     const long c = ccol + (long)(k - 1) * pli, q = qcol + (long)k * plh;
     const double ctr[4] = {zu[R], zv[R], zw[R], zT[R]};  // the cell's values, plane k
     if (xy_int) {
@@ -862,10 +904,10 @@ __global__ __launch_bounds__(256) void k_visc_prep_col(DevState S, int kc) {
       __builtin_amdgcn_wave_barrier();
 #pragma unroll
       for (int v = 0; v < 4; v++) xw[v][lane + R] = ctr[v];
-      if (lane < R || lane >= COL_TX - R) {
+      if (lane < R || lane >= TX - R) {
         // halo cell P.i -+ R, clamped into the row (only interior rows read it)
         const int ih = min(max(lane < R ? P.i - R : P.i + R, 1), S.nx);
-        const long qh = qcol + (ih - P.i) + (long)k * plh;
+        const long qh = qcol + (ih - ic) + (long)k * plh;
         const int slot = lane < R ? lane : lane + 2 * R;
         xw[0][slot] = S.u[qh];
         xw[1][slot] = S.v[qh];
@@ -875,12 +917,15 @@ __global__ __launch_bounds__(256) void k_visc_prep_col(DevState S, int kc) {
       __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
       __builtin_amdgcn_wave_barrier();
     }
-    if (xy_int && central_cell(k, S.nz, R)) {
-      auto dz = [&](const double* z) { return deriv1<CD, true>(S, 2, k, [&](int o) { return z[o + R]; }); };
-      visc_prep_cell<CD, MS, true, MT>(S, c, q, P.i, P.j, k, dz(zu), dz(zv), dz(zw), dz(zT), ctr, &xt);
-    } else {
-      auto dz = [&](const double* z) { return deriv1<CD>(S, 2, k, [&](int o) { return z[o + R]; }); };
-      visc_prep_cell<CD, MS, false, MT>(S, c, q, P.i, P.j, k, dz(zu), dz(zv), dz(zw), dz(zT), ctr);
+    if (act) {
+      if (xy_int && central_cell(k, S.nz, R)) {
+        const YTaps yt{ys[b], lane, wv};
+        auto dz = [&](const double* z) { return deriv1<CD, true>(S, 2, k, [&](int o) { return z[o + R]; }); };
+        visc_prep_cell<CD, MS, true, MT>(S, c, q, P.i, P.j, k, dz(zu), dz(zv), dz(zw), dz(zT), ctr, &xt, &yt);
+      } else {
+        auto dz = [&](const double* z) { return deriv1<CD>(S, 2, k, [&](int o) { return z[o + R]; }); };
+        visc_prep_cell<CD, MS, false, MT>(S, c, q, P.i, P.j, k, dz(zu), dz(zv), dz(zw), dz(zT), ctr);
+      }
     }
 #pragma unroll
     for (int o = 0; o < W - 1; o++) {
@@ -889,6 +934,8 @@ __global__ __launch_bounds__(256) void k_visc_prep_col(DevState S, int kc) {
       zw[o] = zw[o + 1];
       zT[o] = zT[o + 1];
     }
+    __syncthreads();
+    b ^= 1;
   }
 }
 
