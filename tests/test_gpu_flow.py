@@ -181,7 +181,10 @@ def test_column_planes_bitwise(pkg, vis):
     with periodic and with non-periodic span."""
     import oracle
     for case, zper in ((fc.small_flat_plate(48, 32, 19), 1), (fc.small_flat_plate(48, 32, 19), 0),
-                       (fc.small_isotropic(20), 1)):
+                       (fc.small_isotropic(20), 1),
+                       # rows and columns that leave whole wavefronts and part
+                       # of one past the grid edge (tiles are 64 x 4)
+                       (fc.small_flat_plate(40, 30, 13), 0)):
         case = list(case)
         case[0].scheme_vis = getattr(pkg, "SCHEME_" + vis)
         case[0].periodic[2] = zper  # non-periodic z: windows meet the one-sided closures
