@@ -1,14 +1,14 @@
-// ocfd_mlp_tile.h -- device code of the fused MLP forward used inside the
-// chemistry surrogate kernel: one workgroup of NW waves (4 or 8) carries a
-// tile of cells (ROWS = 64, 96 or 128) through every layer, the activations
-// staying in LDS as bfloat16 and each wave streaming its slice of the
-// weights from L2.
+// ocfd_mlp_tile.h -- device code of the MLP forward in the chemistry
+// surrogate's network kernel: one workgroup of NW waves carries a tile of
+// cells (ROWS = 64, 96 or 128) through every layer, the activations staying
+// in LDS as bfloat16 and each wave streaming its slice of the weights, kept
+// in MFMA fragment order (mt_pack_src), from L2.
 //
 // Layer shapes this code takes: input of at most 32 features (zero-padded),
 // hidden widths 256, at most 16 outputs; v_mfma_f32_16x16x32_bf16 with FP32
-// accumulation.  MFMA
-// fragment layouts (16x16x32): lane l holds A[l%16][8(l/16) .. +8] and
-// B[8(l/16) .. +8][l%16]; the accumulator holds D[4(l/16) + r][l%16], r < 4.
+// accumulation.  MFMA fragment layouts (16x16x32): lane l holds
+// A[l%16][8(l/16) .. +8] and B[8(l/16) .. +8][l%16]; the accumulator holds
+// D[4(l/16) + r][l%16], r < 4.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -16,7 +16,6 @@ typedef __bf16 tbf16;
 typedef __bf16 tbf16x8 __attribute__((ext_vector_type(8)));
 typedef float tfloatx4 __attribute__((ext_vector_type(4)));
 
-constexpr int MT_ROWS = 128;  // largest number of cells per workgroup
 constexpr int MT_LDH = 264;   // LDS row pitch (bf16): 256 + 8 against bank conflicts
 
 // GELU in its tanh form, z * sigmoid(2u), u = c (z + 0.044715 z^3), with
