@@ -70,6 +70,10 @@ __device__ __forceinline__ void mt_layer256(tbf16* H, const tbf16* __restrict__ 
   // of 128 x 64 outputs leave no room
   constexpr int PD = IT * JT >= 32 ? 1 : 2;
   const tbf16* Wl = Wp + (long)(cb / 16) * KS * 512 + lane * 8;
+  // MFMA phase at raised wave priority, epilogue at the normal one, so that a
+  // wave of the other workgroup in its epilogue fills the issue slots
+  // between this wave's MFMAs
+  __builtin_amdgcn_s_setprio(1);
   tbf16x8 wf[PD + 1][JT];
 #pragma unroll
   for (int s = 0; s < PD && s < KS; s++)
@@ -92,6 +96,7 @@ __device__ __forceinline__ void mt_layer256(tbf16* H, const tbf16* __restrict__ 
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[s % (PD + 1)][j], a[i], s == 0 ? bias[j] : acc[i][j],
                                                             0, 0, 0);
   }
+  __builtin_amdgcn_s_setprio(0);
   __syncthreads();  // every wave has read its input rows
 #pragma unroll
   for (int i = 0; i < IT; i++)
