@@ -839,6 +839,9 @@ __global__ __launch_bounds__(256) void k_visc_prep_col(DevState S, int kc) {
   constexpr int R = CD == 1 ? 4 : 3, W = 2 * R + 1;
   constexpr int TX = COL_TX, TY = COL_TY, YH = TY + 2 * R;
   static_assert(TX == 64, "one row per wavefront");
+  // raised wave priority: the memory-bound preparation issues its loads
+  // ahead of the VALU-bound sweeps that share the CUs from the other stream
+  __builtin_amdgcn_s_setprio(1);
   ColPos P;
   const bool act = col_pos(S, kc, P);
   const int lane = (int)threadIdx.x % TX, wv = (int)threadIdx.x / TX;
