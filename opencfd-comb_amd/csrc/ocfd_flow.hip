@@ -697,13 +697,14 @@ __host__ __device__ __forceinline__ long ev_index(long ni, int g, long c, int m)
 // the species-diffusion enthalpy flux Qp from k_species_visc_prep.
 // (us, vs, ws, Ts) are the zeta derivatives of u, v, w, T at the cell; ctr
 // (may be null) holds u, v, w, T at the cell when the caller has them; xt,
-// when given (interior form only), returns u, v, w, T (v = 0..3) at xi
-// offset o from the cell.
-// INT: the cell's xi and eta stencils are central (no wrap, no closure).
+// when given (central xi form only), returns u, v, w, T (v = 0..3) at xi
+// offset o from the cell; yt likewise along eta.
+// INTX / INTY: the cell's xi / eta stencil is central (no wrap, no closure),
+// so that derivative runs in straight-line code.
 struct NoXTaps {
   __device__ double operator()(int, int) const { return 0.0; }
 };
-template <int CD, bool MS, bool INT, bool MT, class XT = NoXTaps, class YT = NoXTaps>
+template <int CD, bool MS, bool INTX, bool INTY, bool MT, class XT = NoXTaps, class YT = NoXTaps>
 __device__ __forceinline__ void visc_prep_cell(const DevState& S, long c, long q, int i, int j, int k, double us,
                                                double vs, double ws, double Ts, const double* ctr = nullptr,
                                                const XT* xt = nullptr, const YT* yt = nullptr) {
@@ -724,31 +725,36 @@ __device__ __forceinline__ void visc_prep_cell(const DevState& S, long c, long q
     S.Amk[c] = amk;
   }
 
-  auto dh = [&](const double* F, int dir, int s) {
+  auto dcen = [&](const double* F, int dir, int s) {
     const double* Fq = F + q;
     const int st = dir == 0 ? 1 : S.nxh;
-    if constexpr (INT) {
-      return deriv1<CD, true>(S, dir, s, [&](int o) { return Fq[o * st]; });
-    } else {
-      return deriv1<CD>(S, dir, s, [&](int o) {
-        const int t = wrap(s + o, S.n[dir], S.per[dir]);
-        return Fq[(t - s) * st];
-      });
-    }
+    return deriv1<CD, true>(S, dir, s, [&](int o) { return Fq[o * st]; });
+  };
+  auto dgen = [&](const double* F, int dir, int s) {
+    const double* Fq = F + q;
+    const int st = dir == 0 ? 1 : S.nxh;
+    return deriv1<CD>(S, dir, s, [&](int o) {
+      const int t = wrap(s + o, S.n[dir], S.per[dir]);
+      return Fq[(t - s) * st];
+    });
   };
   double uk, vk, wk, Tk;
-  if (INT && xt) {
+  if (INTX && xt) {
     auto dx = [&](int v) { return deriv1<CD, true>(S, 0, i, [&](int o) { return (*xt)(v, o); }); };
     uk = dx(0); vk = dx(1); wk = dx(2); Tk = dx(3);
+  } else if (INTX) {
+    uk = dcen(S.u, 0, i); vk = dcen(S.v, 0, i); wk = dcen(S.w, 0, i); Tk = dcen(S.T, 0, i);
   } else {
-    uk = dh(S.u, 0, i); vk = dh(S.v, 0, i); wk = dh(S.w, 0, i); Tk = dh(S.T, 0, i);
+    uk = dgen(S.u, 0, i); vk = dgen(S.v, 0, i); wk = dgen(S.w, 0, i); Tk = dgen(S.T, 0, i);
   }
   double ui, vi, wi, Ti;
-  if (INT && yt) {
+  if (INTY && yt) {
     auto dy = [&](int v) { return deriv1<CD, true>(S, 1, j, [&](int o) { return (*yt)(v, o); }); };
     ui = dy(0); vi = dy(1); wi = dy(2); Ti = dy(3);
+  } else if (INTY) {
+    ui = dcen(S.u, 1, j); vi = dcen(S.v, 1, j); wi = dcen(S.w, 1, j); Ti = dcen(S.T, 1, j);
   } else {
-    ui = dh(S.u, 1, j); vi = dh(S.v, 1, j); wi = dh(S.w, 1, j); Ti = dh(S.T, 1, j);
+    ui = dgen(S.u, 1, j); vi = dgen(S.v, 1, j); wi = dgen(S.w, 1, j); Ti = dgen(S.T, 1, j);
   }
   const Metrics M = metrics_at<MT>(S, q, i, j, k);
   const double Akx = M.kx, Aky = M.ky, Akz = M.kz;
@@ -813,7 +819,7 @@ __global__ __launch_bounds__(256) void k_visc_prep(DevState S) {
   const long plh = (long)S.nxh * S.nyh;
   if (wave_all(central_cell(i, S.nx, R) && central_cell(j, S.ny, R) && central_cell(k, S.nz, R))) {
     auto dz = [&](const double* F) { return deriv1<CD, true>(S, 2, k, [&](int o) { return F[q + o * plh]; }); };
-    visc_prep_cell<CD, MS, true, MT>(S, c, q, i, j, k, dz(S.u), dz(S.v), dz(S.w), dz(S.T));
+    visc_prep_cell<CD, MS, true, true, MT>(S, c, q, i, j, k, dz(S.u), dz(S.v), dz(S.w), dz(S.T));
   } else {
     auto dz = [&](const double* F) {
       return deriv1<CD>(S, 2, k, [&](int o) {
@@ -821,7 +827,7 @@ __global__ __launch_bounds__(256) void k_visc_prep(DevState S) {
         return F[q + (long)(t - k) * plh];
       });
     };
-    visc_prep_cell<CD, MS, false, MT>(S, c, q, i, j, k, dz(S.u), dz(S.v), dz(S.w), dz(S.T));
+    visc_prep_cell<CD, MS, false, false, MT>(S, c, q, i, j, k, dz(S.u), dz(S.v), dz(S.w), dz(S.T));
   }
 }
 
@@ -849,7 +855,10 @@ __global__ __launch_bounds__(256) void k_visc_prep_col(DevState S, int kc) {
   const int ic = min(P.i, S.nx), jc = min(P.j, S.ny);  // inactive lanes read a real cell
   const long plh = (long)S.nxh * S.nyh, pli = (long)S.nx * S.ny;
   const long qcol = hq(S, ic, jc, 0), ccol = iq(S, ic, jc, 1);
-  const bool xy_int = wave_all(!act || (central_cell(P.i, S.nx, R) && central_cell(P.j, S.ny, R)));
+  // per-direction central forms: wavefronts at the xi faces (a fifth of them
+  // on 640 cells) keep the central eta and zeta forms
+  const bool xint = wave_all(!act || central_cell(P.i, S.nx, R));
+  const bool yint = wave_all(!act || central_cell(P.j, S.ny, R));
   double zu[W], zv[W], zw[W], zT[W];
   __shared__ double xs[TY][4][TX + 2 * R];
   __shared__ double ys[2][4][YH][TX];
@@ -900,7 +909,7 @@ __global__ __launch_bounds__(256) void k_visc_prep_col(DevState S, int kc) {
     if (k + 1 < P.k1) stage(b ^ 1, k + 1, R + 1);
     const long c = ccol + (long)(k - 1) * pli, q = qcol + (long)k * plh;
     const double ctr[4] = {zu[R], zv[R], zw[R], zT[R]};  // the cell's values, plane k
-    if (xy_int) {
+    if (xint) {
       // the previous plane's reads of xs are done before these writes (a
       // wavefront's LDS operations complete in order)
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -921,14 +930,22 @@ __global__ __launch_bounds__(256) void k_visc_prep_col(DevState S, int kc) {
       __builtin_amdgcn_wave_barrier();
     }
     if (act) {
-      if (xy_int && central_cell(k, S.nz, R)) {
-        const YTaps yt{ys[b], lane, wv};
+      double zd[4];
+      if (central_cell(k, S.nz, R)) {
         auto dz = [&](const double* z) { return deriv1<CD, true>(S, 2, k, [&](int o) { return z[o + R]; }); };
-        visc_prep_cell<CD, MS, true, MT>(S, c, q, P.i, P.j, k, dz(zu), dz(zv), dz(zw), dz(zT), ctr, &xt, &yt);
+        zd[0] = dz(zu); zd[1] = dz(zv); zd[2] = dz(zw); zd[3] = dz(zT);
       } else {
         auto dz = [&](const double* z) { return deriv1<CD>(S, 2, k, [&](int o) { return z[o + R]; }); };
-        visc_prep_cell<CD, MS, false, MT>(S, c, q, P.i, P.j, k, dz(zu), dz(zv), dz(zw), dz(zT), ctr);
+        zd[0] = dz(zu); zd[1] = dz(zv); zd[2] = dz(zw); zd[3] = dz(zT);
       }
+      const YTaps yt{ys[b], lane, wv};
+      if (xint && yint)
+        visc_prep_cell<CD, MS, true, true, MT>(S, c, q, P.i, P.j, k, zd[0], zd[1], zd[2], zd[3], ctr, &xt, &yt);
+      else if (yint)
+        visc_prep_cell<CD, MS, false, true, MT>(S, c, q, P.i, P.j, k, zd[0], zd[1], zd[2], zd[3], ctr,
+                                                (const XTaps*)nullptr, &yt);
+      else
+        visc_prep_cell<CD, MS, false, false, MT>(S, c, q, P.i, P.j, k, zd[0], zd[1], zd[2], zd[3], ctr);
     }
 #pragma unroll
     for (int o = 0; o < W - 1; o++) {
@@ -975,8 +992,8 @@ enum { RK_MASS = 1, RK_VISC = 2, RK_WRITE_DU = 4, RK_UPDATE_F = 8, RK_PRIMS = 16
 // straight-line code and its loads can be in flight together; F = -1 reads
 // the flags at run time (per-stage entry points).
 // zs[m] is the zeta derivative of Ev3(m+1) at the cell (read only with RK_VISC).
-// INT: the cell's xi and eta stencils are central (no wrap, no closure).
-template <int CD, int F, bool INT, bool MT>
+// INTX / INTY: the cell's xi / eta stencil is central (no wrap, no closure).
+template <int CD, int F, bool INTX, bool INTY, bool MT>
 __device__ __forceinline__ void rk_cell(const DevState& S, long c, long q, int i, int j, int k, int krk, double dt,
                                         int flags_rt, const double* zs) {
   const int flags = F >= 0 ? F : flags_rt;
@@ -1007,9 +1024,18 @@ __device__ __forceinline__ void rk_cell(const DevState& S, long c, long q, int i
   if (flags & RK_VISC) {
     const double ajac = jac_at<MT>(S, q, i, j, k);
     double uk[4], ui[4];
-    if constexpr (INT) {
+    constexpr int R = CD == 1 ? 4 : 3;
+    // generic taps (closures, periodic wrap) of component m of direction g
+    auto dv = [&](int g, int m, int dir, int s) {
+      const double* Fc = S.Ev + ev_index(ni, g, c, m);
+      const int st = dir == 0 ? 4 : 4 * S.nx;  // ev_index steps 4 per cell
+      return deriv1<CD>(S, dir, s, [&](int o) {
+        const int t = wrap(s + o, S.n[dir], S.per[dir]);
+        return Fc[(t - s) * st];
+      });
+    };
+    if constexpr (INTX && INTY) {
       // central taps: the four components of a neighbour in two 16-byte loads
-      constexpr int R = CD == 1 ? 4 : 3;
       const double* bx = S.Ev + ev_index(ni, 0, c, 0);
       const double* by = S.Ev + ev_index(ni, 1, c, 0);
       const long sy = 4L * S.nx;
@@ -1027,15 +1053,23 @@ __device__ __forceinline__ void rk_cell(const DevState& S, long c, long q, int i
         uk[m] = deriv1<CD, true>(S, 0, i, [&](int o) { return tx[o + R][m]; });
         ui[m] = deriv1<CD, true>(S, 1, j, [&](int o) { return ty[o + R][m]; });
       }
+    } else if constexpr (INTY) {
+      // xi face: generic xi taps, central eta taps
+#pragma unroll
+      for (int m = 0; m < 4; m++) uk[m] = dv(0, m, 0, i);
+      const double* by = S.Ev + ev_index(ni, 1, c, 0);
+      const long sy = 4L * S.nx;
+      double ty[2 * R + 1][4];
+#pragma unroll
+      for (int o = -R; o <= R; o++) {
+        if (o == 0) continue;
+        const double2 ya = *(const double2*)(by + o * sy), yb = *(const double2*)(by + o * sy + 2);
+        ty[o + R][0] = ya.x; ty[o + R][1] = ya.y; ty[o + R][2] = yb.x; ty[o + R][3] = yb.y;
+      }
+#pragma unroll
+      for (int m = 0; m < 4; m++) ui[m] = deriv1<CD, true>(S, 1, j, [&](int o) { return ty[o + R][m]; });
     } else {
-      auto dv = [&](int g, int m, int dir, int s) {
-        const double* Fc = S.Ev + ev_index(ni, g, c, m);
-        const int st = dir == 0 ? 4 : 4 * S.nx;  // ev_index steps 4 per cell
-        return deriv1<CD>(S, dir, s, [&](int o) {
-          const int t = wrap(s + o, S.n[dir], S.per[dir]);
-          return Fc[(t - s) * st];
-        });
-      };
+      static_assert(!INTX, "a central xi form comes with a central eta form");
 #pragma unroll
       for (int m = 0; m < 4; m++) {
         uk[m] = dv(0, m, 0, i);
@@ -1127,7 +1161,7 @@ __global__ __launch_bounds__(256) void k_rk(DevState S, int krk, double dt, int 
         zs[m] = deriv1<CD, true>(S, 2, k, [&](int o) { return Fv[4 * o * pli]; });
       }
     }
-    rk_cell<CD, F, true, MT>(S, c, q, i, j, k, krk, dt, flags_rt, zs);
+    rk_cell<CD, F, true, true, MT>(S, c, q, i, j, k, krk, dt, flags_rt, zs);
   } else {
     if (flags & RK_VISC) {
 #pragma unroll
@@ -1139,13 +1173,16 @@ __global__ __launch_bounds__(256) void k_rk(DevState S, int krk, double dt, int 
         });
       }
     }
-    rk_cell<CD, F, false, MT>(S, c, q, i, j, k, krk, dt, flags_rt, zs);
+    rk_cell<CD, F, false, false, MT>(S, c, q, i, j, k, krk, dt, flags_rt, zs);
   }
 }
 
 // marching over k planes with the z windows of Ev3(1:4) in registers
 template <int CD, int F, bool MT>
-__global__ __launch_bounds__(256) void k_rk_col(DevState S, int kc, int krk, double dt, int flags_rt) {
+// (the CD6 N_SPEC = 1 stepping path, with its three interior forms, held to
+// 2 waves/SIMD)
+__global__ __launch_bounds__(256, (CD == 0 && F >= 0 && !(F & RK_MS)) ? 2 : 1) void k_rk_col(DevState S, int kc, int krk, double dt,
+                                                                              int flags_rt) {
   constexpr int R = CD == 1 ? 4 : 3, W = 2 * R + 1;
   const int flags = F >= 0 ? F : flags_rt;
   ColPos P;
@@ -1153,7 +1190,10 @@ __global__ __launch_bounds__(256) void k_rk_col(DevState S, int kc, int krk, dou
   const long plh = (long)S.nxh * S.nyh, pli = (long)S.nx * S.ny;
   const long qcol = hq(S, P.i, P.j, 0), ccol = iq(S, P.i, P.j, 1);
   const bool visc = flags & RK_VISC;
-  const bool xy_int = wave_all(central_cell(P.i, S.nx, R) && central_cell(P.j, S.ny, R));
+  // per-direction central forms: wavefronts at the xi faces (a fifth of them
+  // on 640 cells) keep the central eta and zeta forms
+  const bool xint = wave_all(central_cell(P.i, S.nx, R));
+  const bool yint = wave_all(central_cell(P.j, S.ny, R));
   double z[4][W];
   // Ev is stored over the interior only: planes outside 1..nz of a
   // non-periodic z are never read by the one-sided closures and stay zero
@@ -1174,14 +1214,29 @@ __global__ __launch_bounds__(256) void k_rk_col(DevState S, int kc, int krk, dou
     ld(W - 1, k + R);
     const long c = ccol + (long)(k - 1) * pli, q = qcol + (long)k * plh;
     double zs[4];
-    if (xy_int && central_cell(k, S.nz, R)) {
+    // per-direction forms on the CD6 N_SPEC = 1 stepping path; elsewhere the
+    // extra code would push the kernel into register spills, so one interior
+    // form
+    constexpr bool XFACE = CD == 0 && F >= 0 && !(F & RK_MS);
+    if constexpr (XFACE) {
+      if (central_cell(k, S.nz, R)) {
+#pragma unroll
+        for (int m = 0; m < 4; m++) zs[m] = visc ? deriv1<CD, true>(S, 2, k, [&](int o) { return z[m][o + R]; }) : 0.0;
+      } else {
+#pragma unroll
+        for (int m = 0; m < 4; m++) zs[m] = visc ? deriv1<CD>(S, 2, k, [&](int o) { return z[m][o + R]; }) : 0.0;
+      }
+      if (xint && yint) rk_cell<CD, F, true, true, MT>(S, c, q, P.i, P.j, k, krk, dt, flags_rt, zs);
+      else if (yint) rk_cell<CD, F, false, true, MT>(S, c, q, P.i, P.j, k, krk, dt, flags_rt, zs);
+      else rk_cell<CD, F, false, false, MT>(S, c, q, P.i, P.j, k, krk, dt, flags_rt, zs);
+    } else if (xint && yint && central_cell(k, S.nz, R)) {
 #pragma unroll
       for (int m = 0; m < 4; m++) zs[m] = visc ? deriv1<CD, true>(S, 2, k, [&](int o) { return z[m][o + R]; }) : 0.0;
-      rk_cell<CD, F, true, MT>(S, c, q, P.i, P.j, k, krk, dt, flags_rt, zs);
+      rk_cell<CD, F, true, true, MT>(S, c, q, P.i, P.j, k, krk, dt, flags_rt, zs);
     } else {
 #pragma unroll
       for (int m = 0; m < 4; m++) zs[m] = visc ? deriv1<CD>(S, 2, k, [&](int o) { return z[m][o + R]; }) : 0.0;
-      rk_cell<CD, F, false, MT>(S, c, q, P.i, P.j, k, krk, dt, flags_rt, zs);
+      rk_cell<CD, F, false, false, MT>(S, c, q, P.i, P.j, k, krk, dt, flags_rt, zs);
     }
 #pragma unroll
     for (int o = 0; o < W - 1; o++)
