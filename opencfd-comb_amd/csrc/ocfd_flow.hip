@@ -817,18 +817,26 @@ __global__ __launch_bounds__(256) void k_visc_prep(DevState S) {
   const int k = (int)(c / ((long)S.nx * S.ny)) + 1;
   const long q = hq(S, i, j, k);
   const long plh = (long)S.nxh * S.nyh;
-  if (wave_all(central_cell(i, S.nx, R) && central_cell(j, S.ny, R) && central_cell(k, S.nz, R))) {
-    auto dz = [&](const double* F) { return deriv1<CD, true>(S, 2, k, [&](int o) { return F[q + o * plh]; }); };
-    visc_prep_cell<CD, MS, true, true, MT>(S, c, q, i, j, k, dz(S.u), dz(S.v), dz(S.w), dz(S.T));
+  // central forms per direction (a wavefront at the xi faces keeps the
+  // central eta and zeta forms)
+  const bool ix = wave_all(central_cell(i, S.nx, R)), iy = wave_all(central_cell(j, S.ny, R));
+  double zd[4];
+  const double* zf[4] = {S.u, S.v, S.w, S.T};
+  if (wave_all(central_cell(k, S.nz, R))) {
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+      zd[v] = deriv1<CD, true>(S, 2, k, [&](int o) { return zf[v][q + o * plh]; });
   } else {
-    auto dz = [&](const double* F) {
-      return deriv1<CD>(S, 2, k, [&](int o) {
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+      zd[v] = deriv1<CD>(S, 2, k, [&](int o) {
         const int t = wrap(k + o, S.n[2], S.per[2]);
-        return F[q + (long)(t - k) * plh];
+        return zf[v][q + (long)(t - k) * plh];
       });
-    };
-    visc_prep_cell<CD, MS, false, false, MT>(S, c, q, i, j, k, dz(S.u), dz(S.v), dz(S.w), dz(S.T));
   }
+  if (ix && iy) visc_prep_cell<CD, MS, true, true, MT>(S, c, q, i, j, k, zd[0], zd[1], zd[2], zd[3]);
+  else if (iy) visc_prep_cell<CD, MS, false, true, MT>(S, c, q, i, j, k, zd[0], zd[1], zd[2], zd[3]);
+  else visc_prep_cell<CD, MS, false, false, MT>(S, c, q, i, j, k, zd[0], zd[1], zd[2], zd[3]);
 }
 
 // marching over k planes with the z windows of u, v, w, T in registers.

@@ -7,6 +7,8 @@
 // Same conventions as ocfd_device.h: FP64, no FMA contraction, the
 // reference's operation order.  Included once, by ocfd_flow.hip.
 #pragma once
+#include <type_traits>
+
 #include "ocfd_device.h"
 
 #define GAS_NT_MAX 4
@@ -426,7 +428,7 @@ __global__ __launch_bounds__(256) void k_species_conc(DevState S, int m0, int m1
 // Species m0 .. m1-1; Qp starts from zero at the first species (m0 = 0) and
 // from its stored value otherwise, so species taken one at a time add up in
 // the same order as all at once.
-template <int CD, bool INT>
+template <int CD, bool IX, bool IY, bool IZ>
 __device__ __forceinline__ void species_visc_prep_cell(const DevState& S, long c, long q, int i, int j, int k, int m0,
                                                        int m1) {
   const double T0 = S.T[q], d0 = S.d[q];
@@ -445,9 +447,9 @@ __device__ __forceinline__ void species_visc_prep_cell(const DevState& S, long c
   }
   for (int m = m0; m < m1; m++) {
     const double* C = S.cs + (long)m * ni + c;
-    auto dc = [&](int dir, int s) {
+    auto dc = [&](auto central, int dir, int s) {
       const long st = S.si[dir];
-      if constexpr (INT) {
+      if constexpr (decltype(central)::value) {
         return deriv1<CD, true>(S, dir, s, [&](int o) { return C[o * st]; });
       } else {
         return deriv1<CD>(S, dir, s, [&](int o) {
@@ -456,7 +458,8 @@ __device__ __forceinline__ void species_visc_prep_cell(const DevState& S, long c
         });
       }
     };
-    const double uk = dc(0, i), ui = dc(1, j), us = dc(2, k);
+    const double uk = dc(std::bool_constant<IX>{}, 0, i), ui = dc(std::bool_constant<IY>{}, 1, j),
+                 us = dc(std::bool_constant<IZ>{}, 2, k);
     const double dD = d0 * S.AmD[c + (long)m * ni];
     const double hi = gas_hi(m, T0);
     const double cix = uk * Akx + ui * Aix + us * Asx;
@@ -483,24 +486,27 @@ __global__ __launch_bounds__(256) void k_species_visc_prep(DevState S, int m0, i
   int i, j, k;
   cell_ijk(S, c, i, j, k);
   const long q = hq(S, i, j, k);
-  if (wave_all(central_cell(i, S.nx, R) && central_cell(j, S.ny, R) && central_cell(k, S.nz, R)))
-    species_visc_prep_cell<CD, true>(S, c, q, i, j, k, m0, m1);
-  else
-    species_visc_prep_cell<CD, false>(S, c, q, i, j, k, m0, m1);
+  // central forms per direction: a wavefront at the xi faces keeps the
+  // central eta and zeta forms
+  const bool ix = wave_all(central_cell(i, S.nx, R));
+  const bool iyz = wave_all(central_cell(j, S.ny, R) && central_cell(k, S.nz, R));
+  if (ix && iyz) species_visc_prep_cell<CD, true, true, true>(S, c, q, i, j, k, m0, m1);
+  else if (iyz) species_visc_prep_cell<CD, false, true, true>(S, c, q, i, j, k, m0, m1);
+  else species_visc_prep_cell<CD, false, false, false>(S, c, q, i, j, k, m0, m1);
 }
 
 // Species update: Dus += div(diffusive flux) Ajac (OCFD_scaler_viscous.f90:67-77)
 // and Scalar_time_adv_RK3 (OCFD_Time_Adv.f90:134-166) for every species;
 // krk = 0 stores the complete right-hand side in dus instead.
-template <int CD, bool INT>
+template <int CD, bool IX, bool IY, bool IZ>
 __device__ __forceinline__ void species_rk_cell(const DevState& S, long c, long q, int i, int j, int k, int krk,
                                                 double dt, int visc, int m0, int m1) {
   const long ni = S.ni;
   const double ajac = S.Ajac[q];
-  auto dv = [&](const double* F, int dir, int s) {
+  auto dv = [&](auto central, const double* F, int dir, int s) {
     const double* Fc = F + c;
     const long st = S.si[dir];
-    if constexpr (INT) {
+    if constexpr (decltype(central)::value) {
       return deriv1<CD, true>(S, dir, s, [&](int o) { return Fc[o * st]; });
     } else {
       return deriv1<CD>(S, dir, s, [&](int o) {
@@ -518,9 +524,9 @@ __device__ __forceinline__ void species_rk_cell(const DevState& S, long c, long 
     const double dold = krk >= 2 ? S.di[qm] : 0.0;
     double val = S.dus[cm];
     if (visc) {
-      const double uk = dv(S.Evs + (long)(3 * m) * ni, 0, i);
-      const double ui = dv(S.Evs + (long)(3 * m + 1) * ni, 1, j);
-      const double us = dv(S.Evs + (long)(3 * m + 2) * ni, 2, k);
+      const double uk = dv(std::bool_constant<IX>{}, S.Evs + (long)(3 * m) * ni, 0, i);
+      const double ui = dv(std::bool_constant<IY>{}, S.Evs + (long)(3 * m + 1) * ni, 1, j);
+      const double us = dv(std::bool_constant<IZ>{}, S.Evs + (long)(3 * m + 2) * ni, 2, k);
       val = val + (uk + ui + us) * ajac;
     }
     if (krk == 0) {  // right-hand side only
@@ -540,10 +546,11 @@ __global__ __launch_bounds__(256) void k_species_rk(DevState S, int krk, double 
   int i, j, k;
   cell_ijk(S, c, i, j, k);
   const long q = hq(S, i, j, k);
-  if (wave_all(central_cell(i, S.nx, R) && central_cell(j, S.ny, R) && central_cell(k, S.nz, R)))
-    species_rk_cell<CD, true>(S, c, q, i, j, k, krk, dt, visc, m0, m1);
-  else
-    species_rk_cell<CD, false>(S, c, q, i, j, k, krk, dt, visc, m0, m1);
+  const bool ix = wave_all(central_cell(i, S.nx, R));
+  const bool iyz = wave_all(central_cell(j, S.ny, R) && central_cell(k, S.nz, R));
+  if (ix && iyz) species_rk_cell<CD, true, true, true>(S, c, q, i, j, k, krk, dt, visc, m0, m1);
+  else if (iyz) species_rk_cell<CD, false, true, true>(S, c, q, i, j, k, krk, dt, visc, m0, m1);
+  else species_rk_cell<CD, false, false, false>(S, c, q, i, j, k, krk, dt, visc, m0, m1);
 }
 
 // Rescaling_di (Comput_flow_variables.f90:101-121) and comput_T_P (:32-46)
