@@ -2457,7 +2457,8 @@ void enq_stage(int krk, double dt1) {
   if (S.viscous) launch_visc_prep(false, vs);
   else launch_s(vs, "k_transport", k_transport, grid1(S.ni), dim3(256), S);
   // the step's saved solution (fn, din) is first read by this stage's
-  // k_rk, after the join: its copy runs beside the sweeps
+  // k_rk, after the join: its copy runs beside the sweeps (k_save0 is
+  // launched inside stage 1 on either path)
   if (krk == 1) launch_s(vs, "k_save0", k_save0, grid1(S.ni), dim3(256), S);
   if (krk == 1 && G.cfg.scheme_invis == OCFD_SCHEME_HYBRID) enq_rhybrid();
   launch_sweep<0>();
@@ -2537,6 +2538,9 @@ void enq_stage_ms(int krk, double dt1) {
   // the second stream beside the flow sweeps, which read no di; the flow's
   // viscous preparation waits for Qp, k_rk (Rescaling_di) for the whole chain
   const hipStream_t sp = fork_side();
+  // the step's saved solution: din is first read by the species update on
+  // the second stream, fn by k_rk after the join
+  if (krk == 1) launch_s(sp, "k_save0", k_save0, grid1(S.ni), dim3(256), S);
   enq_species(krk, dt1, sp, sp != G.stream ? G.ev_qp : nullptr);
   launch_sweep<0>();
   launch_sweep<1>();
@@ -2556,7 +2560,6 @@ void enq_stage_ms(int krk, double dt1) {
 void enq_step(double dt1) {
   const DevState& S = G.S;
   launch("k_negt_reset", k_negt_reset, dim3(1), dim3(64), S, 1);
-  if (S.ns > 1) launch("k_save0", k_save0, grid1(S.ni), dim3(256), S);  // N_SPEC = 1: inside stage 1
   for (int krk = 1; krk <= 3; krk++) {
     if (S.ns > 1) enq_stage_ms(krk, dt1);
     else enq_stage(krk, dt1);
