@@ -184,7 +184,11 @@ def test_column_planes_bitwise(pkg, vis):
                        (fc.small_isotropic(20), 1),
                        # rows and columns that leave whole wavefronts and part
                        # of one past the grid edge (tiles are 64 x 4)
-                       (fc.small_flat_plate(40, 30, 13), 0)):
+                       (fc.small_flat_plate(40, 30, 13), 0),
+                       # rows of 160 cells: the wavefronts of i = 65..128 (and,
+                       # one cell per thread, of i = 33..96 on odd rows) have
+                       # the central xi stencil and take the central forms
+                       (fc.small_flat_plate(160, 12, 9), 0)):
         case = list(case)
         case[0].scheme_vis = getattr(pkg, "SCHEME_" + vis)
         case[0].periodic[2] = zper  # non-periodic z: windows meet the one-sided closures

@@ -93,6 +93,22 @@ def test_rk3_step(pkg, iflag):
     g.close()
 
 
+def test_rk3_step_wide_rows(pkg):
+    """Rows of 160 cells, so that whole wavefronts have the central xi
+    stencil (i = 65..128 on even rows, 33..96 on odd ones): the central xi,
+    eta and zeta forms of the multi-species viscous kernels and of the flow's
+    viscous kernels run beside the face forms; one RK3 step against the
+    oracle."""
+    case = fc.small_jet(nx=160, ny=12, nz=10)
+    g, o = pair(pkg, case)
+    dt = case[5]["dt"]
+    g.flow_time_advance(dt)
+    o.flow_time_advance(dt)
+    compare_all(g, o)
+    assert g.negative_et_count() == o.negative_et_count()
+    g.close()
+
+
 def test_column_planes_bitwise(pkg):
     """The k-marching viscous-flux and update kernels (non-periodic z, where
     the windows meet the one-sided closures) give the one-thread-per-cell
