@@ -251,7 +251,7 @@ def chem_bench(m, ncell=1 << 21, cpu=True, surrogate_cells=1 << 24, surrogate_ep
     return out, S
 
 
-def jet_bench(m, S, nx=256, ny=192, nz=128, steps=10, warmup=2, tau=2e-5, cfl=0.5):
+def jet_bench(m, S, nx=256, ny=192, nz=128, steps=10, warmup=2, tau=2e-5, cfl=0.5, serial=False):
     """The reacting planar H2 jet (WENO5, 9 species, real gas, Chemkin/Wilke
     transport, BC_JET) in NS_solver steps: flow dt/2, chemistry over dt,
     comput_T_P, flow dt/2.  The field is first pre-reacted for tau so the
@@ -266,6 +266,7 @@ def jet_bench(m, S, nx=256, ny=192, nz=128, steps=10, warmup=2, tau=2e-5, cfl=0.
     mech = info["mech"]
     chemistry.ChemBatch(mech)
     g = m.FlowSolver(cfg)
+    g.set_option(m.OPT_CONCURRENT, 0 if serial else 1)
     g.set_mesh(*mesh, span)
     cases.jet_gas(g, mech)
     g.set_bc_jet(bc["inlet2d"])
@@ -308,7 +309,8 @@ def jet_bench(m, S, nx=256, ny=192, nz=128, steps=10, warmup=2, tau=2e-5, cfl=0.
     out = dict(workload=f"reacting planar H2 jet {nx}x{ny}x{nz}, 9 species / 19 reactions, WENO5 + SW, CD6, "
                         "real gas, Chemkin/Wilke transport, mixture-averaged diffusion, BC_JET",
                dt=dt, cfl=cfl, pre_react_s=t_pre, pre_react_tau=tau, steps=steps,
-               step="NS_solver: flow RK3 dt/2, chemistry dt, comput_T_P, flow RK3 dt/2")
+               step="NS_solver: flow RK3 dt/2, chemistry dt, comput_T_P, flow RK3 dt/2",
+               streams=1 if serial else 2)
     ms_flow = run(m.CHEM_FIXED, reaction=False)
     out["flow_only_ms_per_step"] = ms_flow
     rate = ncell * 2 / ms_flow / 1e3
@@ -431,7 +433,8 @@ def main():
         return
     if args.only_jet:
         m = load_pkg()
-        print(json.dumps({"jet": jet_bench(m, None, args.jet_nx, args.jet_ny, args.jet_nz, steps=args.jet_steps)}))
+        print(json.dumps({"jet": jet_bench(m, None, args.jet_nx, args.jet_ny, args.jet_nz, steps=args.jet_steps,
+                                           serial=args.serial)}))
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -582,7 +585,7 @@ def main():
                                                             surrogate_epochs=args.surrogate_epochs)
     jet = None
     if not args.no_jet:
-        jet = jet_bench(m, S, args.jet_nx, args.jet_ny, args.jet_nz, steps=args.jet_steps)
+        jet = jet_bench(m, S, args.jet_nx, args.jet_ny, args.jet_nz, steps=args.jet_steps, serial=args.serial)
     out = {
         "metric": METRIC, "value": value, "unit": "Mcell-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
