@@ -251,6 +251,34 @@ def chem_bench(m, ncell=1 << 21, cpu=True, surrogate_cells=1 << 24, surrogate_ep
     return out, S
 
 
+def surrogate_inference_only(tile, n=1 << 21, ndata=1 << 18):
+    """The three surrogate inference launches alone on n cells (the C4
+    sampler's states, repeated), untrained network, no fallback: the cost of
+    the inference does not depend on the weights."""
+    import torch
+    load_pkg()
+    from ocfd_amd import chemistry, mechanism, surrogate
+    M = mechanism.Mechanism()
+    ch = chemistry.ChemBatch(M)
+    S = surrogate.Surrogate(ch, seed=0)
+    data = S.dataset(ndata, seed=1)
+    S.configure(data)
+    idx = torch.arange(n, device="cuda") % ndata
+    rho0, E0, dt = data["rho0"][:, idx].contiguous(), data["E0"][idx].contiguous(), data["dt"][idx].contiguous()
+    S.tile(tile)
+    ms = []
+    for it in range(5):
+        rho, E = rho0.clone(), E0.clone()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        S.step(rho, E, dt, fallback=False)
+        b.record()
+        torch.cuda.synchronize()
+        if it >= 2:
+            ms.append(a.elapsed_time(b))
+    return dict(cells=n, tile=S.tile(), ms=ms)
+
+
 def jet_bench(m, S, nx=256, ny=192, nz=128, steps=10, warmup=2, tau=2e-5, cfl=0.5, serial=False):
     """The reacting planar H2 jet (WENO5, 9 species, real gas, Chemkin/Wilke
     transport, BC_JET) in NS_solver steps: flow dt/2, chemistry over dt,
@@ -418,6 +446,9 @@ def main():
     ap.add_argument("--jet-nx", type=int, default=256)
     ap.add_argument("--jet-ny", type=int, default=192)
     ap.add_argument("--jet-nz", type=int, default=128)
+    ap.add_argument("--surrogate-inference-only", type=int, default=-1, metavar="TILE",
+                    help="only the surrogate inference (untrained network, no fallback) on 2^21 cells with the "
+                         "given network tile shape, 3 timed calls: a short program for counter runs")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a plain 512 MiB device copy (3 times): the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
@@ -430,6 +461,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"calib_copy_MiB": 512, "repeats": 3}))
+        return
+    if args.surrogate_inference_only >= 0:
+        print(json.dumps({"surrogate_inference": surrogate_inference_only(args.surrogate_inference_only)}))
         return
     if args.only_jet:
         m = load_pkg()
